@@ -1,0 +1,185 @@
+/*
+ * oneline_hip.h -- C ABI of the MI355X (gfx950) parse-and-search engine.
+ *
+ * Drop-in boundary for the `oneline-gpu` crate's public API
+ * (crates/oneline-gpu/src/lib.rs:27-36).  Every entry point below names the
+ * reference item it replaces.  Plain pointers and sizes only; no C++ or torch
+ * types cross this boundary.
+ *
+ * Conventions (mirroring the reference's behaviour):
+ *   - every call returns an int status: OL_OK (0) or a negative OL_ERR_*;
+ *     ol_last_error() returns the calling thread's last message;
+ *   - bad input is an error, never a crash, and an error does not poison the
+ *     context (crates/oneline-gpu/src/query.rs:112-118);
+ *   - empty input returns empty results without touching the GPU
+ *     (crates/oneline-gpu/src/parse.rs:88-90, document.rs:60-67);
+ *   - truncated outputs keep the true totals (crates/oneline-gpu/src/tests.rs:375-397);
+ *   - arrays returned through `T **out` are allocated by the library and
+ *     released with ol_free(); arrays passed as `T *out` are caller-owned;
+ *   - a context serialises its calls with an internal lock (the reference's
+ *     Mutex-guarded scratch, crates/oneline-gpu/src/parser.rs:15-19), so it may
+ *     be shared between threads.
+ */
+#ifndef ONELINE_HIP_H
+#define ONELINE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OL_OK 0
+#define OL_ERR_INVALID (-1)      /* bad argument */
+#define OL_ERR_TOO_LARGE (-2)    /* input exceeds OL_MAX_GPU_INPUT_LEN */
+#define OL_ERR_PATTERN (-3)      /* empty or too long pattern */
+#define OL_ERR_TOO_MANY (-4)     /* too many patterns, or a count that does not fit */
+#define OL_ERR_DEVICE (-5)       /* HIP runtime / device failure */
+#define OL_ERR_TOO_DEEP (-6)     /* content line nested deeper than OL_MAX_TREE_DEPTH - 1 */
+#define OL_ERR_NOMEM (-7)        /* host or device allocation failed, or HBM budget exceeded */
+
+/* Limits.  MAX_GPU_INPUT_LEN: crates/oneline-gpu/src/util.rs:8.
+ * MAX_PATTERN_LEN: crates/oneline-gpu/src/lib.rs:67.  MAX_TREE_DEPTH:
+ * crates/oneline-gpu/src/tree.rs:12.  The dictionary limits are wider than the
+ * reference's (crates/oneline-gpu/src/many.rs:14-18: 4..=64 bytes, 2,048 patterns):
+ * 1..=64 bytes and up to OL_MAX_PATTERNS patterns. */
+#define OL_MAX_GPU_INPUT_LEN 4294967292ull
+#define OL_MAX_PATTERN_LEN 64
+#define OL_MIN_SET_PATTERN_LEN 1
+#define OL_MAX_SET_PATTERN_LEN 64
+#define OL_MAX_PATTERNS (1u << 22)
+#define OL_MAX_TREE_DEPTH 32
+#define OL_NONE 0xFFFFFFFFu
+
+/* FlatNode, crates/oneline-core/src/lib.rs:87-96 (16 bytes, same layout). */
+typedef struct ol_flat_node {
+    uint32_t depth;
+    uint32_t kind;       /* NodeKind: 0 blank .. 7 comment (lib.rs:41-51) */
+    uint32_t span_start;
+    uint32_t span_len;
+} ol_flat_node;
+
+/* ManyHit, crates/oneline-gpu/src/many.rs:75-82. */
+typedef struct ol_many_hit {
+    uint32_t pattern;
+    uint32_t offset;
+} ol_many_hit;
+
+typedef struct ol_ctx ol_ctx;    /* GpuParser      crates/oneline-gpu/src/parser.rs:8-57 */
+typedef struct ol_doc ol_doc;    /* GpuDocument    crates/oneline-gpu/src/document.rs:16-31 */
+typedef struct ol_tree ol_tree;  /* GpuTree        crates/oneline-gpu/src/tree.rs:29-46 */
+typedef struct ol_dict ol_dict;  /* GpuPatternSet  crates/oneline-gpu/src/many.rs:116-150 */
+
+/* ---- context ------------------------------------------------------------------ */
+/* GpuParser::new (crates/oneline-gpu/src/parser.rs:94).  Uses HIP device 0. */
+int ol_ctx_create(ol_ctx **out);
+void ol_ctx_destroy(ol_ctx *ctx);
+/* GpuParser::adapter_name (parser.rs:470). */
+int ol_ctx_adapter_name(ol_ctx *ctx, char *buf, size_t cap);
+/* GpuParser::is_software (parser.rs:476): always 0 on a HIP device. */
+int ol_ctx_is_software(ol_ctx *ctx);
+/* GpuParser::timestamps_supported (crates/oneline-gpu/src/document.rs:130). */
+int ol_ctx_timestamps_supported(ol_ctx *ctx);
+/* Waits for all work of the context. */
+int ol_ctx_synchronize(ol_ctx *ctx);
+/* Thread-local message of the last failed call ("" if none). */
+const char *ol_last_error(void);
+/* Releases an array returned through a `T **out` parameter. */
+void ol_free(void *p);
+
+/* ---- one-shot parse ------------------------------------------------------------ */
+/* GpuParser::scan_line_offsets (crates/oneline-gpu/src/scan.rs:127). */
+int ol_scan_line_offsets(ol_ctx *ctx, const uint8_t *text, size_t len, uint32_t **offsets,
+                         size_t *line_count);
+/* GpuParser::parse_raw (crates/oneline-gpu/src/parse.rs:86): offsets + nodes. */
+int ol_parse_raw(ol_ctx *ctx, const uint8_t *text, size_t len, ol_flat_node **nodes,
+                 uint32_t **offsets, size_t *line_count);
+/* GpuParser::parse (crates/oneline-gpu/src/parse.rs:104): nodes for caller-given
+ * line offsets (line i spans offsets[i] .. offsets[i+1], the last to the end). */
+int ol_parse(ol_ctx *ctx, const uint8_t *text, size_t len, const uint32_t *offsets,
+             size_t line_count, ol_flat_node **nodes);
+
+/* ---- resident documents ---------------------------------------------------------- */
+/* GpuParser::upload (crates/oneline-gpu/src/document.rs:54): upload, scan and
+ * classify once; text, offsets, kinds, keys and nodes stay in HBM. */
+int ol_doc_upload(ol_ctx *ctx, const uint8_t *text, size_t len, ol_doc **out);
+/* Same as ol_doc_upload without the resident 16-byte FlatNode array: offsets,
+ * kinds and tree keys only (the per-line outputs the tree and searches need). */
+int ol_doc_upload_lean(ol_ctx *ctx, const uint8_t *text, size_t len, ol_doc **out);
+void ol_doc_free(ol_doc *doc);
+/* GpuDocument::line_count / text_len (document.rs:35,47). */
+size_t ol_doc_line_count(const ol_doc *doc);
+size_t ol_doc_text_len(const ol_doc *doc);
+/* Copies the resident per-line arrays to the host (any pointer may be NULL;
+ * each array holds ol_doc_line_count entries).  Classes are NodeKind as u8. */
+int ol_doc_download(ol_ctx *ctx, const ol_doc *doc, uint32_t *offsets, uint8_t *classes,
+                    ol_flat_node *nodes);
+/* Re-runs scan + classify over the resident text into the resident arrays
+ * (same results as ol_doc_upload; asynchronous: no host round trip). */
+int ol_doc_reparse(ol_ctx *ctx, ol_doc *doc);
+
+/* ---- tree links ------------------------------------------------------------------- */
+/* GpuParser::build_tree / build_tree_reusing (crates/oneline-gpu/src/tree.rs:53,60).
+ * `reuse` (nullable) is consumed: its buffers are reused when large enough.
+ * Fails with OL_ERR_TOO_DEEP if a content line is nested >= OL_MAX_TREE_DEPTH. */
+int ol_tree_build(ol_ctx *ctx, const ol_doc *doc, ol_tree *reuse, ol_tree **out);
+void ol_tree_free(ol_tree *tree);
+/* GpuTree::line_count (tree.rs:43). */
+size_t ol_tree_line_count(const ol_tree *tree);
+/* GpuParser::download_links (tree.rs:278), plus the derived first_child and
+ * next_sibling.  Any pointer may be NULL; each holds line_count entries. */
+int ol_tree_download(ol_ctx *ctx, const ol_tree *tree, uint32_t *parent, uint32_t *subtree_end,
+                     uint32_t *first_child, uint32_t *next_sibling);
+/* Rebuilds the links of `doc` into `tree` (same line count) without a host
+ * round trip; the error flag is checked by the next ol_tree_check. */
+int ol_tree_rebuild_async(ol_ctx *ctx, const ol_doc *doc, ol_tree *tree);
+int ol_tree_check(ol_ctx *ctx, const ol_tree *tree);
+
+/* ---- literal search --------------------------------------------------------------- */
+/* GpuParser::search_document (crates/oneline-gpu/src/search.rs:84): 1..=64-byte
+ * pattern; total = every overlapping occurrence; offsets = the first
+ * min(total, max_matches) occurrences in ascending order (library-allocated). */
+int ol_search_document(ol_ctx *ctx, const ol_doc *doc, const uint8_t *pattern, size_t len,
+                       size_t max_matches, uint32_t *total, uint32_t **offsets, size_t *n_offsets);
+/* GpuParser::search_raw (search.rs:107): uploads, then searches. */
+int ol_search_raw(ol_ctx *ctx, const uint8_t *text, size_t text_len, const uint8_t *pattern,
+                  size_t len, size_t max_matches, uint32_t *total, uint32_t **offsets,
+                  size_t *n_offsets);
+/* Device-resident form: u64 offsets written to device memory `d_offsets`
+ * (capacity `cap`), *total = all matches.  Synchronous. */
+int ol_search_device(ol_ctx *ctx, const ol_doc *doc, const uint8_t *pattern, size_t len,
+                     uint64_t *d_offsets, size_t cap, uint64_t *total);
+
+/* ---- dictionary search ------------------------------------------------------------ */
+/* GpuParser::compile_patterns (crates/oneline-gpu/src/many.rs:158).  `doc` is
+ * accepted for signature parity and may be NULL (the set is not bound to it). */
+int ol_dict_compile(ol_ctx *ctx, const ol_doc *doc, const uint8_t *const *patterns,
+                    const size_t *lens, size_t count, ol_dict **out);
+void ol_dict_free(ol_dict *dict);
+/* GpuPatternSet::len (many.rs:141). */
+size_t ol_dict_len(const ol_dict *dict);
+/* GpuParser::search_many (many.rs:391): counts[dict_len] per pattern,
+ * *total_hits = their sum, hits = min(total, max_hits) hits sorted by
+ * (pattern, offset) (library-allocated; none when max_hits == 0). */
+int ol_search_many(ol_ctx *ctx, const ol_doc *doc, const ol_dict *dict, size_t max_hits,
+                   uint32_t *counts, uint64_t *total_hits, ol_many_hit **hits, size_t *n_hits);
+/* Offset-major form: every (offset, pattern id) sorted by offset then id,
+ * written to device memory (d_offsets u64, d_ids u32; capacity `cap`, the
+ * first `cap` kept); *total = all matches.  Synchronous. */
+int ol_search_many_device(ol_ctx *ctx, const ol_doc *doc, const ol_dict *dict, uint64_t *d_offsets,
+                          uint32_t *d_ids, size_t cap, uint64_t *total);
+
+/* ---- profiling (the reference's Profile / StageTimer, crates/oneline-gpu/src/timing.rs:13-123) --- */
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream.  ol_profile_take waits for the stream and returns, per
+ * kernel name, the launch count and the summed milliseconds, then resets. */
+int ol_profile_enable(ol_ctx *ctx, int on);
+int ol_profile_take(ol_ctx *ctx, char *names, size_t name_stride, uint64_t *counts, double *ms,
+                    size_t cap, size_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ONELINE_HIP_H */

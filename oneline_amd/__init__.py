@@ -1,0 +1,422 @@
+"""oneline_amd -- MI355X-native engine for oneline's GPU parse-and-search path.
+
+Python mirror of the reference's ``oneline-gpu`` public API
+(crates/oneline-gpu/src/lib.rs:27-36) over the C ABI in ``include/oneline_hip.h``:
+``GpuParser`` with ``scan_line_offsets``, ``parse_raw``, ``parse``, ``upload``,
+``build_tree`` / ``build_tree_reusing``, ``download_links``, ``search_document``,
+``search_raw``, ``compile_patterns`` and ``search_many``.  Same names, argument
+meaning and error behaviour: bad input raises ``OnelineError`` (the reference's
+``anyhow::Error``) and leaves the parser usable.
+
+The native library must be built (``python -m oneline_amd.build``); nothing here
+falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboneline_hip.so")
+
+NONE = 0xFFFFFFFF
+MAX_GPU_INPUT_LEN = 4294967292
+MAX_PATTERN_LEN = 64
+MIN_SET_PATTERN_LEN = 1
+MAX_SET_PATTERN_LEN = 64
+MAX_PATTERNS = 1 << 22
+MAX_TREE_DEPTH = 32
+
+# NodeKind, crates/oneline-core/src/lib.rs:41-51
+BLANK, TEXT, HEADING, ELEMENT, LIST, CONNECT, ORDERED_LIST, COMMENT = range(8)
+
+# FlatNode, crates/oneline-core/src/lib.rs:87-96
+FLAT_NODE = np.dtype([("depth", "<u4"), ("kind", "<u4"), ("span_start", "<u4"), ("span_len", "<u4")])
+
+OL_OK = 0
+ERR_NAMES = {-1: "invalid", -2: "too_large", -3: "pattern", -4: "too_many", -5: "device",
+             -6: "too_deep", -7: "nomem"}
+
+
+class OnelineError(RuntimeError):
+    """A failed engine call (bad input, device failure, ...)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{message} [{ERR_NAMES.get(code, code)}]")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Loads liboneline_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m oneline_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    S = ctypes.c_size_t
+    U64P = ctypes.POINTER(ctypes.c_uint64)
+    U32P = ctypes.POINTER(ctypes.c_uint32)
+    sig = {
+        "ol_ctx_create": [ctypes.POINTER(P)],
+        "ol_ctx_destroy": [P],
+        "ol_ctx_adapter_name": [P, ctypes.c_char_p, S],
+        "ol_ctx_is_software": [P],
+        "ol_ctx_timestamps_supported": [P],
+        "ol_ctx_synchronize": [P],
+        "ol_free": [P],
+        "ol_scan_line_offsets": [P, P, S, ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_parse_raw": [P, P, S, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_parse": [P, P, S, P, S, ctypes.POINTER(P)],
+        "ol_doc_upload": [P, P, S, ctypes.POINTER(P)],
+        "ol_doc_upload_lean": [P, P, S, ctypes.POINTER(P)],
+        "ol_doc_free": [P],
+        "ol_doc_line_count": [P],
+        "ol_doc_text_len": [P],
+        "ol_doc_download": [P, P, P, P, P],
+        "ol_doc_reparse": [P, P],
+        "ol_tree_build": [P, P, P, ctypes.POINTER(P)],
+        "ol_tree_free": [P],
+        "ol_tree_line_count": [P],
+        "ol_tree_download": [P, P, P, P, P, P],
+        "ol_tree_rebuild_async": [P, P, P],
+        "ol_tree_check": [P, P],
+        "ol_search_document": [P, P, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_search_raw": [P, P, S, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_search_device": [P, P, P, S, P, S, U64P],
+        "ol_dict_compile": [P, P, P, P, S, ctypes.POINTER(P)],
+        "ol_dict_free": [P],
+        "ol_dict_len": [P],
+        "ol_search_many": [P, P, P, S, P, U64P, ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_search_many_device": [P, P, P, P, P, S, U64P],
+        "ol_profile_enable": [P, ctypes.c_int],
+        "ol_profile_take": [P, P, S, P, P, S, ctypes.POINTER(S)],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    for name in ("ol_doc_line_count", "ol_doc_text_len", "ol_tree_line_count", "ol_dict_len"):
+        getattr(lib, name).restype = ctypes.c_size_t
+    for name in ("ol_ctx_destroy", "ol_free", "ol_doc_free", "ol_tree_free", "ol_dict_free"):
+        getattr(lib, name).restype = None
+    lib.ol_last_error.argtypes = []
+    lib.ol_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != OL_OK:
+        msg = _lib.ol_last_error().decode("utf-8", "replace")
+        raise OnelineError(rc, msg)
+
+
+def as_bytes_array(text) -> np.ndarray:
+    """Zero-copy uint8 view of str/bytes/bytearray/ndarray input."""
+    if isinstance(text, np.ndarray):
+        return np.ascontiguousarray(text, dtype=np.uint8).reshape(-1)
+    if isinstance(text, str):
+        text = text.encode("utf-8")
+    return np.frombuffer(bytes(text), dtype=np.uint8) if len(text) else np.zeros(0, np.uint8)
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data) if a.size else ctypes.c_void_p(0)
+
+
+def _take(ptr: ctypes.c_void_p, count: int, dtype) -> np.ndarray:
+    """Copies a library-allocated array into numpy and frees it."""
+    dtype = np.dtype(dtype)
+    if not ptr.value:
+        return np.zeros(0, dtype)
+    try:
+        if count == 0:
+            return np.zeros(0, dtype)
+        buf = (ctypes.c_char * (count * dtype.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dtype, count=count).copy()
+    finally:
+        _lib.ol_free(ptr)
+
+
+# ---- result types (crates/oneline-core/src/lib.rs:113-158, links.rs:22-47,
+#      crates/oneline-gpu/src/search.rs:12-21, many.rs:75-94) -----------------------------
+@dataclass
+class FlatTree:
+    source: np.ndarray
+    nodes: np.ndarray          # FLAT_NODE records
+    line_offsets: np.ndarray   # u32
+
+    def content_of(self, i: int) -> bytes:
+        nd = self.nodes[i]
+        s = int(nd["span_start"])
+        return bytes(self.source[s:s + int(nd["span_len"])])
+
+
+@dataclass
+class TreeLinks:
+    parent: np.ndarray
+    subtree_end: np.ndarray
+    first_child: np.ndarray
+    next_sibling: np.ndarray
+
+    def __len__(self):
+        return len(self.parent)
+
+    def parent_of(self, line: int) -> Optional[int]:
+        p = int(self.parent[line])
+        return None if p == NONE else p
+
+    def subtree(self, line: int) -> range:
+        return range(line, int(self.subtree_end[line]))
+
+
+@dataclass
+class SearchResult:
+    total_matches: int
+    offsets: np.ndarray
+
+
+@dataclass
+class ManyResult:
+    counts: np.ndarray
+    total_hits: int
+    hits: np.ndarray = field(default_factory=lambda: np.zeros(0, np.dtype([("pattern", "<u4"), ("offset", "<u4")])))
+
+
+MANY_HIT = np.dtype([("pattern", "<u4"), ("offset", "<u4")])
+
+
+class GpuDocument:
+    """A document uploaded, scanned and classified once, resident in HBM
+    (crates/oneline-gpu/src/document.rs:16-90)."""
+
+    def __init__(self, parser: "GpuParser", handle: ctypes.c_void_p):
+        self._parser = parser
+        self._h = handle
+
+    def line_count(self) -> int:
+        return int(_lib.ol_doc_line_count(self._h))
+
+    def text_len(self) -> int:
+        return int(_lib.ol_doc_text_len(self._h))
+
+    def download(self, nodes: bool = False):
+        """(offsets u32, classes u8, nodes FLAT_NODE or None) of the resident document."""
+        L = self.line_count()
+        off = np.zeros(L, np.uint32)
+        cls = np.zeros(L, np.uint8)
+        nd = np.zeros(L, FLAT_NODE) if nodes else None
+        _check(_lib.ol_doc_download(self._parser._ctx, self._h, _ptr(off), _ptr(cls),
+                                    _ptr(nd) if nd is not None else None))
+        return off, cls, nd
+
+    def close(self) -> None:
+        if self._h:
+            _lib.ol_doc_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuTree:
+    """parent / subtree_end / first_child / next_sibling of a document, resident
+    in HBM (crates/oneline-gpu/src/tree.rs:29-46)."""
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._h = handle
+
+    def line_count(self) -> int:
+        return int(_lib.ol_tree_line_count(self._h))
+
+    def _release(self) -> ctypes.c_void_p:
+        h, self._h = self._h, None
+        return h
+
+    def close(self) -> None:
+        if self._h:
+            _lib.ol_tree_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuPatternSet:
+    """A compiled dictionary (crates/oneline-gpu/src/many.rs:116-150)."""
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._h = handle
+
+    def __len__(self) -> int:
+        return int(_lib.ol_dict_len(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.ol_dict_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuParser:
+    """GpuParser (crates/oneline-gpu/src/parser.rs:8-57) over HIP device 0."""
+
+    def __init__(self):
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.ol_ctx_create(ctypes.byref(h)))
+        self._ctx = h
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            _lib.ol_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- info -------------------------------------------------------------------
+    def adapter_name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check(_lib.ol_ctx_adapter_name(self._ctx, buf, 256))
+        return buf.value.decode()
+
+    def is_software(self) -> bool:
+        return bool(_lib.ol_ctx_is_software(self._ctx))
+
+    def timestamps_supported(self) -> bool:
+        return bool(_lib.ol_ctx_timestamps_supported(self._ctx))
+
+    def synchronize(self) -> None:
+        _check(_lib.ol_ctx_synchronize(self._ctx))
+
+    # -- parse --------------------------------------------------------------------
+    def scan_line_offsets(self, text) -> np.ndarray:
+        t = as_bytes_array(text)
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        _check(_lib.ol_scan_line_offsets(self._ctx, _ptr(t), t.size, ctypes.byref(p), ctypes.byref(n)))
+        return _take(p, n.value, np.uint32)
+
+    def parse_raw(self, text) -> FlatTree:
+        t = as_bytes_array(text)
+        pn = ctypes.c_void_p()
+        po = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        _check(_lib.ol_parse_raw(self._ctx, _ptr(t), t.size, ctypes.byref(pn), ctypes.byref(po), ctypes.byref(n)))
+        return FlatTree(t, _take(pn, n.value, FLAT_NODE), _take(po, n.value, np.uint32))
+
+    def parse(self, text, line_offsets) -> FlatTree:
+        t = as_bytes_array(text)
+        o = np.ascontiguousarray(line_offsets, dtype=np.uint32)
+        pn = ctypes.c_void_p()
+        _check(_lib.ol_parse(self._ctx, _ptr(t), t.size, _ptr(o), o.size, ctypes.byref(pn)))
+        return FlatTree(t, _take(pn, o.size, FLAT_NODE), o.copy())
+
+    def upload(self, text, nodes: bool = True) -> GpuDocument:
+        t = as_bytes_array(text)
+        h = ctypes.c_void_p()
+        fn = _lib.ol_doc_upload if nodes else _lib.ol_doc_upload_lean
+        _check(fn(self._ctx, _ptr(t), t.size, ctypes.byref(h)))
+        return GpuDocument(self, h)
+
+    # -- tree -----------------------------------------------------------------------
+    def build_tree(self, doc: GpuDocument) -> GpuTree:
+        return self.build_tree_reusing(doc, None)
+
+    def build_tree_reusing(self, doc: GpuDocument, previous: Optional[GpuTree]) -> GpuTree:
+        reuse = previous._release() if previous is not None else None
+        h = ctypes.c_void_p()
+        _check(_lib.ol_tree_build(self._ctx, doc._h, reuse, ctypes.byref(h)))
+        return GpuTree(h)
+
+    def download_links(self, tree: GpuTree) -> TreeLinks:
+        L = tree.line_count()
+        arrs = [np.zeros(L, np.uint32) for _ in range(4)]
+        _check(_lib.ol_tree_download(self._ctx, tree._h, *[_ptr(a) for a in arrs]))
+        return TreeLinks(*arrs)
+
+    # -- literal search ---------------------------------------------------------------
+    def search_document(self, doc: GpuDocument, pattern: bytes, max_matches: int) -> SearchResult:
+        pat = as_bytes_array(pattern)
+        tot = ctypes.c_uint32()
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        if max_matches < 0:
+            raise OnelineError(-1, "max_matches must be >= 0")
+        _check(_lib.ol_search_document(self._ctx, doc._h, _ptr(pat), pat.size, max_matches,
+                                       ctypes.byref(tot), ctypes.byref(p), ctypes.byref(n)))
+        return SearchResult(tot.value, _take(p, n.value, np.uint32))
+
+    def search_raw(self, text, pattern: bytes, max_matches: int) -> SearchResult:
+        t = as_bytes_array(text)
+        pat = as_bytes_array(pattern)
+        tot = ctypes.c_uint32()
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        _check(_lib.ol_search_raw(self._ctx, _ptr(t), t.size, _ptr(pat), pat.size, max_matches,
+                                  ctypes.byref(tot), ctypes.byref(p), ctypes.byref(n)))
+        return SearchResult(tot.value, _take(p, n.value, np.uint32))
+
+    # -- dictionary search --------------------------------------------------------------
+    def compile_patterns(self, doc: Optional[GpuDocument], patterns: Sequence[bytes]) -> GpuPatternSet:
+        arrs = [as_bytes_array(p) for p in patterns]
+        keep = [np.concatenate([a, np.zeros(1, np.uint8)]) for a in arrs]   # non-null even when empty
+        ptrs = (ctypes.c_void_p * max(1, len(keep)))(*[k.ctypes.data for k in keep])
+        lens = (ctypes.c_size_t * max(1, len(keep)))(*[a.size for a in arrs])
+        h = ctypes.c_void_p()
+        _check(_lib.ol_dict_compile(self._ctx, doc._h if doc is not None else None,
+                                    ctypes.cast(ptrs, ctypes.c_void_p), ctypes.cast(lens, ctypes.c_void_p),
+                                    len(arrs), ctypes.byref(h)))
+        return GpuPatternSet(h)
+
+    def search_many(self, doc: GpuDocument, pset: GpuPatternSet, max_hits: int) -> ManyResult:
+        counts = np.zeros(len(pset), np.uint32)
+        tot = ctypes.c_uint64()
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        _check(_lib.ol_search_many(self._ctx, doc._h, pset._h, max_hits, _ptr(counts), ctypes.byref(tot),
+                                   ctypes.byref(p), ctypes.byref(n)))
+        return ManyResult(counts, tot.value, _take(p, n.value, MANY_HIT))
+
+    # -- profiling ----------------------------------------------------------------------
+    def profile_enable(self, on: bool = True) -> None:
+        _check(_lib.ol_profile_enable(self._ctx, 1 if on else 0))
+
+    def profile_take(self) -> dict:
+        """{kernel name: (launches, total ms)} since the last call."""
+        cap, stride = 64, 64
+        names = ctypes.create_string_buffer(cap * stride)
+        counts = np.zeros(cap, np.uint64)
+        ms = np.zeros(cap, np.float64)
+        n = ctypes.c_size_t()
+        _check(_lib.ol_profile_take(self._ctx, names, stride, _ptr(counts), _ptr(ms), cap, ctypes.byref(n)))
+        out = {}
+        raw = names.raw
+        for i in range(min(n.value, cap)):
+            nm = raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode()
+            out[nm] = (int(counts[i]), float(ms[i]))
+        return out

@@ -1,0 +1,1128 @@
+// api.cpp -- the C ABI (include/oneline_hip.h): host orchestration of the HIP
+// kernels.  Mirrors GpuParser and its resident types (crates/oneline-gpu/src/*.rs):
+// one HIP stream per context, device buffers owned by the handles, scratch
+// grown on demand and reused across calls, errors returned as status codes.
+#include "internal.h"
+#include "../../include/oneline_hip.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(OL_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+uint64_t round_up(uint64_t a, uint64_t b) { return ceil_div(a, b) * b; }
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p)
+            return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, b ? b : 16);
+        if (e == hipSuccess)
+            bytes = b;
+        else
+            p = nullptr;
+        return e;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace
+
+// Scratch layout (u64 words): [0, 8) meta, [8, 16) counters (u32 x 16),
+// [16, ...) per-tile look-back words.
+static constexpr size_t SCR_META = 0, SCR_CTR = 8, SCR_STATUS = 16;
+
+struct ol_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    std::mutex mu;
+    DevBuf scratch;
+    DevBuf tree_agg, tree_incl;
+    // profiling
+    bool profiling = false;
+    struct Rec {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+
+    uint64_t *meta() { return (uint64_t *)scratch.p + SCR_META; }
+    uint32_t *ctr() { return (uint32_t *)((uint64_t *)scratch.p + SCR_CTR); }
+    uint64_t *status() { return (uint64_t *)scratch.p + SCR_STATUS; }
+};
+
+struct ol_doc {
+    uint64_t n = 0, lines = 0, cap = 0;
+    bool with_nodes = true;
+    uint8_t *text = nullptr;
+    uint32_t *offsets = nullptr;
+    uint8_t *classes = nullptr, *keys = nullptr;
+    ol_node_dev *nodes = nullptr;
+    uint32_t byte_freq[256];
+};
+
+struct ol_tree {
+    uint64_t lines = 0, cap = 0;
+    uint32_t *parent = nullptr, *fc = nullptr, *ns = nullptr, *se = nullptr;
+};
+
+struct ol_dict {
+    uint32_t count = 0;
+    DictDev d{};
+    void *dev = nullptr;   // one allocation holding filter, tables, entries, pool
+};
+
+namespace {
+
+hipEvent_t take_event(ol_ctx *c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+// Brackets one launch with events when profiling is on.
+struct Prof {
+    ol_ctx *c;
+    const char *name;
+    hipEvent_t a = nullptr;
+    Prof(ol_ctx *c_, const char *n) : c(c_), name(n)
+    {
+        if (c->profiling) {
+            a = take_event(c);
+            if (a)
+                (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Prof()
+    {
+        if (a) {
+            hipEvent_t b = take_event(c);
+            if (b) {
+                (void)hipEventRecord(b, c->stream);
+                c->recs.push_back({name, a, b});
+            } else {
+                c->pool.push_back(a);
+            }
+        }
+    }
+};
+
+hipError_t ensure_scratch(ol_ctx *c, size_t status_words)
+{
+    return c->scratch.ensure((SCR_STATUS + status_words + 64) * 8);
+}
+
+hipError_t clear_scratch(ol_ctx *c, size_t status_words)
+{
+    Prof p(c, "clear");
+    return ol_launch_clear_u64((uint64_t *)c->scratch.p, SCR_STATUS + status_words, c->stream);
+}
+
+int read_meta(ol_ctx *c, uint64_t *out)
+{
+    HIPCHK(hipMemcpyAsync(out, c->meta(), 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OL_OK;
+}
+
+void free_doc_lines(ol_doc *d)
+{
+    if (d->offsets) (void)hipFree(d->offsets);
+    if (d->classes) (void)hipFree(d->classes);
+    if (d->keys) (void)hipFree(d->keys);
+    if (d->nodes) (void)hipFree(d->nodes);
+    d->offsets = nullptr;
+    d->classes = d->keys = nullptr;
+    d->nodes = nullptr;
+    d->cap = 0;
+}
+
+int alloc_doc_lines(ol_doc *d, uint64_t cap)
+{
+    free_doc_lines(d);
+    uint64_t capa = round_up(cap ? cap : 1, LINE_ALIGN);
+    HIPCHK(hipMalloc(&d->offsets, capa * 4));
+    HIPCHK(hipMalloc(&d->classes, capa));
+    HIPCHK(hipMalloc(&d->keys, capa));
+    if (d->with_nodes)
+        HIPCHK(hipMalloc(&d->nodes, capa * sizeof(ol_node_dev)));
+    d->cap = capa;
+    return OL_OK;
+}
+
+int launch_scan(ol_ctx *c, ol_doc *d)
+{
+    uint32_t tiles = (uint32_t)ceil_div(d->n, SCAN_TILE);
+    HIPCHK(ensure_scratch(c, tiles));
+    HIPCHK(clear_scratch(c, tiles));
+    ScanArgs a{};
+    a.text = d->text;
+    a.n = (uint32_t)d->n;
+    a.num_tiles = tiles;
+    a.cap = d->cap;
+    a.offsets = d->offsets;
+    a.classes = d->classes;
+    a.keys = d->keys;
+    a.nodes = d->nodes;
+    a.status = c->status();
+    a.tile_ctr = c->ctr();
+    a.meta = c->meta();
+    Prof p(c, "scan_classify");
+    HIPCHK(ol_launch_scan(a, c->stream));
+    return OL_OK;
+}
+
+void sample_histogram(const uint8_t *t, size_t n, uint32_t *freq)
+{
+    // sample_byte_histogram, crates/oneline-gpu/src/search.rs:176-201
+    std::memset(freq, 0, 256 * sizeof(uint32_t));
+    const size_t SAMPLE = 1u << 20, WINDOWS = 4;
+    if (n <= SAMPLE) {
+        for (size_t i = 0; i < n; i++)
+            freq[t[i]]++;
+    } else {
+        size_t w = SAMPLE / WINDOWS, span = n - w;
+        for (size_t k = 0; k < WINDOWS; k++) {
+            size_t s = span / (WINDOWS - 1) * k;
+            for (size_t i = 0; i < w; i++)
+                freq[t[s + i]]++;
+        }
+    }
+}
+
+int upload_doc(ol_ctx *c, const uint8_t *text, size_t len, bool with_nodes, ol_doc **out)
+{
+    if (!out || (!text && len))
+        return fail(OL_ERR_INVALID, "null argument");
+    if (len > OL_MAX_GPU_INPUT_LEN)
+        return fail(OL_ERR_TOO_LARGE, "input is " + std::to_string(len) +
+                                          " bytes; the GPU path addresses at most " +
+                                          std::to_string(OL_MAX_GPU_INPUT_LEN) +
+                                          " bytes (32-bit offsets)");
+    ol_doc *d = new (std::nothrow) ol_doc();
+    if (!d)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    d->n = len;
+    d->with_nodes = with_nodes;
+    std::memset(d->byte_freq, 0, sizeof d->byte_freq);
+    if (len == 0) {
+        *out = d;
+        return OL_OK;
+    }
+    auto bail = [&](int rc) {
+        free_doc_lines(d);
+        if (d->text) (void)hipFree(d->text);
+        delete d;
+        return rc;
+    };
+    uint64_t padded = round_up(len, TEXT_ALIGN) + TEXT_SLACK;
+    if (hipMalloc(&d->text, padded) != hipSuccess) {
+        d->text = nullptr;
+        return bail(fail(OL_ERR_NOMEM, "device allocation of the text failed"));
+    }
+    if (hipMemcpyAsync(d->text, text, len, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemsetAsync(d->text + len, 0, padded - len, c->stream) != hipSuccess)
+        return bail(fail(OL_ERR_DEVICE, "text upload failed"));
+    uint64_t cap = std::min<uint64_t>(len, len / 16 + 4096);
+    int rc = alloc_doc_lines(d, cap);
+    if (rc)
+        return bail(rc);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        rc = launch_scan(c, d);
+        if (rc)
+            return bail(rc);
+        uint64_t lines = 0;
+        rc = read_meta(c, &lines);
+        if (rc)
+            return bail(rc);
+        d->lines = lines;
+        if (lines <= d->cap)
+            break;
+        rc = alloc_doc_lines(d, lines);
+        if (rc)
+            return bail(rc);
+    }
+    sample_histogram(text, len, d->byte_freq);
+    *out = d;
+    return OL_OK;
+}
+
+int ensure_tree(ol_tree *t, uint64_t lines)
+{
+    if (t->cap >= lines && t->parent)
+        return OL_OK;
+    if (t->parent) (void)hipFree(t->parent);
+    if (t->fc) (void)hipFree(t->fc);
+    if (t->ns) (void)hipFree(t->ns);
+    if (t->se) (void)hipFree(t->se);
+    t->parent = t->fc = t->ns = t->se = nullptr;
+    t->cap = 0;
+    uint64_t capa = round_up(lines ? lines : 1, LINE_ALIGN);
+    HIPCHK(hipMalloc(&t->parent, capa * 4));
+    HIPCHK(hipMalloc(&t->fc, capa * 4));
+    HIPCHK(hipMalloc(&t->ns, capa * 4));
+    HIPCHK(hipMalloc(&t->se, capa * 4));
+    t->cap = capa;
+    return OL_OK;
+}
+
+int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
+{
+    uint32_t tiles = (uint32_t)ceil_div(d->lines, TREE_TILE);
+    size_t flag_words = ceil_div(tiles, 2);
+    HIPCHK(ensure_scratch(c, flag_words));
+    HIPCHK(c->tree_agg.ensure((size_t)tiles * TREE_D * 4));
+    HIPCHK(c->tree_incl.ensure((size_t)tiles * TREE_D * 4));
+    HIPCHK(clear_scratch(c, flag_words));
+    TreeArgs a{};
+    a.keys = d->keys;
+    a.lines = d->lines;
+    a.num_tiles = tiles;
+    a.parent = t->parent;
+    a.first_child = t->fc;
+    a.next_sibling = t->ns;
+    a.subtree_end = t->se;
+    a.flags = (uint32_t *)c->status();
+    a.agg = (uint32_t *)c->tree_agg.p;
+    a.incl = (uint32_t *)c->tree_incl.p;
+    a.tile_ctr = c->ctr();
+    a.err = c->ctr() + 1;
+    Prof p(c, "tree");
+    HIPCHK(ol_launch_tree(a, c->stream));
+    return OL_OK;
+}
+
+int check_tree_error(ol_ctx *c)
+{
+    uint32_t err = 0;
+    HIPCHK(hipMemcpyAsync(&err, c->ctr() + 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (err)
+        return fail(OL_ERR_TOO_DEEP,
+                    "a content line is nested deeper than the " + std::to_string(OL_MAX_TREE_DEPTH) +
+                        " levels the GPU tree tracks (MAX_TREE_DEPTH = " +
+                        std::to_string(OL_MAX_TREE_DEPTH) + "); use the CPU builder");
+    return OL_OK;
+}
+
+int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_t *out64,
+               uint32_t *out32, uint64_t cap, uint64_t *total)
+{
+    uint32_t tiles = (uint32_t)ceil_div(d->n, SEARCH_TILE);
+    HIPCHK(ensure_scratch(c, tiles));
+    HIPCHK(clear_scratch(c, tiles));
+    SearchArgs a{};
+    a.text = d->text;
+    a.n = (uint32_t)d->n;
+    a.num_tiles = tiles;
+    a.m = (uint32_t)m;
+    uint32_t k = (uint32_t)std::min<size_t>(m, 4);
+    a.win = 0;
+    for (uint32_t i = 0; i < k; i++)
+        a.win |= (uint32_t)pat[i] << (8 * i);
+    a.win_mask = (k == 4) ? 0xFFFFFFFFu : ((1u << (8 * k)) - 1u);
+    std::memset(a.pat, 0, sizeof a.pat);
+    std::memcpy(a.pat, pat, m);
+    a.cap = cap;
+    a.out64 = out64;
+    a.out32 = out32;
+    a.status = c->status();
+    a.tile_ctr = c->ctr();
+    a.meta = c->meta();
+    {
+        Prof p(c, "search");
+        HIPCHK(ol_launch_search(a, c->stream));
+    }
+    return read_meta(c, total);
+}
+
+int check_pattern(size_t len)
+{
+    if (len == 0 || len > OL_MAX_PATTERN_LEN)
+        return fail(OL_ERR_PATTERN, "pattern must be 1..=" + std::to_string(OL_MAX_PATTERN_LEN) +
+                                        " bytes, got " + std::to_string(len));
+    return OL_OK;
+}
+
+int search_doc_host(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, size_t max_matches,
+                    uint32_t *total, uint32_t **offsets, size_t *n_offsets)
+{
+    if (!total || !offsets || !n_offsets || (!pat && m))
+        return fail(OL_ERR_INVALID, "null argument");
+    int rc = check_pattern(m);
+    if (rc)
+        return rc;
+    if (max_matches > 0xFFFFFFFFull)
+        return fail(OL_ERR_TOO_MANY, "max_matches " + std::to_string(max_matches) + " does not fit in u32");
+    *total = 0;
+    *offsets = nullptr;
+    *n_offsets = 0;
+    if (d->n == 0)
+        return OL_OK;
+    uint64_t cap = std::min<uint64_t>(max_matches, d->n);
+    DevBuf out;
+    HIPCHK(out.ensure((cap ? cap : 1) * 4));
+    uint64_t tot = 0;
+    rc = run_search(c, d, pat, m, nullptr, cap ? (uint32_t *)out.p : nullptr, cap, &tot);
+    if (rc) {
+        out.release();
+        return rc;
+    }
+    uint64_t keep = std::min<uint64_t>(tot, cap);
+    uint32_t *h = (uint32_t *)std::malloc((keep ? keep : 1) * 4);
+    if (!h) {
+        out.release();
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    }
+    if (keep) {
+        hipError_t e = hipMemcpy(h, out.p, keep * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            std::free(h);
+            out.release();
+            return fail(OL_ERR_DEVICE, std::string("download: ") + hipGetErrorString(e));
+        }
+    }
+    out.release();
+    *total = (uint32_t)tot;
+    *offsets = h;
+    *n_offsets = keep;
+    return OL_OK;
+}
+
+int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, uint32_t *out_id,
+             uint64_t cap, uint64_t *total)
+{
+    uint32_t tiles = (uint32_t)ceil_div(d->n, SEARCH_TILE);
+    HIPCHK(ensure_scratch(c, tiles));
+    HIPCHK(clear_scratch(c, tiles));
+    ManyArgs a{};
+    a.text = d->text;
+    a.n = (uint32_t)d->n;
+    a.num_tiles = tiles;
+    a.d = s->d;
+    a.cap = cap;
+    a.out_off = out_off;
+    a.out_id = out_id;
+    a.status = c->status();
+    a.tile_ctr = c->ctr();
+    a.meta = c->meta();
+    {
+        Prof p(c, "many");
+        HIPCHK(ol_launch_many(a, c->stream));
+    }
+    return read_meta(c, total);
+}
+
+uint32_t ceil_log2(uint64_t v)
+{
+    uint32_t b = 0;
+    while ((1ull << b) < v)
+        b++;
+    return b;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char *ol_last_error(void) { return g_err.c_str(); }
+
+void ol_free(void *p) { std::free(p); }
+
+int ol_ctx_create(ol_ctx **out)
+{
+    if (!out)
+        return fail(OL_ERR_INVALID, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(OL_ERR_DEVICE, "no HIP device available");
+    ol_ctx *c = new (std::nothrow) ol_ctx();
+    if (!c)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    c->device = 0;
+    if (hipSetDevice(0) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(OL_ERR_DEVICE, "stream creation failed");
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
+        c->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    if (c->scratch.ensure((SCR_STATUS + 4096) * 8) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(OL_ERR_NOMEM, "scratch allocation failed");
+    }
+    *out = c;
+    return OL_OK;
+}
+
+void ol_ctx_destroy(ol_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &r : c->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c->pool)
+        (void)hipEventDestroy(e);
+    c->scratch.release();
+    c->tree_agg.release();
+    c->tree_incl.release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ol_ctx_adapter_name(ol_ctx *c, char *buf, size_t cap)
+{
+    if (!c || !buf || cap == 0)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::snprintf(buf, cap, "%s", c->name.c_str());
+    return OL_OK;
+}
+
+int ol_ctx_is_software(ol_ctx *c) { return c ? 0 : fail(OL_ERR_INVALID, "null argument"); }
+
+int ol_ctx_timestamps_supported(ol_ctx *c) { return c ? 1 : fail(OL_ERR_INVALID, "null argument"); }
+
+int ol_ctx_synchronize(ol_ctx *c)
+{
+    if (!c)
+        return fail(OL_ERR_INVALID, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OL_OK;
+}
+
+int ol_doc_upload(ol_ctx *c, const uint8_t *text, size_t len, ol_doc **out)
+{
+    if (!c)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return upload_doc(c, text, len, true, out);
+}
+
+int ol_doc_upload_lean(ol_ctx *c, const uint8_t *text, size_t len, ol_doc **out)
+{
+    if (!c)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return upload_doc(c, text, len, false, out);
+}
+
+void ol_doc_free(ol_doc *d)
+{
+    if (!d)
+        return;
+    free_doc_lines(d);
+    if (d->text)
+        (void)hipFree(d->text);
+    delete d;
+}
+
+size_t ol_doc_line_count(const ol_doc *d) { return d ? d->lines : 0; }
+size_t ol_doc_text_len(const ol_doc *d) { return d ? d->n : 0; }
+
+int ol_doc_download(ol_ctx *c, const ol_doc *d, uint32_t *offsets, uint8_t *classes, ol_flat_node *nodes)
+{
+    if (!c || !d)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (d->lines == 0)
+        return OL_OK;
+    if (nodes && !d->nodes)
+        return fail(OL_ERR_INVALID, "document was uploaded without nodes");
+    if (offsets)
+        HIPCHK(hipMemcpyAsync(offsets, d->offsets, d->lines * 4, hipMemcpyDeviceToHost, c->stream));
+    if (classes)
+        HIPCHK(hipMemcpyAsync(classes, d->classes, d->lines, hipMemcpyDeviceToHost, c->stream));
+    if (nodes)
+        HIPCHK(hipMemcpyAsync(nodes, d->nodes, d->lines * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OL_OK;
+}
+
+int ol_doc_reparse(ol_ctx *c, ol_doc *d)
+{
+    if (!c || !d)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (d->n == 0)
+        return OL_OK;
+    return launch_scan(c, d);
+}
+
+int ol_scan_line_offsets(ol_ctx *c, const uint8_t *text, size_t len, uint32_t **offsets, size_t *line_count)
+{
+    if (!c || !offsets || !line_count)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    *offsets = nullptr;
+    *line_count = 0;
+    ol_doc *d = nullptr;
+    int rc = upload_doc(c, text, len, false, &d);
+    if (rc)
+        return rc;
+    uint32_t *h = (uint32_t *)std::malloc((d->lines ? d->lines : 1) * 4);
+    if (!h) {
+        ol_doc_free(d);
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    }
+    if (d->lines && hipMemcpy(h, d->offsets, d->lines * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        std::free(h);
+        ol_doc_free(d);
+        return fail(OL_ERR_DEVICE, "download failed");
+    }
+    *offsets = h;
+    *line_count = d->lines;
+    ol_doc_free(d);
+    return OL_OK;
+}
+
+int ol_parse_raw(ol_ctx *c, const uint8_t *text, size_t len, ol_flat_node **nodes, uint32_t **offsets,
+                 size_t *line_count)
+{
+    if (!c || !nodes || !offsets || !line_count)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    *nodes = nullptr;
+    *offsets = nullptr;
+    *line_count = 0;
+    ol_doc *d = nullptr;
+    int rc = upload_doc(c, text, len, true, &d);
+    if (rc)
+        return rc;
+    size_t L = d->lines;
+    uint32_t *ho = (uint32_t *)std::malloc((L ? L : 1) * 4);
+    ol_flat_node *hn = (ol_flat_node *)std::malloc((L ? L : 1) * 16);
+    if (!ho || !hn) {
+        std::free(ho);
+        std::free(hn);
+        ol_doc_free(d);
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    }
+    if (L && (hipMemcpy(ho, d->offsets, L * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(hn, d->nodes, L * 16, hipMemcpyDeviceToHost) != hipSuccess)) {
+        std::free(ho);
+        std::free(hn);
+        ol_doc_free(d);
+        return fail(OL_ERR_DEVICE, "download failed");
+    }
+    ol_doc_free(d);
+    *nodes = hn;
+    *offsets = ho;
+    *line_count = L;
+    return OL_OK;
+}
+
+int ol_parse(ol_ctx *c, const uint8_t *text, size_t len, const uint32_t *offsets, size_t line_count,
+             ol_flat_node **nodes)
+{
+    if (!c || !nodes || (!offsets && line_count) || (!text && len))
+        return fail(OL_ERR_INVALID, "null argument");
+    if (len > OL_MAX_GPU_INPUT_LEN)
+        return fail(OL_ERR_TOO_LARGE, "input is " + std::to_string(len) + " bytes; too large");
+    std::lock_guard<std::mutex> g(c->mu);
+    *nodes = nullptr;
+    ol_flat_node *hn = (ol_flat_node *)std::malloc((line_count ? line_count : 1) * 16);
+    if (!hn)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    if (line_count == 0) {
+        *nodes = hn;
+        return OL_OK;
+    }
+    DevBuf t, o, nd;
+    auto done = [&](int rc) {
+        t.release();
+        o.release();
+        nd.release();
+        if (rc)
+            std::free(hn);
+        return rc;
+    };
+    if (t.ensure(len + 16) != hipSuccess || o.ensure(line_count * 4) != hipSuccess ||
+        nd.ensure(line_count * 16) != hipSuccess)
+        return done(fail(OL_ERR_NOMEM, "device allocation failed"));
+    if ((len && hipMemcpyAsync(t.p, text, len, hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
+        hipMemcpyAsync(o.p, offsets, line_count * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return done(fail(OL_ERR_DEVICE, "upload failed"));
+    {
+        Prof p(c, "parse_given");
+        if (ol_launch_parse_given((const uint8_t *)t.p, (uint32_t)len, (const uint32_t *)o.p, line_count,
+                                  (ol_node_dev *)nd.p, c->stream) != hipSuccess)
+            return done(fail(OL_ERR_DEVICE, "launch failed"));
+    }
+    if (hipMemcpyAsync(hn, nd.p, line_count * 16, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return done(fail(OL_ERR_DEVICE, "download failed"));
+    *nodes = hn;
+    return done(OL_OK);
+}
+
+// ---- tree -------------------------------------------------------------------------
+int ol_tree_build(ol_ctx *c, const ol_doc *d, ol_tree *reuse, ol_tree **out)
+{
+    if (!c || !d || !out)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    ol_tree *t = reuse ? reuse : new (std::nothrow) ol_tree();
+    if (!t)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    *out = nullptr;
+    t->lines = d->lines;
+    if (d->lines == 0) {
+        *out = t;
+        return OL_OK;
+    }
+    int rc = ensure_tree(t, d->lines);
+    if (!rc)
+        rc = launch_tree(c, d, t);
+    if (!rc)
+        rc = check_tree_error(c);
+    if (rc) {
+        ol_tree_free(t);
+        return rc;
+    }
+    *out = t;
+    return OL_OK;
+}
+
+int ol_tree_rebuild_async(ol_ctx *c, const ol_doc *d, ol_tree *t)
+{
+    if (!c || !d || !t)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (d->lines == 0)
+        return OL_OK;
+    if (t->cap < d->lines)
+        return fail(OL_ERR_INVALID, "tree capacity is smaller than the document");
+    t->lines = d->lines;
+    return launch_tree(c, d, t);
+}
+
+int ol_tree_check(ol_ctx *c, const ol_tree *t)
+{
+    if (!c || !t)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (t->lines == 0)
+        return OL_OK;
+    return check_tree_error(c);
+}
+
+void ol_tree_free(ol_tree *t)
+{
+    if (!t)
+        return;
+    if (t->parent) (void)hipFree(t->parent);
+    if (t->fc) (void)hipFree(t->fc);
+    if (t->ns) (void)hipFree(t->ns);
+    if (t->se) (void)hipFree(t->se);
+    delete t;
+}
+
+size_t ol_tree_line_count(const ol_tree *t) { return t ? t->lines : 0; }
+
+int ol_tree_download(ol_ctx *c, const ol_tree *t, uint32_t *parent, uint32_t *subtree_end,
+                     uint32_t *first_child, uint32_t *next_sibling)
+{
+    if (!c || !t)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    size_t b = t->lines * 4;
+    if (!b)
+        return OL_OK;
+    if (parent) HIPCHK(hipMemcpyAsync(parent, t->parent, b, hipMemcpyDeviceToHost, c->stream));
+    if (subtree_end) HIPCHK(hipMemcpyAsync(subtree_end, t->se, b, hipMemcpyDeviceToHost, c->stream));
+    if (first_child) HIPCHK(hipMemcpyAsync(first_child, t->fc, b, hipMemcpyDeviceToHost, c->stream));
+    if (next_sibling) HIPCHK(hipMemcpyAsync(next_sibling, t->ns, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OL_OK;
+}
+
+// ---- literal search -------------------------------------------------------------
+int ol_search_document(ol_ctx *c, const ol_doc *d, const uint8_t *pattern, size_t len, size_t max_matches,
+                       uint32_t *total, uint32_t **offsets, size_t *n_offsets)
+{
+    if (!c || !d)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return search_doc_host(c, d, pattern, len, max_matches, total, offsets, n_offsets);
+}
+
+int ol_search_raw(ol_ctx *c, const uint8_t *text, size_t text_len, const uint8_t *pattern, size_t len,
+                  size_t max_matches, uint32_t *total, uint32_t **offsets, size_t *n_offsets)
+{
+    if (!c)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_pattern(len);
+    if (rc)
+        return rc;
+    if (text_len > OL_MAX_GPU_INPUT_LEN)
+        return fail(OL_ERR_TOO_LARGE, "input is " + std::to_string(text_len) + " bytes; too large");
+    if (!total || !offsets || !n_offsets)
+        return fail(OL_ERR_INVALID, "null argument");
+    if (text_len == 0) {
+        *total = 0;
+        *offsets = nullptr;
+        *n_offsets = 0;
+        return OL_OK;
+    }
+    // The one-shot path needs only the text on the device, no line scan.
+    ol_doc tmp;
+    tmp.n = text_len;
+    uint64_t padded = round_up(text_len, TEXT_ALIGN) + TEXT_SLACK;
+    HIPCHK(hipMalloc(&tmp.text, padded));
+    if (hipMemcpyAsync(tmp.text, text, text_len, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemsetAsync(tmp.text + text_len, 0, padded - text_len, c->stream) != hipSuccess) {
+        (void)hipFree(tmp.text);
+        return fail(OL_ERR_DEVICE, "upload failed");
+    }
+    rc = search_doc_host(c, &tmp, pattern, len, max_matches, total, offsets, n_offsets);
+    (void)hipFree(tmp.text);
+    tmp.text = nullptr;
+    return rc;
+}
+
+int ol_search_device(ol_ctx *c, const ol_doc *d, const uint8_t *pattern, size_t len, uint64_t *d_offsets,
+                     size_t cap, uint64_t *total)
+{
+    if (!c || !d || !total || (!pattern && len))
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_pattern(len);
+    if (rc)
+        return rc;
+    *total = 0;
+    if (d->n == 0)
+        return OL_OK;
+    return run_search(c, d, pattern, len, d_offsets, nullptr, d_offsets ? cap : 0, total);
+}
+
+// ---- dictionary ------------------------------------------------------------------
+int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns, const size_t *lens,
+                    size_t count, ol_dict **out)
+{
+    (void)doc;
+    if (!c || !out || (count && (!patterns || !lens)))
+        return fail(OL_ERR_INVALID, "null argument");
+    if (count == 0 || count > OL_MAX_PATTERNS)
+        return fail(OL_ERR_TOO_MANY, "need 1..=" + std::to_string(OL_MAX_PATTERNS) + " patterns, got " +
+                                         std::to_string(count));
+    for (size_t i = 0; i < count; i++) {
+        if (lens[i] < OL_MIN_SET_PATTERN_LEN || lens[i] > OL_MAX_SET_PATTERN_LEN || !patterns[i])
+            return fail(OL_ERR_PATTERN, "pattern " + std::to_string(i) + " is " + std::to_string(lens[i]) +
+                                            " bytes; each must be " + std::to_string(OL_MIN_SET_PATTERN_LEN) +
+                                            "..=" + std::to_string(OL_MAX_SET_PATTERN_LEN));
+    }
+    std::lock_guard<std::mutex> g(c->mu);
+    // Key each pattern by its first min(len, 4) bytes.
+    std::vector<uint32_t> cls(count), key(count), order(count);
+    size_t pool_bytes = 0;
+    uint32_t max_len = 0;
+    for (size_t i = 0; i < count; i++) {
+        uint32_t k = (uint32_t)std::min<size_t>(lens[i], 4);
+        uint32_t v = 0;
+        for (uint32_t b = 0; b < k; b++)
+            v |= (uint32_t)patterns[i][b] << (8 * b);
+        cls[i] = k;
+        key[i] = v;
+        order[i] = (uint32_t)i;
+        pool_bytes += lens[i];
+        max_len = std::max<uint32_t>(max_len, (uint32_t)lens[i]);
+    }
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        if (cls[a] != cls[b]) return cls[a] < cls[b];
+        if (key[a] != key[b]) return key[a] < key[b];
+        return a < b;
+    });
+    // Entries (in sorted order) and the pattern byte pool.
+    std::vector<uint32_t> ent_id(count), ent_len(count), ent_off(count);
+    std::vector<uint8_t> pool(pool_bytes ? pool_bytes : 1);
+    size_t at = 0;
+    for (size_t e = 0; e < count; e++) {
+        uint32_t i = order[e];
+        ent_id[e] = i;
+        ent_len[e] = (uint32_t)lens[i];
+        ent_off[e] = (uint32_t)at;
+        std::memcpy(pool.data() + at, patterns[i], lens[i]);
+        at += lens[i];
+    }
+    // Per class: distinct keys with their entry ranges.
+    struct Bucket {
+        uint32_t key, begin, end;
+    };
+    std::vector<Bucket> buckets[5];
+    for (size_t e = 0; e < count;) {
+        uint32_t i = order[e];
+        size_t f = e + 1;
+        while (f < count && cls[order[f]] == cls[i] && key[order[f]] == key[i])
+            f++;
+        buckets[cls[i]].push_back({key[i], (uint32_t)e, (uint32_t)f});
+        e = f;
+    }
+    DictDev dd{};
+    dd.max_len = max_len;
+    std::vector<std::vector<uint32_t>> filt(5);
+    std::vector<std::vector<uint32_t>> tab(5);   // 4 words per slot
+    uint32_t fwords = 0;
+    for (uint32_t k = 1; k <= 4; k++) {
+        size_t nb = buckets[k].size();
+        if (!nb)
+            continue;
+        dd.class_mask |= 1u << k;
+        uint32_t fb;
+        if (k == 1) fb = 8;
+        else if (k == 2) fb = 16;
+        else if (k == 3) fb = std::min<uint32_t>(16, std::max<uint32_t>(10, ceil_log2(nb) + 3));
+        else fb = std::min<uint32_t>(17, std::max<uint32_t>(10, ceil_log2(nb) + 2));
+        dd.fbits[k] = fb;
+        dd.foff[k] = fwords;
+        uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
+        filt[k].assign(words, 0);
+        fwords += words;
+        uint32_t tb = std::max<uint32_t>(4, ceil_log2(nb * 2));
+        dd.tbits[k] = tb;
+        tab[k].assign((size_t)4 << tb, 0);
+        uint32_t tmask = (1u << tb) - 1;
+        for (const Bucket &b : buckets[k]) {
+            uint32_t fi = (k <= 2) ? b.key : (uint32_t)((b.key * 0x9E3779B1u) >> (32 - fb));
+            filt[k][fi >> 5] |= 1u << (fi & 31);
+            uint32_t h = (uint32_t)(((b.key ^ (b.key >> 15)) * 0x85EBCA6Bu) >> (32 - tb));
+            while (tab[k][(size_t)h * 4 + 2] != 0)
+                h = (h + 1) & tmask;
+            tab[k][(size_t)h * 4 + 0] = b.key;
+            tab[k][(size_t)h * 4 + 1] = b.begin;
+            tab[k][(size_t)h * 4 + 2] = b.end;
+        }
+    }
+    dd.fwords = fwords;
+    // One device allocation: [filter][tables][ent_id][ent_len][ent_off][pool]
+    size_t bytes = 0;
+    auto place = [&](size_t n) {
+        size_t o = round_up(bytes, 256);
+        bytes = o + n;
+        return o;
+    };
+    size_t o_f = place((size_t)fwords * 4);
+    size_t o_t[5] = {0, 0, 0, 0, 0};
+    for (uint32_t k = 1; k <= 4; k++)
+        if (!tab[k].empty())
+            o_t[k] = place(tab[k].size() * 4);
+    size_t o_id = place(count * 4), o_len = place(count * 4), o_off = place(count * 4);
+    size_t o_pool = place(pool.size() + 64);
+    ol_dict *s = new (std::nothrow) ol_dict();
+    if (!s)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    if (hipMalloc(&s->dev, bytes) != hipSuccess) {
+        delete s;
+        return fail(OL_ERR_NOMEM, "device allocation of the dictionary failed");
+    }
+    uint8_t *base = (uint8_t *)s->dev;
+    std::vector<uint8_t> host(bytes, 0);
+    for (uint32_t k = 1; k <= 4; k++) {
+        if (!filt[k].empty())
+            std::memcpy(host.data() + o_f + (size_t)dd.foff[k] * 4, filt[k].data(), filt[k].size() * 4);
+        if (!tab[k].empty())
+            std::memcpy(host.data() + o_t[k], tab[k].data(), tab[k].size() * 4);
+    }
+    std::memcpy(host.data() + o_id, ent_id.data(), count * 4);
+    std::memcpy(host.data() + o_len, ent_len.data(), count * 4);
+    std::memcpy(host.data() + o_off, ent_off.data(), count * 4);
+    std::memcpy(host.data() + o_pool, pool.data(), pool.size());
+    if (hipMemcpy(base, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(s->dev);
+        delete s;
+        return fail(OL_ERR_DEVICE, "dictionary upload failed");
+    }
+    dd.filter = (const uint32_t *)(base + o_f);
+    for (uint32_t k = 1; k <= 4; k++)
+        dd.table[k] = tab[k].empty() ? nullptr : (const uint4 *)(base + o_t[k]);
+    dd.ent_id = (const uint32_t *)(base + o_id);
+    dd.ent_len = (const uint32_t *)(base + o_len);
+    dd.ent_off = (const uint32_t *)(base + o_off);
+    dd.pool = base + o_pool;
+    s->d = dd;
+    s->count = (uint32_t)count;
+    *out = s;
+    return OL_OK;
+}
+
+void ol_dict_free(ol_dict *s)
+{
+    if (!s)
+        return;
+    if (s->dev)
+        (void)hipFree(s->dev);
+    delete s;
+}
+
+size_t ol_dict_len(const ol_dict *s) { return s ? s->count : 0; }
+
+int ol_search_many_device(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *d_offsets, uint32_t *d_ids,
+                          size_t cap, uint64_t *total)
+{
+    if (!c || !d || !s || !total)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    *total = 0;
+    if (d->n == 0)
+        return OL_OK;
+    return run_many(c, d, s, d_offsets, d_ids, (d_offsets || d_ids) ? cap : 0, total);
+}
+
+int ol_search_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, size_t max_hits, uint32_t *counts,
+                   uint64_t *total_hits, ol_many_hit **hits, size_t *n_hits)
+{
+    if (!c || !d || !s || !counts || !total_hits || !hits || !n_hits)
+        return fail(OL_ERR_INVALID, "null argument");
+    if (max_hits > 0xFFFFFFFFull)
+        return fail(OL_ERR_TOO_MANY, "max_hits " + std::to_string(max_hits) + " does not fit in u32");
+    std::lock_guard<std::mutex> g(c->mu);
+    std::memset(counts, 0, (size_t)s->count * 4);
+    *total_hits = 0;
+    *hits = nullptr;
+    *n_hits = 0;
+    if (d->n == 0)
+        return OL_OK;
+    uint64_t total = 0;
+    int rc = run_many(c, d, s, nullptr, nullptr, 0, &total);
+    if (rc)
+        return rc;
+    *total_hits = total;
+    if (total == 0)
+        return OL_OK;
+    // Exact per-pattern counts come from the full offset-major list.
+    std::vector<uint32_t> ids(total);
+    std::vector<uint64_t> offs64(total);
+    {
+        DevBuf d64, did;
+        if (d64.ensure(total * 8) != hipSuccess || did.ensure(total * 4) != hipSuccess)
+            return fail(OL_ERR_NOMEM, "device allocation of the hit list failed");
+        uint64_t again = 0;
+        rc = run_many(c, d, s, (uint64_t *)d64.p, (uint32_t *)did.p, total, &again);
+        if (!rc && (hipMemcpy(offs64.data(), d64.p, total * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(ids.data(), did.p, total * 4, hipMemcpyDeviceToHost) != hipSuccess))
+            rc = fail(OL_ERR_DEVICE, "hit list download failed");
+        d64.release();
+        did.release();
+        if (rc)
+            return rc;
+    }
+    for (uint64_t i = 0; i < total; i++)
+        counts[ids[i]]++;
+    if (max_hits == 0)
+        return OL_OK;
+    // Stable counting sort by pattern: the list is offset-ordered, so the result
+    // is sorted by (pattern, offset) (ManyResult::hits, crates/oneline-gpu/src/many.rs:89-93).
+    std::vector<uint64_t> start(s->count + 1, 0);
+    for (uint32_t p = 0; p < s->count; p++)
+        start[p + 1] = start[p] + counts[p];
+    uint64_t keep = std::min<uint64_t>(total, max_hits);
+    ol_many_hit *h = (ol_many_hit *)std::malloc(keep * sizeof(ol_many_hit));
+    if (!h)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    std::vector<uint64_t> pos(start.begin(), start.end() - 1);
+    for (uint64_t i = 0; i < total; i++) {
+        uint64_t at = pos[ids[i]]++;
+        if (at < keep)
+            h[at] = ol_many_hit{ids[i], (uint32_t)offs64[i]};
+    }
+    *hits = h;
+    *n_hits = keep;
+    return OL_OK;
+}
+
+// ---- profiling --------------------------------------------------------------------
+int ol_profile_enable(ol_ctx *c, int on)
+{
+    if (!c)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->profiling = on != 0;
+    return OL_OK;
+}
+
+int ol_profile_take(ol_ctx *c, char *names, size_t name_stride, uint64_t *counts, double *ms, size_t cap,
+                    size_t *n)
+{
+    if (!c || !n)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<std::string> keys;
+    std::vector<uint64_t> cnt;
+    std::vector<double> sum;
+    for (auto &r : c->recs) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, r.a, r.b);
+        size_t i = 0;
+        while (i < keys.size() && keys[i] != r.name)
+            i++;
+        if (i == keys.size()) {
+            keys.push_back(r.name);
+            cnt.push_back(0);
+            sum.push_back(0.0);
+        }
+        cnt[i]++;
+        sum[i] += t;
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    size_t m = std::min(cap, keys.size());
+    for (size_t i = 0; i < m; i++) {
+        if (names && name_stride) {
+            std::snprintf(names + i * name_stride, name_stride, "%s", keys[i].c_str());
+        }
+        if (counts) counts[i] = cnt[i];
+        if (ms) ms[i] = sum[i];
+    }
+    *n = keys.size();
+    return OL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+// internal.h -- shared declarations of the HIP engine (device helpers, kernel
+// argument blocks and host-side launchers).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define OL_NONE 0xFFFFFFFFu
+
+// NodeKind values, crates/oneline-core/src/lib.rs:41-51.
+enum : uint32_t {
+    OLK_BLANK = 0, OLK_TEXT = 1, OLK_HEADING = 2, OLK_ELEMENT = 3,
+    OLK_LIST = 4, OLK_CONNECT = 5, OLK_ORDERED = 6, OLK_COMMENT = 7
+};
+
+// Tree key byte per line: depth for content lines, KEY_DEEP when the depth
+// does not fit a byte, KEY_BLANK for blank lines (outside the tree).
+#define KEY_BLANK 255u
+#define KEY_DEEP 254u
+
+// Device text buffers are padded with zeros to a multiple of TEXT_ALIGN plus
+// TEXT_SLACK bytes, so tile loads never need a bounds check.
+#define TEXT_ALIGN (64u * 1024u)
+#define TEXT_SLACK (64u * 1024u)
+// Per-line device arrays are padded to a multiple of LINE_ALIGN entries.
+#define LINE_ALIGN 4096u
+
+struct ol_node_dev {
+    uint32_t depth, kind, span_start, span_len;
+};
+
+// ---- decoupled look-back status words ------------------------------------
+// One 64-bit word per tile: flag in the top two bits, value in the rest.
+#define LB_FLAG_AGG 1ull
+#define LB_FLAG_INCL 2ull
+#define LB_SHIFT 62
+#define LB_MASK ((1ull << LB_SHIFT) - 1)
+
+__device__ __forceinline__ uint64_t ld_acquire_u64(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release_u64(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_acquire_u32(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release_u32(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Exclusive look-back over per-tile status words for the sum of the values of
+// all tiles before `t` (one wave calls this; every lane gets the result).
+// Tile t must already have published its own aggregate.
+__device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_t t)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t prefix = 0;
+    int64_t top = (int64_t)t - 1;
+    while (true) {
+        int64_t idx = top - lane;
+        uint64_t st = (idx >= 0) ? ld_acquire_u64(status + idx) : (LB_FLAG_INCL << LB_SHIFT);
+        uint64_t flag = st >> LB_SHIFT;
+        uint64_t incl = __ballot(flag == LB_FLAG_INCL);
+        uint64_t notready = __ballot(flag == 0);
+        int first = incl ? (int)__builtin_ctzll(incl) : 64;
+        uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
+        if (notready & window) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t contrib = (lane <= first) ? (st & LB_MASK) : 0;
+        prefix += wave_sum_u64(contrib);
+        if (first < 64)
+            break;
+        top -= 64;
+    }
+    return prefix;
+}
+
+// ---- kernel argument blocks ----------------------------------------------
+struct ScanArgs {
+    const uint8_t *text;     // padded device text
+    uint32_t n;              // text length (bytes)
+    uint32_t num_tiles;
+    uint64_t cap;            // capacity of the per-line outputs (lines)
+    uint32_t *offsets;       // line start offsets (u32)          [cap]
+    uint8_t *classes;        // NodeKind as u8                     [cap]
+    uint8_t *keys;           // tree key byte                      [cap]
+    ol_node_dev *nodes;      // FlatNode (nullable)                [cap]
+    uint64_t *status;        // look-back words                    [num_tiles]
+    uint32_t *tile_ctr;      // dynamic tile counter
+    uint64_t *meta;          // meta[0] <- line count
+};
+
+struct TreeArgs {
+    const uint8_t *keys;     // tree key byte per line (padded to LINE_ALIGN)
+    uint64_t lines;          // line count
+    uint32_t num_tiles;
+    uint32_t *parent, *first_child, *next_sibling, *subtree_end;  // nullable each
+    uint32_t *flags;         // look-back flag per tile
+    uint32_t *agg, *incl;    // per tile x level: last line index + 1 (0 = none)
+    uint32_t *tile_ctr;
+    uint32_t *err;           // set to 1 if a content line is too deep
+};
+
+struct SearchArgs {
+    const uint8_t *text;
+    uint32_t n;
+    uint32_t num_tiles;
+    uint32_t m;              // pattern length 1..64
+    uint32_t win, win_mask;  // first min(m,4) pattern bytes and their mask
+    uint8_t pat[64];
+    uint64_t cap;            // output capacity (matches)
+    uint64_t *out64;         // offsets as u64 (nullable)
+    uint32_t *out32;         // offsets as u32 (nullable)
+    uint64_t *status;
+    uint32_t *tile_ctr;
+    uint64_t *meta;          // meta[0] <- total matches
+};
+
+// Compiled dictionary in device memory.  Patterns are keyed by their first
+// k = min(len, 4) bytes ("class" k).  Per class: a filter bitmap (copied to
+// LDS by every workgroup) and an open-addressing table of 16-byte slots
+// {key, begin, end, 0} giving the range of that key's entries.  Entries are
+// sorted by (class, key, pattern id).
+struct DictDev {
+    uint32_t class_mask;     // bit k set if class k (1..4) is non-empty
+    uint32_t fbits[5];       // log2 filter bits per class
+    uint32_t foff[5];        // word offset of each class's filter in `filter`
+    uint32_t fwords;         // total filter words
+    uint32_t tbits[5];       // log2 table slots per class
+    const uint32_t *filter;
+    const uint4 *table[5];
+    const uint32_t *ent_id;
+    const uint32_t *ent_len;
+    const uint32_t *ent_off;
+    const uint8_t *pool;
+    uint32_t max_len;
+};
+
+struct ManyArgs {
+    const uint8_t *text;
+    uint32_t n;
+    uint32_t num_tiles;
+    DictDev d;
+    uint64_t cap;
+    uint64_t *out_off;       // u64 offsets (nullable)
+    uint32_t *out_id;        // pattern ids (nullable)
+    uint64_t *status;
+    uint32_t *tile_ctr;
+    uint64_t *meta;          // meta[0] <- total matches
+};
+
+// ---- launchers (each returns the launch error, if any) ---------------------
+hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s);
+hipError_t ol_launch_scan(const ScanArgs &a, hipStream_t s);
+hipError_t ol_launch_tree(const TreeArgs &a, hipStream_t s);
+hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
+hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
+
+// Tile geometry of each kernel.
+constexpr uint32_t SCAN_THREADS = 256;
+constexpr uint32_t SCAN_BPT = 64;
+constexpr uint32_t SCAN_TILE = SCAN_THREADS * SCAN_BPT;
+constexpr uint32_t SCAN_HALO = 256;
+constexpr uint32_t TREE_THREADS = 256;
+constexpr uint32_t TREE_RUN = 16;
+constexpr uint32_t TREE_TILE = TREE_THREADS * TREE_RUN;
+constexpr uint32_t TREE_D = 32;
+constexpr uint32_t SEARCH_THREADS = 256;
+constexpr uint32_t SEARCH_BPT = 64;
+constexpr uint32_t SEARCH_TILE = SEARCH_THREADS * SEARCH_BPT;
+hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t *offsets,
+                                 uint64_t lines, ol_node_dev *nodes, hipStream_t s);
+size_t ol_many_lds_bytes(uint32_t fwords);

@@ -1,0 +1,312 @@
+/*
+ * oracle.c -- CPU restatement of the reference's parse-and-search semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU path is compared
+ * against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it; the product library (oneline_amd/liboneline_hip.so) never links or
+ * calls it, and the product path fails loudly when its HIP build is missing.
+ *
+ * Every function restates one piece of the reference (Rust, tomet-lang/oneline)
+ * from its source; the file:line it follows is given above each function.  The
+ * reference cannot be compiled here (no cargo/rustc; its crates are not vendored),
+ * so the restatement is pinned instead by:
+ *   - the reference's own inline known answers (tests/golden/known_answers.json),
+ *   - a second, independently written byte-loop parser (tests/test_oracle.py),
+ *     checked exhaustively over the 8-symbol alphabet up to length 6,
+ *   - brute-force definitions of the tree relations and of search on small inputs.
+ *
+ * Build: oracle/Makefile -> oracle/liboneline_oracle.so (plain C, gcc).
+ */
+#include <stdint.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_NONE 0xFFFFFFFFu
+
+/* NodeKind values, crates/oneline-core/src/lib.rs:41-51 */
+enum {
+    K_BLANK = 0, K_TEXT = 1, K_HEADING = 2, K_ELEMENT = 3,
+    K_LIST = 4, K_CONNECT = 5, K_ORDERED = 6, K_COMMENT = 7
+};
+
+/* FlatNode, crates/oneline-core/src/lib.rs:87-96: four u32, 16 bytes. */
+typedef struct {
+    uint32_t depth, kind, span_start, span_len;
+} orc_node;
+
+static int is_blank(uint8_t b) { return b == ' ' || b == '\t'; }
+
+/* classify_line + FIRST_BYTE, crates/oneline-simd/src/line.rs:46-81.
+ * `c` is the content (after prefix and blanks, CR/LF trimmed), n >= 1. */
+static uint32_t classify(const uint8_t *c, size_t n)
+{
+    switch (c[0]) {
+    case '=': return K_HEADING;
+    case ':': return K_CONNECT;
+    case '@': return K_ELEMENT;
+    case '/': return (n > 1 && c[1] == '/') ? K_COMMENT : K_TEXT;
+    case '-':
+        if (n > 1 && is_blank(c[1])) return K_LIST;
+        if (n > 2 && c[1] == '.' && is_blank(c[2])) return K_ORDERED;
+        return K_TEXT;
+    default: return K_TEXT;
+    }
+}
+
+/* parse_line, crates/oneline-simd/src/line.rs:8-44.
+ * Trailing run of CR/LF is dropped; depth = number of '|', where blank runs
+ * between '|' runs belong to the prefix only when another '|' follows. */
+void orc_parse_line(const uint8_t *s, size_t len, uint32_t start, orc_node *o)
+{
+    size_t eff = len;
+    while (eff > 0 && (s[eff - 1] == '\r' || s[eff - 1] == '\n'))
+        eff--;
+    if (eff == 0) {
+        o->depth = 0; o->kind = K_BLANK; o->span_start = start; o->span_len = 0;
+        return;
+    }
+    size_t pos = 0;
+    while (pos < eff && s[pos] == '|')
+        pos++;
+    size_t depth = pos;
+    size_t bl = 0;
+    while (pos + bl < eff && is_blank(s[pos + bl]))
+        bl++;
+    while (depth > 0 && pos + bl < eff && s[pos + bl] == '|') {
+        pos += bl;
+        size_t run = 0;
+        while (pos + run < eff && s[pos + run] == '|')
+            run++;
+        depth += run;
+        pos += run;
+        bl = 0;
+        while (pos + bl < eff && is_blank(s[pos + bl]))
+            bl++;
+    }
+    size_t cs = pos + bl;
+    o->depth = (uint32_t)depth;
+    o->span_start = start + (uint32_t)cs;
+    if (cs == eff) {
+        o->kind = K_BLANK;
+        o->span_len = 0;
+        return;
+    }
+    o->kind = classify(s + cs, eff - cs);
+    o->span_len = (uint32_t)(eff - cs);
+}
+
+/* Number of lines: one per '\n' plus one for an unterminated tail.
+ * lines_in_chunk, crates/oneline-simd/src/lib.rs:38-44. */
+size_t orc_line_count(const uint8_t *t, size_t n)
+{
+    if (n == 0)
+        return 0;
+    size_t c = 0;
+    for (size_t i = 0; i < n; i++)
+        c += (t[i] == '\n');
+    return c + (t[n - 1] != '\n');
+}
+
+/* scan_line_offsets, crates/oneline-simd/src/lib.rs:49-66 (its test oracle,
+ * lib.rs:493-513, is the same split-on-'\n' rule): 0, then p+1 for every
+ * '\n' at p < n-1.  Writes min(count, cap) offsets; returns count. */
+size_t orc_scan_line_offsets(const uint8_t *t, size_t n, uint32_t *out, size_t cap)
+{
+    if (n == 0)
+        return 0;
+    size_t k = 0;
+    if (k < cap) out[k] = 0;
+    k++;
+    for (size_t i = 0; i + 1 < n; i++) {
+        if (t[i] == '\n') {
+            if (k < cap) out[k] = (uint32_t)(i + 1);
+            k++;
+        }
+    }
+    return k;
+}
+
+/* Whole-document parse: split on '\n', parse each piece
+ * (crates/oneline-simd/src/lib.rs:493-513).  offsets/nodes hold line_count
+ * entries (orc_line_count).  Returns the line count. */
+size_t orc_parse(const uint8_t *t, size_t n, uint32_t *offsets, orc_node *nodes)
+{
+    if (n == 0)
+        return 0;
+    size_t k = 0, s = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (t[i] == '\n') {
+            if (offsets) offsets[k] = (uint32_t)s;
+            if (nodes) orc_parse_line(t + s, i + 1 - s, (uint32_t)s, &nodes[k]);
+            k++;
+            s = i + 1;
+        }
+    }
+    if (s < n) {
+        if (offsets) offsets[k] = (uint32_t)s;
+        if (nodes) orc_parse_line(t + s, n - s, (uint32_t)s, &nodes[k]);
+        k++;
+    }
+    return k;
+}
+
+/* TreeLinks::from_nodes_sequential, crates/oneline-core/src/links.rs:52-81.
+ * Monotonic stack over content lines; blank lines get parent NONE and
+ * subtree_end i+1.  `kinds`/`depths` are per-line arrays. */
+void orc_links_sequential(const uint32_t *depths, const uint32_t *kinds, size_t n,
+                          uint32_t *parent, uint32_t *subtree_end)
+{
+    uint32_t *stack = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    size_t sp = 0;
+    for (size_t i = 0; i < n; i++) {
+        parent[i] = ORC_NONE;
+        subtree_end[i] = (uint32_t)(i + 1);
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (kinds[i] == K_BLANK)
+            continue;
+        while (sp > 0 && depths[stack[sp - 1]] >= depths[i]) {
+            subtree_end[stack[sp - 1]] = (uint32_t)i;
+            sp--;
+        }
+        parent[i] = sp ? stack[sp - 1] : ORC_NONE;
+        stack[sp++] = (uint32_t)i;
+    }
+    for (size_t k = 0; k < sp; k++)
+        subtree_end[stack[k]] = (uint32_t)n;
+    free(stack);
+}
+
+/* first_child / next_sibling.  The reference stores only parent and
+ * subtree_end (crates/oneline-core/src/links.rs:23-26); these two follow by
+ * closed form from its definitions:
+ *   first_child[i]  = first content line j in (i, subtree_end[i]), else NONE
+ *   next_sibling[i] = k = subtree_end[i] if k < n and parent[k] == parent[i],
+ *                     else NONE
+ * Both are NONE for blank lines. */
+void orc_links_derived(const uint32_t *kinds, const uint32_t *parent,
+                       const uint32_t *subtree_end, size_t n,
+                       uint32_t *first_child, uint32_t *next_sibling)
+{
+    /* next content line after i, computed right to left */
+    uint32_t next_content = ORC_NONE;
+    for (size_t ii = n; ii-- > 0;) {
+        if (kinds[ii] == K_BLANK) {
+            first_child[ii] = ORC_NONE;
+            next_sibling[ii] = ORC_NONE;
+            continue;
+        }
+        uint32_t se = subtree_end[ii];
+        first_child[ii] = (next_content != ORC_NONE && next_content < se) ? next_content : ORC_NONE;
+        next_sibling[ii] = (se < n && parent[se] == parent[ii]) ? se : ORC_NONE;
+        next_content = (uint32_t)ii;
+    }
+}
+
+/* All overlapping occurrences of pat in t, ascending; the naive
+ * all-positions compare of crates/oneline-simd/src/search.rs:106-114, which
+ * the reference uses to pin find_all (search.rs:21-26, 88-99).
+ * Empty pattern or text shorter than it -> none.  Writes min(count, cap)
+ * offsets; returns count. */
+size_t orc_find_all(const uint8_t *t, size_t n, const uint8_t *pat, size_t m,
+                    uint64_t *out, size_t cap)
+{
+    if (m == 0 || n < m)
+        return 0;
+    size_t k = 0;
+    for (size_t i = 0; i + m <= n; i++) {
+        if (t[i] == pat[0] && memcmp(t + i, pat, m) == 0) {
+            if (k < cap) out[k] = i;
+            k++;
+        }
+    }
+    return k;
+}
+
+/* Dictionary search: every (start, id) with t[start..start+len_id) == pat_id,
+ * sorted by start then id.  This is the closed form of what the reference's
+ * ManyPatterns (aho-corasick 1.1.5, MatchKind::Standard, overlapping
+ * iteration; crates/oneline-simd/src/search.rs:206-283) reports: every
+ * occurrence of every pattern, a duplicate pattern reporting each of its ids
+ * (crates/oneline-gpu/src/tests.rs:805-826).  Patterns are given as one byte
+ * pool plus per-pattern offset and length.
+ *
+ * Implementation: patterns are grouped by length; for each start and each
+ * length present, the candidate slice is looked up in a sorted index of the
+ * patterns of that length (binary search), so cost is O(n * lengths * log P).
+ * Writes min(count, cap) hits; returns count. */
+typedef struct {
+    const uint8_t *pool;
+    const uint64_t *poff;
+    const uint32_t *plen;
+} orc_pats;
+
+static orc_pats g_pats;     /* qsort context (single-threaded oracle) */
+
+static int cmp_pat(const void *a, const void *b)
+{
+    uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    uint32_t lx = g_pats.plen[x], ly = g_pats.plen[y];
+    if (lx != ly) return lx < ly ? -1 : 1;
+    int c = memcmp(g_pats.pool + g_pats.poff[x], g_pats.pool + g_pats.poff[y], lx);
+    if (c) return c;
+    return x < y ? -1 : (x > y);
+}
+
+size_t orc_many_find_all(const uint8_t *t, size_t n, const uint8_t *pool,
+                         const uint64_t *poff, const uint32_t *plen, size_t np,
+                         uint64_t *out_off, uint32_t *out_id, size_t cap)
+{
+    if (np == 0 || n == 0)
+        return 0;
+    uint32_t *order = (uint32_t *)malloc(np * sizeof(uint32_t));
+    for (size_t i = 0; i < np; i++)
+        order[i] = (uint32_t)i;
+    g_pats.pool = pool; g_pats.poff = poff; g_pats.plen = plen;
+    qsort(order, np, sizeof(uint32_t), cmp_pat);
+    /* ranges of equal length in `order` */
+    size_t nl = 0;
+    size_t *lbeg = (size_t *)malloc((np + 1) * sizeof(size_t));
+    uint32_t *lval = (uint32_t *)malloc((np + 1) * sizeof(uint32_t));
+    for (size_t i = 0; i < np; i++) {
+        if (i == 0 || plen[order[i]] != plen[order[i - 1]]) {
+            lbeg[nl] = i;
+            lval[nl] = plen[order[i]];
+            nl++;
+        }
+    }
+    lbeg[nl] = np;
+    size_t k = 0;
+    uint32_t *ids = (uint32_t *)malloc(np * sizeof(uint32_t));
+    for (size_t p = 0; p < n; p++) {
+        size_t nid = 0;
+        for (size_t g = 0; g < nl; g++) {
+            uint32_t L = lval[g];
+            if (L == 0 || p + L > n)
+                continue;
+            /* lower bound of slice among patterns of this length */
+            size_t lo = lbeg[g], hi = lbeg[g + 1];
+            while (lo < hi) {
+                size_t mid = (lo + hi) / 2;
+                int c = memcmp(pool + poff[order[mid]], t + p, L);
+                if (c < 0) lo = mid + 1; else hi = mid;
+            }
+            while (lo < lbeg[g + 1] && memcmp(pool + poff[order[lo]], t + p, L) == 0)
+                ids[nid++] = order[lo++];
+        }
+        /* ids of different lengths interleave: sort this position's ids */
+        for (size_t a = 1; a < nid; a++) {
+            uint32_t v = ids[a];
+            size_t b = a;
+            while (b > 0 && ids[b - 1] > v) { ids[b] = ids[b - 1]; b--; }
+            ids[b] = v;
+        }
+        for (size_t a = 0; a < nid; a++) {
+            if (k < cap) { out_off[k] = p; out_id[k] = ids[a]; }
+            k++;
+        }
+    }
+    free(ids); free(order); free(lbeg); free(lval);
+    return k;
+}

@@ -1,0 +1,122 @@
+"""Dictionary search on the HIP path against the oracle, in both output forms:
+the offset-major (offset, id) list and the drop-in ManyResult (per-pattern
+counts, hits sorted by (pattern, offset)) (crates/oneline-gpu/src/tests.rs:732-974)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oneline_amd as ol
+from oneline_amd import datagen
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def device_list(parser, doc, pset, cap):
+    """Offset-major list via ol_search_many_device (device buffers via hipMalloc)."""
+    lib = ol.load_library()
+    hip = ctypes.CDLL("libamdhip64.so")
+    d_off = ctypes.c_void_p()
+    d_id = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d_off), ctypes.c_size_t(max(1, cap) * 8)) == 0
+    assert hip.hipMalloc(ctypes.byref(d_id), ctypes.c_size_t(max(1, cap) * 4)) == 0
+    try:
+        tot = ctypes.c_uint64()
+        ol._check(lib.ol_search_many_device(parser._ctx, doc._h, pset._h, d_off, d_id, cap, ctypes.byref(tot)))
+        keep = min(cap, tot.value)
+        off = np.zeros(keep, np.uint64)
+        ids = np.zeros(keep, np.uint32)
+        if keep:
+            assert hip.hipMemcpy(ctypes.c_void_p(off.ctypes.data), d_off, ctypes.c_size_t(keep * 8), 2) == 0
+            assert hip.hipMemcpy(ctypes.c_void_p(ids.ctypes.data), d_id, ctypes.c_size_t(keep * 4), 2) == 0
+        return tot.value, off, ids
+    finally:
+        hip.hipFree(d_off)
+        hip.hipFree(d_id)
+
+
+def assert_many_matches(parser, text, patterns, what=""):
+    text = ol.as_bytes_array(text)
+    want_off, want_id = orc.many_find_all(text, patterns)
+    doc = parser.upload(text, nodes=False)
+    pset = parser.compile_patterns(doc, patterns)
+    assert len(pset) == len(patterns)
+    tot, off, ids = device_list(parser, doc, pset, want_off.size)
+    assert tot == want_off.size, f"{what}: total"
+    assert np.array_equal(off, want_off), f"{what}: offsets"
+    assert np.array_equal(ids, want_id), f"{what}: ids"
+    counts = np.bincount(want_id, minlength=len(patterns)).astype(np.uint32)
+    counted = parser.search_many(doc, pset, 0)
+    assert np.array_equal(counted.counts, counts), f"{what}: counts (count-only)"
+    assert counted.total_hits == want_off.size and counted.hits.size == 0
+    full = parser.search_many(doc, pset, max(1, want_off.size))
+    assert np.array_equal(full.counts, counts), f"{what}: counts"
+    order = np.lexsort((want_off, want_id))
+    assert np.array_equal(full.hits["pattern"], want_id[order]), f"{what}: hit patterns"
+    assert np.array_equal(full.hits["offset"], want_off[order].astype(np.uint32)), f"{what}: hit offsets"
+
+
+def test_corpus(parser, known):
+    c = known["many_corpus"]
+    assert_many_matches(parser, c["text"].encode(), [p.encode() for p in c["patterns"]], "nested/overlapping")
+    for text, pats in c["also"]:
+        assert_many_matches(parser, text.encode(), [p.encode() for p in pats], text)
+
+
+def test_lengths_1_to_64_at_all_alignments(parser):
+    text = b""
+    pats = []
+    for ln in range(1, 65):
+        p = bytes(ord("a") + (i * 7 + ln) % 26 for i in range(ln))
+        for shift in range(4):
+            text += b"#" * shift + p + b"\n"
+        pats.append(p)
+    assert_many_matches(parser, text, pats, "lengths 1..=64")
+
+
+def test_generated_text_and_dictionaries(parser):
+    text = datagen.document(3 << 20, seed=12)
+    rng = np.random.default_rng(77)
+    for n in (1, 2, 7, 64, 500, 2048):
+        pats = []
+        for i in range(n):
+            if i % 3 == 2:
+                ln = int(rng.integers(1, 65))
+                at = int(rng.integers(0, text.size - ln))
+                pats.append(bytes(text[at:at + ln]))
+            else:
+                pats.append(datagen.vocab_word(int(rng.integers(0, 10_000))))
+        assert_many_matches(parser, text, pats, f"{n} patterns")
+
+
+def test_benchmark_dictionaries(parser):
+    text = datagen.document(4 << 20, seed=0, crlf=10, empty=5)
+    for count in (1024, 65536):
+        assert_many_matches(parser, text, datagen.patterns(count, seed=0xC3B), f"{count}-pattern set")
+
+
+def test_validation_and_truncation(parser):
+    doc = parser.upload(datagen.document(2 << 20, seed=3))
+    with pytest.raises(ol.OnelineError):
+        parser.compile_patterns(doc, [])
+    with pytest.raises(ol.OnelineError):
+        parser.compile_patterns(doc, [b""])
+    with pytest.raises(ol.OnelineError):
+        parser.compile_patterns(doc, [b"a" * 65])
+    pset = parser.compile_patterns(doc, [datagen.vocab_word(0), datagen.vocab_word(1)])
+    all_ = parser.search_many(doc, pset, 0)
+    assert all_.total_hits > 1000
+    few = parser.search_many(doc, pset, 50)
+    assert np.array_equal(few.counts, all_.counts) and few.total_hits == all_.total_hits
+    assert few.hits.size == 50
+    for _ in range(3):
+        assert np.array_equal(parser.search_many(doc, pset, 0).counts, all_.counts)
+    empty = parser.upload(b"")
+    assert parser.search_many(empty, pset, 8).counts.tolist() == [0, 0]
+
+
+def test_start_before_text_and_saturation(parser):
+    text = b"bcdq" + b"\0" * 5000 + b"xyz"
+    assert_many_matches(parser, text, [b"\0bcdq", b"zzzz"], "start before the text")
+    assert_many_matches(parser, b"a" * (1 << 18), [b"aaaa", b"a", b"aa", b"zq001x"], "saturated")

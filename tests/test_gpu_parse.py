@@ -1,0 +1,118 @@
+"""Scan + classify on the HIP path, bit-exact against the oracle
+(the reference's own GPU tests: crates/oneline-gpu/src/tests.rs:27-279)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oneline_amd as ol
+from oneline_amd import datagen
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_parse_matches(parser, text, what=""):
+    text = ol.as_bytes_array(text)
+    off, nodes = orc.parse(text)
+    tree = parser.parse_raw(text)
+    assert tree.line_offsets.size == off.size, f"{what}: line count"
+    assert np.array_equal(tree.line_offsets, off), f"{what}: offsets"
+    if not np.array_equal(tree.nodes, nodes):
+        i = int(np.nonzero(tree.nodes != nodes)[0][0])
+        raise AssertionError(f"{what}: node {i} gpu {tree.nodes[i]} cpu {nodes[i]}")
+
+
+def test_known_answers(parser, known):
+    for case in known["scan_line_offsets"]:
+        assert parser.scan_line_offsets(case["text"]).tolist() == case["offsets"], case["src"]
+    for case in known["parse_document"]:
+        tree = parser.parse_raw(case["text"])
+        got = [[int(n["depth"]), int(n["kind"])] for n in tree.nodes]
+        assert got == case["depth_kind"], case["src"]
+        offs = orc.scan_line_offsets(case["text"].encode())
+        given = parser.parse(case["text"], offs)
+        assert [[int(n["depth"]), int(n["kind"])] for n in given.nodes] == case["depth_kind"]
+
+
+def test_edge_corpus(parser, known):
+    for case in known["parse_edge_corpus"]["cases"] + known["scan_corpus"]["cases"]:
+        assert_parse_matches(parser, case, repr(case))
+        assert parser.scan_line_offsets(case).tolist() == orc.scan_line_offsets(case.encode()).tolist()
+
+
+def test_parse_with_given_offsets(parser, known):
+    for case in known["parse_edge_corpus"]["cases"]:
+        off, nodes = orc.parse(case.encode())
+        got = parser.parse(case, off)
+        assert np.array_equal(got.nodes, nodes), repr(case)
+
+
+def test_awkward_shapes(parser):
+    cases = {
+        "long lines": b"a" * 100_000 + b"\n|" + b"b" * 300_000,
+        "one giant prefix": b"|" * 1_000_000,
+        "crlf only": b"x\r\n" * 300_000,
+        "all newlines": b"\n" * 400_000,
+        "tail without newline": b"line\n" * 200_000 + b"tail without newline",
+        "cr runs": b"a\r\r\r\n" * 5000 + b"\r\r",
+        "long blank prefix": (b"| " * 2000 + b"x\n") * 50,
+    }
+    for name, text in cases.items():
+        assert_parse_matches(parser, text, name)
+
+
+def test_tile_boundaries(parser):
+    base = datagen.document(1 << 20, seed=11)
+    for n in (1, 15, 16, 17, 255, 256, 257, 16383, 16384, 16385, 32768, 65535, 65536, 65537, 200_000):
+        t = base[:n].copy()
+        assert_parse_matches(parser, t, f"n={n}")
+        t[-1] = ord("x")   # unterminated last line
+        assert_parse_matches(parser, t, f"n={n} unterminated")
+
+
+def test_newline_at_every_position_near_a_tile_edge(parser):
+    for p in range(16384 - 70, 16384 + 70, 7):
+        t = np.full(40_000, ord("a"), np.uint8)
+        t[p] = 0x0A
+        t[p + 1:p + 5] = np.frombuffer(b"|| -", np.uint8)
+        assert_parse_matches(parser, t, f"newline at {p}")
+
+
+def test_generated_documents(parser):
+    text = datagen.document(1 << 20, seed=0)
+    assert_parse_matches(parser, text, "1 MiB seed 0")
+    text = datagen.document(37 << 20, seed=3, crlf=10, empty=5)
+    assert_parse_matches(parser, text, "37 MiB crlf/empty")
+
+
+def test_resident_document_arrays(parser):
+    text = datagen.document(9 << 20, seed=5, crlf=10, empty=5)
+    off, nodes = orc.parse(text)
+    doc = parser.upload(text)
+    assert doc.line_count() == off.size and doc.text_len() == text.size
+    o, c, nd = doc.download(nodes=True)
+    assert np.array_equal(o, off)
+    assert np.array_equal(c, nodes["kind"].astype(np.uint8))
+    assert np.array_equal(nd, nodes)
+    lean = parser.upload(text, nodes=False)
+    o2, c2, _ = lean.download()
+    assert np.array_equal(o2, off) and np.array_equal(c2, c)
+
+
+def test_empty_input(parser):
+    assert parser.scan_line_offsets(b"").size == 0
+    assert parser.parse_raw(b"").nodes.size == 0
+    doc = parser.upload(b"")
+    assert doc.line_count() == 0 and doc.text_len() == 0
+
+
+def test_oversized_input_is_an_error(parser):
+    lib = ol.load_library()
+    buf = np.zeros(16, np.uint8)
+    h = ctypes.c_void_p()
+    rc = lib.ol_doc_upload(parser._ctx, ctypes.c_void_p(buf.ctypes.data), ol.MAX_GPU_INPUT_LEN + 1, ctypes.byref(h))
+    assert rc == -2
+    assert "bytes" in lib.ol_last_error().decode()
+    # still usable
+    assert parser.scan_line_offsets(b"a\nb").tolist() == [0, 2]
