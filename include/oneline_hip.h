@@ -170,6 +170,15 @@ int ol_search_many(ol_ctx *ctx, const ol_doc *doc, const ol_dict *dict, size_t m
 int ol_search_many_device(ol_ctx *ctx, const ol_doc *doc, const ol_dict *dict, uint64_t *d_offsets,
                           uint32_t *d_ids, size_t cap, uint64_t *total);
 
+/* ---- launch tuning (no counterpart in the reference: its tile sizes are
+ * compile-time constants, e.g. crates/oneline-gpu/src/scan.rs:11).  The
+ * autotuner (oneline_amd/autotune.py) sets these from measurements:
+ *   "scan_bpt"      bytes per lane of the scan kernel: 64, 128 or 256
+ *                   (tile = 256 lanes x bpt bytes);
+ *   "tree_threads"  lanes per tree tile: 64 or 128 (tile = lanes x 16 lines). */
+int ol_tune_set(ol_ctx *ctx, const char *name, int value);
+int ol_tune_get(ol_ctx *ctx, const char *name, int *value);
+
 /* ---- profiling (the reference's Profile / StageTimer, crates/oneline-gpu/src/timing.rs:13-123) --- */
 /* When enabled, every kernel launch is bracketed by HIP events on the
  * context's stream.  ol_profile_take waits for the stream and returns, per

@@ -98,6 +98,8 @@ def load_library() -> ctypes.CDLL:
         "ol_search_many": [P, P, P, S, P, U64P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_many_device": [P, P, P, P, P, S, U64P],
         "ol_profile_enable": [P, ctypes.c_int],
+        "ol_tune_set": [P, ctypes.c_char_p, ctypes.c_int],
+        "ol_tune_get": [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
         "ol_profile_take": [P, P, S, P, P, S, ctypes.POINTER(S)],
     }
     for name, args in sig.items():
@@ -401,6 +403,15 @@ class GpuParser:
         _check(_lib.ol_search_many(self._ctx, doc._h, pset._h, max_hits, _ptr(counts), ctypes.byref(tot),
                                    ctypes.byref(p), ctypes.byref(n)))
         return ManyResult(counts, tot.value, _take(p, n.value, MANY_HIT))
+
+    # -- launch tuning ------------------------------------------------------------------
+    def tune_set(self, name: str, value: int) -> None:
+        _check(_lib.ol_tune_set(self._ctx, name.encode(), int(value)))
+
+    def tune_get(self, name: str) -> int:
+        v = ctypes.c_int()
+        _check(_lib.ol_tune_get(self._ctx, name.encode(), ctypes.byref(v)))
+        return v.value
 
     # -- profiling ----------------------------------------------------------------------
     def profile_enable(self, on: bool = True) -> None:

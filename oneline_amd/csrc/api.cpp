@@ -71,6 +71,9 @@ struct ol_ctx {
     std::mutex mu;
     DevBuf scratch;
     DevBuf tree_agg, tree_incl;
+    // launch geometry picked by the autotuner (ol_tune_set)
+    uint32_t scan_bpt = SCAN_BPT;          // bytes per lane of scan_classify (tile = 256 lanes x bpt)
+    uint32_t tree_threads = TREE_THREADS;  // lanes per tree tile (tile = lanes x TREE_RUN lines)
     // profiling
     bool profiling = false;
     struct Rec {
@@ -193,7 +196,7 @@ int alloc_doc_lines(ol_doc *d, uint64_t cap)
 
 int launch_scan(ol_ctx *c, ol_doc *d)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->n, SCAN_TILE);
+    uint32_t tiles = (uint32_t)ceil_div(d->n, (uint64_t)SCAN_THREADS * c->scan_bpt);
     HIPCHK(ensure_scratch(c, tiles));
     HIPCHK(clear_scratch(c, tiles));
     ScanArgs a{};
@@ -209,7 +212,7 @@ int launch_scan(ol_ctx *c, ol_doc *d)
     a.tile_ctr = c->ctr();
     a.meta = c->meta();
     Prof p(c, "scan_classify");
-    HIPCHK(ol_launch_scan(a, c->stream));
+    HIPCHK(ol_launch_scan(a, c->scan_bpt, c->stream));
     return OL_OK;
 }
 
@@ -309,12 +312,11 @@ int ensure_tree(ol_tree *t, uint64_t lines)
 
 int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->lines, TREE_TILE);
-    size_t flag_words = ceil_div(tiles, 2);
-    HIPCHK(ensure_scratch(c, flag_words));
+    uint32_t tiles = (uint32_t)ceil_div(d->lines, (uint64_t)c->tree_threads * TREE_RUN);
+    HIPCHK(ensure_scratch(c, 0));
     HIPCHK(c->tree_agg.ensure((size_t)tiles * TREE_D * 4));
     HIPCHK(c->tree_incl.ensure((size_t)tiles * TREE_D * 4));
-    HIPCHK(clear_scratch(c, flag_words));
+    HIPCHK(clear_scratch(c, 0));
     TreeArgs a{};
     a.keys = d->keys;
     a.lines = d->lines;
@@ -323,13 +325,11 @@ int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
     a.first_child = t->fc;
     a.next_sibling = t->ns;
     a.subtree_end = t->se;
-    a.flags = (uint32_t *)c->status();
-    a.agg = (uint32_t *)c->tree_agg.p;
-    a.incl = (uint32_t *)c->tree_incl.p;
-    a.tile_ctr = c->ctr();
+    a.summary = (uint32_t *)c->tree_agg.p;
+    a.incoming = (uint32_t *)c->tree_incl.p;
     a.err = c->ctr() + 1;
     Prof p(c, "tree");
-    HIPCHK(ol_launch_tree(a, c->stream));
+    HIPCHK(ol_launch_tree(a, c->tree_threads, c->stream));
     return OL_OK;
 }
 
@@ -1073,6 +1073,42 @@ int ol_search_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, size_t max_hits
     }
     *hits = h;
     *n_hits = keep;
+    return OL_OK;
+}
+
+// ---- launch tuning ----------------------------------------------------------------
+int ol_tune_set(ol_ctx *c, const char *name, int value)
+{
+    if (!c || !name)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    std::string n(name);
+    if (n == "scan_bpt") {
+        if (value != 64 && value != 128 && value != 256)
+            return fail(OL_ERR_INVALID, "scan_bpt must be 64, 128 or 256");
+        c->scan_bpt = (uint32_t)value;
+    } else if (n == "tree_threads") {
+        if (value != 64 && value != 128)
+            return fail(OL_ERR_INVALID, "tree_threads must be 64 or 128");
+        c->tree_threads = (uint32_t)value;
+    } else {
+        return fail(OL_ERR_INVALID, "unknown tuning parameter " + n);
+    }
+    return OL_OK;
+}
+
+int ol_tune_get(ol_ctx *c, const char *name, int *value)
+{
+    if (!c || !name || !value)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    std::string n(name);
+    if (n == "scan_bpt")
+        *value = (int)c->scan_bpt;
+    else if (n == "tree_threads")
+        *value = (int)c->tree_threads;
+    else
+        return fail(OL_ERR_INVALID, "unknown tuning parameter " + n);
     return OL_OK;
 }
 

@@ -37,21 +37,17 @@ struct ol_node_dev {
 #define LB_SHIFT 62
 #define LB_MASK ((1ull << LB_SHIFT) - 1)
 
-__device__ __forceinline__ uint64_t ld_acquire_u64(const uint64_t *p)
+// Look-back status words are self-describing (flag + value in one word), so
+// they are read and written with RELAXED agent-scope atomics.  Acquire/release
+// at agent scope would compile to whole-L2 write-back / invalidate on this
+// multi-XCD part (each XCD has its own L2), which serialises the whole chip.
+__device__ __forceinline__ uint64_t ld_relaxed_u64(const uint64_t *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release_u64(uint64_t *p, uint64_t v)
+__device__ __forceinline__ void st_relaxed_u64(uint64_t *p, uint64_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t ld_acquire_u32(const uint32_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_release_u32(uint32_t *p, uint32_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
@@ -63,30 +59,50 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 }
 
 // Exclusive look-back over per-tile status words for the sum of the values of
-// all tiles before `t` (one wave calls this; every lane gets the result).
-// Tile t must already have published its own aggregate.
+// all tiles before `t` (one full wave calls this; every lane gets the result).
+// Tile t must already have published its own aggregate.  Each round trip reads
+// a window of LB_WINDOWS x 64 predecessors, so the look-back keeps up with
+// tiles arriving faster than one memory latency per tile.
+#define LB_WINDOWS 4
 __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_t t)
 {
     const int lane = threadIdx.x & 63;
     uint64_t prefix = 0;
     int64_t top = (int64_t)t - 1;
     while (true) {
-        int64_t idx = top - lane;
-        uint64_t st = (idx >= 0) ? ld_acquire_u64(status + idx) : (LB_FLAG_INCL << LB_SHIFT);
-        uint64_t flag = st >> LB_SHIFT;
-        uint64_t incl = __ballot(flag == LB_FLAG_INCL);
-        uint64_t notready = __ballot(flag == 0);
-        int first = incl ? (int)__builtin_ctzll(incl) : 64;
-        uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
-        if (notready & window) {
-            __builtin_amdgcn_s_sleep(1);
+        uint64_t st[LB_WINDOWS];
+#pragma unroll
+        for (int k = 0; k < LB_WINDOWS; k++) {
+            int64_t idx = top - 64 * k - lane;
+            st[k] = (idx >= 0) ? ld_relaxed_u64(status + idx) : (LB_FLAG_INCL << LB_SHIFT);
+        }
+        bool retry = false, done = false;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < LB_WINDOWS; k++) {
+            uint64_t flag = st[k] >> LB_SHIFT;
+            uint64_t incl = __ballot(flag == LB_FLAG_INCL);
+            uint64_t notready = __ballot(flag == 0);
+            int first = incl ? (int)__builtin_ctzll(incl) : 64;
+            uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
+            if (notready & window) {
+                retry = true;
+                break;
+            }
+            acc += wave_sum_u64((lane <= first) ? (st[k] & LB_MASK) : 0);
+            if (first < 64) {
+                done = true;
+                break;
+            }
+        }
+        if (retry) {
+            __builtin_amdgcn_s_sleep(2);
             continue;
         }
-        uint64_t contrib = (lane <= first) ? (st & LB_MASK) : 0;
-        prefix += wave_sum_u64(contrib);
-        if (first < 64)
+        prefix += acc;
+        if (done)
             break;
-        top -= 64;
+        top -= 64 * LB_WINDOWS;
     }
     return prefix;
 }
@@ -111,9 +127,8 @@ struct TreeArgs {
     uint64_t lines;          // line count
     uint32_t num_tiles;
     uint32_t *parent, *first_child, *next_sibling, *subtree_end;  // nullable each
-    uint32_t *flags;         // look-back flag per tile
-    uint32_t *agg, *incl;    // per tile x level: last line index + 1 (0 = none)
-    uint32_t *tile_ctr;
+    uint32_t *summary;       // per tile x level: last line index + 1 in the tile (0 = none)
+    uint32_t *incoming;      // per tile x level: the same over all earlier tiles
     uint32_t *err;           // set to 1 if a content line is too deep
 };
 
@@ -167,22 +182,21 @@ struct ManyArgs {
 
 // ---- launchers (each returns the launch error, if any) ---------------------
 hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s);
-hipError_t ol_launch_scan(const ScanArgs &a, hipStream_t s);
-hipError_t ol_launch_tree(const TreeArgs &a, hipStream_t s);
+hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, hipStream_t s);   // bpt: 64, 128 or 256
+hipError_t ol_launch_tree(const TreeArgs &a, uint32_t threads, hipStream_t s);   // summary + scan + walk; threads 128 or 256
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
 
 // Tile geometry of each kernel.
 constexpr uint32_t SCAN_THREADS = 256;
-constexpr uint32_t SCAN_BPT = 64;
-constexpr uint32_t SCAN_TILE = SCAN_THREADS * SCAN_BPT;
+constexpr uint32_t SCAN_BPT = 64;                           // default bytes per lane
+constexpr uint32_t SCAN_TILE_MAX = SCAN_THREADS * 256;
 constexpr uint32_t SCAN_HALO = 256;
-constexpr uint32_t TREE_THREADS = 256;
+constexpr uint32_t TREE_THREADS = 128;                      // default lanes per tile
 constexpr uint32_t TREE_RUN = 16;
-constexpr uint32_t TREE_TILE = TREE_THREADS * TREE_RUN;
 constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
-constexpr uint32_t SEARCH_BPT = 64;
+constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks
 constexpr uint32_t SEARCH_TILE = SEARCH_THREADS * SEARCH_BPT;
 hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t *offsets,
                                  uint64_t lines, ol_node_dev *nodes, hipStream_t s);

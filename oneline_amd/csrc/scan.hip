@@ -40,13 +40,44 @@ hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s)
     return hipGetLastError();
 }
 
-// 4-bit mask of the '\n' bytes of a little-endian word (exact SWAR test).
-__device__ __forceinline__ uint32_t nl_nibble(uint32_t w)
+// 4-bit mask of the bytes of a little-endian word equal to the byte replicated
+// in `pat` (exact SWAR test: no false positives from borrows).
+__device__ __forceinline__ uint32_t eq_nibble(uint32_t w, uint32_t pat)
 {
-    uint32_t x = w ^ 0x0A0A0A0Au;
-    uint32_t nz = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;   // bit 7 of each byte: byte != '\n'
+    uint32_t x = w ^ pat;
+    uint32_t nz = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;   // bit 7 of each byte: byte != pat
     uint32_t z = ~nz & 0x80808080u;
     return (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
+}
+__device__ __forceinline__ uint32_t nl_nibble(uint32_t w) { return eq_nibble(w, 0x0A0A0A0Au); }
+
+// '\n', '|' and blank (space / tab) masks of 64 LDS bytes.
+__device__ __forceinline__ void masks64(const uint4 *src, uint64_t &nl, uint64_t &pipe, uint64_t &blank)
+{
+    nl = pipe = blank = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint4 v = src[k];
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        uint64_t qn = 0, qp = 0, qb = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            qn |= (uint64_t)eq_nibble(w[i], 0x0A0A0A0Au) << (4 * i);
+            qp |= (uint64_t)eq_nibble(w[i], 0x7C7C7C7Cu) << (4 * i);
+            qb |= (uint64_t)(eq_nibble(w[i], 0x20202020u) | eq_nibble(w[i], 0x09090909u)) << (4 * i);
+        }
+        nl |= qn << (16 * k);
+        pipe |= qp << (16 * k);
+        blank |= qb << (16 * k);
+    }
+}
+
+// 64 mask bits starting at bit `b` of a mask array.
+__device__ __forceinline__ uint64_t window64(const uint64_t *m, uint32_t b)
+{
+    uint32_t w = b >> 6, sh = b & 63;
+    uint64_t lo = m[w] >> sh;
+    return sh ? (lo | (m[w + 1] << (64 - sh))) : lo;
 }
 
 __device__ __forceinline__ bool is_blank(uint32_t b) { return b == ' ' || b == '\t'; }
@@ -67,17 +98,6 @@ __device__ __forceinline__ uint32_t classify(uint32_t c0, uint32_t c1, uint32_t 
     default: return OLK_TEXT;
     }
 }
-
-// Byte source for line parsing: the LDS window [lo, hi) or global memory.
-struct ByteSrc {
-    const uint8_t *lds;   // lds[0] is text position `lo`
-    const uint8_t *g;
-    uint32_t lo, hi;
-    __device__ __forceinline__ uint32_t operator()(uint32_t pos) const
-    {
-        return (pos >= lo && pos < hi) ? lds[pos - lo] : g[pos];
-    }
-};
 
 // parse_line, crates/oneline-simd/src/line.rs:8-44, over text[s ..= e].
 template <class Src>
@@ -161,16 +181,79 @@ __device__ uint32_t find_line_start_global(const uint8_t *text, uint32_t end)
     return 0;
 }
 
+// parse_line (crates/oneline-simd/src/line.rs:8-44) for text[s ..= e] from the
+// '|' / blank bit masks instead of a byte loop.  With P = '|' bits and
+// B = blank bits from the line start: a line starting with '|' has its prefix
+// = the leading run of P|B (depth = '|' count in it, content after it); one
+// starting with a blank skips the leading blank run at depth 0.  Returns false
+// (caller falls back to the byte loop) when the prefix runs past the 64-bit
+// window.  `rel` is the bit index of s in the mask arrays.
+template <class Src>
+__device__ __forceinline__ bool line_head(const uint64_t *pm, const uint64_t *bm, const Src &b, uint32_t rel,
+                                          uint32_t s, uint32_t e_incl, ol_node_dev &nd)
+{
+    uint32_t eff = e_incl + 1;
+    while (eff > s) {
+        uint32_t c = b(eff - 1);
+        if (c != '\n' && c != '\r')
+            break;
+        eff--;
+    }
+    if (eff == s) {
+        nd.depth = 0; nd.kind = OLK_BLANK; nd.span_start = s; nd.span_len = 0;
+        return true;
+    }
+    const uint32_t len = eff - s;
+    uint64_t P = window64(pm, rel), B = window64(bm, rel);
+    if (len < 64) {
+        const uint64_t lm = (1ull << len) - 1;
+        P &= lm;
+        B &= lm;
+    }
+    uint32_t depth = 0, r = 0;
+    if (P & 1) {
+        const uint64_t nq = ~(P | B);
+        if (!nq)
+            return false;                       // prefix longer than the window
+        r = (uint32_t)__builtin_ctzll(nq);
+        depth = (uint32_t)__popcll(P & ((1ull << r) - 1));
+    } else if (B & 1) {
+        const uint64_t nb = ~B;
+        if (!nb)
+            return false;
+        r = (uint32_t)__builtin_ctzll(nb);
+    }
+    const uint32_t cs = s + r;
+    nd.depth = depth;
+    nd.span_start = cs;
+    if (cs >= eff) {
+        nd.span_start = eff;
+        nd.kind = OLK_BLANK;
+        nd.span_len = 0;
+        return true;
+    }
+    const uint32_t c0 = b(cs);
+    const uint32_t c1 = (cs + 1 < eff) ? b(cs + 1) : 256u;
+    const uint32_t c2 = (cs + 2 < eff) ? b(cs + 2) : 256u;
+    nd.kind = classify(c0, c1, c2);
+    nd.span_len = eff - cs;
+    return true;
+}
+
 template <int THREADS, int BPT>
 __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
 {
     constexpr int TILE = THREADS * BPT;
     constexpr int HALO = SCAN_HALO;
     constexpr int NW = THREADS / 64;
-    static_assert(BPT % 16 == 0 && BPT <= 64, "BPT");
+    constexpr int NM = BPT / 64;   // 64-bit terminator masks per lane
+    static_assert(BPT % 64 == 0 && NM >= 1, "BPT");
     static_assert(NW >= 2, "needs two waves");
 
+    constexpr int MW = (HALO + TILE) / 64 + 1;   // mask words (+1 so a window may read past the end)
     __shared__ __align__(16) uint8_t s_text[HALO + TILE];
+    __shared__ uint64_t s_pm[MW];   // '|' bits of the staged window
+    __shared__ uint64_t s_bm[MW];   // blank bits of the staged window
     __shared__ uint32_t s_wcnt[NW];
     __shared__ int32_t s_wlast[NW];
     __shared__ uint32_t s_tile;
@@ -187,35 +270,56 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     {
         const uint4 *src = (const uint4 *)(a.text + base);
         uint4 *dst = (uint4 *)(s_text + HALO);
+        uint4 v[BPT / 16];
 #pragma unroll
         for (int k = 0; k < BPT / 16; k++)
-            dst[k * THREADS + tid] = src[k * THREADS + tid];
+            v[k] = src[k * THREADS + tid];
+#pragma unroll
+        for (int k = 0; k < BPT / 16; k++)
+            dst[k * THREADS + tid] = v[k];
         if (tid < HALO / 16) {
-            uint4 v = make_uint4(0, 0, 0, 0);
+            uint4 h = make_uint4(0, 0, 0, 0);
             if (t > 0)
-                v = ((const uint4 *)(a.text + base - HALO))[tid];
-            ((uint4 *)s_text)[tid] = v;
+                h = ((const uint4 *)(a.text + base - HALO))[tid];
+            ((uint4 *)s_text)[tid] = h;
         }
     }
     __syncthreads();
 
+    // Lane-contiguous BPT bytes -> NM masks of terminators, plus the '|' and
+    // blank masks the line heads are decoded from.
     const uint32_t lo = tid * BPT;
-    uint64_t mask = 0;
+    uint64_t mask[NM];
+    uint32_t cnt = 0;
+    int32_t mylast = -1;
     {
         const uint4 *mine = (const uint4 *)(s_text + HALO + lo);
-#pragma unroll
-        for (int k = 0; k < BPT / 16; k++) {
-            uint4 v = mine[k];
-            uint64_t q = nl_nibble(v.x) | nl_nibble(v.y) << 4 | nl_nibble(v.z) << 8 | nl_nibble(v.w) << 12;
-            mask |= q << (16 * k);
-        }
         const uint32_t last = a.n - 1;   // always a terminator
-        if (last >= base + lo && last < base + lo + BPT)
-            mask |= 1ull << (last - base - lo);
-        // bytes at or past n are padding zeros and never '\n'
+        if (tid < HALO / 64) {
+            uint64_t nl, pm, bm;
+            masks64((const uint4 *)(s_text + 64 * tid), nl, pm, bm);
+            s_pm[tid] = pm;
+            s_bm[tid] = bm;
+        }
+        if (tid == 0) {
+            s_pm[MW - 1] = 0;
+            s_bm[MW - 1] = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NM; j++) {
+            uint64_t mk, pm, bm;
+            masks64(mine + 4 * j, mk, pm, bm);
+            s_pm[(HALO + lo) / 64 + j] = pm;
+            s_bm[(HALO + lo) / 64 + j] = bm;
+            const uint32_t b0 = base + lo + 64 * j;
+            if (last >= b0 && last < b0 + 64)
+                mk |= 1ull << (last - b0);
+            mask[j] = mk;
+            cnt += (uint32_t)__popcll(mk);
+            if (mk)
+                mylast = (int32_t)(lo + 64 * j + 63 - __clzll(mk));
+        }
     }
-    const uint32_t cnt = (uint32_t)__popcll(mask);
-    const int32_t mylast = mask ? (int32_t)(lo + 63 - __clzll(mask)) : -1;
 
     uint32_t ic = cnt;
     int32_t il = mylast;
@@ -253,13 +357,13 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         uint64_t prefix = 0;
         if (t == 0) {
             if (lane == 0)
-                st_release_u64(a.status, (LB_FLAG_INCL << LB_SHIFT) | total);
+                st_relaxed_u64(a.status, (LB_FLAG_INCL << LB_SHIFT) | total);
         } else {
             if (lane == 0)
-                st_release_u64(a.status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
+                st_relaxed_u64(a.status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
             prefix = lookback_sum(a.status, t);
             if (lane == 0)
-                st_release_u64(a.status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
+                st_relaxed_u64(a.status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
         }
         if (lane == 0)
             s_prefix = prefix;
@@ -288,36 +392,59 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     const uint64_t prefix = s_prefix;
     if (tid == 0 && t == a.num_tiles - 1)
         a.meta[0] = prefix + total;
-    if (!mask)
+    if (!cnt)
         return;
 
-    ByteSrc src{s_text, a.text, base - (t > 0 ? HALO : 0u), base + TILE};
-    if (t == 0)
-        src.lds = s_text + HALO;
+    // Lines that start inside the staged window (all but possibly the tile's
+    // first) read only LDS; the accessor has no global branch, so the reads
+    // compile to ds_read (a mixed accessor makes the compiler select between
+    // the two base pointers and emit slower generic-address loads).
+    const uint32_t lds_lo = (t > 0) ? base - HALO : 0u;
+    auto lds_byte = [&](uint32_t pos) -> uint32_t { return s_text[pos + (uint32_t)HALO - base]; };
+    auto any_byte = [&](uint32_t pos) -> uint32_t {
+        uint32_t rel = pos + (uint32_t)HALO - base;
+        return (rel < (uint32_t)(HALO + TILE)) ? (uint32_t)s_text[rel] : (uint32_t)a.text[pos];
+    };
     uint64_t r = prefix + excl_c;
     uint32_t s = (excl_l >= 0) ? base + (uint32_t)excl_l + 1 : s_first;
-    uint64_t m = mask;
-    while (m) {
-        int bit = __builtin_ctzll(m);
-        m &= m - 1;
-        uint32_t e = base + lo + (uint32_t)bit;
-        if (r < a.cap) {
-            ol_node_dev nd = parse_span(src, s, e);
-            a.offsets[r] = s;
-            a.classes[r] = (uint8_t)nd.kind;
-            a.keys[r] = key_of(nd);
-            if (a.nodes)
-                a.nodes[r] = nd;
+#pragma unroll
+    for (int j = 0; j < NM; j++) {
+        uint64_t m = mask[j];
+        while (m) {
+            int bit = __builtin_ctzll(m);
+            m &= m - 1;
+            uint32_t e = base + lo + 64 * j + (uint32_t)bit;
+            if (r < a.cap) {
+                ol_node_dev nd;
+                if (s < lds_lo || !line_head(s_pm, s_bm, lds_byte, s + (uint32_t)HALO - base, s, e, nd))
+                    nd = (s >= lds_lo) ? parse_span(lds_byte, s, e) : parse_span(any_byte, s, e);
+                a.offsets[r] = s;
+                a.classes[r] = (uint8_t)nd.kind;
+                a.keys[r] = key_of(nd);
+                if (a.nodes)
+                    a.nodes[r] = nd;
+            }
+            s = e + 1;
+            r++;
         }
-        s = e + 1;
-        r++;
     }
 }
 
-hipError_t ol_launch_scan(const ScanArgs &a, hipStream_t s)
+hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, hipStream_t s)
 {
-    hipLaunchKernelGGL((scan_classify_kernel<SCAN_THREADS, SCAN_BPT>), dim3(a.num_tiles),
-                       dim3(SCAN_THREADS), 0, s, a);
+    switch (bpt) {
+    case 64:
+        hipLaunchKernelGGL((scan_classify_kernel<SCAN_THREADS, 64>), dim3(a.num_tiles), dim3(SCAN_THREADS), 0, s, a);
+        break;
+    case 128:
+        hipLaunchKernelGGL((scan_classify_kernel<SCAN_THREADS, 128>), dim3(a.num_tiles), dim3(SCAN_THREADS), 0, s, a);
+        break;
+    case 256:
+        hipLaunchKernelGGL((scan_classify_kernel<SCAN_THREADS, 256>), dim3(a.num_tiles), dim3(SCAN_THREADS), 0, s, a);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -66,13 +66,13 @@ __device__ __forceinline__ uint64_t tile_prefix(uint64_t *status, uint32_t t, ui
         uint64_t prefix = 0;
         if (t == 0) {
             if (lane == 0)
-                st_release_u64(status, (LB_FLAG_INCL << LB_SHIFT) | total);
+                st_relaxed_u64(status, (LB_FLAG_INCL << LB_SHIFT) | total);
         } else {
             if (lane == 0)
-                st_release_u64(status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
+                st_relaxed_u64(status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
             prefix = lookback_sum(status, t);
             if (lane == 0)
-                st_release_u64(status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
+                st_relaxed_u64(status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
         }
         if (lane == 0)
             *s_prefix = prefix;

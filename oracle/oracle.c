@@ -232,10 +232,11 @@ size_t orc_find_all(const uint8_t *t, size_t n, const uint8_t *pat, size_t m,
  * (crates/oneline-gpu/src/tests.rs:805-826).  Patterns are given as one byte
  * pool plus per-pattern offset and length.
  *
- * Implementation: patterns are grouped by length; for each start and each
- * length present, the candidate slice is looked up in a sorted index of the
- * patterns of that length (binary search), so cost is O(n * lengths * log P).
- * Writes min(count, cap) hits; returns count. */
+ * Implementation: patterns sorted lexicographically (shorter prefix first,
+ * then id).  From each start the range of patterns that have t[start..start+k)
+ * as prefix is narrowed one byte at a time (binary search on byte k-1); the
+ * patterns of length exactly k at the front of the range are the matches of
+ * length k.  Writes min(count, cap) hits; returns count. */
 typedef struct {
     const uint8_t *pool;
     const uint64_t *poff;
@@ -248,9 +249,10 @@ static int cmp_pat(const void *a, const void *b)
 {
     uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
     uint32_t lx = g_pats.plen[x], ly = g_pats.plen[y];
-    if (lx != ly) return lx < ly ? -1 : 1;
-    int c = memcmp(g_pats.pool + g_pats.poff[x], g_pats.pool + g_pats.poff[y], lx);
+    uint32_t l = lx < ly ? lx : ly;
+    int c = memcmp(g_pats.pool + g_pats.poff[x], g_pats.pool + g_pats.poff[y], l);
     if (c) return c;
+    if (lx != ly) return lx < ly ? -1 : 1;
     return x < y ? -1 : (x > y);
 }
 
@@ -265,35 +267,37 @@ size_t orc_many_find_all(const uint8_t *t, size_t n, const uint8_t *pool,
         order[i] = (uint32_t)i;
     g_pats.pool = pool; g_pats.poff = poff; g_pats.plen = plen;
     qsort(order, np, sizeof(uint32_t), cmp_pat);
-    /* ranges of equal length in `order` */
-    size_t nl = 0;
-    size_t *lbeg = (size_t *)malloc((np + 1) * sizeof(size_t));
-    uint32_t *lval = (uint32_t *)malloc((np + 1) * sizeof(uint32_t));
-    for (size_t i = 0; i < np; i++) {
-        if (i == 0 || plen[order[i]] != plen[order[i - 1]]) {
-            lbeg[nl] = i;
-            lval[nl] = plen[order[i]];
-            nl++;
-        }
-    }
-    lbeg[nl] = np;
-    size_t k = 0;
-    uint32_t *ids = (uint32_t *)malloc(np * sizeof(uint32_t));
+    size_t k = 0, idcap = 64;
+    uint32_t *ids = (uint32_t *)malloc(idcap * sizeof(uint32_t));
     for (size_t p = 0; p < n; p++) {
         size_t nid = 0;
-        for (size_t g = 0; g < nl; g++) {
-            uint32_t L = lval[g];
-            if (L == 0 || p + L > n)
-                continue;
-            /* lower bound of slice among patterns of this length */
-            size_t lo = lbeg[g], hi = lbeg[g + 1];
-            while (lo < hi) {
-                size_t mid = (lo + hi) / 2;
-                int c = memcmp(pool + poff[order[mid]], t + p, L);
-                if (c < 0) lo = mid + 1; else hi = mid;
+        size_t lo = 0, hi = np;
+        for (size_t d = 0; lo < hi && p + d < n; d++) {
+            /* [lo, hi): patterns whose first d bytes equal t[p..p+d).  Those of
+             * length d (a match, recorded at the previous step) sort first. */
+            while (lo < hi && plen[order[lo]] == d)
+                lo++;
+            uint8_t c = t[p + d];
+            size_t a = lo, b = hi;
+            while (a < b) {   /* first with byte d >= c */
+                size_t mid = (a + b) / 2;
+                if (pool[poff[order[mid]] + d] < c) a = mid + 1; else b = mid;
             }
-            while (lo < lbeg[g + 1] && memcmp(pool + poff[order[lo]], t + p, L) == 0)
-                ids[nid++] = order[lo++];
+            size_t lo2 = a;
+            b = hi;
+            while (a < b) {   /* first with byte d > c */
+                size_t mid = (a + b) / 2;
+                if (pool[poff[order[mid]] + d] <= c) a = mid + 1; else b = mid;
+            }
+            lo = lo2;
+            hi = a;
+            for (size_t q = lo; q < hi && plen[order[q]] == d + 1; q++) {
+                if (nid == idcap) {
+                    idcap *= 2;
+                    ids = (uint32_t *)realloc(ids, idcap * sizeof(uint32_t));
+                }
+                ids[nid++] = order[q];
+            }
         }
         /* ids of different lengths interleave: sort this position's ids */
         for (size_t a = 1; a < nid; a++) {
@@ -307,6 +311,6 @@ size_t orc_many_find_all(const uint8_t *t, size_t n, const uint8_t *pool,
             k++;
         }
     }
-    free(ids); free(order); free(lbeg); free(lval);
+    free(ids); free(order);
     return k;
 }

@@ -116,3 +116,33 @@ def test_oversized_input_is_an_error(parser):
     assert "bytes" in lib.ol_last_error().decode()
     # still usable
     assert parser.scan_line_offsets(b"a\nb").tolist() == [0, 2]
+
+
+def test_random_lines_over_the_grammar_alphabet(parser):
+    """Documents of random short lines over the reference's random-line alphabet
+    (crates/oneline-simd/src/line.rs:148), so prefixes, blank runs, markers and
+    CR runs land at every offset relative to lanes, masks and tiles."""
+    rng = np.random.default_rng(4242)
+    alphabet = np.frombuffer(b"|| \t=:@->*`0123456789.\rxy", np.uint8)
+    for n_lines in (5_000, 60_000):
+        lens = rng.integers(0, 14, n_lines)
+        parts = []
+        for ln in lens:
+            parts.append(alphabet[rng.integers(0, alphabet.size, ln)].tobytes() + b"\n")
+        assert_parse_matches(parser, b"".join(parts), f"{n_lines} random lines")
+
+
+def test_prefixes_around_the_mask_window(parser):
+    """Prefixes whose '|' / blank run ends just before, at and after 64 bytes."""
+    lines = []
+    for plen in range(50, 80):
+        for unit in (b"|", b"| ", b"|\t", b"|  "):
+            pre = (unit * plen)[:plen]
+            for tail in (b"x", b"- y", b"", b"  ", b"-. z", b"//c"):
+                lines.append(pre + tail + b"\n")
+                lines.append(b" " + pre + tail + b"\r\n")
+                lines.append(b"\t" * (plen % 7) + tail + b"\n")
+    text = b"".join(lines)
+    assert_parse_matches(parser, text, "prefix lengths 50..79")
+    for shift in (1, 7, 33, 63):
+        assert_parse_matches(parser, b"q" * shift + b"\n" + text, f"shifted {shift}")
