@@ -71,8 +71,11 @@ struct ol_ctx {
     std::mutex mu;
     DevBuf scratch;
     DevBuf tree_agg, tree_incl;
+    DevBuf fix;                            // list of lines decoded by the scan fixup pass
     // launch geometry picked by the autotuner (ol_tune_set)
     uint32_t scan_bpt = SCAN_BPT;          // bytes per lane of scan_classify (tile = 256 lanes x bpt)
+    uint32_t scan_grid = 0;                // persistent workgroups of scan_classify (0 = all co-resident)
+    uint32_t scan_max_grid[3] = {0, 0, 0}; // co-resident limit per bpt (64, 128, 256), from the occupancy API
     uint32_t tree_threads = TREE_THREADS;  // lanes per tree tile (tile = lanes x TREE_RUN lines)
     // profiling
     bool profiling = false;
@@ -209,10 +212,28 @@ int launch_scan(ol_ctx *c, ol_doc *d)
     a.keys = d->keys;
     a.nodes = d->nodes;
     a.status = c->status();
-    a.tile_ctr = c->ctr();
     a.meta = c->meta();
-    Prof p(c, "scan_classify");
-    HIPCHK(ol_launch_scan(a, c->scan_bpt, c->stream));
+    // a line needing the fixup pass is longer than the scan halo: at most n / 256 of them
+    a.fix_cap = (uint32_t)(d->n / SCAN_HALO + 64);
+    HIPCHK(c->fix.ensure((size_t)a.fix_cap * 4));
+    a.fix_list = (uint32_t *)c->fix.p;
+    a.fix_count = c->ctr() + 2;
+    const int vi = c->scan_bpt == 64 ? 0 : (c->scan_bpt == 128 ? 1 : 2);
+    if (!c->scan_max_grid[vi])
+        c->scan_max_grid[vi] = ol_scan_max_grid(c->scan_bpt);
+    if (!c->scan_max_grid[vi])
+        return fail(OL_ERR_DEVICE, "occupancy query for the scan kernel failed");
+    uint32_t grid = c->scan_max_grid[vi];
+    if (c->scan_grid && c->scan_grid < grid)
+        grid = c->scan_grid;
+    if (grid > tiles)
+        grid = tiles;
+    {
+        Prof p(c, "scan_classify");
+        HIPCHK(ol_launch_scan(a, c->scan_bpt, grid, c->stream));
+    }
+    Prof p(c, "scan_fixup");
+    HIPCHK(ol_launch_scan_fixup(a, c->stream));
     return OL_OK;
 }
 
@@ -368,7 +389,6 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
     a.out64 = out64;
     a.out32 = out32;
     a.status = c->status();
-    a.tile_ctr = c->ctr();
     a.meta = c->meta();
     {
         Prof p(c, "search");
@@ -445,7 +465,6 @@ int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, ui
     a.out_off = out_off;
     a.out_id = out_id;
     a.status = c->status();
-    a.tile_ctr = c->ctr();
     a.meta = c->meta();
     {
         Prof p(c, "many");
@@ -512,6 +531,7 @@ void ol_ctx_destroy(ol_ctx *c)
     c->scratch.release();
     c->tree_agg.release();
     c->tree_incl.release();
+    c->fix.release();
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1087,6 +1107,10 @@ int ol_tune_set(ol_ctx *c, const char *name, int value)
         if (value != 64 && value != 128 && value != 256)
             return fail(OL_ERR_INVALID, "scan_bpt must be 64, 128 or 256");
         c->scan_bpt = (uint32_t)value;
+    } else if (n == "scan_grid") {
+        if (value < 0)
+            return fail(OL_ERR_INVALID, "scan_grid must be >= 0 (0 = all co-resident workgroups)");
+        c->scan_grid = (uint32_t)value;
     } else if (n == "tree_threads") {
         if (value != 64 && value != 128)
             return fail(OL_ERR_INVALID, "tree_threads must be 64 or 128");
@@ -1105,6 +1129,8 @@ int ol_tune_get(ol_ctx *c, const char *name, int *value)
     std::string n(name);
     if (n == "scan_bpt")
         *value = (int)c->scan_bpt;
+    else if (n == "scan_grid")
+        *value = (int)c->scan_grid;
     else if (n == "tree_threads")
         *value = (int)c->tree_threads;
     else

@@ -60,49 +60,39 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 
 // Exclusive look-back over per-tile status words for the sum of the values of
 // all tiles before `t` (one full wave calls this; every lane gets the result).
-// Tile t must already have published its own aggregate.  Each round trip reads
-// a window of LB_WINDOWS x 64 predecessors, so the look-back keeps up with
-// tiles arriving faster than one memory latency per tile.
-#define LB_WINDOWS 4
+// Tile t must already have published its own aggregate.
+//
+// The status words are read with uncached (agent-scope) loads, and on this
+// part their throughput, not their latency, is what limits a look-back: a wave
+// reads one 64-tile window, keeps every ready entry up to the first
+// not-ready one, and then polls only that tile, with one lane and a sleep,
+// instead of re-reading the window.
 __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_t t)
 {
     const int lane = threadIdx.x & 63;
     uint64_t prefix = 0;
     int64_t top = (int64_t)t - 1;
     while (true) {
-        uint64_t st[LB_WINDOWS];
-#pragma unroll
-        for (int k = 0; k < LB_WINDOWS; k++) {
-            int64_t idx = top - 64 * k - lane;
-            st[k] = (idx >= 0) ? ld_relaxed_u64(status + idx) : (LB_FLAG_INCL << LB_SHIFT);
-        }
-        bool retry = false, done = false;
-        uint64_t acc = 0;
-#pragma unroll
-        for (int k = 0; k < LB_WINDOWS; k++) {
-            uint64_t flag = st[k] >> LB_SHIFT;
-            uint64_t incl = __ballot(flag == LB_FLAG_INCL);
-            uint64_t notready = __ballot(flag == 0);
-            int first = incl ? (int)__builtin_ctzll(incl) : 64;
-            uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
-            if (notready & window) {
-                retry = true;
-                break;
+        const int64_t idx = top - lane;
+        const uint64_t st = (idx >= 0) ? ld_relaxed_u64(status + idx) : (LB_FLAG_INCL << LB_SHIFT);
+        const uint64_t flag = st >> LB_SHIFT;
+        const uint64_t incl = __ballot(flag == LB_FLAG_INCL);
+        const uint64_t notready = __ballot(flag == 0);
+        const int fi = incl ? (int)__builtin_ctzll(incl) : 64;
+        const int fn = notready ? (int)__builtin_ctzll(notready) : 64;
+        if (fn < fi) {
+            prefix += wave_sum_u64((lane < fn) ? (st & LB_MASK) : 0);
+            top -= fn;
+            if (lane == 0) {
+                while ((ld_relaxed_u64(status + top) >> LB_SHIFT) == 0)
+                    __builtin_amdgcn_s_sleep(8);
             }
-            acc += wave_sum_u64((lane <= first) ? (st[k] & LB_MASK) : 0);
-            if (first < 64) {
-                done = true;
-                break;
-            }
-        }
-        if (retry) {
-            __builtin_amdgcn_s_sleep(2);
             continue;
         }
-        prefix += acc;
-        if (done)
+        prefix += wave_sum_u64((lane <= fi) ? (st & LB_MASK) : 0);
+        if (fi < 64)
             break;
-        top -= 64 * LB_WINDOWS;
+        top -= 64;
     }
     return prefix;
 }
@@ -118,8 +108,10 @@ struct ScanArgs {
     uint8_t *keys;           // tree key byte                      [cap]
     ol_node_dev *nodes;      // FlatNode (nullable)                [cap]
     uint64_t *status;        // look-back words                    [num_tiles]
-    uint32_t *tile_ctr;      // dynamic tile counter
     uint64_t *meta;          // meta[0] <- line count
+    uint32_t *fix_list;      // lines whose head was not in LDS (decoded by the fixup pass)
+    uint32_t *fix_count;     // entries appended to fix_list
+    uint32_t fix_cap;        // capacity of fix_list (>= lines longer than the halo)
 };
 
 struct TreeArgs {
@@ -143,7 +135,6 @@ struct SearchArgs {
     uint64_t *out64;         // offsets as u64 (nullable)
     uint32_t *out32;         // offsets as u32 (nullable)
     uint64_t *status;
-    uint32_t *tile_ctr;
     uint64_t *meta;          // meta[0] <- total matches
 };
 
@@ -176,20 +167,24 @@ struct ManyArgs {
     uint64_t *out_off;       // u64 offsets (nullable)
     uint32_t *out_id;        // pattern ids (nullable)
     uint64_t *status;
-    uint32_t *tile_ctr;
     uint64_t *meta;          // meta[0] <- total matches
 };
 
 // ---- launchers (each returns the launch error, if any) ---------------------
 hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s);
-hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, hipStream_t s);   // bpt: 64, 128 or 256
+// Persistent scan: `grid` co-resident workgroups (cooperative launch), at most
+// ol_scan_max_grid(bpt); bpt (bytes per lane per tile) is 64, 128 or 256, i.e.
+// tiles of 1, 2 or 4 LDS-staged quarters of 256 x 64 bytes.
+hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, hipStream_t s);
+uint32_t ol_scan_max_grid(uint32_t bpt);
+hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s);   // reads the line count from meta[0]
 hipError_t ol_launch_tree(const TreeArgs &a, uint32_t threads, hipStream_t s);   // summary + scan + walk; threads 128 or 256
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
 
 // Tile geometry of each kernel.
 constexpr uint32_t SCAN_THREADS = 256;
-constexpr uint32_t SCAN_BPT = 64;                           // default bytes per lane
+constexpr uint32_t SCAN_BPT = 256;                          // default bytes per lane (64 KiB tiles)
 constexpr uint32_t SCAN_TILE_MAX = SCAN_THREADS * 256;
 constexpr uint32_t SCAN_HALO = 256;
 constexpr uint32_t TREE_THREADS = 128;                      // default lanes per tile

@@ -49,35 +49,11 @@ __device__ __forceinline__ uint32_t eq_nibble(uint32_t w, uint32_t pat)
     uint32_t z = ~nz & 0x80808080u;
     return (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
 }
-__device__ __forceinline__ uint32_t nl_nibble(uint32_t w) { return eq_nibble(w, 0x0A0A0A0Au); }
-
-// '\n', '|' and blank (space / tab) masks of 64 LDS bytes.
-__device__ __forceinline__ void masks64(const uint4 *src, uint64_t &nl, uint64_t &pipe, uint64_t &blank)
+__device__ __forceinline__ uint32_t nl_nibble(uint32_t w)
 {
-    nl = pipe = blank = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        uint4 v = src[k];
-        uint32_t w[4] = {v.x, v.y, v.z, v.w};
-        uint64_t qn = 0, qp = 0, qb = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            qn |= (uint64_t)eq_nibble(w[i], 0x0A0A0A0Au) << (4 * i);
-            qp |= (uint64_t)eq_nibble(w[i], 0x7C7C7C7Cu) << (4 * i);
-            qb |= (uint64_t)(eq_nibble(w[i], 0x20202020u) | eq_nibble(w[i], 0x09090909u)) << (4 * i);
-        }
-        nl |= qn << (16 * k);
-        pipe |= qp << (16 * k);
-        blank |= qb << (16 * k);
-    }
-}
-
-// 64 mask bits starting at bit `b` of a mask array.
-__device__ __forceinline__ uint64_t window64(const uint64_t *m, uint32_t b)
-{
-    uint32_t w = b >> 6, sh = b & 63;
-    uint64_t lo = m[w] >> sh;
-    return sh ? (lo | (m[w + 1] << (64 - sh))) : lo;
+    const uint32_t x = w ^ 0x0A0A0A0Au;
+    const uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+    return ((z >> 7) | (z >> 14) | (z >> 21) | (z >> 28)) & 0xFu;
 }
 
 __device__ __forceinline__ bool is_blank(uint32_t b) { return b == ' ' || b == '\t'; }
@@ -181,280 +157,446 @@ __device__ uint32_t find_line_start_global(const uint8_t *text, uint32_t end)
     return 0;
 }
 
-// parse_line (crates/oneline-simd/src/line.rs:8-44) for text[s ..= e] from the
-// '|' / blank bit masks instead of a byte loop.  With P = '|' bits and
-// B = blank bits from the line start: a line starting with '|' has its prefix
-// = the leading run of P|B (depth = '|' count in it, content after it); one
-// starting with a blank skips the leading blank run at depth 0.  Returns false
-// (caller falls back to the byte loop) when the prefix runs past the 64-bit
-// window.  `rel` is the bit index of s in the mask arrays.
-template <class Src>
-__device__ __forceinline__ bool line_head(const uint64_t *pm, const uint64_t *bm, const Src &b, uint32_t rel,
-                                          uint32_t s, uint32_t e_incl, ol_node_dev &nd)
+struct GlobalSrc {
+    const uint8_t *g;
+    __device__ __forceinline__ uint32_t operator()(uint32_t pos) const { return g[pos]; }
+};
+
+// High bit of each byte of x that is zero (exact SWAR test).
+__device__ __forceinline__ uint32_t zbytes(uint32_t x)
 {
-    uint32_t eff = e_incl + 1;
-    while (eff > s) {
-        uint32_t c = b(eff - 1);
-        if (c != '\n' && c != '\r')
-            break;
-        eff--;
+    return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+// Bit 7 of each byte -> bits 0..3 (shifts only: a 32-bit multiply is quarter rate).
+__device__ __forceinline__ uint32_t squeeze(uint32_t z)
+{
+    return ((z >> 7) | (z >> 14) | (z >> 21) | (z >> 28)) & 0xFu;
+}
+
+// Prefix run of `nb` bytes starting at line position `off` (words W[0..], the
+// line's bytes from LDS, aligned so byte 0 is the line start): returns the
+// length of the leading '|'/blank run when `from_pipe` (the line starts with
+// '|'), else of the leading blank run; `pipes` gets the '|' count inside it.
+// Returns nb when the run covers the whole window.
+template <int NWORDS>
+__device__ __forceinline__ uint32_t prefix_run(const uint32_t (&A)[NWORDS], uint32_t avail, bool from_pipe,
+                                               uint32_t &pipes)
+{
+    uint64_t pipe = 0, stop = 0;
+#pragma unroll
+    for (int j = 0; j < NWORDS; j++) {
+        const uint32_t w = A[j];
+        const uint32_t zp = zbytes(w ^ 0x7C7C7C7Cu);
+        const uint32_t zb = zbytes(w ^ 0x20202020u) | zbytes(w ^ 0x09090909u);
+        const uint32_t keep = from_pipe ? (zp | zb) : zb;
+        pipe |= (uint64_t)squeeze(zp) << (4 * j);
+        stop |= (uint64_t)squeeze(~keep & 0x80808080u) << (4 * j);
     }
-    if (eff == s) {
-        nd.depth = 0; nd.kind = OLK_BLANK; nd.span_start = s; nd.span_len = 0;
-        return true;
-    }
+    constexpr uint32_t NB = 4 * NWORDS;
+    if (avail < NB)
+        stop |= ~((1ull << avail) - 1ull);
+    const uint32_t r = stop ? (uint32_t)__builtin_ctzll(stop) : NB;
+    pipes = (uint32_t)__popcll(pipe & ((r >= 64) ? ~0ull : ((1ull << r) - 1ull)));
+    return r > NB ? NB : r;
+}
+
+// Loads NWORDS line words starting `off` bytes after the line start (LDS byte
+// index rs of the line start).
+template <int NWORDS>
+__device__ __forceinline__ void load_words(const uint8_t *s_text, uint32_t pos, uint32_t (&A)[NWORDS])
+{
+    const uint32_t *wp = (const uint32_t *)(s_text + (pos & ~3u));
+    const uint32_t sh = pos & 3u;
+    uint32_t W[NWORDS + 1];
+#pragma unroll
+    for (int j = 0; j <= NWORDS; j++)
+        W[j] = wp[j];
+#pragma unroll
+    for (int j = 0; j < NWORDS; j++)
+        A[j] = __builtin_amdgcn_alignbyte(W[j + 1], W[j], sh);
+}
+
+// parse_line (crates/oneline-simd/src/line.rs:8-44) for the non-empty line
+// text[s .. eff) held in LDS at byte index rs (48 bytes readable from there).
+// A line starting with '|' has the leading run of '|'/blank bytes as its prefix
+// (depth = the '|' count in it, content after it); a line starting with a blank
+// skips the leading blank run at depth 0.  Fast tier: count leading "| | "
+// words with one compare each, then decode an 8-byte tail with SWAR; general
+// tier: SWAR over 48 bytes.  Returns false (the caller falls back to the byte
+// loop) when the prefix is longer than that.
+__device__ __forceinline__ bool decode_head(const uint8_t *s_text, uint32_t rs, uint32_t s, uint32_t eff,
+                                            ol_node_dev &nd)
+{
     const uint32_t len = eff - s;
-    uint64_t P = window64(pm, rel), B = window64(bm, rel);
-    if (len < 64) {
-        const uint64_t lm = (1ull << len) - 1;
-        P &= lm;
-        B &= lm;
-    }
     uint32_t depth = 0, r = 0;
-    if (P & 1) {
-        const uint64_t nq = ~(P | B);
-        if (!nq)
-            return false;                       // prefix longer than the window
-        r = (uint32_t)__builtin_ctzll(nq);
-        depth = (uint32_t)__popcll(P & ((1ull << r) - 1));
-    } else if (B & 1) {
-        const uint64_t nb = ~B;
-        if (!nb)
-            return false;
-        r = (uint32_t)__builtin_ctzll(nb);
+    const uint32_t c0 = s_text[rs];
+    if (c0 == '|' || c0 == ' ' || c0 == '\t') {
+        const bool from_pipe = (c0 == '|');
+        bool done = false;
+        if (from_pipe) {
+            uint32_t A[10];
+            load_words<10>(s_text, rs, A);
+            uint32_t eq = 0;
+#pragma unroll
+            for (int j = 0; j < 10; j++)
+                eq |= (uint32_t)(A[j] == 0x207C207Cu) << j;   // "| | "
+            uint32_t k = (uint32_t)__builtin_ctz(~eq);
+            if (4 * k > len)
+                k = len / 4;
+            if (k <= 8) {
+                uint32_t T[2];
+                load_words<2>(s_text, rs + 4 * k, T);
+                uint32_t pipes;
+                const uint32_t rt = prefix_run<2>(T, len - 4 * k, true, pipes);
+                if (rt < 8 || 4 * k + 8 >= len) {
+                    r = 4 * k + rt;
+                    depth = 2 * k + pipes;
+                    done = true;
+                }
+            }
+        }
+        if (!done) {
+            uint32_t A[12];
+            load_words<12>(s_text, rs, A);
+            uint32_t pipes;
+            r = prefix_run<12>(A, len, from_pipe, pipes);
+            if (r == 48 && len > 48)
+                return false;
+            depth = from_pipe ? pipes : 0;
+        }
     }
     const uint32_t cs = s + r;
     nd.depth = depth;
-    nd.span_start = cs;
     if (cs >= eff) {
         nd.span_start = eff;
         nd.kind = OLK_BLANK;
         nd.span_len = 0;
         return true;
     }
-    const uint32_t c0 = b(cs);
-    const uint32_t c1 = (cs + 1 < eff) ? b(cs + 1) : 256u;
-    const uint32_t c2 = (cs + 2 < eff) ? b(cs + 2) : 256u;
-    nd.kind = classify(c0, c1, c2);
+    nd.span_start = cs;
+    const uint32_t d0 = s_text[rs + r];
+    const uint32_t d1 = (cs + 1 < eff) ? (uint32_t)s_text[rs + r + 1] : 256u;
+    const uint32_t d2 = (cs + 2 < eff) ? (uint32_t)s_text[rs + r + 2] : 256u;
+    nd.kind = classify(d0, d1, d2);
     nd.span_len = eff - cs;
     return true;
 }
 
-template <int THREADS, int BPT>
+// One tile = Q quarters of THREADS x 64 bytes.  The tile is loaded into
+// registers (16-byte vectors, coalesced); its '\n' masks go to LDS once and
+// serve both the tile count (one look-back per tile) and the lines' positions.
+// The quarters are then staged through LDS in turn, each with the 256 bytes
+// before it as a halo, and every lane decodes the lines that end in its 64
+// bytes.  A line that starts before the staged window (longer than the halo)
+// is written later by scan_fixup_kernel from global memory, so the decode loop
+// touches LDS only.
+template <int THREADS, int Q>
 __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
 {
-    constexpr int TILE = THREADS * BPT;
+    constexpr int QB = THREADS * 64;   // bytes per quarter
+    constexpr int TILE = QB * Q;
     constexpr int HALO = SCAN_HALO;
     constexpr int NW = THREADS / 64;
-    constexpr int NM = BPT / 64;   // 64-bit terminator masks per lane
-    static_assert(BPT % 64 == 0 && NM >= 1, "BPT");
+    constexpr int NV = Q * 4;          // 16-byte vectors per lane per tile
     static_assert(NW >= 2, "needs two waves");
+    static_assert(HALO % 64 == 0 && HALO <= QB, "halo");
 
-    constexpr int MW = (HALO + TILE) / 64 + 1;   // mask words (+1 so a window may read past the end)
-    __shared__ __align__(16) uint8_t s_text[HALO + TILE];
-    __shared__ uint64_t s_pm[MW];   // '|' bits of the staged window
-    __shared__ uint64_t s_bm[MW];   // blank bits of the staged window
-    __shared__ uint32_t s_wcnt[NW];
-    __shared__ int32_t s_wlast[NW];
-    __shared__ uint32_t s_tile;
+    __shared__ __align__(16) uint8_t s_text[HALO + QB + 64];
+    __shared__ __align__(16) uint8_t s_halo[HALO];      // the 256 bytes before the tile
+    __shared__ __align__(8) uint16_t s_nm[TILE / 16];   // '\n' bits per 16-byte vector, in byte order
+    __shared__ uint32_t s_wcnt[NW][Q];
+    __shared__ int32_t s_wlast[NW][Q];
+    constexpr int LIST = 1024;          // lines per quarter decoded from the compacted list
+    __shared__ uint32_t s_ls[LIST], s_le[LIST];   // line start / terminator of the quarter's lines
     __shared__ uint64_t s_prefix;
     __shared__ uint32_t s_first;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0)
-        s_tile = atomicAdd(a.tile_ctr, 1u);
-    __syncthreads();
-    const uint32_t t = s_tile;
-    const uint32_t base = t * TILE;
+    const uint32_t G = gridDim.x;
+    const uint32_t last = a.n - 1;   // always a terminator, '\n' or not
 
-    {
-        const uint4 *src = (const uint4 *)(a.text + base);
-        uint4 *dst = (uint4 *)(s_text + HALO);
-        uint4 v[BPT / 16];
+    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += G) {
+        const uint32_t base = t * TILE;
+        uint4 v[NV];
+        uint4 hv = make_uint4(0, 0, 0, 0);
+        {
+            const uint4 *src = (const uint4 *)(a.text + (size_t)base);
 #pragma unroll
-        for (int k = 0; k < BPT / 16; k++)
-            v[k] = src[k * THREADS + tid];
-#pragma unroll
-        for (int k = 0; k < BPT / 16; k++)
-            dst[k * THREADS + tid] = v[k];
-        if (tid < HALO / 16) {
-            uint4 h = make_uint4(0, 0, 0, 0);
-            if (t > 0)
-                h = ((const uint4 *)(a.text + base - HALO))[tid];
-            ((uint4 *)s_text)[tid] = h;
+            for (int k = 0; k < NV; k++)
+                v[k] = src[k * THREADS + tid];
+            if (t > 0 && tid < HALO / 16)
+                hv = ((const uint4 *)(a.text + base - HALO))[tid];
         }
-    }
-    __syncthreads();
+        if (tid < HALO / 16)
+            ((uint4 *)s_halo)[tid] = hv;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const uint32_t m = nl_nibble(v[k].x) | nl_nibble(v[k].y) << 4 | nl_nibble(v[k].z) << 8 |
+                               nl_nibble(v[k].w) << 12;
+            s_nm[k * THREADS + tid] = (uint16_t)m;
+        }
+        if (last >= base && last < base + TILE) {   // the tile holding the text's last byte
+            __syncthreads();
+            if (tid == 0)
+                s_nm[(last - base) >> 4] |= (uint16_t)(1u << ((last - base) & 15));
+        }
+        __syncthreads();
 
-    // Lane-contiguous BPT bytes -> NM masks of terminators, plus the '|' and
-    // blank masks the line heads are decoded from.
-    const uint32_t lo = tid * BPT;
-    uint64_t mask[NM];
-    uint32_t cnt = 0;
-    int32_t mylast = -1;
-    {
-        const uint4 *mine = (const uint4 *)(s_text + HALO + lo);
-        const uint32_t last = a.n - 1;   // always a terminator
-        if (tid < HALO / 64) {
-            uint64_t nl, pm, bm;
-            masks64((const uint4 *)(s_text + 64 * tid), nl, pm, bm);
-            s_pm[tid] = pm;
-            s_bm[tid] = bm;
-        }
-        if (tid == 0) {
-            s_pm[MW - 1] = 0;
-            s_bm[MW - 1] = 0;
+        // Lane-contiguous 64-byte masks of each quarter, counted and scanned
+        // together (one block scan per tile).
+        uint64_t mk[Q];
+        uint32_t cnt[Q], ic[Q];
+        int32_t il[Q], prev_l[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            mk[q] = *(const uint64_t *)(s_nm + q * (QB / 16) + 4 * tid);
+            cnt[q] = (uint32_t)__popcll(mk[q]);
+            ic[q] = cnt[q];
+            il[q] = mk[q] ? (int32_t)(tid * 64 + 63 - __clzll(mk[q])) : -1;
         }
 #pragma unroll
-        for (int j = 0; j < NM; j++) {
-            uint64_t mk, pm, bm;
-            masks64(mine + 4 * j, mk, pm, bm);
-            s_pm[(HALO + lo) / 64 + j] = pm;
-            s_bm[(HALO + lo) / 64 + j] = bm;
-            const uint32_t b0 = base + lo + 64 * j;
-            if (last >= b0 && last < b0 + 64)
-                mk |= 1ull << (last - b0);
-            mask[j] = mk;
-            cnt += (uint32_t)__popcll(mk);
-            if (mk)
-                mylast = (int32_t)(lo + 64 * j + 63 - __clzll(mk));
-        }
-    }
-
-    uint32_t ic = cnt;
-    int32_t il = mylast;
+        for (int o = 1; o < 64; o <<= 1) {
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t c = __shfl_up(ic, o, 64);
-        int32_t l = __shfl_up(il, o, 64);
-        if (lane >= o) {
-            ic += c;
-            il = il > l ? il : l;
-        }
-    }
-    int32_t prev_l = __shfl_up(il, 1, 64);
-    if (lane == 0)
-        prev_l = -1;
-    if (lane == 63) {
-        s_wcnt[wid] = ic;
-        s_wlast[wid] = il;
-    }
-    __syncthreads();
-    uint32_t wbc = 0, total = 0;
-    int32_t wbl = -1;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-        if (w < wid) {
-            wbc += s_wcnt[w];
-            wbl = wbl > s_wlast[w] ? wbl : s_wlast[w];
-        }
-        total += s_wcnt[w];
-    }
-    const uint32_t excl_c = wbc + ic - cnt;
-    const int32_t excl_l = wbl > prev_l ? wbl : prev_l;
-
-    if (wid == 0) {
-        uint64_t prefix = 0;
-        if (t == 0) {
-            if (lane == 0)
-                st_relaxed_u64(a.status, (LB_FLAG_INCL << LB_SHIFT) | total);
-        } else {
-            if (lane == 0)
-                st_relaxed_u64(a.status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
-            prefix = lookback_sum(a.status, t);
-            if (lane == 0)
-                st_relaxed_u64(a.status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
-        }
-        if (lane == 0)
-            s_prefix = prefix;
-    } else if (wid == 1) {
-        // Start of the first line that ends in this tile: after the last '\n'
-        // before the tile (in the halo, else further back in global memory).
-        uint32_t start = 0;
-        if (t > 0 && total > 0) {
-            uint32_t w4 = ((const uint32_t *)s_text)[lane];   // halo bytes 4*lane .. +3
-            uint32_t m = nl_nibble(w4);
-            int32_t best = m ? (int32_t)(4 * lane + 31 - __clz(m)) : -1;
-            uint64_t found = __ballot(best >= 0);
-            if (found) {
-                int src = 63 - (int)__clzll(found);
-                int32_t p = __shfl(best, src, 64);
-                start = base - HALO + (uint32_t)p + 1;
-            } else {
-                start = find_line_start_global(a.text, base - HALO);
+            for (int q = 0; q < Q; q++) {
+                uint32_t cc = __shfl_up(ic[q], o, 64);
+                int32_t l = __shfl_up(il[q], o, 64);
+                if (lane >= o) {
+                    ic[q] += cc;
+                    il[q] = il[q] > l ? il[q] : l;
+                }
             }
         }
-        if (lane == 0)
-            s_first = start;
-    }
-    __syncthreads();
-
-    const uint64_t prefix = s_prefix;
-    if (tid == 0 && t == a.num_tiles - 1)
-        a.meta[0] = prefix + total;
-    if (!cnt)
-        return;
-
-    // Lines that start inside the staged window (all but possibly the tile's
-    // first) read only LDS; the accessor has no global branch, so the reads
-    // compile to ds_read (a mixed accessor makes the compiler select between
-    // the two base pointers and emit slower generic-address loads).
-    const uint32_t lds_lo = (t > 0) ? base - HALO : 0u;
-    auto lds_byte = [&](uint32_t pos) -> uint32_t { return s_text[pos + (uint32_t)HALO - base]; };
-    auto any_byte = [&](uint32_t pos) -> uint32_t {
-        uint32_t rel = pos + (uint32_t)HALO - base;
-        return (rel < (uint32_t)(HALO + TILE)) ? (uint32_t)s_text[rel] : (uint32_t)a.text[pos];
-    };
-    uint64_t r = prefix + excl_c;
-    uint32_t s = (excl_l >= 0) ? base + (uint32_t)excl_l + 1 : s_first;
 #pragma unroll
-    for (int j = 0; j < NM; j++) {
-        uint64_t m = mask[j];
-        while (m) {
-            int bit = __builtin_ctzll(m);
-            m &= m - 1;
-            uint32_t e = base + lo + 64 * j + (uint32_t)bit;
-            if (r < a.cap) {
-                ol_node_dev nd;
-                if (s < lds_lo || !line_head(s_pm, s_bm, lds_byte, s + (uint32_t)HALO - base, s, e, nd))
-                    nd = (s >= lds_lo) ? parse_span(lds_byte, s, e) : parse_span(any_byte, s, e);
+        for (int q = 0; q < Q; q++) {
+            prev_l[q] = __shfl_up(il[q], 1, 64);
+            if (lane == 0)
+                prev_l[q] = -1;
+            if (lane == 63) {
+                s_wcnt[wid][q] = ic[q];
+                s_wlast[wid][q] = il[q];
+            }
+        }
+        __syncthreads();
+        uint32_t total = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+                total += s_wcnt[w][q];
+
+        if (wid == 0) {
+            uint64_t prefix = 0;
+            if (t == 0) {
+                if (lane == 0)
+                    st_relaxed_u64(a.status, (LB_FLAG_INCL << LB_SHIFT) | total);
+            } else {
+                if (lane == 0)
+                    st_relaxed_u64(a.status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
+                prefix = lookback_sum(a.status, t);
+                if (lane == 0)
+                    st_relaxed_u64(a.status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
+            }
+            if (lane == 0)
+                s_prefix = prefix;
+        } else if (wid == 1) {
+            // Start of the first line that ends in this tile: after the last '\n'
+            // before the tile (in the 256 bytes before it, else further back).
+            uint32_t start = 0;
+            if (t > 0 && total > 0) {
+                uint32_t w4 = ((const uint32_t *)s_halo)[lane];
+                uint32_t m = nl_nibble(w4);
+                int32_t best = m ? (int32_t)(4 * lane + 31 - __clz(m)) : -1;
+                uint64_t found = __ballot(best >= 0);
+                if (found) {
+                    int srcl = 63 - (int)__clzll(found);
+                    int32_t p = __shfl(best, srcl, 64);
+                    start = base - HALO + (uint32_t)p + 1;
+                } else {
+                    start = find_line_start_global(a.text, base - HALO);
+                }
+            }
+            if (lane == 0)
+                s_first = start;
+        }
+        __syncthreads();
+        uint64_t running = s_prefix;       // index of the first line of the quarter
+        uint32_t carry = s_first;          // start of the line open at the quarter start
+        if (tid == 0 && t == a.num_tiles - 1)
+            a.meta[0] = running + total;
+
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const uint32_t qbase = base + q * QB;
+            uint32_t wb = 0, qtot = 0;
+            int32_t wl = -1, ql = -1;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                if (w < wid) {
+                    wb += s_wcnt[w][q];
+                    wl = wl > s_wlast[w][q] ? wl : s_wlast[w][q];
+                }
+                qtot += s_wcnt[w][q];
+                ql = ql > s_wlast[w][q] ? ql : s_wlast[w][q];
+            }
+            const uint32_t excl_c = wb + ic[q] - cnt[q];
+            const int32_t excl_l = wl > prev_l[q] ? wl : prev_l[q];
+
+            // stage the quarter (+ the 256 bytes before it)
+            if (q == 0) {
+                if (tid < HALO / 16)
+                    ((uint4 *)s_text)[tid] = hv;
+            } else {
+                uint4 h;
+                if (tid < HALO / 16)
+                    h = ((const uint4 *)(s_text + QB))[tid];
+                __syncthreads();
+                if (tid < HALO / 16)
+                    ((uint4 *)s_text)[tid] = h;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                ((uint4 *)(s_text + HALO))[j * THREADS + tid] = v[4 * q + j];
+            __syncthreads();
+
+            const uint32_t lds_lo = (qbase >= (uint32_t)HALO) ? qbase - HALO : 0u;
+            auto lds_byte = [&](uint32_t pos) -> uint32_t { return s_text[pos + (uint32_t)HALO - qbase]; };
+            // Decode line (s, e) and write it at output index r.
+            auto emit = [&](uint64_t r, uint32_t s, uint32_t e) {
+                if (r >= a.cap)
+                    return;
                 a.offsets[r] = s;
+                if (s < lds_lo) {
+                    // head not in LDS: decoded by the fixup pass
+                    uint32_t slot = atomicAdd(a.fix_count, 1u);
+                    if (slot < a.fix_cap)
+                        a.fix_list[slot] = (uint32_t)r;
+                    return;
+                }
+                uint32_t eff = e + 1;
+                while (eff > s) {
+                    const uint32_t c = lds_byte(eff - 1);
+                    if (c != '\n' && c != '\r')
+                        break;
+                    eff--;
+                }
+                ol_node_dev nd;
+                if (eff == s) {
+                    nd.depth = 0; nd.kind = OLK_BLANK; nd.span_start = s; nd.span_len = 0;
+                } else if (!decode_head(s_text, s + (uint32_t)HALO - qbase, s, eff, nd)) {
+                    nd = parse_span(lds_byte, s, e);
+                }
                 a.classes[r] = (uint8_t)nd.kind;
                 a.keys[r] = key_of(nd);
                 if (a.nodes)
                     a.nodes[r] = nd;
+            };
+            if (qtot <= (uint32_t)LIST) {
+                // Compact the quarter's lines into LDS, then lane i decodes line
+                // i: balanced work and coalesced output writes.
+                uint32_t sp = (excl_l >= 0) ? qbase + (uint32_t)excl_l + 1 : carry;
+                uint32_t at = excl_c;
+                uint64_t m = mk[q];
+                while (m) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const uint32_t e = qbase + tid * 64 + (uint32_t)bit;
+                    s_ls[at] = sp;
+                    s_le[at] = e;
+                    at++;
+                    sp = e + 1;
+                }
+                __syncthreads();
+                for (uint32_t i = tid; i < qtot; i += THREADS)
+                    emit(running + i, s_ls[i], s_le[i]);
+            } else if (cnt[q]) {
+                // dense quarter: each lane decodes the lines ending in its bytes
+                uint64_t r = running + excl_c;
+                uint32_t sp = (excl_l >= 0) ? qbase + (uint32_t)excl_l + 1 : carry;
+                uint64_t m = mk[q];
+                while (m) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const uint32_t e = qbase + tid * 64 + (uint32_t)bit;
+                    emit(r, sp, e);
+                    sp = e + 1;
+                    r++;
+                }
             }
-            s = e + 1;
-            r++;
+            running += qtot;
+            if (ql >= 0)
+                carry = qbase + (uint32_t)ql + 1;
+            __syncthreads();   // the next quarter (or tile) overwrites LDS
         }
     }
 }
 
-hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, hipStream_t s)
+// Lines whose head was not in LDS: decode them from global memory.
+// Line r spans offsets[r] ..= offsets[r+1] - 1 (the last line: to the text end).
+__global__ void scan_fixup_kernel(ScanArgs a)
+{
+    const uint32_t cnt = *a.fix_count < a.fix_cap ? *a.fix_count : a.fix_cap;
+    const uint64_t lines = a.meta[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint32_t r = a.fix_list[i];
+        if (r + 1 < lines && r + 1 >= a.cap)
+            continue;   // outputs overflowed: the caller re-runs with a larger capacity
+        const uint32_t s = a.offsets[r];
+        const uint32_t e = (r + 1 < lines) ? a.offsets[r + 1] - 1 : a.n - 1;
+        const ol_node_dev nd = parse_span(GlobalSrc{a.text}, s, e);
+        a.classes[r] = (uint8_t)nd.kind;
+        a.keys[r] = key_of(nd);
+        if (a.nodes)
+            a.nodes[r] = nd;
+    }
+}
+
+hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(scan_fixup_kernel, dim3(64), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int Q>
+static const void *scan_fn()
+{
+    return (const void *)scan_classify_kernel<SCAN_THREADS, Q>;
+}
+
+static const void *scan_fn_bpt(uint32_t bpt)
 {
     switch (bpt) {
-    case 64:
-        hipLaunchKernelGGL((scan_classify_kernel<SCAN_THREADS, 64>), dim3(a.num_tiles), dim3(SCAN_THREADS), 0, s, a);
-        break;
-    case 128:
-        hipLaunchKernelGGL((scan_classify_kernel<SCAN_THREADS, 128>), dim3(a.num_tiles), dim3(SCAN_THREADS), 0, s, a);
-        break;
-    case 256:
-        hipLaunchKernelGGL((scan_classify_kernel<SCAN_THREADS, 256>), dim3(a.num_tiles), dim3(SCAN_THREADS), 0, s, a);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case 64: return scan_fn<1>();
+    case 128: return scan_fn<2>();
+    case 256: return scan_fn<4>();
+    default: return nullptr;
     }
-    return hipGetLastError();
+}
+
+// Co-resident workgroups of the scan kernel for a given bytes-per-lane.
+uint32_t ol_scan_max_grid(uint32_t bpt)
+{
+    int dev = 0, ncu = 0, per_cu = 0;
+    const void *f = scan_fn_bpt(bpt);
+    if (!f || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, SCAN_THREADS, 0) != hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu * ncu);
+}
+
+hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, hipStream_t s)
+{
+    const void *f = scan_fn_bpt(bpt);
+    if (!f || grid == 0)
+        return hipErrorInvalidValue;
+    ScanArgs args = a;
+    void *params[] = {&args};
+    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SCAN_THREADS), params, 0, s);
 }
 
 // ---- parse with caller-given line offsets ------------------------------------
 // GpuParser::parse (crates/oneline-gpu/src/parse.rs:104-124 + shader.wgsl:34-134):
 // line i spans offsets[i] .. offsets[i+1] (or the text end for the last line).
-struct GlobalSrc {
-    const uint8_t *g;
-    __device__ __forceinline__ uint32_t operator()(uint32_t pos) const { return g[pos]; }
-};
 
 __global__ void parse_given_kernel(const uint8_t *text, uint32_t n, const uint32_t *offsets,
                                    uint64_t lines, ol_node_dev *nodes)

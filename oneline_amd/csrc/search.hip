@@ -81,7 +81,20 @@ __device__ __forceinline__ uint64_t tile_prefix(uint64_t *status, uint32_t t, ui
     return *s_prefix;
 }
 
+// Co-resident workgroups of a kernel (for the persistent, cooperative launches).
+static uint32_t max_grid(const void *f, int threads, size_t lds)
+{
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, lds) != hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu * ncu);
+}
+
 // ---- single literal -----------------------------------------------------------
+// Persistent: tile t = blockIdx + k * gridDim (all workgroups co-resident, each
+// walking its tiles in increasing order, so look-backs always make progress).
 template <int THREADS, int BPT>
 __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
 {
@@ -89,78 +102,83 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
     static_assert(BPT == 64, "64-bit lane masks");
     __shared__ __align__(16) uint8_t s_text[TILE + HALO_AFTER];
     __shared__ uint32_t s_w[THREADS / 64];
-    __shared__ uint32_t s_tile;
     __shared__ uint64_t s_prefix;
 
     const int tid = threadIdx.x;
-    if (tid == 0)
-        s_tile = atomicAdd(a.tile_ctr, 1u);
-    __syncthreads();
-    const uint32_t t = s_tile;
-    const uint32_t base = t * TILE;
-    load_tile<THREADS, BPT>(s_text, a.text, base);
-    __syncthreads();
+    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
+        const uint32_t base = t * TILE;
+        load_tile<THREADS, BPT>(s_text, a.text, base);
+        __syncthreads();
 
-    const uint32_t lo = tid * BPT;
-    const uint32_t g0 = base + lo;
-    // positions p with g0 + p + m <= n
-    uint32_t lim = 0;
-    if ((uint64_t)g0 + a.m <= a.n)
-        lim = a.n - a.m - g0 + 1;
-    if (lim > BPT)
-        lim = BPT;
-    uint64_t match = 0;
-    if (lim) {
-        const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
-        uint32_t w[BPT / 4 + 1];
+        const uint32_t lo = tid * BPT;
+        const uint32_t g0 = base + lo;
+        // positions p with g0 + p + m <= n
+        uint32_t lim = 0;
+        if ((uint64_t)g0 + a.m <= a.n)
+            lim = a.n - a.m - g0 + 1;
+        if (lim > BPT)
+            lim = BPT;
+        uint64_t match = 0;
+        if (lim) {
+            const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
+            uint32_t w[BPT / 4 + 1];
 #pragma unroll
-        for (int i = 0; i < BPT / 4 + 1; i++)
-            w[i] = wsrc[i];
-        uint64_t cand = 0;
+            for (int i = 0; i < BPT / 4 + 1; i++)
+                w[i] = wsrc[i];
+            uint64_t cand = 0;
 #pragma unroll
-        for (int p = 0; p < BPT; p++) {
-            uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w[(p >> 2) + 1], w[p >> 2], p & 3)
-                                   : w[p >> 2];
-            cand |= (uint64_t)((win & a.win_mask) == a.win) << p;
-        }
-        if (lim < BPT)
-            cand &= (1ull << lim) - 1;
-        while (cand) {
-            int p = __builtin_ctzll(cand);
-            cand &= cand - 1;
-            bool ok = true;
-            for (uint32_t i = 4; i < a.m; i++) {
-                if (s_text[lo + p + i] != a.pat[i]) {
-                    ok = false;
-                    break;
-                }
+            for (int p = 0; p < BPT; p++) {
+                uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w[(p >> 2) + 1], w[p >> 2], p & 3)
+                                       : w[p >> 2];
+                cand |= (uint64_t)((win & a.win_mask) == a.win) << p;
             }
-            if (ok)
-                match |= 1ull << p;
+            if (lim < BPT)
+                cand &= (1ull << lim) - 1;
+            while (cand) {
+                int p = __builtin_ctzll(cand);
+                cand &= cand - 1;
+                bool ok = true;
+                for (uint32_t i = 4; i < a.m; i++) {
+                    if (s_text[lo + p + i] != a.pat[i]) {
+                        ok = false;
+                        break;
+                    }
+                }
+                if (ok)
+                    match |= 1ull << p;
+            }
         }
-    }
-    uint32_t total;
-    const uint32_t excl = block_excl_sum<THREADS>((uint32_t)__popcll(match), s_w, total);
-    const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix);
-    if (tid == 0 && t == a.num_tiles - 1)
-        a.meta[0] = prefix + total;
-    uint64_t r = prefix + excl;
-    while (match) {
-        int p = __builtin_ctzll(match);
-        match &= match - 1;
-        if (r < a.cap) {
-            if (a.out64) a.out64[r] = g0 + p;
-            if (a.out32) a.out32[r] = g0 + p;
+        uint32_t total;
+        const uint32_t excl = block_excl_sum<THREADS>((uint32_t)__popcll(match), s_w, total);
+        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix);
+        if (tid == 0 && t == a.num_tiles - 1)
+            a.meta[0] = prefix + total;
+        uint64_t r = prefix + excl;
+        while (match) {
+            int p = __builtin_ctzll(match);
+            match &= match - 1;
+            if (r < a.cap) {
+                if (a.out64) a.out64[r] = g0 + p;
+                if (a.out32) a.out32[r] = g0 + p;
+            }
+            r++;
         }
-        r++;
+        __syncthreads();   // LDS is reused by the next tile
     }
 }
 
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((search_kernel<SEARCH_THREADS, SEARCH_BPT>), dim3(a.num_tiles),
-                       dim3(SEARCH_THREADS), 0, s, a);
-    return hipGetLastError();
+    static uint32_t cap = 0;   // co-resident limit (same device for the process)
+    const void *f = (const void *)search_kernel<SEARCH_THREADS, SEARCH_BPT>;
+    if (!cap)
+        cap = max_grid(f, SEARCH_THREADS, 0);
+    if (!cap)
+        return hipErrorInvalidValue;
+    uint32_t grid = a.num_tiles < cap ? a.num_tiles : cap;
+    SearchArgs args = a;
+    void *params[] = {&args};
+    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SEARCH_THREADS), params, 0, s);
 }
 
 // ---- dictionary -----------------------------------------------------------------
@@ -261,58 +279,56 @@ __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
     static_assert(BPT == 64, "64-bit lane masks");
     extern __shared__ __align__(16) uint32_t s_dyn[];
     __shared__ uint32_t s_w[THREADS / 64];
-    __shared__ uint32_t s_tile;
     __shared__ uint64_t s_prefix;
 
     uint8_t *s_text = (uint8_t *)s_dyn;                       // TILE + HALO_AFTER bytes
     uint32_t *s_filter = s_dyn + (TILE + HALO_AFTER) / 4;   // fwords
 
     const int tid = threadIdx.x;
-    if (tid == 0)
-        s_tile = atomicAdd(a.tile_ctr, 1u);
     for (uint32_t i = tid; i < a.d.fwords; i += THREADS)
         s_filter[i] = a.d.filter[i];
-    __syncthreads();
-    const uint32_t t = s_tile;
-    const uint32_t base = t * TILE;
-    load_tile<THREADS, BPT>(s_text, a.text, base);
-    __syncthreads();
+    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
+        const uint32_t base = t * TILE;
+        load_tile<THREADS, BPT>(s_text, a.text, base);
+        __syncthreads();
 
-    const uint32_t lo = tid * BPT;
-    const uint32_t g0 = base + lo;
-    uint32_t lim = (g0 < a.n) ? a.n - g0 : 0;
-    if (lim > BPT)
-        lim = BPT;
+        const uint32_t lo = tid * BPT;
+        const uint32_t g0 = base + lo;
+        uint32_t lim = (g0 < a.n) ? a.n - g0 : 0;
+        if (lim > BPT)
+            lim = BPT;
 
-    const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
-    uint64_t has = 0;
-    uint32_t count = 0;
-    uint64_t dummy = 0;
-    for (uint32_t p = 0; p < lim; p++) {
-        uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
-        uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
-        uint32_t c = probe_position(a.d, s_filter, s_text, lo + p, g0 + p, a.n, win, false,
-                                    nullptr, nullptr, dummy, 0);
-        if (c) {
-            has |= 1ull << p;
-            count += c;
+        const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
+        uint64_t has = 0;
+        uint32_t count = 0;
+        uint64_t dummy = 0;
+        for (uint32_t p = 0; p < lim; p++) {
+            uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
+            uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
+            uint32_t c = probe_position(a.d, s_filter, s_text, lo + p, g0 + p, a.n, win, false,
+                                        nullptr, nullptr, dummy, 0);
+            if (c) {
+                has |= 1ull << p;
+                count += c;
+            }
         }
-    }
-    uint32_t total;
-    const uint32_t excl = block_excl_sum<THREADS>(count, s_w, total);
-    const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix);
-    if (tid == 0 && t == a.num_tiles - 1)
-        a.meta[0] = prefix + total;
-    if (!a.out_off && !a.out_id)
-        return;
-    uint64_t r = prefix + excl;
-    while (has && r < a.cap) {
-        uint32_t p = (uint32_t)__builtin_ctzll(has);
-        has &= has - 1;
-        uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
-        uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
-        probe_position(a.d, s_filter, s_text, lo + p, g0 + p, a.n, win, true, a.out_off, a.out_id,
-                       r, a.cap);
+        uint32_t total;
+        const uint32_t excl = block_excl_sum<THREADS>(count, s_w, total);
+        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix);
+        if (tid == 0 && t == a.num_tiles - 1)
+            a.meta[0] = prefix + total;
+        if (a.out_off || a.out_id) {
+            uint64_t r = prefix + excl;
+            while (has && r < a.cap) {
+                uint32_t p = (uint32_t)__builtin_ctzll(has);
+                has &= has - 1;
+                uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
+                uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
+                probe_position(a.d, s_filter, s_text, lo + p, g0 + p, a.n, win, true, a.out_off, a.out_id,
+                               r, a.cap);
+            }
+        }
+        __syncthreads();   // LDS is reused by the next tile
     }
 }
 
@@ -323,8 +339,13 @@ size_t ol_many_lds_bytes(uint32_t fwords)
 
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
 {
-    size_t lds = ol_many_lds_bytes(a.d.fwords);
-    hipLaunchKernelGGL((many_kernel<SEARCH_THREADS, SEARCH_BPT>), dim3(a.num_tiles),
-                       dim3(SEARCH_THREADS), lds, s, a);
-    return hipGetLastError();
+    const size_t lds = ol_many_lds_bytes(a.d.fwords);
+    const void *f = (const void *)many_kernel<SEARCH_THREADS, SEARCH_BPT>;
+    uint32_t cap = max_grid(f, SEARCH_THREADS, lds);
+    if (!cap)
+        return hipErrorInvalidValue;
+    uint32_t grid = a.num_tiles < cap ? a.num_tiles : cap;
+    ManyArgs args = a;
+    void *params[] = {&args};
+    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SEARCH_THREADS), params, lds, s);
 }
