@@ -176,8 +176,7 @@ int ol_search_many_device(ol_ctx *ctx, const ol_doc *doc, const ol_dict *dict, u
  *   "scan_bpt"      bytes per lane of the scan kernel: 64, 128 or 256
  *                   (tile = 256 lanes x bpt bytes);
  *   "scan_grid"     persistent workgroups of the scan kernel (0 = as many as
- *                   are co-resident on the device, the default);
- *   "tree_threads"  lanes per tree tile: 64 or 128 (tile = lanes x 16 lines). */
+ *                   are co-resident on the device, the default). */
 int ol_tune_set(ol_ctx *ctx, const char *name, int value);
 int ol_tune_get(ol_ctx *ctx, const char *name, int *value);
 

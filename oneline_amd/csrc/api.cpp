@@ -70,13 +70,12 @@ struct ol_ctx {
     std::string name;
     std::mutex mu;
     DevBuf scratch;
-    DevBuf tree_agg, tree_incl;
+    DevBuf tree_sum;                       // tree per-tile level states (4 arrays of tiles x TREE_D)
     DevBuf fix;                            // list of lines decoded by the scan fixup pass
     // launch geometry picked by the autotuner (ol_tune_set)
     uint32_t scan_bpt = SCAN_BPT;          // bytes per lane of scan_classify (tile = 256 lanes x bpt)
     uint32_t scan_grid = 0;                // persistent workgroups of scan_classify (0 = all co-resident)
     uint32_t scan_max_grid[3] = {0, 0, 0}; // co-resident limit per bpt (64, 128, 256), from the occupancy API
-    uint32_t tree_threads = TREE_THREADS;  // lanes per tree tile (tile = lanes x TREE_RUN lines)
     // profiling
     bool profiling = false;
     struct Rec {
@@ -333,11 +332,12 @@ int ensure_tree(ol_tree *t, uint64_t lines)
 
 int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->lines, (uint64_t)c->tree_threads * TREE_RUN);
+    uint32_t tiles = (uint32_t)ceil_div(d->lines ? d->lines : 1, (uint64_t)ol_tree_tile_lines());
+    size_t part = (size_t)tiles * TREE_D;
     HIPCHK(ensure_scratch(c, 0));
-    HIPCHK(c->tree_agg.ensure((size_t)tiles * TREE_D * 4));
-    HIPCHK(c->tree_incl.ensure((size_t)tiles * TREE_D * 4));
+    HIPCHK(c->tree_sum.ensure(part * 4 * 4));
     HIPCHK(clear_scratch(c, 0));
+    uint32_t *base = (uint32_t *)c->tree_sum.p;
     TreeArgs a{};
     a.keys = d->keys;
     a.lines = d->lines;
@@ -346,11 +346,13 @@ int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
     a.first_child = t->fc;
     a.next_sibling = t->ns;
     a.subtree_end = t->se;
-    a.summary = (uint32_t *)c->tree_agg.p;
-    a.incoming = (uint32_t *)c->tree_incl.p;
+    a.sum_last = base;
+    a.sum_first = base + part;
+    a.in_last = base + 2 * part;
+    a.out_first = base + 3 * part;
     a.err = c->ctr() + 1;
     Prof p(c, "tree");
-    HIPCHK(ol_launch_tree(a, c->tree_threads, c->stream));
+    HIPCHK(ol_launch_tree(a, c->stream));
     return OL_OK;
 }
 
@@ -529,8 +531,7 @@ void ol_ctx_destroy(ol_ctx *c)
     for (auto e : c->pool)
         (void)hipEventDestroy(e);
     c->scratch.release();
-    c->tree_agg.release();
-    c->tree_incl.release();
+    c->tree_sum.release();
     c->fix.release();
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1111,10 +1112,6 @@ int ol_tune_set(ol_ctx *c, const char *name, int value)
         if (value < 0)
             return fail(OL_ERR_INVALID, "scan_grid must be >= 0 (0 = all co-resident workgroups)");
         c->scan_grid = (uint32_t)value;
-    } else if (n == "tree_threads") {
-        if (value != 64 && value != 128)
-            return fail(OL_ERR_INVALID, "tree_threads must be 64 or 128");
-        c->tree_threads = (uint32_t)value;
     } else {
         return fail(OL_ERR_INVALID, "unknown tuning parameter " + n);
     }
@@ -1131,8 +1128,6 @@ int ol_tune_get(ol_ctx *c, const char *name, int *value)
         *value = (int)c->scan_bpt;
     else if (n == "scan_grid")
         *value = (int)c->scan_grid;
-    else if (n == "tree_threads")
-        *value = (int)c->tree_threads;
     else
         return fail(OL_ERR_INVALID, "unknown tuning parameter " + n);
     return OL_OK;

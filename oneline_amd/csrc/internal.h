@@ -117,10 +117,12 @@ struct ScanArgs {
 struct TreeArgs {
     const uint8_t *keys;     // tree key byte per line (padded to LINE_ALIGN)
     uint64_t lines;          // line count
-    uint32_t num_tiles;
+    uint32_t num_tiles;      // ceil(lines / ol_tree_tile_lines())
     uint32_t *parent, *first_child, *next_sibling, *subtree_end;  // nullable each
-    uint32_t *summary;       // per tile x level: last line index + 1 in the tile (0 = none)
-    uint32_t *incoming;      // per tile x level: the same over all earlier tiles
+    uint32_t *sum_last;      // per tile x level: last content line + 1 in the tile (0 = none)
+    uint32_t *sum_first;     // per tile x level: first content line in the tile (OL_NONE = none)
+    uint32_t *in_last;       // per tile x level: sum_last over all earlier tiles (max)
+    uint32_t *out_first;     // per tile x level: sum_first over all later tiles (min)
     uint32_t *err;           // set to 1 if a content line is too deep
 };
 
@@ -178,7 +180,8 @@ hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s);
 hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, hipStream_t s);
 uint32_t ol_scan_max_grid(uint32_t bpt);
 hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s);   // reads the line count from meta[0]
-hipError_t ol_launch_tree(const TreeArgs &a, uint32_t threads, hipStream_t s);   // summary + scan + walk; threads 128 or 256
+hipError_t ol_launch_tree(const TreeArgs &a, hipStream_t s);   // summary + scan + walk
+uint32_t ol_tree_tile_lines();
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
 
@@ -187,8 +190,6 @@ constexpr uint32_t SCAN_THREADS = 256;
 constexpr uint32_t SCAN_BPT = 256;                          // default bytes per lane (64 KiB tiles)
 constexpr uint32_t SCAN_TILE_MAX = SCAN_THREADS * 256;
 constexpr uint32_t SCAN_HALO = 256;
-constexpr uint32_t TREE_THREADS = 128;                      // default lanes per tile
-constexpr uint32_t TREE_RUN = 16;
 constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks

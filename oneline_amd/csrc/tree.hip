@@ -1,387 +1,321 @@
 // tree.hip -- parent / first_child / next_sibling / subtree_end from the
-// per-line key bytes.
+// per-line key bytes, one line per lane.
 //
-// The reference builds parent + subtree_end with seven passes (extract keys,
-// per-run forward/backward summaries, per-level scans, two assign passes;
-// crates/oneline-gpu/src/tree.wgsl:127-375).  Here:
+// The reference builds parent + subtree_end with seven passes of per-thread
+// 32-line runs (crates/oneline-gpu/src/tree.wgsl:127-375) and checks them
+// against the monotonic stack of TreeLinks::from_nodes_sequential
+// (crates/oneline-core/src/links.rs:52-81).  Here every line is resolved
+// independently from two per-level states:
 //
-//   * State.  F[d] = index of the last content line at depth d seen so far.
-//     The monotonic stack of TreeLinks::from_nodes_sequential
-//     (crates/oneline-core/src/links.rs:52-81) is exactly the set of levels d
-//     with F[d] > max(F[0..d)): the lines not yet closed by a later line of
-//     depth <= theirs.  F combines across line ranges by element-wise max.
-//   * Three launches (reduce-then-scan; no inter-workgroup waiting, which on
-//     this multi-XCD part would need cross-L2 traffic per tile):
-//       tree_summary: per tile (threads x TREE_RUN lines), F of the tile alone;
-//       tree_scan:    per level, the exclusive max-scan of the tile summaries,
-//                     i.e. the F in force at each tile's first line;
-//       tree_walk:    the links (below).
-//   * Walk.  Each lane owns a run of TREE_RUN consecutive lines; an exclusive
-//     max-scan over the runs of a tile (one level per pass) gives each run its
-//     incoming F.  The lane then replays the monotonic stack over its run: a
-//     local stack in LDS for its own lines, and the incoming stack read from F
-//     ("virtual" entries, popped by lowering a level bound).  When line i
-//     arrives: the previous content line p gets first_child = i unless i
-//     closes it; every popped line j gets subtree_end = i; the last popped one
-//     is the line directly above i's parent, so it gets next_sibling = i and
-//     the others NONE; parent = the top after popping.
-//   * Exactly-once writes.  A line's subtree_end / next_sibling are written
-//     by whichever lane pops it (possibly in a later tile), its first_child
-//     by the lane of the next content line; the last tile closes the lines
-//     still open at the end of the document.
+//   F[d] = the last content line at depth d before a point (forward),
+//   B[d] = the first content line at depth d after a point (backward).
 //
-// Blank lines are outside the tree: parent NONE, subtree_end i + 1 and
-// (new outputs) first_child = next_sibling = NONE.  A content line nested
-// TREE_D or more levels deep sets the error flag (the reference's
-// MAX_TREE_DEPTH contract, crates/oneline-gpu/src/tree.rs:10-12,218-222).
+//   parent[i]       = max over d < x_i of F[d]            (nearest previous smaller)
+//   subtree_end[i]  = min over d <= x_i of B[d], else n   (nearest next smaller-or-equal)
+//   first_child[i]  = the next content line j if x_j > x_i, else NONE
+//   next_sibling[i] = k = subtree_end[i] if k < n and depth(parent[i]) < x_k
+//                     (or parent[i] is NONE), else NONE
+//
+// The last rule restates "parent[k] == parent[i]": the content lines between i
+// and k are deeper than x_i >= x_k, so parent[k] is the nearest line before i
+// shallower than x_k, which is parent[i] exactly when parent[i] is shallower
+// than x_k.  first_child restates "first content line in (i, subtree_end[i])".
+//
+// Work split: a wave resolves 64 consecutive lines per round, lane = line.  For
+// each level d the wave takes ballot(key < d); a lane's nearest smaller line
+// inside the round is then one masked clz / ctz, and only lines whose answer
+// lies outside the round use F / B, kept one level per lane (prefix max / min
+// over the lanes = over the levels, read with one shuffle).  Three launches:
+//
+//   tree_summary: per tile (4 waves x 16 rounds x 64 lines), F and B of the tile;
+//   tree_scan:    per level, max over earlier tiles / min over later tiles;
+//   tree_walk:    the four outputs, written once each and coalesced.
+//
+// Blank lines are outside the tree: parent NONE, subtree_end i + 1,
+// first_child = next_sibling = NONE.  A content line nested TREE_D or more
+// levels deep sets the error flag (the reference's MAX_TREE_DEPTH contract,
+// crates/oneline-gpu/src/tree.rs:10-12,218-222).
 #include "internal.h"
 
-// Loads the R keys of this lane's run (keys past the last line read as blank).
-template <int R>
-__device__ __forceinline__ void load_run_keys(const uint8_t *keys, uint32_t rbase, uint64_t L, uint8_t (&key)[R])
+constexpr int TW = 256;              // threads per tile (4 waves)
+constexpr int TR = 16;               // rounds of 64 lines per wave
+constexpr int TLINES = TW * TR;      // lines per tile
+constexpr int D = TREE_D;            // levels tracked
+static_assert(D <= 63, "levels + 1 must fit a wave");
+
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// Keys of this lane's 16 rounds (lines past the end read as blank).
+__device__ __forceinline__ void load_keys(const uint8_t *keys, uint32_t wbase, uint64_t L, uint32_t (&key)[TR])
 {
-    static_assert(R == 16, "one 16-byte key load per run");
-    uint4 v = *(const uint4 *)(keys + rbase);
-    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < R; k++) {
-        uint32_t x = (w[k >> 2] >> (8 * (k & 3))) & 0xFF;
-        key[k] = (rbase + k < L) ? (uint8_t)x : (uint8_t)KEY_BLANK;
+    for (int r = 0; r < TR; r++) {
+        const uint32_t g = wbase + r * 64 + lane;
+        key[r] = (g < L) ? (uint32_t)keys[g] : KEY_BLANK;
     }
 }
 
-// Fills s_run[d][tid] with this run's last local index + 1 at depth d (0 = none);
-// returns true if the run holds a content line that is too deep.
-template <int THREADS, int R, int D>
-__device__ __forceinline__ bool run_summary(const uint8_t (&key)[R], uint16_t (*s_run)[THREADS])
+// Lane d (d <= D) gets ballot(key < d) of the round.
+__device__ __forceinline__ uint64_t level_masks(uint32_t key)
 {
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    uint64_t my = 0;
 #pragma unroll
-    for (int d = 0; d < D; d++)
-        s_run[d][tid] = 0;
+    for (int d = 0; d <= D; d++) {
+        const uint64_t m = __ballot(key < (uint32_t)d);
+        my = (lane == d) ? m : my;
+    }
+    return my;
+}
+
+// Per tile: for each level the last content line (+1, 0 = none) and the first
+// (NONE = none); error flag for content lines at depth >= D.
+__global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
+{
+    __shared__ uint32_t s_l[TW / 64][D], s_f[TW / 64][D];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t t = blockIdx.x;
+    const uint32_t wbase = t * TLINES + w * (TR * 64);
+    uint32_t key[TR];
+    load_keys(a.keys, wbase, a.lines, key);
+    uint32_t last = 0, first = OL_NONE;
     bool deep = false;
 #pragma unroll
-    for (int k = 0; k < R; k++) {
-        uint32_t x = key[k];
-        if (x < (uint32_t)D)
-            s_run[x][tid] = (uint16_t)(tid * R + k + 1);
-        else if (x != KEY_BLANK)
-            deep = true;
+    for (int r = 0; r < TR; r++) {
+        const uint64_t my = level_masks(key[r]);
+        const uint64_t up = __shfl_down(my, 1, 64);
+        const uint64_t eq = up & ~my;   // lanes whose key == lane id
+        const uint32_t rb = wbase + r * 64;
+        if (lane < D && eq) {
+            last = rb + 64 - (uint32_t)__clzll(eq);     // index + 1
+            if (first == OL_NONE)
+                first = rb + (uint32_t)__builtin_ctzll(eq);
+        }
+        deep |= (key[r] != KEY_BLANK && key[r] >= (uint32_t)D);
     }
-    return deep;
-}
-
-// Exclusive max-scan of s_run over the runs, one level per pass; s_agg[d] gets
-// the tile total.
-template <int THREADS, int D>
-__device__ __forceinline__ void run_scan(uint16_t (*s_run)[THREADS], uint16_t *s_agg)
-{
-    constexpr int NW = THREADS / 64;
-    constexpr int RPL = THREADS / 64;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int d = wid; d < D; d += NW) {
-        uint16_t v[RPL];
-        uint32_t lmax = 0;
-#pragma unroll
-        for (int j = 0; j < RPL; j++) {
-            v[j] = s_run[d][lane * RPL + j];
-            lmax = lmax > v[j] ? lmax : v[j];
-        }
-        uint32_t inc = lmax;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(inc, o, 64);
-            if (lane >= o)
-                inc = inc > y ? inc : y;
-        }
-        uint32_t run = __shfl_up(inc, 1, 64);
-        if (lane == 0)
-            run = 0;
-#pragma unroll
-        for (int j = 0; j < RPL; j++) {
-            s_run[d][lane * RPL + j] = (uint16_t)run;
-            run = run > v[j] ? run : v[j];
-        }
-        if (lane == 63)
-            s_agg[d] = (uint16_t)inc;
-    }
-}
-
-template <int THREADS, int R, int D>
-__global__ __launch_bounds__(THREADS) void tree_summary_kernel(TreeArgs a)
-{
-    constexpr int TL = THREADS * R;
-    __shared__ uint16_t s_run[D][THREADS];
-    __shared__ uint16_t s_agg[D];
-    const uint32_t t = blockIdx.x;
-    const uint32_t tbase = t * TL;
-    uint8_t key[R];
-    load_run_keys<R>(a.keys, tbase + threadIdx.x * R, a.lines, key);
-    if (run_summary<THREADS, R, D>(key, s_run))
+    if (__ballot(deep) && lane == 0)
         atomicOr(a.err, 1u);
-    __syncthreads();
-    // tile total per level: max over runs
-    constexpr int NW = THREADS / 64;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int d = wid; d < D; d += NW) {
-        uint32_t m = 0;
-        for (int j = lane; j < THREADS; j += 64)
-            m = m > s_run[d][j] ? m : s_run[d][j];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            uint32_t y = __shfl_xor(m, o, 64);
-            m = m > y ? m : y;
-        }
-        if (lane == 0)
-            s_agg[d] = (uint16_t)m;
+    if (lane < D) {
+        s_l[w][lane] = last;
+        s_f[w][lane] = first;
     }
     __syncthreads();
-    if (threadIdx.x < D) {
-        uint32_t v = s_agg[threadIdx.x];
-        a.summary[(size_t)t * D + threadIdx.x] = v ? tbase + v : 0u;
+    if (w == 0 && lane < D) {
+        uint32_t l = 0, f = OL_NONE;
+#pragma unroll
+        for (int k = 0; k < TW / 64; k++) {
+            l = umax(l, s_l[k][lane]);
+            f = umin(f, s_f[k][lane]);
+        }
+        a.sum_last[(size_t)t * D + lane] = l;
+        a.sum_first[(size_t)t * D + lane] = f;
     }
 }
 
-// One workgroup per level: exclusive max-scan of the tile summaries.
-template <int SCAN_T>
-__global__ __launch_bounds__(SCAN_T) void tree_scan_kernel(TreeArgs a, uint32_t D)
+// One workgroup per level: in_last[t] = max of sum_last over tiles < t,
+// out_first[t] = min of sum_first over tiles > t.  Each thread owns a contiguous
+// chunk of tiles.
+template <int ST>
+__global__ __launch_bounds__(ST) void tree_scan_kernel(TreeArgs a)
 {
+    constexpr int NWS = ST / 64;
+    __shared__ uint32_t s_w[NWS];
     const uint32_t d = blockIdx.x;
     const uint32_t T = a.num_tiles;
-    __shared__ uint32_t s_w[SCAN_T / 64];
-    __shared__ uint32_t s_carry;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0)
-        s_carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < T; base += SCAN_T) {
-        uint32_t i = base + tid;
-        uint32_t v = (i < T) ? a.summary[(size_t)i * D + d] : 0u;
-        uint32_t inc = v;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t K = (T + ST - 1) / ST;
+    const uint32_t t0 = tid * K, t1 = umin(T, t0 + K);
+
+    // forward: exclusive prefix max
+    uint32_t v = 0;
+    for (uint32_t t = t0; t < t1; t++)
+        v = umax(v, a.sum_last[(size_t)t * D + d]);
+    uint32_t inc = v;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(inc, o, 64);
-            if (lane >= o)
-                inc = inc > y ? inc : y;
-        }
-        if (lane == 63)
-            s_w[wid] = inc;
-        __syncthreads();
-        uint32_t carry = s_carry, wb = 0, tot = 0;
-        for (int w = 0; w < SCAN_T / 64; w++) {
-            if (w < wid)
-                wb = wb > s_w[w] ? wb : s_w[w];
-            tot = tot > s_w[w] ? tot : s_w[w];
-        }
-        uint32_t excl = __shfl_up(inc, 1, 64);
-        if (lane == 0)
-            excl = 0;
-        excl = excl > wb ? excl : wb;
-        excl = excl > carry ? excl : carry;
-        if (i < T)
-            a.incoming[(size_t)i * D + d] = excl;
-        __syncthreads();
-        if (tid == 0)
-            s_carry = carry > tot ? carry : tot;
-        __syncthreads();
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o)
+            inc = umax(inc, y);
+    }
+    if (lane == 63)
+        s_w[w] = inc;
+    __syncthreads();
+    uint32_t run = __shfl_up(inc, 1, 64);
+    if (lane == 0)
+        run = 0;
+    for (int k = 0; k < w; k++)
+        run = umax(run, s_w[k]);
+    for (uint32_t t = t0; t < t1; t++) {
+        a.in_last[(size_t)t * D + d] = run;
+        run = umax(run, a.sum_last[(size_t)t * D + d]);
+    }
+    __syncthreads();
+
+    // backward: exclusive suffix min
+    v = OL_NONE;
+    for (uint32_t t = t0; t < t1; t++)
+        v = umin(v, a.sum_first[(size_t)t * D + d]);
+    inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_down(inc, o, 64);
+        if (lane + o < 64)
+            inc = umin(inc, y);
+    }
+    if (lane == 0)
+        s_w[w] = inc;
+    __syncthreads();
+    run = __shfl_down(inc, 1, 64);
+    if (lane == 63)
+        run = OL_NONE;
+    for (int k = w + 1; k < NWS; k++)
+        run = umin(run, s_w[k]);
+    for (uint32_t t = t1; t-- > t0;) {
+        a.out_first[(size_t)t * D + d] = run;
+        run = umin(run, a.sum_first[(size_t)t * D + d]);
     }
 }
 
-#define TREE_UNSET 0xFFFFFFFEu   // staging slot not written by this tile (never a valid value)
-
-template <int THREADS, int R, int D>
-__global__ __launch_bounds__(THREADS) void tree_walk_kernel(TreeArgs a)
+__global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
 {
-    constexpr int TL = THREADS * R;
-    static_assert(D <= 64 && TL <= 65535, "level / index width");
+    constexpr int NWV = TW / 64;
+    __shared__ uint64_t s_m[NWV][TR][D + 1];   // per round: lane d's ballot(key < d)
+    __shared__ uint32_t s_b[NWV][TR][D];       // per round: first line per level (then: B after the round)
+    __shared__ uint32_t s_l[NWV][D], s_f[NWV][D];
 
-    __shared__ uint16_t s_run[D][THREADS];   // exclusive scan over runs (local index + 1)
-    __shared__ uint16_t s_agg[D];            // tile aggregate (local index + 1)
-    __shared__ uint32_t s_fin[D];            // incoming from earlier tiles (global index + 1)
-    __shared__ uint16_t s_stack[R][THREADS]; // local stack: (k << 8) | depth
-    // Staged parent / first_child / next_sibling / subtree_end.  Line
-    // rel = run * R + k sits at slot k * (THREADS + 1) + run: lanes writing
-    // their k-th lines hit consecutive banks, and the padding keeps the
-    // line-order copy-out nearly conflict-free.
-    constexpr int SLOTS = R * (THREADS + 1);
-    __shared__ uint32_t s_out[4][SLOTS];
-
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t t = blockIdx.x;
     const uint64_t L = a.lines;
-    const uint32_t tbase = t * TL;
-    const uint32_t rbase = tbase + tid * R;
+    const uint32_t wbase = t * TLINES + w * (TR * 64);
+    uint32_t key[TR];
+    load_keys(a.keys, wbase, L, key);
 
-    uint8_t key[R];
-    load_run_keys<R>(a.keys, rbase, L, key);
-    run_summary<THREADS, R, D>(key, s_run);
-    if (tid < D)
-        s_fin[tid] = a.incoming[(size_t)t * D + tid];
-    for (int i = tid; i < 4 * SLOTS; i += THREADS)
-        (&s_out[0][0])[i] = TREE_UNSET;
-    __syncthreads();
-    run_scan<THREADS, D>(s_run, s_agg);
-    __syncthreads();
-
-    uint32_t *const outs[4] = {a.parent, a.first_child, a.next_sibling, a.subtree_end};
-    // Lines of this tile are staged in LDS and written out coalesced at the end;
-    // lines of earlier tiles (closed from here) are written straight to memory.
-    auto slot = [](uint32_t rel) -> uint32_t { return (rel % R) * (THREADS + 1) + rel / R; };
-    auto put = [&](int which, uint32_t line, uint32_t v) {
-        uint32_t rel = line - tbase;
-        if (rel < (uint32_t)TL)
-            s_out[which][slot(rel)] = v;
-        else if (outs[which])
-            outs[which][line] = v;
-    };
-    enum { P = 0, FC = 1, NS = 2, SE = 3 };
-
-    // Incoming F of this run at level d, as global index + 1 (0 = none).
-    auto fin = [&](int d) -> uint32_t {
-        uint32_t l = s_run[d][tid];
-        return l ? tbase + l : s_fin[d];
-    };
-    // Top of the incoming stack among levels < vb: (depth, index + 1).
-    auto vtop = [&](int vb, uint32_t &vd, uint32_t &vj) {
-        vd = 0;
-        vj = 0;
-        for (int d = 0; d < vb; d++) {
-            uint32_t f = fin(d);
-            if (f > vj) {
-                vj = f;
-                vd = (uint32_t)d;
-            }
-        }
-    };
-
-    int vb = D;
-    uint32_t cvd, cvj;
-    vtop(vb, cvd, cvj);
-    int sp = 0;
+    // pass 1: masks and per-round first line per level
+    uint32_t last = 0, first = OL_NONE;
 #pragma unroll
-    for (int k = 0; k < R; k++) {
-        const uint32_t g = rbase + k;
-        if (g >= L)
-            break;
-        const uint32_t x = key[k];
-        if (x == KEY_BLANK) {
-            put(P, g, OL_NONE);
-            put(FC, g, OL_NONE);
-            put(NS, g, OL_NONE);
-            put(SE, g, g + 1);
-            continue;
+    for (int r = 0; r < TR; r++) {
+        const uint64_t my = level_masks(key[r]);
+        if (lane <= D)
+            s_m[w][r][lane] = my;
+        const uint64_t up = __shfl_down(my, 1, 64);
+        const uint64_t eq = up & ~my;
+        const uint32_t rb = wbase + r * 64;
+        uint32_t f = OL_NONE;
+        if (lane < D && eq) {
+            last = rb + 64 - (uint32_t)__clzll(eq);
+            f = rb + (uint32_t)__builtin_ctzll(eq);
+            if (first == OL_NONE)
+                first = f;
         }
-        if (x >= (uint32_t)D)
-            continue;   // too deep: flagged by the summary pass, the build fails
-        // previous content line: the stack top
-        uint32_t p, pd;
-        if (sp > 0) {
-            uint32_t e = s_stack[sp - 1][tid];
-            p = rbase + (e >> 8);
-            pd = e & 0xFF;
-        } else {
-            p = cvj ? cvj - 1 : OL_NONE;
-            pd = cvd;
-        }
-        if (p != OL_NONE)
-            put(FC, p, (pd >= x) ? OL_NONE : g);
-        uint32_t pend = OL_NONE;
-        while (sp > 0) {
-            uint32_t e = s_stack[sp - 1][tid];
-            if ((e & 0xFF) < x)
-                break;
-            uint32_t jj = rbase + (e >> 8);
-            put(SE, jj, g);
-            if (pend != OL_NONE)
-                put(NS, pend, OL_NONE);
-            pend = jj;
-            sp--;
-        }
-        uint32_t par;
-        if (sp > 0) {
-            par = rbase + (s_stack[sp - 1][tid] >> 8);
-        } else {
-            while (cvj && cvd >= x) {
-                uint32_t jj = cvj - 1;
-                put(SE, jj, g);
-                if (pend != OL_NONE)
-                    put(NS, pend, OL_NONE);
-                pend = jj;
-                vb = (int)cvd;
-                vtop(vb, cvd, cvj);
-            }
-            if (vb > (int)x)
-                vb = (int)x;   // the cached top (depth < x) stays valid
-            par = cvj ? cvj - 1 : OL_NONE;
-        }
-        if (pend != OL_NONE)
-            put(NS, pend, g);
-        put(P, g, par);
-        s_stack[sp][tid] = (uint16_t)((k << 8) | x);
-        sp++;
+        if (lane < D)
+            s_b[w][r][lane] = f;
+    }
+    if (lane < D) {
+        s_l[w][lane] = last;
+        s_f[w][lane] = first;
     }
     __syncthreads();
 
-    // The last tile closes what is still open at the end of the document.
-    if (t == a.num_tiles - 1 && tid == 0) {
-        auto ffin = [&](int d) -> uint32_t { return s_agg[d] ? tbase + s_agg[d] : s_fin[d]; };
-        int b = D;
-        bool first = true;
-        while (true) {
-            uint32_t vd = 0, vj = 0;
-            for (int d = 0; d < b; d++) {
-                uint32_t f = ffin(d);
-                if (f > vj) {
-                    vj = f;
-                    vd = (uint32_t)d;
-                }
-            }
-            if (!vj)
-                break;
-            uint32_t jj = vj - 1;
-            put(SE, jj, (uint32_t)L);
-            put(NS, jj, OL_NONE);
-            if (first)
-                put(FC, jj, OL_NONE);
-            first = false;
-            b = (int)vd;
+    // states at the wave's first and last line
+    uint32_t F = 0, Bw = OL_NONE;
+    if (lane < D) {
+        F = a.in_last[(size_t)t * D + lane];
+        Bw = a.out_first[(size_t)t * D + lane];
+        for (int k = 0; k < w; k++)
+            F = umax(F, s_l[k][lane]);
+        for (int k = w + 1; k < NWV; k++)
+            Bw = umin(Bw, s_f[k][lane]);
+        // B after each round (suffix over the later rounds of this wave)
+        uint32_t bc = Bw;
+        for (int r = TR - 1; r >= 0; r--) {
+            const uint32_t f = s_b[w][r][lane];
+            s_b[w][r][lane] = bc;
+            bc = umin(bc, f);
         }
     }
-    __syncthreads();
 
-    // Coalesced copy-out of the staged slots this tile wrote.
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t above = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
+    uint32_t *const P = a.parent, *const FC = a.first_child, *const NS = a.next_sibling, *const SE = a.subtree_end;
+
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-        uint32_t *dst = outs[w];
-        if (!dst)
-            continue;
-        for (int i = tid; i < TL; i += THREADS) {
-            uint32_t v = s_out[w][slot((uint32_t)i)];
-            if (v != TREE_UNSET && tbase + i < L)
-                dst[tbase + i] = v;
+    for (int r = 0; r < TR; r++) {
+        const uint32_t rb = wbase + r * 64;
+        const uint32_t x = key[r];
+        // exclusive prefix max over levels of F (value = line + 1, with its level)
+        uint32_t pv = (lane < D) ? F : 0u, pl = (uint32_t)lane;
+        // inclusive prefix min over levels of B after this round
+        uint32_t sv = (lane < D) ? s_b[w][r][lane] : OL_NONE, sl = (uint32_t)lane;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t pv2 = __shfl_up(pv, o, 64), pl2 = __shfl_up(pl, o, 64);
+            const uint32_t sv2 = __shfl_up(sv, o, 64), sl2 = __shfl_up(sl, o, 64);
+            if (lane >= o && pv2 > pv) {
+                pv = pv2;
+                pl = pl2;
+            }
+            if (lane >= o && sv2 < sv) {
+                sv = sv2;
+                sl = sl2;
+            }
+        }
+        uint32_t pfx = __shfl_up(pv, 1, 64), pfl = __shfl_up(pl, 1, 64);         // max over d < lane
+        if (lane == 0)
+            pfx = 0;
+        const uint32_t nxt = __shfl(sv, D - 1, 64), nxtl = __shfl(sl, D - 1, 64); // first content after the round
+
+        const bool valid = x < (uint32_t)D;
+        const uint32_t xi = valid ? x : 0u;
+        const uint64_t mlt = s_m[w][r][xi], mle = s_m[w][r][xi + 1];
+        const uint64_t pm = mlt & below, sm = mle & above;
+        const uint64_t fm = __ballot(x != KEY_BLANK) & above;
+        const uint32_t jp = pm ? 63u - (uint32_t)__clzll(pm) : 0u;
+        const uint32_t js = sm ? (uint32_t)__builtin_ctzll(sm) : 0u;
+        const uint32_t jf = fm ? (uint32_t)__builtin_ctzll(fm) : 0u;
+        const uint32_t kp = __shfl(x, jp, 64), ks = __shfl(x, js, 64), kf = __shfl(x, jf, 64);
+        const uint32_t opv = __shfl(pfx, xi, 64), opl = __shfl(pfl, xi, 64);
+        const uint32_t osv = __shfl(sv, xi, 64), osl = __shfl(sl, xi, 64);
+
+        const uint32_t g = rb + lane;
+        uint32_t par, se, fc, ns;
+        if (x == KEY_BLANK || !valid) {
+            par = OL_NONE;
+            se = g + 1;
+            fc = OL_NONE;
+            ns = OL_NONE;
+        } else {
+            par = pm ? rb + jp : (opv ? opv - 1 : OL_NONE);
+            const uint32_t plv = pm ? kp : opl;
+            se = sm ? rb + js : (osv != OL_NONE ? osv : (uint32_t)L);
+            const uint32_t slv = sm ? ks : osl;
+            const uint32_t fi = fm ? rb + jf : nxt;
+            const uint32_t fl = fm ? kf : nxtl;
+            fc = (fi != OL_NONE && fl > x) ? fi : OL_NONE;
+            ns = (se < L && (par == OL_NONE || plv < slv)) ? se : OL_NONE;
+        }
+        if (g < L) {
+            if (P) P[g] = par;
+            if (SE) SE[g] = se;
+            if (FC) FC[g] = fc;
+            if (NS) NS[g] = ns;
+        }
+        // F after this round: the last line per level
+        if (lane < D) {
+            const uint64_t eq = s_m[w][r][lane + 1] & ~s_m[w][r][lane];
+            if (eq)
+                F = rb + 64 - (uint32_t)__clzll(eq);
         }
     }
 }
 
-template <int THREADS>
-static void launch_tree_cfg(const TreeArgs &a, hipStream_t s)
+hipError_t ol_launch_tree(const TreeArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((tree_summary_kernel<THREADS, TREE_RUN, TREE_D>), dim3(a.num_tiles), dim3(THREADS), 0, s, a);
-    hipLaunchKernelGGL((tree_scan_kernel<1024>), dim3(TREE_D), dim3(1024), 0, s, a, (uint32_t)TREE_D);
-    hipLaunchKernelGGL((tree_walk_kernel<THREADS, TREE_RUN, TREE_D>), dim3(a.num_tiles), dim3(THREADS), 0, s, a);
-}
-
-hipError_t ol_launch_tree(const TreeArgs &a, uint32_t threads, hipStream_t s)
-{
-    switch (threads) {
-    case 64:
-        launch_tree_cfg<64>(a, s);
-        break;
-    case 128:
-        launch_tree_cfg<128>(a, s);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(tree_summary_kernel, dim3(a.num_tiles), dim3(TW), 0, s, a);
+    hipLaunchKernelGGL((tree_scan_kernel<1024>), dim3(D), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(tree_walk_kernel, dim3(a.num_tiles), dim3(TW), 0, s, a);
     return hipGetLastError();
 }
+
+uint32_t ol_tree_tile_lines() { return TLINES; }
