@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256 << 20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after timing, copy 1 GiB device-to-device twice (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -115,6 +118,7 @@ def main():
     k_avg_ms = k_ms / max(1, k_launch)
     k_bytes = n + 5 * L
     k_gbs = k_bytes / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
+    traffic = load_traffic(kname, n)
     kernels = {k: {"launches": c, "avg_us": round(t / max(1, c) * 1e3, 2)} for k, (c, t) in prof.items()}
     log(f"[bench] {ms:.3f} ms/step, {value:.1f} GB/s; kernels {kernels}")
 
@@ -128,18 +132,30 @@ def main():
     value2 = alg_bytes / (dt2 / args.steps) / 1e9
     log(f"[bench] without events: {ms2:.3f} ms/step, {value2:.1f} GB/s")
 
+    if args.copy_calibration:
+        copy_calibration(1 << 30)
+
     # CPU baseline: the oracle on a bounded sample of the same document.
-    from oracle import pyoracle as orc
-    sample = text[: min(args.cpu_sample, n)]
-    last_nl = int(np.flatnonzero(sample == 0x0A)[-1]) + 1 if sample.size else 0
-    sample = sample[:last_nl]
-    t0 = time.perf_counter()
-    so, sn = orc.parse(sample)
-    orc.links(sn)
-    cpu_s = time.perf_counter() - t0
-    cpu_bytes = sample.size + 17 * so.size
-    cpu_gbs = cpu_bytes / cpu_s / 1e9
-    log(f"[bench] cpu oracle: {sample.size} B in {cpu_s:.2f}s = {cpu_gbs:.3f} GB/s")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+        sample = text[: min(args.cpu_sample, n)]
+        last_nl = int(np.flatnonzero(sample == 0x0A)[-1]) + 1 if sample.size else 0
+        sample = sample[:last_nl]
+        t0 = time.perf_counter()
+        so, sn = orc.parse(sample)
+        orc.links(sn)
+        cpu_s = time.perf_counter() - t0
+        cpu_bytes = sample.size + 17 * so.size
+        cpu_gbs = cpu_bytes / cpu_s / 1e9
+        log(f"[bench] cpu oracle: {sample.size} B in {cpu_s:.2f}s = {cpu_gbs:.3f} GB/s")
+        cpu = {
+            "value": round(cpu_gbs, 3),
+            "unit": "GB/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"first {sample.size} bytes of the same document: oracle parse + sequential tree",
+        }
 
     extra = {}
     if not args.no_search:
@@ -175,21 +191,47 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(k_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic.get("bytes_per_launch"),
+            "traffic_source": traffic.get("source"),
             "kernel": kname,
             "kernel_avg_us": round(k_avg_ms * 1e3, 2),
             "kernel_algorithmic_bytes": k_bytes,
         },
-        "cpu_baseline": {
-            "value": round(cpu_gbs, 3),
-            "unit": "GB/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"first {sample.size} bytes of the same document: oracle parse + sequential tree",
-        },
+        "cpu_baseline": cpu,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def load_traffic(kname, n):
+    """HBM bytes per launch of `kname` as measured by rocprofv3 counters
+    (FETCH_SIZE, WRITE_SIZE, scaled by a device copy of known size), kept in
+    profiles/traffic.json; empty if absent or measured on another size."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kname)
+    except (OSError, ValueError):
+        return {}
+    if not rec or rec.get("text_bytes") != n:
+        return {}
+    return {"bytes_per_launch": rec["bytes_per_launch"], "source": f"profiles/traffic.json ({rec['profile']})"}
+
+
+def copy_calibration(nbytes):
+    """Two device-to-device copies of `nbytes`: a plain copy kernel of known
+    size whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: 2 x {nbytes} bytes device-to-device")
 
 
 def search_benchmarks(p, doc, text, args):
@@ -209,28 +251,33 @@ def search_benchmarks(p, doc, text, args):
         assert hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(max(16, nbytes))) == 0
         return ptr
 
-    # literal: a 16-byte phrase of vocabulary words
-    pat = datagen.literal(16, kind=0, seed=0xC3A)
-    tot = ctypes.c_uint64()
-    ol._check(lib.ol_search_device(p._ctx, doc._h, pat, len(pat), None, 0, ctypes.byref(tot)))
-    cap = tot.value
-    d_off = dalloc(cap * 8)
-    reps = 5
-    p.profile_enable(True)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ol._check(lib.ol_search_device(p._ctx, doc._h, pat, len(pat), d_off, cap, ctypes.byref(tot)))
-    dt = (time.perf_counter() - t0) / reps
-    p.profile_enable(False)
-    prof = p.profile_take()
-    k = prof.get("search", (1, 0.0))
-    kms = k[1] / max(1, k[0])
-    b = n + 8 * tot.value
-    res["literal_search"] = {"pattern_len": len(pat), "matches": tot.value, "call_ms": round(dt * 1e3, 3),
-                             "kernel_ms": round(kms, 4), "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1),
-                             "frac_of_hbm_peak": round(b / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
-    hip.hipFree(d_off)
-    log(f"[bench] literal {res['literal_search']}")
+    # literal (C3a): lengths 4..64, vocabulary phrases (kind 0) and random lowercase (kind 1)
+    lits = []
+    for kind in (0, 1):
+        for ln in (4, 8, 16, 32, 64):
+            pat = datagen.literal(ln, kind=kind, seed=0xC3A)
+            tot = ctypes.c_uint64()
+            ol._check(lib.ol_search_device(p._ctx, doc._h, pat, len(pat), None, 0, ctypes.byref(tot)))
+            cap = tot.value
+            d_off = dalloc(cap * 8)
+            reps = 3
+            p.profile_enable(True)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ol._check(lib.ol_search_device(p._ctx, doc._h, pat, len(pat), d_off, cap, ctypes.byref(tot)))
+            dt = (time.perf_counter() - t0) / reps
+            p.profile_enable(False)
+            prof = p.profile_take()
+            k = prof.get("search", (1, 0.0))
+            kms = k[1] / max(1, k[0])
+            b = n + 8 * tot.value
+            lits.append({"pattern": pat.decode(), "kind": ("vocabulary", "random")[kind], "matches": tot.value,
+                         "call_ms": round(dt * 1e3, 3), "kernel_ms": round(kms, 4),
+                         "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1),
+                         "frac_of_hbm_peak": round(b / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+            hip.hipFree(d_off)
+            log(f"[bench] literal {lits[-1]}")
+    res["literal_search"] = lits
 
     for count in (1024, 65536):
         pats = datagen.patterns(count, seed=0xC3B)

@@ -153,15 +153,15 @@ struct Prof {
     }
 };
 
-hipError_t ensure_scratch(ol_ctx *c, size_t status_words)
+hipError_t ensure_scratch(ol_ctx *c, size_t tiles)
 {
-    return c->scratch.ensure((SCR_STATUS + status_words + 64) * 8);
+    return c->scratch.ensure((SCR_STATUS + tiles * LB_STRIDE + 64) * 8);
 }
 
-hipError_t clear_scratch(ol_ctx *c, size_t status_words)
+hipError_t clear_scratch(ol_ctx *c, size_t tiles)
 {
     Prof p(c, "clear");
-    return ol_launch_clear_u64((uint64_t *)c->scratch.p, SCR_STATUS + status_words, c->stream);
+    return ol_launch_clear_u64((uint64_t *)c->scratch.p, SCR_STATUS + tiles * LB_STRIDE, c->stream);
 }
 
 int read_meta(ol_ctx *c, uint64_t *out)
@@ -468,6 +468,7 @@ int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, ui
     a.out_id = out_id;
     a.status = c->status();
     a.meta = c->meta();
+    a.tile_ctr = c->ctr() + 3;
     {
         Prof p(c, "many");
         HIPCHK(ol_launch_many(a, c->stream));

@@ -67,32 +67,55 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 // reads one 64-tile window, keeps every ready entry up to the first
 // not-ready one, and then polls only that tile, with one lane and a sleep,
 // instead of re-reading the window.
+// Status word of tile t is status[t * LB_STRIDE]: spreading the words of
+// neighbouring tiles over memory channels.
+#ifndef LB_STRIDE
+#define LB_STRIDE 8
+#endif
+
+template <int NWIN = 1>
 __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_t t)
 {
     const int lane = threadIdx.x & 63;
     uint64_t prefix = 0;
     int64_t top = (int64_t)t - 1;
     while (true) {
-        const int64_t idx = top - lane;
-        const uint64_t st = (idx >= 0) ? ld_relaxed_u64(status + idx) : (LB_FLAG_INCL << LB_SHIFT);
-        const uint64_t flag = st >> LB_SHIFT;
-        const uint64_t incl = __ballot(flag == LB_FLAG_INCL);
-        const uint64_t notready = __ballot(flag == 0);
-        const int fi = incl ? (int)__builtin_ctzll(incl) : 64;
-        const int fn = notready ? (int)__builtin_ctzll(notready) : 64;
-        if (fn < fi) {
-            prefix += wave_sum_u64((lane < fn) ? (st & LB_MASK) : 0);
-            top -= fn;
-            if (lane == 0) {
-                while ((ld_relaxed_u64(status + top) >> LB_SHIFT) == 0)
-                    __builtin_amdgcn_s_sleep(8);
-            }
-            continue;
+        // NWIN windows of 64 status words in flight at once: window k covers
+        // top - 64k - 63 ..= top - 64k, lane l reading top - 64k - l.
+        uint64_t st[NWIN];
+#pragma unroll
+        for (int k = 0; k < NWIN; k++) {
+            const int64_t idx = top - 64 * k - lane;
+            st[k] = (idx >= 0) ? ld_relaxed_u64(status + idx * LB_STRIDE) : (LB_FLAG_INCL << LB_SHIFT);
         }
-        prefix += wave_sum_u64((lane <= fi) ? (st & LB_MASK) : 0);
-        if (fi < 64)
+        bool done = false, waited = false;
+#pragma unroll
+        for (int k = 0; k < NWIN; k++) {
+            const uint64_t flag = st[k] >> LB_SHIFT;
+            const uint64_t incl = __ballot(flag == LB_FLAG_INCL);
+            const uint64_t notready = __ballot(flag == 0);
+            const int fi = incl ? (int)__builtin_ctzll(incl) : 64;
+            const int fn = notready ? (int)__builtin_ctzll(notready) : 64;
+            if (fn < fi) {
+                prefix += wave_sum_u64((lane < fn) ? (st[k] & LB_MASK) : 0);
+                top -= 64 * k + fn;
+                if (lane == 0) {
+                    while ((ld_relaxed_u64(status + top * LB_STRIDE) >> LB_SHIFT) == 0)
+                        __builtin_amdgcn_s_sleep(8);
+                }
+                waited = true;
+                break;
+            }
+            prefix += wave_sum_u64((lane <= fi) ? (st[k] & LB_MASK) : 0);
+            if (fi < 64) {
+                done = true;
+                break;
+            }
+        }
+        if (done)
             break;
-        top -= 64;
+        if (!waited)
+            top -= 64 * NWIN;
     }
     return prefix;
 }
@@ -170,6 +193,7 @@ struct ManyArgs {
     uint32_t *out_id;        // pattern ids (nullable)
     uint64_t *status;
     uint64_t *meta;          // meta[0] <- total matches
+    uint32_t *tile_ctr;      // tile ticket, zero at launch
 };
 
 // ---- launchers (each returns the launch error, if any) ---------------------
@@ -187,9 +211,12 @@ hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
 
 // Tile geometry of each kernel.
 constexpr uint32_t SCAN_THREADS = 256;
-constexpr uint32_t SCAN_BPT = 256;                          // default bytes per lane (64 KiB tiles)
+constexpr uint32_t SCAN_BPT = 128;                          // default bytes per lane (32 KiB tiles)
 constexpr uint32_t SCAN_TILE_MAX = SCAN_THREADS * 256;
 constexpr uint32_t SCAN_HALO = 256;
+#ifndef SCAN_LB_WINDOWS
+#define SCAN_LB_WINDOWS 1                                  // look-back windows of 64 tiles read per step
+#endif
 constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks

@@ -282,251 +282,276 @@ __device__ __forceinline__ bool decode_head(const uint8_t *s_text, uint32_t rs, 
     return true;
 }
 
-// One tile = Q quarters of THREADS x 64 bytes.  The tile is loaded into
-// registers (16-byte vectors, coalesced); its '\n' masks go to LDS once and
-// serve both the tile count (one look-back per tile) and the lines' positions.
-// The quarters are then staged through LDS in turn, each with the 256 bytes
-// before it as a halo, and every lane decodes the lines that end in its 64
-// bytes.  A line that starts before the staged window (longer than the halo)
-// is written later by scan_fixup_kernel from global memory, so the decode loop
-// touches LDS only.
+// One tile = THREADS lanes x 64*Q contiguous bytes.  The tile arrives in
+// registers (coalesced 16-byte vectors), is staged whole in LDS behind a
+// 256-byte halo, and the registers are then refilled with the workgroup's next
+// tile while this one is decoded (software pipelining: the loads of tile t+G
+// are in flight during the look-back, compaction and decode of tile t; wave 0
+// issues its share after the look-back so its status loads do not wait
+// behind them).  Each lane owns 64*Q contiguous bytes: their '\n' masks are
+// counted and scanned once per tile, the line ends are compacted into LDS in
+// byte order, and lane i then decodes line i (balanced, coalesced writes).  A
+// line that starts before the staged window (longer than the halo) is written
+// later by scan_fixup_kernel from global memory, so the decode touches LDS only.
 template <int THREADS, int Q>
 __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
 {
-    constexpr int QB = THREADS * 64;   // bytes per quarter
-    constexpr int TILE = QB * Q;
+    constexpr int LB = 64 * Q;          // bytes per lane
+    constexpr int TILE = THREADS * LB;
     constexpr int HALO = SCAN_HALO;
     constexpr int NW = THREADS / 64;
-    constexpr int NV = Q * 4;          // 16-byte vectors per lane per tile
+    constexpr int NV = LB / 16;         // 16-byte vectors per lane
+    constexpr int LIST = TILE / 16;     // lines per tile decoded from the compacted list
     static_assert(NW >= 2, "needs two waves");
-    static_assert(HALO % 64 == 0 && HALO <= QB, "halo");
+    static_assert(HALO % 16 == 0 && HALO / 16 <= THREADS, "halo");
+    static_assert(TILE <= 65536, "tile positions are u16");
 
-    __shared__ __align__(16) uint8_t s_text[HALO + QB + 64];
-    __shared__ __align__(16) uint8_t s_halo[HALO];      // the 256 bytes before the tile
-    __shared__ __align__(8) uint16_t s_nm[TILE / 16];   // '\n' bits per 16-byte vector, in byte order
-    __shared__ uint32_t s_wcnt[NW][Q];
-    __shared__ int32_t s_wlast[NW][Q];
-    constexpr int LIST = 1024;          // lines per quarter decoded from the compacted list
-    __shared__ uint32_t s_ls[LIST], s_le[LIST];   // line start / terminator of the quarter's lines
+    __shared__ __align__(16) uint8_t s_text[HALO + TILE + 64];
+    __shared__ __align__(16) uint16_t s_nm[TILE / 16];   // '\n' bits per 16-byte vector, byte order
+    __shared__ uint16_t s_le[LIST];                      // tile position of each line end
+    __shared__ uint16_t s_rc[LIST];                      // decoded class | key << 8 (FIX: fixup pass)
+    __shared__ uint32_t s_wcnt[NW];
+    __shared__ int32_t s_wlast[NW];
     __shared__ uint64_t s_prefix;
     __shared__ uint32_t s_first;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t G = gridDim.x;
+    const uint32_t T = a.num_tiles;
     const uint32_t last = a.n - 1;   // always a terminator, '\n' or not
 
-    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += G) {
-        const uint32_t base = t * TILE;
-        uint4 v[NV];
-        uint4 hv = make_uint4(0, 0, 0, 0);
-        {
-            const uint4 *src = (const uint4 *)(a.text + (size_t)base);
+    uint4 v[NV];
+    uint4 hv;
+    auto load_tile = [&](uint32_t tt) {
+        const uint32_t b = tt * TILE;
+        const uint4 *src = (const uint4 *)(a.text + (size_t)b);
 #pragma unroll
-            for (int k = 0; k < NV; k++)
-                v[k] = src[k * THREADS + tid];
-            if (t > 0 && tid < HALO / 16)
-                hv = ((const uint4 *)(a.text + base - HALO))[tid];
-        }
-        if (tid < HALO / 16)
-            ((uint4 *)s_halo)[tid] = hv;
+        for (int k = 0; k < NV; k++)
+            v[k] = src[k * THREADS + tid];
+        hv = make_uint4(0, 0, 0, 0);
+        if (tt > 0 && tid < HALO / 16)
+            hv = ((const uint4 *)(a.text + b - HALO))[tid];
+    };
+
+    uint32_t t = blockIdx.x;
+    if (t < T)
+        load_tile(t);
+    for (; t < T; t += G) {
+        const uint32_t base = t * TILE;
+        const uint32_t tn = t + G;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             const uint32_t m = nl_nibble(v[k].x) | nl_nibble(v[k].y) << 4 | nl_nibble(v[k].z) << 8 |
                                nl_nibble(v[k].w) << 12;
             s_nm[k * THREADS + tid] = (uint16_t)m;
+            ((uint4 *)(s_text + HALO))[k * THREADS + tid] = v[k];
         }
-        if (last >= base && last < base + TILE) {   // the tile holding the text's last byte
+        if (tid < HALO / 16)
+            ((uint4 *)s_text)[tid] = hv;
+        if (wid != 0 && tn < T)
+            load_tile(tn);
+        if (last >= base && last - base < (uint32_t)TILE) {   // the tile holding the text's last byte
             __syncthreads();
             if (tid == 0)
                 s_nm[(last - base) >> 4] |= (uint16_t)(1u << ((last - base) & 15));
         }
         __syncthreads();
 
-        // Lane-contiguous 64-byte masks of each quarter, counted and scanned
-        // together (one block scan per tile).
+        // this lane's bytes [tid*LB, tid*LB + LB): masks, count, last terminator
         uint64_t mk[Q];
-        uint32_t cnt[Q], ic[Q];
-        int32_t il[Q], prev_l[Q];
+        uint32_t cnt = 0;
+        int32_t il = -1;
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            mk[q] = *(const uint64_t *)(s_nm + q * (QB / 16) + 4 * tid);
-            cnt[q] = (uint32_t)__popcll(mk[q]);
-            ic[q] = cnt[q];
-            il[q] = mk[q] ? (int32_t)(tid * 64 + 63 - __clzll(mk[q])) : -1;
+            mk[q] = ((const uint64_t *)s_nm)[tid * Q + q];
+            cnt += (uint32_t)__popcll(mk[q]);
+            if (mk[q])
+                il = (int32_t)(tid * LB + 64 * q + 63 - __clzll(mk[q]));
         }
+        uint32_t ic = cnt;
+        int32_t ilm = il;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t cc = __shfl_up(ic, o, 64);
+            const int32_t l = __shfl_up(ilm, o, 64);
+            if (lane >= o) {
+                ic += cc;
+                ilm = ilm > l ? ilm : l;
+            }
+        }
+        int32_t prev_l = __shfl_up(ilm, 1, 64);
+        if (lane == 0)
+            prev_l = -1;
+        if (lane == 63) {
+            s_wcnt[wid] = ic;
+            s_wlast[wid] = ilm;
+        }
+        __syncthreads();
+        uint32_t total = 0, wb = 0;
+        int32_t wl = -1;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const uint32_t c = s_wcnt[w];
+            total += c;
+            if (w < wid) {
+                wb += c;
+                wl = wl > s_wlast[w] ? wl : s_wlast[w];
+            }
+        }
+        const uint32_t excl_c = wb + ic - cnt;
+        const int32_t excl_l = wl > prev_l ? wl : prev_l;
+
+        const bool listed = total <= (uint32_t)LIST;
+        const bool staged = listed && !a.nodes;   // decode overlapped with the look-back
+
+        // Publish the tile's count at once, then compact the line ends.
+        if (wid == 0 && lane == 0)
+            st_relaxed_u64(a.status + (size_t)t * LB_STRIDE, ((t == 0 ? LB_FLAG_INCL : LB_FLAG_AGG) << LB_SHIFT) | total);
+        if (listed) {
+            uint32_t at = excl_c;
 #pragma unroll
             for (int q = 0; q < Q; q++) {
-                uint32_t cc = __shfl_up(ic[q], o, 64);
-                int32_t l = __shfl_up(il[q], o, 64);
-                if (lane >= o) {
-                    ic[q] += cc;
-                    il[q] = il[q] > l ? il[q] : l;
+                uint64_t m = mk[q];
+                while (m) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1;
+                    s_le[at++] = (uint16_t)(tid * LB + 64 * q + bit);
                 }
             }
         }
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            prev_l[q] = __shfl_up(il[q], 1, 64);
-            if (lane == 0)
-                prev_l[q] = -1;
-            if (lane == 63) {
-                s_wcnt[wid][q] = ic[q];
-                s_wlast[wid][q] = il[q];
-            }
-        }
         __syncthreads();
-        uint32_t total = 0;
-#pragma unroll
-        for (int q = 0; q < Q; q++)
-#pragma unroll
-            for (int w = 0; w < NW; w++)
-                total += s_wcnt[w][q];
 
+        uint32_t first_w1 = 0;   // wave 1: start of the tile's first line
         if (wid == 0) {
             uint64_t prefix = 0;
-            if (t == 0) {
+            if (t > 0) {
+                prefix = lookback_sum<SCAN_LB_WINDOWS>(a.status, t);
                 if (lane == 0)
-                    st_relaxed_u64(a.status, (LB_FLAG_INCL << LB_SHIFT) | total);
-            } else {
-                if (lane == 0)
-                    st_relaxed_u64(a.status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
-                prefix = lookback_sum(a.status, t);
-                if (lane == 0)
-                    st_relaxed_u64(a.status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
+                    st_relaxed_u64(a.status + (size_t)t * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
             }
             if (lane == 0)
                 s_prefix = prefix;
+            if (tn < T)
+                load_tile(tn);
         } else if (wid == 1) {
             // Start of the first line that ends in this tile: after the last '\n'
-            // before the tile (in the 256 bytes before it, else further back).
-            uint32_t start = 0;
+            // before the tile (in the halo, else further back).
             if (t > 0 && total > 0) {
-                uint32_t w4 = ((const uint32_t *)s_halo)[lane];
-                uint32_t m = nl_nibble(w4);
-                int32_t best = m ? (int32_t)(4 * lane + 31 - __clz(m)) : -1;
-                uint64_t found = __ballot(best >= 0);
+                const uint32_t w4 = ((const uint32_t *)s_text)[lane];
+                const uint32_t m = nl_nibble(w4);
+                const int32_t best = m ? (int32_t)(4 * lane + 31 - __clz(m)) : -1;
+                const uint64_t found = __ballot(best >= 0);
                 if (found) {
-                    int srcl = 63 - (int)__clzll(found);
-                    int32_t p = __shfl(best, srcl, 64);
-                    start = base - HALO + (uint32_t)p + 1;
+                    const int srcl = 63 - (int)__clzll(found);
+                    const int32_t p = __shfl(best, srcl, 64);
+                    first_w1 = base - HALO + (uint32_t)p + 1;
                 } else {
-                    start = find_line_start_global(a.text, base - HALO);
+                    first_w1 = find_line_start_global(a.text, base - HALO);
                 }
             }
             if (lane == 0)
-                s_first = start;
+                s_first = first_w1;
+        }
+
+        const uint32_t lds_lo = (base >= (uint32_t)HALO) ? base - HALO : 0u;
+        auto lds_byte = [&](uint32_t pos) -> uint32_t { return s_text[pos + (uint32_t)HALO - base]; };
+        // parse_line of text[s ..= e] from LDS (s >= lds_lo)
+        auto decode = [&](uint32_t s, uint32_t e) -> ol_node_dev {
+            uint32_t eff = e + 1;
+            while (eff > s) {
+                const uint32_t c = lds_byte(eff - 1);
+                if (c != '\n' && c != '\r')
+                    break;
+                eff--;
+            }
+            ol_node_dev nd;
+            if (eff == s) {
+                nd.depth = 0; nd.kind = OLK_BLANK; nd.span_start = s; nd.span_len = 0;
+            } else if (!decode_head(s_text, s + (uint32_t)HALO - base, s, eff, nd)) {
+                nd = parse_span(lds_byte, s, e);
+            }
+            return nd;
+        };
+        auto add_fix = [&](uint64_t r) {
+            const uint32_t slot = atomicAdd(a.fix_count, 1u);
+            if (slot < a.fix_cap)
+                a.fix_list[slot] = (uint32_t)r;
+        };
+        // Decode line (s, e) and write it at output index r.
+        auto emit = [&](uint64_t r, uint32_t s, uint32_t e) {
+            if (r >= a.cap)
+                return;
+            a.offsets[r] = s;
+            if (s < lds_lo) {   // head not in LDS: decoded by the fixup pass
+                add_fix(r);
+                return;
+            }
+            const ol_node_dev nd = decode(s, e);
+            a.classes[r] = (uint8_t)nd.kind;
+            a.keys[r] = key_of(nd);
+            if (a.nodes)
+                a.nodes[r] = nd;
+        };
+        constexpr uint16_t FIX = 0xFFFF;   // never a class | key << 8 (classes are < 8)
+        if (staged) {
+            // Decode into LDS while wave 0 resolves the prefix.  Line i goes to
+            // thread (i + 64) mod THREADS, so line 0 is wave 1's, which has
+            // just found its start.
+            for (uint32_t i = (tid + THREADS - 64) % THREADS; i < total; i += THREADS) {
+                const uint32_t e = base + s_le[i];
+                const uint32_t s = i ? base + s_le[i - 1] + 1u : first_w1;
+                uint16_t rc = FIX;
+                if (s >= lds_lo) {
+                    const ol_node_dev nd = decode(s, e);
+                    rc = (uint16_t)(nd.kind | (uint32_t)key_of(nd) << 8);
+                }
+                s_rc[i] = rc;
+            }
         }
         __syncthreads();
-        uint64_t running = s_prefix;       // index of the first line of the quarter
-        uint32_t carry = s_first;          // start of the line open at the quarter start
-        if (tid == 0 && t == a.num_tiles - 1)
+        const uint64_t running = s_prefix;   // index of the tile's first line
+        const uint32_t first = s_first;      // start of the tile's first line
+        if (tid == 0 && t == T - 1)
             a.meta[0] = running + total;
 
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const uint32_t qbase = base + q * QB;
-            uint32_t wb = 0, qtot = 0;
-            int32_t wl = -1, ql = -1;
-#pragma unroll
-            for (int w = 0; w < NW; w++) {
-                if (w < wid) {
-                    wb += s_wcnt[w][q];
-                    wl = wl > s_wlast[w][q] ? wl : s_wlast[w][q];
-                }
-                qtot += s_wcnt[w][q];
-                ql = ql > s_wlast[w][q] ? ql : s_wlast[w][q];
-            }
-            const uint32_t excl_c = wb + ic[q] - cnt[q];
-            const int32_t excl_l = wl > prev_l[q] ? wl : prev_l[q];
-
-            // stage the quarter (+ the 256 bytes before it)
-            if (q == 0) {
-                if (tid < HALO / 16)
-                    ((uint4 *)s_text)[tid] = hv;
-            } else {
-                uint4 h;
-                if (tid < HALO / 16)
-                    h = ((const uint4 *)(s_text + QB))[tid];
-                __syncthreads();
-                if (tid < HALO / 16)
-                    ((uint4 *)s_text)[tid] = h;
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                ((uint4 *)(s_text + HALO))[j * THREADS + tid] = v[4 * q + j];
-            __syncthreads();
-
-            const uint32_t lds_lo = (qbase >= (uint32_t)HALO) ? qbase - HALO : 0u;
-            auto lds_byte = [&](uint32_t pos) -> uint32_t { return s_text[pos + (uint32_t)HALO - qbase]; };
-            // Decode line (s, e) and write it at output index r.
-            auto emit = [&](uint64_t r, uint32_t s, uint32_t e) {
+        if (staged) {
+            for (uint32_t i = tid; i < total; i += THREADS) {
+                const uint64_t r = running + i;
                 if (r >= a.cap)
-                    return;
-                a.offsets[r] = s;
-                if (s < lds_lo) {
-                    // head not in LDS: decoded by the fixup pass
-                    uint32_t slot = atomicAdd(a.fix_count, 1u);
-                    if (slot < a.fix_cap)
-                        a.fix_list[slot] = (uint32_t)r;
-                    return;
+                    break;
+                a.offsets[r] = i ? base + s_le[i - 1] + 1u : first;
+                const uint32_t rc = s_rc[i];
+                if (rc == FIX) {
+                    add_fix(r);
+                } else {
+                    a.classes[r] = (uint8_t)(rc & 0xFFu);
+                    a.keys[r] = (uint8_t)(rc >> 8);
                 }
-                uint32_t eff = e + 1;
-                while (eff > s) {
-                    const uint32_t c = lds_byte(eff - 1);
-                    if (c != '\n' && c != '\r')
-                        break;
-                    eff--;
-                }
-                ol_node_dev nd;
-                if (eff == s) {
-                    nd.depth = 0; nd.kind = OLK_BLANK; nd.span_start = s; nd.span_len = 0;
-                } else if (!decode_head(s_text, s + (uint32_t)HALO - qbase, s, eff, nd)) {
-                    nd = parse_span(lds_byte, s, e);
-                }
-                a.classes[r] = (uint8_t)nd.kind;
-                a.keys[r] = key_of(nd);
-                if (a.nodes)
-                    a.nodes[r] = nd;
-            };
-            if (qtot <= (uint32_t)LIST) {
-                // Compact the quarter's lines into LDS, then lane i decodes line
-                // i: balanced work and coalesced output writes.
-                uint32_t sp = (excl_l >= 0) ? qbase + (uint32_t)excl_l + 1 : carry;
-                uint32_t at = excl_c;
+            }
+        } else if (listed) {
+            for (uint32_t i = tid; i < total; i += THREADS) {
+                const uint32_t e = base + s_le[i];
+                const uint32_t s = i ? base + s_le[i - 1] + 1u : first;
+                emit(running + i, s, e);
+            }
+        } else if (cnt) {
+            // dense tile: each lane decodes the lines ending in its bytes
+            uint64_t r = running + excl_c;
+            uint32_t sp = (excl_l >= 0) ? base + (uint32_t)excl_l + 1 : first;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
                 uint64_t m = mk[q];
                 while (m) {
                     const int bit = __builtin_ctzll(m);
                     m &= m - 1;
-                    const uint32_t e = qbase + tid * 64 + (uint32_t)bit;
-                    s_ls[at] = sp;
-                    s_le[at] = e;
-                    at++;
-                    sp = e + 1;
-                }
-                __syncthreads();
-                for (uint32_t i = tid; i < qtot; i += THREADS)
-                    emit(running + i, s_ls[i], s_le[i]);
-            } else if (cnt[q]) {
-                // dense quarter: each lane decodes the lines ending in its bytes
-                uint64_t r = running + excl_c;
-                uint32_t sp = (excl_l >= 0) ? qbase + (uint32_t)excl_l + 1 : carry;
-                uint64_t m = mk[q];
-                while (m) {
-                    const int bit = __builtin_ctzll(m);
-                    m &= m - 1;
-                    const uint32_t e = qbase + tid * 64 + (uint32_t)bit;
+                    const uint32_t e = base + tid * LB + 64 * q + (uint32_t)bit;
                     emit(r, sp, e);
                     sp = e + 1;
                     r++;
                 }
             }
-            running += qtot;
-            if (ql >= 0)
-                carry = qbase + (uint32_t)ql + 1;
-            __syncthreads();   // the next quarter (or tile) overwrites LDS
         }
+        __syncthreads();   // the next tile overwrites LDS
     }
 }
 

@@ -69,10 +69,10 @@ __device__ __forceinline__ uint64_t tile_prefix(uint64_t *status, uint32_t t, ui
                 st_relaxed_u64(status, (LB_FLAG_INCL << LB_SHIFT) | total);
         } else {
             if (lane == 0)
-                st_relaxed_u64(status + t, (LB_FLAG_AGG << LB_SHIFT) | total);
+                st_relaxed_u64(status + (size_t)t * LB_STRIDE, (LB_FLAG_AGG << LB_SHIFT) | total);
             prefix = lookback_sum(status, t);
             if (lane == 0)
-                st_relaxed_u64(status + t, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
+                st_relaxed_u64(status + (size_t)t * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
         }
         if (lane == 0)
             *s_prefix = prefix;
@@ -284,10 +284,21 @@ __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
     uint8_t *s_text = (uint8_t *)s_dyn;                       // TILE + HALO_AFTER bytes
     uint32_t *s_filter = s_dyn + (TILE + HALO_AFTER) / 4;   // fwords
 
+    __shared__ uint32_t s_tile;
+
     const int tid = threadIdx.x;
     for (uint32_t i = tid; i < a.d.fwords; i += THREADS)
         s_filter[i] = a.d.filter[i];
-    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
+    // Tiles are claimed in order from a ticket: a tile's look-back then only
+    // waits on tiles already claimed by running workgroups, whatever the
+    // residency of the launch (its LDS depends on the dictionary).
+    while (true) {
+        if (tid == 0)
+            s_tile = atomicAdd(a.tile_ctr, 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= a.num_tiles)
+            break;
         const uint32_t base = t * TILE;
         load_tile<THREADS, BPT>(s_text, a.text, base);
         __syncthreads();
@@ -345,7 +356,6 @@ hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
     if (!cap)
         return hipErrorInvalidValue;
     uint32_t grid = a.num_tiles < cap ? a.num_tiles : cap;
-    ManyArgs args = a;
-    void *params[] = {&args};
-    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SEARCH_THREADS), params, lds, s);
+    hipLaunchKernelGGL((many_kernel<SEARCH_THREADS, SEARCH_BPT>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
+    return hipGetLastError();
 }

@@ -120,3 +120,23 @@ def test_start_before_text_and_saturation(parser):
     text = b"bcdq" + b"\0" * 5000 + b"xyz"
     assert_many_matches(parser, text, [b"\0bcdq", b"zzzz"], "start before the text")
     assert_many_matches(parser, b"a" * (1 << 18), [b"aaaa", b"a", b"aa", b"zq001x"], "saturated")
+
+
+def test_many_tiles_per_workgroup_equal_single_pattern_searches(parser):
+    """64 MiB (4,096 tiles, more than one per resident workgroup): each
+    pattern's hits equal its own literal search (the reference's
+    many_patterns_match_single_pattern_search, crates/oneline-simd/src/search.rs:290-312)."""
+    text = datagen.document(64 << 20, seed=5, crlf=10, empty=5)
+    pats = datagen.patterns(48, seed=0xC3B) + [datagen.vocab_word(i) for i in (0, 1, 2, 7, 50, 999)]
+    doc = parser.upload(text, nodes=False)
+    pset = parser.compile_patterns(doc, pats)
+    per = [parser.search_document(doc, p, 1 << 26) for p in pats]
+    want_off = np.concatenate([np.asarray(r.offsets, np.uint64) for r in per])
+    want_id = np.concatenate([np.full(len(r.offsets), i, np.uint32) for i, r in enumerate(per)])
+    order = np.lexsort((want_id, want_off))
+    want_off, want_id = want_off[order], want_id[order]
+    assert want_off.size > 100_000
+    tot, off, ids = device_list(parser, doc, pset, want_off.size)
+    assert tot == want_off.size
+    assert np.array_equal(off, want_off)
+    assert np.array_equal(ids, want_id)
