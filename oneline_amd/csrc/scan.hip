@@ -56,6 +56,20 @@ __device__ __forceinline__ uint32_t nl_nibble(uint32_t w)
     return ((z >> 7) | (z >> 14) | (z >> 21) | (z >> 28)) & 0xFu;
 }
 
+// 16-bit mask of the '\n' bytes of a 16-byte vector (bit i = byte i): an exact
+// SWAR zero-byte test per word leaves 0x80 in each matching byte, and u8 dot
+// products with the bit weights gather those flags, two words per result.
+__device__ __forceinline__ uint32_t nl_mask16(const uint4 &v)
+{
+    auto z = [](uint32_t w) {
+        const uint32_t x = w ^ 0x0A0A0A0Au;
+        return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+    };
+    const uint32_t lo = __builtin_amdgcn_udot4(z(v.y), 0x80402010u, __builtin_amdgcn_udot4(z(v.x), 0x08040201u, 0u, false), false);
+    const uint32_t hi = __builtin_amdgcn_udot4(z(v.w), 0x80402010u, __builtin_amdgcn_udot4(z(v.z), 0x08040201u, 0u, false), false);
+    return (lo >> 7) | ((hi >> 7) << 8);
+}
+
 __device__ __forceinline__ bool is_blank(uint32_t b) { return b == ' ' || b == '\t'; }
 
 // classify_line, crates/oneline-simd/src/line.rs:59-81.  c1/c2 are the second and
@@ -282,17 +296,56 @@ __device__ __forceinline__ bool decode_head(const uint8_t *s_text, uint32_t rs, 
     return true;
 }
 
+// Class and tree key of the line text[s ..= e] (e its terminator) staged in
+// LDS at byte rs, for the common shape: a prefix of "| " pairs and then at
+// least three content bytes before any '\r'/'\n', or an empty line ("\n",
+// "\r\n", possibly after a "| " prefix).  Only these need no trailing trim:
+// classify (crates/oneline-simd/src/line.rs:59-81) reads the first three
+// content bytes, and when none of them is '\r'/'\n' the content reaches past
+// them.  Returns false for every other line (the general decoder handles it).
+__device__ __forceinline__ bool decode_common(const uint8_t *s_text, uint32_t rs, uint32_t s, uint32_t e,
+                                              uint32_t &rc)
+{
+    uint32_t A[10];
+    load_words<10>(s_text, rs, A);
+    uint32_t eq = 0;
+#pragma unroll
+    for (int j = 0; j < 10; j++)
+        eq |= (uint32_t)(A[j] == 0x207C207Cu) << j;   // "| | "
+    const uint32_t k = (uint32_t)__builtin_ctz(~eq);   // whole "| | " words (a '\n' ends the run)
+    if (k >= 10)
+        return false;
+    uint32_t T[2];
+    load_words<2>(s_text, rs + 4 * k, T);
+    const bool half = (T[0] & 0xFFFFu) == 0x207Cu;     // one more "| "
+    const uint32_t x = half ? __builtin_amdgcn_alignbyte(T[1], T[0], 2) : T[0];
+    const uint32_t cs = s + 4 * k + (half ? 2u : 0u);
+    const uint32_t x0 = x & 0xFFu, x1 = (x >> 8) & 0xFFu, x2 = (x >> 16) & 0xFFu;
+    if ((x0 == '\n' && cs == e) || (x0 == '\r' && x1 == '\n' && cs + 1 == e)) {
+        rc = OLK_BLANK | (uint32_t)KEY_BLANK << 8;
+        return true;
+    }
+    if (cs + 2 > e || x0 == '|' || x0 == ' ' || x0 == '\t' || x0 == '\r' || x0 == '\n' || x1 == '\r' ||
+        x1 == '\n' || x2 == '\r' || x2 == '\n')
+        return false;
+    rc = classify(x0, x1, x2) | (2 * k + (half ? 1u : 0u)) << 8;   // depth <= 19: the key is the depth
+    return true;
+}
+
 // One tile = THREADS lanes x 64*Q contiguous bytes.  The tile arrives in
 // registers (coalesced 16-byte vectors), is staged whole in LDS behind a
 // 256-byte halo, and the registers are then refilled with the workgroup's next
-// tile while this one is decoded (software pipelining: the loads of tile t+G
-// are in flight during the look-back, compaction and decode of tile t; wave 0
-// issues its share after the look-back so its status loads do not wait
-// behind them).  Each lane owns 64*Q contiguous bytes: their '\n' masks are
-// counted and scanned once per tile, the line ends are compacted into LDS in
-// byte order, and lane i then decodes line i (balanced, coalesced writes).  A
-// line that starts before the staged window (longer than the halo) is written
-// later by scan_fixup_kernel from global memory, so the decode touches LDS only.
+// tile (software pipelining).  Each lane owns 64*Q contiguous bytes: their
+// '\n' masks are counted and scanned once per tile, the line ends are
+// compacted into LDS in byte order, and lane i decodes line i into an LDS
+// result slot (class | key << 8).  The tile's output position (its look-back
+// over the per-tile status words) is resolved one iteration later, while the
+// workgroup already works on its next tile, and only then are the staged
+// results written (coalesced).  Tiles whose lines do not fit the LDS list, and
+// calls that want the 16-byte FlatNodes, are resolved and written at once
+// instead.  A line that starts before the staged window (longer than the
+// halo) is written later by scan_fixup_kernel from global memory, so the
+// decode touches LDS only.
 template <int THREADS, int Q>
 __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
 {
@@ -308,17 +361,18 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
 
     __shared__ __align__(16) uint8_t s_text[HALO + TILE + 64];
     __shared__ __align__(16) uint16_t s_nm[TILE / 16];   // '\n' bits per 16-byte vector, byte order
-    __shared__ uint16_t s_le[LIST];                      // tile position of each line end
-    __shared__ uint16_t s_rc[LIST];                      // decoded class | key << 8 (FIX: fixup pass)
+    __shared__ uint16_t s_le[2][LIST];                   // tile position of each line end
+    __shared__ uint16_t s_rc[2][LIST];                   // decoded class | key << 8 (FIX: fixup pass)
     __shared__ uint32_t s_wcnt[NW];
     __shared__ int32_t s_wlast[NW];
-    __shared__ uint64_t s_prefix;
-    __shared__ uint32_t s_first;
+    __shared__ uint64_t s_prefix, s_pprefix;
+    __shared__ uint32_t s_first[2];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t G = gridDim.x;
     const uint32_t T = a.num_tiles;
     const uint32_t last = a.n - 1;   // always a terminator, '\n' or not
+    constexpr uint16_t FIX = 0xFFFF;  // never a class | key << 8 (classes are < 8)
 
     uint4 v[NV];
     uint4 hv;
@@ -332,18 +386,54 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         if (tt > 0 && tid < HALO / 16)
             hv = ((const uint4 *)(a.text + b - HALO))[tid];
     };
+    // wave 0: the tile's exclusive prefix (look-back), then its inclusive status
+    auto resolve = [&](uint32_t tt, uint32_t tot) -> uint64_t {
+        uint64_t prefix = 0;
+        if (tt > 0) {
+            prefix = lookback_sum<SCAN_LB_WINDOWS>(a.status, tt);
+            if (lane == 0)
+                st_relaxed_u64(a.status + (size_t)tt * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + tot));
+        }
+        return prefix;
+    };
+    auto add_fix = [&](uint64_t r) {
+        const uint32_t slot = atomicAdd(a.fix_count, 1u);
+        if (slot < a.fix_cap)
+            a.fix_list[slot] = (uint32_t)r;
+    };
+    // writes a staged tile: lines i < tot of buffer pb, first line at pfirst
+    auto write_staged = [&](int pb, uint32_t pbase, uint32_t pfirst, uint32_t tot, uint64_t running, bool is_last) {
+        if (tid == 0 && is_last)
+            a.meta[0] = running + tot;
+        for (uint32_t i = tid; i < tot; i += THREADS) {
+            const uint64_t r = running + i;
+            if (r >= a.cap)
+                break;
+            a.offsets[r] = i ? pbase + s_le[pb][i - 1] + 1u : pfirst;
+            const uint32_t rc = s_rc[pb][i];
+            if (rc == FIX) {
+                add_fix(r);
+            } else {
+                a.classes[r] = (uint8_t)(rc & 0xFFu);
+                a.keys[r] = (uint8_t)(rc >> 8);
+            }
+        }
+    };
+
+    // the workgroup's previous tile, decoded and waiting for its prefix
+    bool pending = false;
+    uint32_t p_t = 0, p_total = 0;
 
     uint32_t t = blockIdx.x;
+    int b = 0;
     if (t < T)
         load_tile(t);
-    for (; t < T; t += G) {
+    for (; t < T; t += G, b ^= 1) {
         const uint32_t base = t * TILE;
         const uint32_t tn = t + G;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
-            const uint32_t m = nl_nibble(v[k].x) | nl_nibble(v[k].y) << 4 | nl_nibble(v[k].z) << 8 |
-                               nl_nibble(v[k].w) << 12;
-            s_nm[k * THREADS + tid] = (uint16_t)m;
+            s_nm[k * THREADS + tid] = (uint16_t)nl_mask16(v[k]);
             ((uint4 *)(s_text + HALO))[k * THREADS + tid] = v[k];
         }
         if (tid < HALO / 16)
@@ -402,11 +492,12 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         const int32_t excl_l = wl > prev_l ? wl : prev_l;
 
         const bool listed = total <= (uint32_t)LIST;
-        const bool staged = listed && !a.nodes;   // decode overlapped with the look-back
+        const bool staged = listed && !a.nodes;   // written one iteration later
 
         // Publish the tile's count at once, then compact the line ends.
         if (wid == 0 && lane == 0)
-            st_relaxed_u64(a.status + (size_t)t * LB_STRIDE, ((t == 0 ? LB_FLAG_INCL : LB_FLAG_AGG) << LB_SHIFT) | total);
+            st_relaxed_u64(a.status + (size_t)t * LB_STRIDE,
+                           ((t == 0 ? LB_FLAG_INCL : LB_FLAG_AGG) << LB_SHIFT) | total);
         if (listed) {
             uint32_t at = excl_c;
 #pragma unroll
@@ -415,7 +506,7 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
                 while (m) {
                     const int bit = __builtin_ctzll(m);
                     m &= m - 1;
-                    s_le[at++] = (uint16_t)(tid * LB + 64 * q + bit);
+                    s_le[b][at++] = (uint16_t)(tid * LB + 64 * q + bit);
                 }
             }
         }
@@ -423,14 +514,18 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
 
         uint32_t first_w1 = 0;   // wave 1: start of the tile's first line
         if (wid == 0) {
-            uint64_t prefix = 0;
-            if (t > 0) {
-                prefix = lookback_sum<SCAN_LB_WINDOWS>(a.status, t);
+            // the previous tile's prefix (its predecessors have long published
+            // their counts), and this tile's when it is written at once
+            if (pending) {
+                const uint64_t pp = resolve(p_t, p_total);
                 if (lane == 0)
-                    st_relaxed_u64(a.status + (size_t)t * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
+                    s_pprefix = pp;
             }
-            if (lane == 0)
-                s_prefix = prefix;
+            if (!staged) {
+                const uint64_t pr = resolve(t, total);
+                if (lane == 0)
+                    s_prefix = pr;
+            }
             if (tn < T)
                 load_tile(tn);
         } else if (wid == 1) {
@@ -450,7 +545,7 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
                 }
             }
             if (lane == 0)
-                s_first = first_w1;
+                s_first[b] = first_w1;
         }
 
         const uint32_t lds_lo = (base >= (uint32_t)HALO) ? base - HALO : 0u;
@@ -472,11 +567,6 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             }
             return nd;
         };
-        auto add_fix = [&](uint64_t r) {
-            const uint32_t slot = atomicAdd(a.fix_count, 1u);
-            if (slot < a.fix_cap)
-                a.fix_list[slot] = (uint32_t)r;
-        };
         // Decode line (s, e) and write it at output index r.
         auto emit = [&](uint64_t r, uint32_t s, uint32_t e) {
             if (r >= a.cap)
@@ -492,66 +582,66 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             if (a.nodes)
                 a.nodes[r] = nd;
         };
-        constexpr uint16_t FIX = 0xFFFF;   // never a class | key << 8 (classes are < 8)
         if (staged) {
-            // Decode into LDS while wave 0 resolves the prefix.  Line i goes to
-            // thread (i + 64) mod THREADS, so line 0 is wave 1's, which has
-            // just found its start.
+            // Decode into LDS.  Line i goes to thread (i + 64) mod THREADS, so
+            // line 0 is wave 1's, which has just found its start.
             for (uint32_t i = (tid + THREADS - 64) % THREADS; i < total; i += THREADS) {
-                const uint32_t e = base + s_le[i];
-                const uint32_t s = i ? base + s_le[i - 1] + 1u : first_w1;
-                uint16_t rc = FIX;
-                if (s >= lds_lo) {
+                const uint32_t e = base + s_le[b][i];
+                const uint32_t s = i ? base + s_le[b][i - 1] + 1u : first_w1;
+                uint32_t rc = FIX;
+                if (s >= lds_lo && !decode_common(s_text, s + (uint32_t)HALO - base, s, e, rc)) {
                     const ol_node_dev nd = decode(s, e);
-                    rc = (uint16_t)(nd.kind | (uint32_t)key_of(nd) << 8);
+                    rc = nd.kind | (uint32_t)key_of(nd) << 8;
                 }
-                s_rc[i] = rc;
+                s_rc[b][i] = (uint16_t)rc;
             }
         }
         __syncthreads();
-        const uint64_t running = s_prefix;   // index of the tile's first line
-        const uint32_t first = s_first;      // start of the tile's first line
-        if (tid == 0 && t == T - 1)
-            a.meta[0] = running + total;
 
-        if (staged) {
-            for (uint32_t i = tid; i < total; i += THREADS) {
-                const uint64_t r = running + i;
-                if (r >= a.cap)
-                    break;
-                a.offsets[r] = i ? base + s_le[i - 1] + 1u : first;
-                const uint32_t rc = s_rc[i];
-                if (rc == FIX) {
-                    add_fix(r);
-                } else {
-                    a.classes[r] = (uint8_t)(rc & 0xFFu);
-                    a.keys[r] = (uint8_t)(rc >> 8);
+        if (pending)
+            write_staged(b ^ 1, p_t * TILE, s_first[b ^ 1], p_total, s_pprefix, p_t == T - 1);
+        if (!staged) {
+            const uint64_t running = s_prefix;   // index of the tile's first line
+            const uint32_t first = s_first[b];   // start of the tile's first line
+            if (tid == 0 && t == T - 1)
+                a.meta[0] = running + total;
+            if (listed) {
+                for (uint32_t i = tid; i < total; i += THREADS) {
+                    const uint32_t e = base + s_le[b][i];
+                    const uint32_t s = i ? base + s_le[b][i - 1] + 1u : first;
+                    emit(running + i, s, e);
                 }
-            }
-        } else if (listed) {
-            for (uint32_t i = tid; i < total; i += THREADS) {
-                const uint32_t e = base + s_le[i];
-                const uint32_t s = i ? base + s_le[i - 1] + 1u : first;
-                emit(running + i, s, e);
-            }
-        } else if (cnt) {
-            // dense tile: each lane decodes the lines ending in its bytes
-            uint64_t r = running + excl_c;
-            uint32_t sp = (excl_l >= 0) ? base + (uint32_t)excl_l + 1 : first;
+            } else if (cnt) {
+                // dense tile: each lane decodes the lines ending in its bytes
+                uint64_t r = running + excl_c;
+                uint32_t sp = (excl_l >= 0) ? base + (uint32_t)excl_l + 1 : first;
 #pragma unroll
-            for (int q = 0; q < Q; q++) {
-                uint64_t m = mk[q];
-                while (m) {
-                    const int bit = __builtin_ctzll(m);
-                    m &= m - 1;
-                    const uint32_t e = base + tid * LB + 64 * q + (uint32_t)bit;
-                    emit(r, sp, e);
-                    sp = e + 1;
-                    r++;
+                for (int q = 0; q < Q; q++) {
+                    uint64_t m = mk[q];
+                    while (m) {
+                        const int bit = __builtin_ctzll(m);
+                        m &= m - 1;
+                        const uint32_t e = base + tid * LB + 64 * q + (uint32_t)bit;
+                        emit(r, sp, e);
+                        sp = e + 1;
+                        r++;
+                    }
                 }
             }
         }
+        pending = staged;
+        p_t = t;
+        p_total = total;
         __syncthreads();   // the next tile overwrites LDS
+    }
+    if (pending) {
+        if (wid == 0) {
+            const uint64_t pp = resolve(p_t, p_total);
+            if (lane == 0)
+                s_pprefix = pp;
+        }
+        __syncthreads();
+        write_staged(b ^ 1, p_t * TILE, s_first[b ^ 1], p_total, s_pprefix, p_t == T - 1);
     }
 }
 

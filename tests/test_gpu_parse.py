@@ -146,3 +146,12 @@ def test_prefixes_around_the_mask_window(parser):
     assert_parse_matches(parser, text, "prefix lengths 50..79")
     for shift in (1, 7, 33, 63):
         assert_parse_matches(parser, b"q" * shift + b"\n" + text, f"shifted {shift}")
+
+
+def test_newline_bytes_at_every_vector_lane_and_neighbours(parser):
+    """'\\n' at each byte of a 16-byte vector, next to bytes that differ from
+    it by one bit (0x0B, 0x0E, 0x8A, 0x2A) -- the exact zero-byte test."""
+    rng = np.random.default_rng(8)
+    near = np.array([0x0B, 0x0E, 0x08, 0x8A, 0x2A, 0x09, 0x0A], np.uint8)
+    text = rng.choice(near, size=3 << 20).astype(np.uint8)
+    assert_parse_matches(parser, text, "near-newline bytes")
