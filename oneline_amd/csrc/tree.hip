@@ -41,7 +41,7 @@ constexpr int TW = 256;              // threads per tile (4 waves)
 constexpr int TR = 16;               // rounds of 64 lines per wave
 constexpr int TLINES = TW * TR;      // lines per tile
 constexpr int D = TREE_D;            // levels tracked
-static_assert(D <= 63, "levels + 1 must fit a wave");
+static_assert(D == 32, "five key bits, lanes 0..32");
 
 __device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
@@ -57,17 +57,36 @@ __device__ __forceinline__ void load_keys(const uint8_t *keys, uint32_t wbase, u
     }
 }
 
-// Lane d (d <= D) gets ballot(key < d) of the round.
-__device__ __forceinline__ uint64_t level_masks(uint32_t key)
+// Bit-sliced level masks of a round (64 lines, lane = line, x = its key):
+// lane d (0 <= d < 32) gets lt = {lines with key < d} and eq = {lines with key
+// == d}, lane 32 gets lt = {all content lines}.  Five ballots of the key bits
+// feed a per-lane comparator against d, most significant bit first, instead
+// of one ballot per level.  `dm[b]` is all ones in lanes whose bit b is set.
+__device__ __forceinline__ void level_masks(uint32_t x, const uint64_t (&dm)[5], uint64_t &lt, uint64_t &eq)
 {
     const int lane = threadIdx.x & 63;
-    uint64_t my = 0;
+    const uint64_t V = __ballot(x < (uint32_t)D);
+    uint64_t B[5];
 #pragma unroll
-    for (int d = 0; d <= D; d++) {
-        const uint64_t m = __ballot(key < (uint32_t)d);
-        my = (lane == d) ? m : my;
+    for (int b = 0; b < 5; b++)
+        B[b] = __ballot((x >> b) & 1u);
+    uint64_t l = 0, e = V;
+#pragma unroll
+    for (int b = 4; b >= 0; b--) {
+        const uint64_t t = e & ~B[b];          // equal so far, key bit 0
+        l |= t & dm[b];                        // ... and d's bit 1: key < d
+        e &= ~(B[b] ^ dm[b]);                  // key bit == d's bit
     }
-    return my;
+    lt = (lane == D) ? V : l;
+    eq = e;
+}
+
+__device__ __forceinline__ void lane_bit_masks(uint64_t (&dm)[5])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int b = 0; b < 5; b++)
+        dm[b] = ((lane >> b) & 1) ? ~0ull : 0ull;
 }
 
 // Per tile: for each level the last content line (+1, 0 = none) and the first
@@ -80,13 +99,14 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
     const uint32_t wbase = t * TLINES + w * (TR * 64);
     uint32_t key[TR];
     load_keys(a.keys, wbase, a.lines, key);
+    uint64_t dm[5];
+    lane_bit_masks(dm);
     uint32_t last = 0, first = OL_NONE;
     bool deep = false;
 #pragma unroll
     for (int r = 0; r < TR; r++) {
-        const uint64_t my = level_masks(key[r]);
-        const uint64_t up = __shfl_down(my, 1, 64);
-        const uint64_t eq = up & ~my;   // lanes whose key == lane id
+        uint64_t lt, eq;   // eq: lanes whose key == lane id
+        level_masks(key[r], dm, lt, eq);
         const uint32_t rb = wbase + r * 64;
         if (lane < D && eq) {
             last = rb + 64 - (uint32_t)__clzll(eq);     // index + 1
@@ -109,108 +129,168 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
             l = umax(l, s_l[k][lane]);
             f = umin(f, s_f[k][lane]);
         }
-        a.sum_last[(size_t)t * D + lane] = l;
-        a.sum_first[(size_t)t * D + lane] = f;
+        a.sum_last[(size_t)lane * a.num_tiles + t] = l;   // level-major
+        a.sum_first[(size_t)lane * a.num_tiles + t] = f;
     }
 }
 
-// One workgroup per level: in_last[t] = max of sum_last over tiles < t,
-// out_first[t] = min of sum_first over tiles > t.  Each thread owns a contiguous
-// chunk of tiles.
+// One workgroup per level (arrays level-major, one row of num_tiles per
+// level): in_last[t] = max of sum_last over tiles < t, out_first[t] = min of
+// sum_first over tiles > t.  The row goes through LDS in chunks: coalesced
+// loads, each thread scans a contiguous run of the chunk, a block scan joins
+// the runs, coalesced stores.
 template <int ST>
 __global__ __launch_bounds__(ST) void tree_scan_kernel(TreeArgs a)
 {
     constexpr int NWS = ST / 64;
+    constexpr int K = 8;                 // tiles per thread per chunk
+    constexpr int CH = ST * K;
+    __shared__ uint32_t s_v[CH];
     __shared__ uint32_t s_w[NWS];
-    const uint32_t d = blockIdx.x;
     const uint32_t T = a.num_tiles;
+    const size_t row = (size_t)blockIdx.x * T;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t K = (T + ST - 1) / ST;
-    const uint32_t t0 = tid * K, t1 = umin(T, t0 + K);
 
-    // forward: exclusive prefix max
-    uint32_t v = 0;
-    for (uint32_t t = t0; t < t1; t++)
-        v = umax(v, a.sum_last[(size_t)t * D + d]);
-    uint32_t inc = v;
+    // forward: exclusive prefix max of sum_last
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < T; c0 += CH) {
+        const uint32_t n = (T - c0) < (uint32_t)CH ? T - c0 : (uint32_t)CH;
+        for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
+            s_v[i] = i < n ? a.sum_last[row + c0 + i] : 0u;
+        __syncthreads();
+        uint32_t loc[K], m = 0;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(inc, o, 64);
-        if (lane >= o)
-            inc = umax(inc, y);
+        for (int k = 0; k < K; k++) {
+            loc[k] = s_v[tid * K + k];
+            m = umax(m, loc[k]);
+        }
+        uint32_t inc = m;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(inc, o, 64);
+            if (lane >= o)
+                inc = umax(inc, y);
+        }
+        if (lane == 63)
+            s_w[w] = inc;
+        __syncthreads();
+        uint32_t run = __shfl_up(inc, 1, 64);
+        if (lane == 0)
+            run = 0;
+        uint32_t tot = 0;
+        for (int k = 0; k < NWS; k++) {
+            if (k < w)
+                run = umax(run, s_w[k]);
+            tot = umax(tot, s_w[k]);
+        }
+        run = umax(run, carry);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            s_v[tid * K + k] = run;
+            run = umax(run, loc[k]);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += ST)
+            a.in_last[row + c0 + i] = s_v[i];
+        carry = umax(carry, tot);
+        __syncthreads();
     }
-    if (lane == 63)
-        s_w[w] = inc;
-    __syncthreads();
-    uint32_t run = __shfl_up(inc, 1, 64);
-    if (lane == 0)
-        run = 0;
-    for (int k = 0; k < w; k++)
-        run = umax(run, s_w[k]);
-    for (uint32_t t = t0; t < t1; t++) {
-        a.in_last[(size_t)t * D + d] = run;
-        run = umax(run, a.sum_last[(size_t)t * D + d]);
-    }
-    __syncthreads();
 
-    // backward: exclusive suffix min
-    v = OL_NONE;
-    for (uint32_t t = t0; t < t1; t++)
-        v = umin(v, a.sum_first[(size_t)t * D + d]);
-    inc = v;
+    // backward: exclusive suffix min of sum_first
+    carry = OL_NONE;
+    for (uint32_t c1 = T; c1 > 0;) {
+        const uint32_t n = c1 < (uint32_t)CH ? c1 : (uint32_t)CH;
+        const uint32_t c0 = c1 - n;
+        for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
+            s_v[i] = i < n ? a.sum_first[row + c0 + i] : OL_NONE;
+        __syncthreads();
+        uint32_t loc[K], m = OL_NONE;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_down(inc, o, 64);
-        if (lane + o < 64)
-            inc = umin(inc, y);
-    }
-    if (lane == 0)
-        s_w[w] = inc;
-    __syncthreads();
-    run = __shfl_down(inc, 1, 64);
-    if (lane == 63)
-        run = OL_NONE;
-    for (int k = w + 1; k < NWS; k++)
-        run = umin(run, s_w[k]);
-    for (uint32_t t = t1; t-- > t0;) {
-        a.out_first[(size_t)t * D + d] = run;
-        run = umin(run, a.sum_first[(size_t)t * D + d]);
+        for (int k = 0; k < K; k++) {
+            loc[k] = s_v[tid * K + k];
+            m = umin(m, loc[k]);
+        }
+        uint32_t inc = m;   // inclusive suffix min over lanes >= lane
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_down(inc, o, 64);
+            if (lane + o < 64)
+                inc = umin(inc, y);
+        }
+        if (lane == 0)
+            s_w[w] = inc;
+        __syncthreads();
+        uint32_t run = __shfl_down(inc, 1, 64);
+        if (lane == 63)
+            run = OL_NONE;
+        uint32_t tot = OL_NONE;
+        for (int k = 0; k < NWS; k++) {
+            if (k > w)
+                run = umin(run, s_w[k]);
+            tot = umin(tot, s_w[k]);
+        }
+        run = umin(run, carry);
+#pragma unroll
+        for (int k = K - 1; k >= 0; k--) {
+            s_v[tid * K + k] = run;
+            run = umin(run, loc[k]);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += ST)
+            a.out_first[row + c0 + i] = s_v[i];
+        carry = umin(carry, tot);
+        c1 = c0;
+        __syncthreads();
     }
 }
 
+// Per round (lane k = line rb + k, key x): the nearest smaller-key line inside
+// the round is a masked clz / ctz on the round's level masks (LDS).  Lines
+// whose answer lies outside the round read two per-lane states, lane d holding
+// level d: G[d] = last line before the round with key < d (+1, 0 = none) and
+// its key, and H[d] = first line after the round with key <= d and its key.
+// G moves forward round by round from the round's own masks (the last line of
+// {key < d}); H is filled backwards over the rounds in LDS before the walk.
 __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
 {
     constexpr int NWV = TW / 64;
-    __shared__ uint64_t s_m[NWV][TR][D + 1];   // per round: lane d's ballot(key < d)
-    __shared__ uint32_t s_b[NWV][TR][D];       // per round: first line per level (then: B after the round)
+    __shared__ uint64_t s_m[NWV][TR][D + 1];   // lane d's {key < d} of each round
+    __shared__ uint32_t s_h[NWV][TR][D];       // first line with key <= d: in the round, then after it
+    __shared__ uint8_t s_hl[NWV][TR][D];       // that line's key
     __shared__ uint32_t s_l[NWV][D], s_f[NWV][D];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t t = blockIdx.x;
+    const uint32_t T = a.num_tiles;
     const uint64_t L = a.lines;
     const uint32_t wbase = t * TLINES + w * (TR * 64);
     uint32_t key[TR];
     load_keys(a.keys, wbase, L, key);
 
-    // pass 1: masks and per-round first line per level
+    // pass 1: masks; per level the wave's last / first line; per round the
+    // first line with key <= d
+    uint64_t dm[5];
+    lane_bit_masks(dm);
     uint32_t last = 0, first = OL_NONE;
 #pragma unroll
     for (int r = 0; r < TR; r++) {
-        const uint64_t my = level_masks(key[r]);
+        uint64_t lt, eq;
+        level_masks(key[r], dm, lt, eq);
         if (lane <= D)
-            s_m[w][r][lane] = my;
-        const uint64_t up = __shfl_down(my, 1, 64);
-        const uint64_t eq = up & ~my;
+            s_m[w][r][lane] = lt;
         const uint32_t rb = wbase + r * 64;
-        uint32_t f = OL_NONE;
-        if (lane < D && eq) {
-            last = rb + 64 - (uint32_t)__clzll(eq);
-            f = rb + (uint32_t)__builtin_ctzll(eq);
-            if (first == OL_NONE)
-                first = f;
+        const uint64_t le = lt | eq;
+        const int j = le ? (int)__builtin_ctzll(le) : 0;
+        const uint32_t kj = __shfl(key[r], j, 64);
+        if (lane < D) {
+            if (eq) {
+                last = rb + 64 - (uint32_t)__clzll(eq);
+                if (first == OL_NONE)
+                    first = rb + (uint32_t)__builtin_ctzll(eq);
+            }
+            s_h[w][r][lane] = le ? rb + (uint32_t)j : OL_NONE;
+            s_hl[w][r][lane] = (uint8_t)kj;
         }
-        if (lane < D)
-            s_b[w][r][lane] = f;
     }
     if (lane < D) {
         s_l[w][lane] = last;
@@ -218,23 +298,51 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     }
     __syncthreads();
 
-    // states at the wave's first and last line
+    // per-level states at the wave's first line (F, last line + 1) and after
+    // its last line (B, first line)
     uint32_t F = 0, Bw = OL_NONE;
     if (lane < D) {
-        F = a.in_last[(size_t)t * D + lane];
-        Bw = a.out_first[(size_t)t * D + lane];
+        F = a.in_last[(size_t)lane * T + t];
+        Bw = a.out_first[(size_t)lane * T + t];
         for (int k = 0; k < w; k++)
             F = umax(F, s_l[k][lane]);
         for (int k = w + 1; k < NWV; k++)
             Bw = umin(Bw, s_f[k][lane]);
-        // B after each round (suffix over the later rounds of this wave)
-        uint32_t bc = Bw;
-        for (int r = TR - 1; r >= 0; r--) {
-            const uint32_t f = s_b[w][r][lane];
-            s_b[w][r][lane] = bc;
-            bc = umin(bc, f);
+    }
+    // G = exclusive prefix max of F over levels, H = inclusive prefix min of B
+    uint32_t pv = (lane < D) ? F : 0u, pl = (uint32_t)lane;
+    uint32_t sv = (lane < D) ? Bw : OL_NONE, sl = (uint32_t)lane;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const uint32_t pv2 = __shfl_up(pv, o, 64), pl2 = __shfl_up(pl, o, 64);
+        const uint32_t sv2 = __shfl_up(sv, o, 64), sl2 = __shfl_up(sl, o, 64);
+        if (lane >= o && pv2 > pv) {
+            pv = pv2;
+            pl = pl2;
+        }
+        if (lane >= o && sv2 < sv) {
+            sv = sv2;
+            sl = sl2;
         }
     }
+    uint32_t G = __shfl_up(pv, 1, 64), GL = __shfl_up(pl, 1, 64);
+    if (lane == 0)
+        G = 0;
+    // H after each round, backwards over the rounds
+    if (lane < D) {
+        uint32_t cv = sv, cl = sl;
+        for (int r = TR - 1; r >= 0; r--) {
+            const uint32_t fv = s_h[w][r][lane];
+            const uint32_t fl = s_hl[w][r][lane];
+            s_h[w][r][lane] = cv;
+            s_hl[w][r][lane] = (uint8_t)cl;
+            if (fv < cv) {
+                cv = fv;
+                cl = fl;
+            }
+        }
+    }
+    __syncthreads();
 
     const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t above = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
@@ -244,28 +352,6 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     for (int r = 0; r < TR; r++) {
         const uint32_t rb = wbase + r * 64;
         const uint32_t x = key[r];
-        // exclusive prefix max over levels of F (value = line + 1, with its level)
-        uint32_t pv = (lane < D) ? F : 0u, pl = (uint32_t)lane;
-        // inclusive prefix min over levels of B after this round
-        uint32_t sv = (lane < D) ? s_b[w][r][lane] : OL_NONE, sl = (uint32_t)lane;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const uint32_t pv2 = __shfl_up(pv, o, 64), pl2 = __shfl_up(pl, o, 64);
-            const uint32_t sv2 = __shfl_up(sv, o, 64), sl2 = __shfl_up(sl, o, 64);
-            if (lane >= o && pv2 > pv) {
-                pv = pv2;
-                pl = pl2;
-            }
-            if (lane >= o && sv2 < sv) {
-                sv = sv2;
-                sl = sl2;
-            }
-        }
-        uint32_t pfx = __shfl_up(pv, 1, 64), pfl = __shfl_up(pl, 1, 64);         // max over d < lane
-        if (lane == 0)
-            pfx = 0;
-        const uint32_t nxt = __shfl(sv, D - 1, 64), nxtl = __shfl(sl, D - 1, 64); // first content after the round
-
         const bool valid = x < (uint32_t)D;
         const uint32_t xi = valid ? x : 0u;
         const uint64_t mlt = s_m[w][r][xi], mle = s_m[w][r][xi + 1];
@@ -275,23 +361,24 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
         const uint32_t js = sm ? (uint32_t)__builtin_ctzll(sm) : 0u;
         const uint32_t jf = fm ? (uint32_t)__builtin_ctzll(fm) : 0u;
         const uint32_t kp = __shfl(x, jp, 64), ks = __shfl(x, js, 64), kf = __shfl(x, jf, 64);
-        const uint32_t opv = __shfl(pfx, xi, 64), opl = __shfl(pfl, xi, 64);
-        const uint32_t osv = __shfl(sv, xi, 64), osl = __shfl(sl, xi, 64);
+        const uint32_t gv = __shfl(G, xi, 64), gl = __shfl(GL, xi, 64);
+        const uint32_t hv = s_h[w][r][xi], hl = s_hl[w][r][xi];
+        const uint32_t nv = s_h[w][r][D - 1], nl = s_hl[w][r][D - 1];   // first content line after the round
 
         const uint32_t g = rb + lane;
         uint32_t par, se, fc, ns;
-        if (x == KEY_BLANK || !valid) {
+        if (!valid) {   // blank (and too deep: reported by the error flag)
             par = OL_NONE;
             se = g + 1;
             fc = OL_NONE;
             ns = OL_NONE;
         } else {
-            par = pm ? rb + jp : (opv ? opv - 1 : OL_NONE);
-            const uint32_t plv = pm ? kp : opl;
-            se = sm ? rb + js : (osv != OL_NONE ? osv : (uint32_t)L);
-            const uint32_t slv = sm ? ks : osl;
-            const uint32_t fi = fm ? rb + jf : nxt;
-            const uint32_t fl = fm ? kf : nxtl;
+            par = pm ? rb + jp : (gv ? gv - 1 : OL_NONE);
+            const uint32_t plv = pm ? kp : gl;
+            se = sm ? rb + js : (hv != OL_NONE ? hv : (uint32_t)L);
+            const uint32_t slv = sm ? ks : hl;
+            const uint32_t fi = fm ? rb + jf : nv;
+            const uint32_t fl = fm ? kf : nl;
             fc = (fi != OL_NONE && fl > x) ? fi : OL_NONE;
             ns = (se < L && (par == OL_NONE || plv < slv)) ? se : OL_NONE;
         }
@@ -301,11 +388,13 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
             if (FC) FC[g] = fc;
             if (NS) NS[g] = ns;
         }
-        // F after this round: the last line per level
-        if (lane < D) {
-            const uint64_t eq = s_m[w][r][lane + 1] & ~s_m[w][r][lane];
-            if (eq)
-                F = rb + 64 - (uint32_t)__clzll(eq);
+        // G after this round: the round's last line with key < d, if any
+        const uint64_t ltd = s_m[w][r][lane < D ? lane : 0];
+        const uint32_t jd = ltd ? 63u - (uint32_t)__clzll(ltd) : 0u;
+        const uint32_t kd = __shfl(x, jd, 64);
+        if (lane < D && ltd) {
+            G = rb + jd + 1;
+            GL = kd;
         }
     }
 }
