@@ -76,6 +76,7 @@ struct ol_ctx {
     uint32_t scan_bpt = SCAN_BPT;          // bytes per lane of scan_classify (tile = 256 lanes x bpt)
     uint32_t scan_grid = 0;                // persistent workgroups of scan_classify (0 = all co-resident)
     uint32_t scan_max_grid[3] = {0, 0, 0}; // co-resident limit per bpt (64, 128, 256), from the occupancy API
+    uint32_t tree_rounds = 16;             // rounds of 64 lines per tree wave (tile = 256 x rounds lines)
     // profiling
     bool profiling = false;
     struct Rec {
@@ -164,10 +165,16 @@ hipError_t clear_scratch(ol_ctx *c, size_t tiles)
     return ol_launch_clear_u64((uint64_t *)c->scratch.p, SCR_STATUS + tiles * LB_STRIDE, c->stream);
 }
 
+// Waits for the stream and reads meta[0]; fails if a look-back wait gave up.
 int read_meta(ol_ctx *c, uint64_t *out)
 {
-    HIPCHK(hipMemcpyAsync(out, c->meta(), 8, hipMemcpyDeviceToHost, c->stream));
+    uint64_t head[SCR_STATUS];
+    HIPCHK(hipMemcpyAsync(head, c->scratch.p, sizeof head, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t *ctr = (const uint32_t *)(head + SCR_CTR);
+    if (ctr[4])
+        return fail(OL_ERR_DEVICE, "a look-back wait gave up: not every workgroup of a persistent launch was resident");
+    *out = head[SCR_META];
     return OL_OK;
 }
 
@@ -212,10 +219,11 @@ int launch_scan(ol_ctx *c, ol_doc *d)
     a.nodes = d->nodes;
     a.status = c->status();
     a.meta = c->meta();
+    a.lb_err = c->ctr() + 4;
     // a line needing the fixup pass is longer than the scan halo: at most n / 256 of them
     a.fix_cap = (uint32_t)(d->n / SCAN_HALO + 64);
-    HIPCHK(c->fix.ensure((size_t)a.fix_cap * 4));
-    a.fix_list = (uint32_t *)c->fix.p;
+    HIPCHK(c->fix.ensure((size_t)a.fix_cap * 16));
+    a.fix_list = (uint4 *)c->fix.p;
     a.fix_count = c->ctr() + 2;
     const int vi = c->scan_bpt == 64 ? 0 : (c->scan_bpt == 128 ? 1 : 2);
     if (!c->scan_max_grid[vi])
@@ -332,7 +340,7 @@ int ensure_tree(ol_tree *t, uint64_t lines)
 
 int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->lines ? d->lines : 1, (uint64_t)ol_tree_tile_lines());
+    uint32_t tiles = (uint32_t)ceil_div(d->lines ? d->lines : 1, (uint64_t)ol_tree_tile_lines(c->tree_rounds));
     size_t part = (size_t)tiles * TREE_D;
     HIPCHK(ensure_scratch(c, 0));
     HIPCHK(c->tree_sum.ensure(part * 4 * 4));
@@ -352,7 +360,7 @@ int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
     a.out_first = base + 3 * part;
     a.err = c->ctr() + 1;
     Prof p(c, "tree");
-    HIPCHK(ol_launch_tree(a, c->stream));
+    HIPCHK(ol_launch_tree(a, c->tree_rounds, c->stream));
     return OL_OK;
 }
 
@@ -392,6 +400,7 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
     a.out32 = out32;
     a.status = c->status();
     a.meta = c->meta();
+    a.lb_err = c->ctr() + 4;
     {
         Prof p(c, "search");
         HIPCHK(ol_launch_search(a, c->stream));
@@ -468,6 +477,7 @@ int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, ui
     a.out_id = out_id;
     a.status = c->status();
     a.meta = c->meta();
+    a.lb_err = c->ctr() + 4;
     a.tile_ctr = c->ctr() + 3;
     {
         Prof p(c, "many");
@@ -1113,6 +1123,10 @@ int ol_tune_set(ol_ctx *c, const char *name, int value)
         if (value < 0)
             return fail(OL_ERR_INVALID, "scan_grid must be >= 0 (0 = all co-resident workgroups)");
         c->scan_grid = (uint32_t)value;
+    } else if (n == "tree_rounds") {
+        if (value != 1 && value != 4 && value != 16)
+            return fail(OL_ERR_INVALID, "tree_rounds must be 1, 4 or 16");
+        c->tree_rounds = (uint32_t)value;
     } else {
         return fail(OL_ERR_INVALID, "unknown tuning parameter " + n);
     }
@@ -1129,6 +1143,8 @@ int ol_tune_get(ol_ctx *c, const char *name, int *value)
         *value = (int)c->scan_bpt;
     else if (n == "scan_grid")
         *value = (int)c->scan_grid;
+    else if (n == "tree_rounds")
+        *value = (int)c->tree_rounds;
     else
         return fail(OL_ERR_INVALID, "unknown tuning parameter " + n);
     return OL_OK;

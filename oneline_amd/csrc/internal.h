@@ -73,8 +73,13 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 #define LB_STRIDE 8
 #endif
 
+// Polls of one not-ready status word before the look-back gives up (about a
+// second): it then sets *err and returns, so a launch whose workgroups are not
+// all resident ends with an error instead of waiting forever.
+constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
+
 template <int NWIN = 1>
-__device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_t t)
+__device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_t t, uint32_t *err)
 {
     const int lane = threadIdx.x & 63;
     uint64_t prefix = 0;
@@ -99,10 +104,15 @@ __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_
             if (fn < fi) {
                 prefix += wave_sum_u64((lane < fn) ? (st[k] & LB_MASK) : 0);
                 top -= 64 * k + fn;
+                uint32_t spins = 0;
                 if (lane == 0) {
-                    while ((ld_relaxed_u64(status + top * LB_STRIDE) >> LB_SHIFT) == 0)
+                    while ((ld_relaxed_u64(status + top * LB_STRIDE) >> LB_SHIFT) == 0 && ++spins < LB_SPIN_LIMIT)
                         __builtin_amdgcn_s_sleep(8);
+                    if (spins >= LB_SPIN_LIMIT)
+                        atomicOr(err, 1u);
                 }
+                if (__shfl(spins, 0, 64) >= LB_SPIN_LIMIT)
+                    return prefix;   // gave up: the result is flagged invalid
                 waited = true;
                 break;
             }
@@ -120,6 +130,13 @@ __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_
     return prefix;
 }
 
+// Workgroups of kernel f that can be resident at once on the device, for the
+// persistent launches whose look-backs need every workgroup resident: the
+// occupancy API's count per CU, capped by the CU's LDS divided by the
+// kernel's LDS rounded up to 4 KiB (margin for the allocation granule, which
+// the API does not model).  0 on failure.
+uint32_t ol_coresident(const void *f, int threads, size_t dyn_lds);
+
 // ---- kernel argument blocks ----------------------------------------------
 struct ScanArgs {
     const uint8_t *text;     // padded device text
@@ -132,21 +149,36 @@ struct ScanArgs {
     ol_node_dev *nodes;      // FlatNode (nullable)                [cap]
     uint64_t *status;        // look-back words                    [num_tiles]
     uint64_t *meta;          // meta[0] <- line count
-    uint32_t *fix_list;      // lines whose head was not in LDS (decoded by the fixup pass)
+    uint32_t *lb_err;        // set when a look-back wait gave up
+    uint4 *fix_list;         // {line, start, end, 0} of lines whose head was not in LDS (fixup pass)
     uint32_t *fix_count;     // entries appended to fix_list
     uint32_t fix_cap;        // capacity of fix_list (>= lines longer than the halo)
+    // Batches (ndocs > 0): the text is the documents back to back, each ending
+    // in '\n'.  Offsets are written relative to the line's document, and the
+    // index of each document's first line to doc_line_start.
+    uint32_t ndocs;
+    const uint32_t *doc_starts;   // [ndocs] byte start of each document (ascending)
+    const uint32_t *tile_doc;     // [num_tiles + 1] first document starting at or after tile t
+    uint32_t *doc_line_start;     // [ndocs] out
 };
 
 struct TreeArgs {
     const uint8_t *keys;     // tree key byte per line (padded to LINE_ALIGN)
     uint64_t lines;          // line count
-    uint32_t num_tiles;      // ceil(lines / ol_tree_tile_lines())
+    uint32_t num_tiles;      // ceil(lines / ol_tree_tile_lines(rounds)), or the batch's tile count
     uint32_t *parent, *first_child, *next_sibling, *subtree_end;  // nullable each
     uint32_t *sum_last;      // per tile x level: last content line + 1 in the tile (0 = none)
     uint32_t *sum_first;     // per tile x level: first content line in the tile (OL_NONE = none)
     uint32_t *in_last;       // per tile x level: sum_last over all earlier tiles (max)
     uint32_t *out_first;     // per tile x level: sum_first over all later tiles (min)
     uint32_t *err;           // set to 1 if a content line is too deep
+    // Batches (nsegs > 0): one tree per document (segment); a tile holds lines
+    // of one document only, and every link is local to its document.
+    uint32_t nsegs;
+    const uint32_t *seg_line0;   // [nsegs] global index of the document's first line
+    const uint32_t *seg_lines;   // [nsegs] its line count
+    const uint32_t *seg_tile0;   // [nsegs] its first tile
+    const uint32_t *tile_seg;    // [num_tiles] document of each tile
 };
 
 struct SearchArgs {
@@ -157,6 +189,7 @@ struct SearchArgs {
     uint32_t win, win_mask;  // first min(m,4) pattern bytes and their mask
     uint8_t pat[64];
     uint64_t cap;            // output capacity (matches)
+    uint32_t *lb_err;        // set when a look-back wait gave up
     uint64_t *out64;         // offsets as u64 (nullable)
     uint32_t *out32;         // offsets as u32 (nullable)
     uint64_t *status;
@@ -194,6 +227,13 @@ struct ManyArgs {
     uint64_t *status;
     uint64_t *meta;          // meta[0] <- total matches
     uint32_t *tile_ctr;      // tile ticket, zero at launch
+    uint32_t *lb_err;        // set when a look-back wait gave up
+    // Batches (ndocs > 0): a match must end inside the document it starts in
+    // (the bytes between documents belong to none).
+    uint32_t ndocs;
+    const uint32_t *doc_starts;   // [ndocs] ascending
+    const uint32_t *doc_ends;     // [ndocs] exclusive end of each document's bytes
+    const uint32_t *tile_doc;     // [num_tiles] last document starting at or before tile t
 };
 
 // ---- launchers (each returns the launch error, if any) ---------------------
@@ -204,8 +244,9 @@ hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s);
 hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, hipStream_t s);
 uint32_t ol_scan_max_grid(uint32_t bpt);
 hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s);   // reads the line count from meta[0]
-hipError_t ol_launch_tree(const TreeArgs &a, hipStream_t s);   // summary + scan + walk
-uint32_t ol_tree_tile_lines();
+// summary + scan + walk; `rounds` of 64 lines per wave: 1, 4 or 16 (tile = 256 x rounds lines)
+hipError_t ol_launch_tree(const TreeArgs &a, uint32_t rounds, hipStream_t s);
+uint32_t ol_tree_tile_lines(uint32_t rounds);
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
 

@@ -354,7 +354,7 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     constexpr int HALO = SCAN_HALO;
     constexpr int NW = THREADS / 64;
     constexpr int NV = LB / 16;         // 16-byte vectors per lane
-    constexpr int LIST = TILE / 16;     // lines per tile decoded from the compacted list
+    constexpr int LIST = TILE / 24 / 64 * 64;   // lines per tile decoded from the compacted list
     static_assert(NW >= 2, "needs two waves");
     static_assert(HALO % 16 == 0 && HALO / 16 <= THREADS, "halo");
     static_assert(TILE <= 65536, "tile positions are u16");
@@ -367,6 +367,9 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     __shared__ int32_t s_wlast[NW];
     __shared__ uint64_t s_prefix, s_pprefix;
     __shared__ uint32_t s_first[2];
+    constexpr int MAXD = 64;                              // document starts staged per tile
+    __shared__ uint32_t s_ds[2][MAXD];                   // starts of the documents starting in the tile
+    __shared__ uint32_t s_dj[2][2];                      // their index range [j0, j1)
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t G = gridDim.x;
@@ -390,16 +393,49 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     auto resolve = [&](uint32_t tt, uint32_t tot) -> uint64_t {
         uint64_t prefix = 0;
         if (tt > 0) {
-            prefix = lookback_sum<SCAN_LB_WINDOWS>(a.status, tt);
+            prefix = lookback_sum<SCAN_LB_WINDOWS>(a.status, tt, a.lb_err);
             if (lane == 0)
                 st_relaxed_u64(a.status + (size_t)tt * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + tot));
         }
         return prefix;
     };
-    auto add_fix = [&](uint64_t r) {
+    auto add_fix = [&](uint64_t r, uint32_t s, uint32_t e) {
         const uint32_t slot = atomicAdd(a.fix_count, 1u);
         if (slot < a.fix_cap)
-            a.fix_list[slot] = (uint32_t)r;
+            a.fix_list[slot] = make_uint4((uint32_t)r, s, e, 0u);
+    };
+    // offsets[r] for the line starting at s (relative to its document in a
+    // batch; the first line of a document also records r in doc_line_start)
+    auto put_offset = [&](int pb, uint64_t r, uint32_t s) {
+        if (a.ndocs == 0) {
+            a.offsets[r] = s;
+            return;
+        }
+        // the document holding s: the last one starting at or before s
+        const uint32_t j0 = s_dj[pb][0], j1 = s_dj[pb][1];
+        uint32_t lo = j0, hi = j1;   // first j in [j0, j1) with start > s
+        if (j1 - j0 <= (uint32_t)MAXD) {
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_ds[pb][mid - j0] <= s)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+        } else {
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.doc_starts[mid] <= s)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+        }
+        const uint32_t d = lo - 1;   // a line starts in or after its document
+        const uint32_t ds = a.doc_starts[d];
+        a.offsets[r] = s - ds;
+        if (s == ds)
+            a.doc_line_start[d] = (uint32_t)r;
     };
     // writes a staged tile: lines i < tot of buffer pb, first line at pfirst
     auto write_staged = [&](int pb, uint32_t pbase, uint32_t pfirst, uint32_t tot, uint64_t running, bool is_last) {
@@ -409,10 +445,11 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             const uint64_t r = running + i;
             if (r >= a.cap)
                 break;
-            a.offsets[r] = i ? pbase + s_le[pb][i - 1] + 1u : pfirst;
+            const uint32_t ls = i ? pbase + s_le[pb][i - 1] + 1u : pfirst;
+            put_offset(pb, r, ls);
             const uint32_t rc = s_rc[pb][i];
             if (rc == FIX) {
-                add_fix(r);
+                add_fix(r, ls, pbase + s_le[pb][i]);
             } else {
                 a.classes[r] = (uint8_t)(rc & 0xFFu);
                 a.keys[r] = (uint8_t)(rc >> 8);
@@ -546,6 +583,13 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             }
             if (lane == 0)
                 s_first[b] = first_w1;
+            if (a.ndocs) {
+                const uint32_t j0 = a.tile_doc[t], j1 = a.tile_doc[t + 1];
+                if (lane < 2)
+                    s_dj[b][lane] = lane ? j1 : j0;
+                for (uint32_t j = j0 + lane; j < j1 && j < j0 + MAXD; j += 64)
+                    s_ds[b][j - j0] = a.doc_starts[j];
+            }
         }
 
         const uint32_t lds_lo = (base >= (uint32_t)HALO) ? base - HALO : 0u;
@@ -571,9 +615,9 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         auto emit = [&](uint64_t r, uint32_t s, uint32_t e) {
             if (r >= a.cap)
                 return;
-            a.offsets[r] = s;
+            put_offset(b, r, s);
             if (s < lds_lo) {   // head not in LDS: decoded by the fixup pass
-                add_fix(r);
+                add_fix(r, s, e);
                 return;
             }
             const ol_node_dev nd = decode(s, e);
@@ -645,23 +689,18 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     }
 }
 
-// Lines whose head was not in LDS: decode them from global memory.
-// Line r spans offsets[r] ..= offsets[r+1] - 1 (the last line: to the text end).
+// Lines whose head was not in LDS: decode them from global memory
+// (each entry carries the line's index and byte span).
 __global__ void scan_fixup_kernel(ScanArgs a)
 {
     const uint32_t cnt = *a.fix_count < a.fix_cap ? *a.fix_count : a.fix_cap;
-    const uint64_t lines = a.meta[0];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const uint32_t r = a.fix_list[i];
-        if (r + 1 < lines && r + 1 >= a.cap)
-            continue;   // outputs overflowed: the caller re-runs with a larger capacity
-        const uint32_t s = a.offsets[r];
-        const uint32_t e = (r + 1 < lines) ? a.offsets[r + 1] - 1 : a.n - 1;
-        const ol_node_dev nd = parse_span(GlobalSrc{a.text}, s, e);
-        a.classes[r] = (uint8_t)nd.kind;
-        a.keys[r] = key_of(nd);
+        const uint4 f = a.fix_list[i];
+        const ol_node_dev nd = parse_span(GlobalSrc{a.text}, f.y, f.z);
+        a.classes[f.x] = (uint8_t)nd.kind;
+        a.keys[f.x] = key_of(nd);
         if (a.nodes)
-            a.nodes[r] = nd;
+            a.nodes[f.x] = nd;
     }
 }
 
@@ -690,13 +729,8 @@ static const void *scan_fn_bpt(uint32_t bpt)
 // Co-resident workgroups of the scan kernel for a given bytes-per-lane.
 uint32_t ol_scan_max_grid(uint32_t bpt)
 {
-    int dev = 0, ncu = 0, per_cu = 0;
     const void *f = scan_fn_bpt(bpt);
-    if (!f || hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, SCAN_THREADS, 0) != hipSuccess)
-        return 0;
-    return (uint32_t)(per_cu * ncu);
+    return f ? ol_coresident(f, SCAN_THREADS, 0) : 0u;
 }
 
 hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, hipStream_t s)

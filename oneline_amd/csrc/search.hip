@@ -59,7 +59,7 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *s_w, ui
 
 // Tile prefix via look-back; wave 0 computes it, everyone reads it.
 __device__ __forceinline__ uint64_t tile_prefix(uint64_t *status, uint32_t t, uint64_t total,
-                                                uint64_t *s_prefix)
+                                                uint64_t *s_prefix, uint32_t *err)
 {
     if ((threadIdx.x >> 6) == 0) {
         const int lane = threadIdx.x & 63;
@@ -70,7 +70,7 @@ __device__ __forceinline__ uint64_t tile_prefix(uint64_t *status, uint32_t t, ui
         } else {
             if (lane == 0)
                 st_relaxed_u64(status + (size_t)t * LB_STRIDE, (LB_FLAG_AGG << LB_SHIFT) | total);
-            prefix = lookback_sum(status, t);
+            prefix = lookback_sum(status, t, err);
             if (lane == 0)
                 st_relaxed_u64(status + (size_t)t * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
         }
@@ -81,16 +81,26 @@ __device__ __forceinline__ uint64_t tile_prefix(uint64_t *status, uint32_t t, ui
     return *s_prefix;
 }
 
-// Co-resident workgroups of a kernel (for the persistent, cooperative launches).
-static uint32_t max_grid(const void *f, int threads, size_t lds)
+uint32_t ol_coresident(const void *f, int threads, size_t dyn_lds)
 {
-    int dev = 0, ncu = 0, per_cu = 0;
+    int dev = 0, ncu = 0, per_cu = 0, lds_cu = 0;
+    hipFuncAttributes fa{};
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, lds) != hipSuccess)
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        hipFuncGetAttributes(&fa, f) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, dyn_lds) != hipSuccess)
         return 0;
-    return (uint32_t)(per_cu * ncu);
+    const size_t lds = ((size_t)fa.sharedSizeBytes + dyn_lds + 4095) / 4096 * 4096;
+    if (lds > 0) {
+        const int by_lds = (int)((size_t)lds_cu / lds);
+        if (by_lds < per_cu)
+            per_cu = by_lds;
+    }
+    return per_cu > 0 ? (uint32_t)(per_cu * ncu) : 0u;
 }
+
+static uint32_t max_grid(const void *f, int threads, size_t lds) { return ol_coresident(f, threads, lds); }
 
 // ---- single literal -----------------------------------------------------------
 // Persistent: tile t = blockIdx + k * gridDim (all workgroups co-resident, each
@@ -150,7 +160,7 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         }
         uint32_t total;
         const uint32_t excl = block_excl_sum<THREADS>((uint32_t)__popcll(match), s_w, total);
-        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix);
+        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix, a.lb_err);
         if (tid == 0 && t == a.num_tiles - 1)
             a.meta[0] = prefix + total;
         uint64_t r = prefix + excl;
@@ -196,11 +206,12 @@ __device__ __forceinline__ uint32_t table_hash(uint32_t key, uint32_t bits)
 
 // Matches starting at local position lp (global gp).  When `emit`, writes them
 // at out[r..] (up to cap) in pattern-id order.  Returns the number of matches.
+// `lim` (> gp) bounds the match end: the text end, or the end of gp's document.
 __device__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
-                                   uint32_t lp, uint32_t gp, uint32_t n, uint32_t w, bool emit,
+                                   uint32_t lp, uint32_t gp, uint32_t lim, uint32_t w, bool emit,
                                    uint64_t *out_off, uint32_t *out_id, uint64_t &r, uint64_t cap)
 {
-    const uint32_t rem = n - gp;
+    const uint32_t rem = lim - gp;
     uint32_t beg[4], end[4], kk[4];
     int nr = 0;
 #pragma unroll
@@ -310,13 +321,36 @@ __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
             lim = BPT;
 
         const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
+        // match-end bound per position: the text end, or (batches) the end of
+        // the position's document; positions between documents match nothing
+        uint32_t dlim = a.n, dnext = 0xFFFFFFFFu;
+        uint32_t di = 0;
+        if (a.ndocs) {
+            di = a.tile_doc[t];   // then the last document starting at or before g0
+            while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= g0)
+                di++;
+            dlim = a.doc_ends[di];
+            dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
+        }
+        auto bound = [&](uint32_t gp) -> uint32_t {   // 0: no match may start at gp
+            while (gp >= dnext) {
+                di++;
+                dlim = a.doc_ends[di];
+                dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
+            }
+            return gp < dlim ? dlim : 0u;
+        };
+        const uint32_t di0 = di, dlim0 = dlim, dnext0 = dnext;
         uint64_t has = 0;
         uint32_t count = 0;
         uint64_t dummy = 0;
         for (uint32_t p = 0; p < lim; p++) {
+            const uint32_t bl = bound(g0 + p);
+            if (!bl)
+                continue;
             uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
             uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
-            uint32_t c = probe_position(a.d, s_filter, s_text, lo + p, g0 + p, a.n, win, false,
+            uint32_t c = probe_position(a.d, s_filter, s_text, lo + p, g0 + p, bl, win, false,
                                         nullptr, nullptr, dummy, 0);
             if (c) {
                 has |= 1ull << p;
@@ -325,18 +359,21 @@ __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
         }
         uint32_t total;
         const uint32_t excl = block_excl_sum<THREADS>(count, s_w, total);
-        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix);
+        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix, a.lb_err);
         if (tid == 0 && t == a.num_tiles - 1)
             a.meta[0] = prefix + total;
         if (a.out_off || a.out_id) {
             uint64_t r = prefix + excl;
+            di = di0;
+            dlim = dlim0;
+            dnext = dnext0;
             while (has && r < a.cap) {
                 uint32_t p = (uint32_t)__builtin_ctzll(has);
                 has &= has - 1;
                 uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
                 uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
-                probe_position(a.d, s_filter, s_text, lo + p, g0 + p, a.n, win, true, a.out_off, a.out_id,
-                               r, a.cap);
+                probe_position(a.d, s_filter, s_text, lo + p, g0 + p, bound(g0 + p), win, true, a.out_off,
+                               a.out_id, r, a.cap);
             }
         }
         __syncthreads();   // LDS is reused by the next tile

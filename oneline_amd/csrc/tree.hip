@@ -37,23 +37,41 @@
 // crates/oneline-gpu/src/tree.rs:10-12,218-222).
 #include "internal.h"
 
-constexpr int TW = 256;              // threads per tile (4 waves)
-constexpr int TR = 16;               // rounds of 64 lines per wave
-constexpr int TLINES = TW * TR;      // lines per tile
+constexpr int TW = 256;              // threads per tile (4 waves); a tile = TW x TR lines
 constexpr int D = TREE_D;            // levels tracked
 static_assert(D == 32, "five key bits, lanes 0..32");
 
 __device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
-// Keys of this lane's 16 rounds (lines past the end read as blank).
-__device__ __forceinline__ void load_keys(const uint8_t *keys, uint32_t wbase, uint64_t L, uint32_t (&key)[TR])
+// The tile's document: its first global line, line count and the tile's
+// index inside it (a single document when nsegs == 0).
+struct TileSeg {
+    uint64_t g0;
+    uint64_t L;
+    uint32_t lt;
+};
+__device__ __forceinline__ TileSeg tile_seg(const TreeArgs &a, uint32_t t)
+{
+    TileSeg ts{0, a.lines, t};
+    if (a.nsegs) {
+        const uint32_t sg = a.tile_seg[t];
+        ts.g0 = a.seg_line0[sg];
+        ts.L = a.seg_lines[sg];
+        ts.lt = t - a.seg_tile0[sg];
+    }
+    return ts;
+}
+
+// Keys of this lane's TR rounds (lines past the end read as blank).
+template <int TR>
+__device__ __forceinline__ void load_keys(const uint8_t *keys, const TileSeg &ts, uint32_t wbase, uint32_t (&key)[TR])
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int r = 0; r < TR; r++) {
         const uint32_t g = wbase + r * 64 + lane;
-        key[r] = (g < L) ? (uint32_t)keys[g] : KEY_BLANK;
+        key[r] = (g < ts.L) ? (uint32_t)keys[ts.g0 + g] : KEY_BLANK;
     }
 }
 
@@ -91,14 +109,16 @@ __device__ __forceinline__ void lane_bit_masks(uint64_t (&dm)[5])
 
 // Per tile: for each level the last content line (+1, 0 = none) and the first
 // (NONE = none); error flag for content lines at depth >= D.
+template <int TR>
 __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
 {
     __shared__ uint32_t s_l[TW / 64][D], s_f[TW / 64][D];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t t = blockIdx.x;
-    const uint32_t wbase = t * TLINES + w * (TR * 64);
+    const TileSeg ts = tile_seg(a, t);
+    const uint32_t wbase = ts.lt * (TW * TR) + w * (TR * 64);
     uint32_t key[TR];
-    load_keys(a.keys, wbase, a.lines, key);
+    load_keys<TR>(a.keys, ts, wbase, key);
     uint64_t dm[5];
     lane_bit_masks(dm);
     uint32_t last = 0, first = OL_NONE;
@@ -134,111 +154,122 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
     }
 }
 
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
 // One workgroup per level (arrays level-major, one row of num_tiles per
-// level): in_last[t] = max of sum_last over tiles < t, out_first[t] = min of
-// sum_first over tiles > t.  The row goes through LDS in chunks: coalesced
-// loads, each thread scans a contiguous run of the chunk, a block scan joins
-// the runs, coalesced stores.
+// level): in_last[t] = max of sum_last over the earlier tiles of t's document,
+// out_first[t] = min of sum_first over its later tiles.  The scan runs over
+// (document << 32 | value): the maximum over earlier tiles carries t's own
+// document exactly when one of them belongs to it (documents are contiguous
+// runs of tiles), and likewise the minimum over later tiles.  The row goes
+// through LDS in chunks: coalesced loads, each thread scans a contiguous run
+// of the chunk, a block scan joins the runs, coalesced stores.
 template <int ST>
 __global__ __launch_bounds__(ST) void tree_scan_kernel(TreeArgs a)
 {
     constexpr int NWS = ST / 64;
-    constexpr int K = 8;                 // tiles per thread per chunk
+    constexpr int K = 4;                 // tiles per thread per chunk
     constexpr int CH = ST * K;
-    __shared__ uint32_t s_v[CH];
-    __shared__ uint32_t s_w[NWS];
+    __shared__ uint64_t s_v[CH];
+    __shared__ uint64_t s_w[NWS];
     const uint32_t T = a.num_tiles;
     const size_t row = (size_t)blockIdx.x * T;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    auto seg = [&](uint32_t t) -> uint64_t { return a.nsegs ? (uint64_t)a.tile_seg[t] : 0ull; };
 
-    // forward: exclusive prefix max of sum_last
-    uint32_t carry = 0;
+    // forward: exclusive prefix max
+    uint64_t carry = 0;
     for (uint32_t c0 = 0; c0 < T; c0 += CH) {
         const uint32_t n = (T - c0) < (uint32_t)CH ? T - c0 : (uint32_t)CH;
         for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
-            s_v[i] = i < n ? a.sum_last[row + c0 + i] : 0u;
+            s_v[i] = i < n ? (seg(c0 + i) << 32 | a.sum_last[row + c0 + i]) : 0ull;
         __syncthreads();
-        uint32_t loc[K], m = 0;
+        uint64_t loc[K], m = 0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             loc[k] = s_v[tid * K + k];
-            m = umax(m, loc[k]);
+            m = umax64(m, loc[k]);
         }
-        uint32_t inc = m;
+        uint64_t inc = m;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(inc, o, 64);
+            const uint64_t y = __shfl_up(inc, o, 64);
             if (lane >= o)
-                inc = umax(inc, y);
+                inc = umax64(inc, y);
         }
         if (lane == 63)
             s_w[w] = inc;
         __syncthreads();
-        uint32_t run = __shfl_up(inc, 1, 64);
+        uint64_t run = __shfl_up(inc, 1, 64);
         if (lane == 0)
             run = 0;
-        uint32_t tot = 0;
+        uint64_t tot = 0;
         for (int k = 0; k < NWS; k++) {
             if (k < w)
-                run = umax(run, s_w[k]);
-            tot = umax(tot, s_w[k]);
+                run = umax64(run, s_w[k]);
+            tot = umax64(tot, s_w[k]);
         }
-        run = umax(run, carry);
+        run = umax64(run, carry);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             s_v[tid * K + k] = run;
-            run = umax(run, loc[k]);
+            run = umax64(run, loc[k]);
         }
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += ST)
-            a.in_last[row + c0 + i] = s_v[i];
-        carry = umax(carry, tot);
+        for (uint32_t i = tid; i < n; i += ST) {
+            const uint64_t r = s_v[i];
+            a.in_last[row + c0 + i] = (r >> 32) == seg(c0 + i) ? (uint32_t)r : 0u;
+        }
+        carry = umax64(carry, tot);
         __syncthreads();
     }
 
-    // backward: exclusive suffix min of sum_first
-    carry = OL_NONE;
+    // backward: exclusive suffix min
+    carry = ~0ull;
     for (uint32_t c1 = T; c1 > 0;) {
         const uint32_t n = c1 < (uint32_t)CH ? c1 : (uint32_t)CH;
         const uint32_t c0 = c1 - n;
         for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
-            s_v[i] = i < n ? a.sum_first[row + c0 + i] : OL_NONE;
+            s_v[i] = i < n ? (seg(c0 + i) << 32 | a.sum_first[row + c0 + i]) : ~0ull;
         __syncthreads();
-        uint32_t loc[K], m = OL_NONE;
+        uint64_t loc[K], m = ~0ull;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             loc[k] = s_v[tid * K + k];
-            m = umin(m, loc[k]);
+            m = umin64(m, loc[k]);
         }
-        uint32_t inc = m;   // inclusive suffix min over lanes >= lane
+        uint64_t inc = m;   // inclusive suffix min over lanes >= lane
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_down(inc, o, 64);
+            const uint64_t y = __shfl_down(inc, o, 64);
             if (lane + o < 64)
-                inc = umin(inc, y);
+                inc = umin64(inc, y);
         }
         if (lane == 0)
             s_w[w] = inc;
         __syncthreads();
-        uint32_t run = __shfl_down(inc, 1, 64);
+        uint64_t run = __shfl_down(inc, 1, 64);
         if (lane == 63)
-            run = OL_NONE;
-        uint32_t tot = OL_NONE;
+            run = ~0ull;
+        uint64_t tot = ~0ull;
         for (int k = 0; k < NWS; k++) {
             if (k > w)
-                run = umin(run, s_w[k]);
-            tot = umin(tot, s_w[k]);
+                run = umin64(run, s_w[k]);
+            tot = umin64(tot, s_w[k]);
         }
-        run = umin(run, carry);
+        run = umin64(run, carry);
 #pragma unroll
         for (int k = K - 1; k >= 0; k--) {
             s_v[tid * K + k] = run;
-            run = umin(run, loc[k]);
+            run = umin64(run, loc[k]);
         }
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += ST)
-            a.out_first[row + c0 + i] = s_v[i];
-        carry = umin(carry, tot);
+        for (uint32_t i = tid; i < n; i += ST) {
+            const uint64_t r = s_v[i];
+            a.out_first[row + c0 + i] = (r >> 32) == seg(c0 + i) ? (uint32_t)r : OL_NONE;
+        }
+        carry = umin64(carry, tot);
         c1 = c0;
         __syncthreads();
     }
@@ -251,6 +282,7 @@ __global__ __launch_bounds__(ST) void tree_scan_kernel(TreeArgs a)
 // its key, and H[d] = first line after the round with key <= d and its key.
 // G moves forward round by round from the round's own masks (the last line of
 // {key < d}); H is filled backwards over the rounds in LDS before the walk.
+template <int TR>
 __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
 {
     constexpr int NWV = TW / 64;
@@ -262,10 +294,11 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t t = blockIdx.x;
     const uint32_t T = a.num_tiles;
-    const uint64_t L = a.lines;
-    const uint32_t wbase = t * TLINES + w * (TR * 64);
+    const TileSeg ts = tile_seg(a, t);
+    const uint64_t L = ts.L;
+    const uint32_t wbase = ts.lt * (TW * TR) + w * (TR * 64);
     uint32_t key[TR];
-    load_keys(a.keys, wbase, L, key);
+    load_keys<TR>(a.keys, ts, wbase, key);
 
     // pass 1: masks; per level the wave's last / first line; per round the
     // first line with key <= d
@@ -383,10 +416,11 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
             ns = (se < L && (par == OL_NONE || plv < slv)) ? se : OL_NONE;
         }
         if (g < L) {
-            if (P) P[g] = par;
-            if (SE) SE[g] = se;
-            if (FC) FC[g] = fc;
-            if (NS) NS[g] = ns;
+            const uint64_t o = ts.g0 + g;
+            if (P) P[o] = par;
+            if (SE) SE[o] = se;
+            if (FC) FC[o] = fc;
+            if (NS) NS[o] = ns;
         }
         // G after this round: the round's last line with key < d, if any
         const uint64_t ltd = s_m[w][r][lane < D ? lane : 0];
@@ -399,12 +433,23 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     }
 }
 
-hipError_t ol_launch_tree(const TreeArgs &a, hipStream_t s)
+template <int TR>
+static void launch_tree_tr(const TreeArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(tree_summary_kernel, dim3(a.num_tiles), dim3(TW), 0, s, a);
+    hipLaunchKernelGGL((tree_summary_kernel<TR>), dim3(a.num_tiles), dim3(TW), 0, s, a);
     hipLaunchKernelGGL((tree_scan_kernel<1024>), dim3(D), dim3(1024), 0, s, a);
-    hipLaunchKernelGGL(tree_walk_kernel, dim3(a.num_tiles), dim3(TW), 0, s, a);
+    hipLaunchKernelGGL((tree_walk_kernel<TR>), dim3(a.num_tiles), dim3(TW), 0, s, a);
+}
+
+hipError_t ol_launch_tree(const TreeArgs &a, uint32_t rounds, hipStream_t s)
+{
+    switch (rounds) {
+    case 1: launch_tree_tr<1>(a, s); break;
+    case 4: launch_tree_tr<4>(a, s); break;
+    case 16: launch_tree_tr<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-uint32_t ol_tree_tile_lines() { return TLINES; }
+uint32_t ol_tree_tile_lines(uint32_t rounds) { return TW * rounds; }
