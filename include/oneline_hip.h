@@ -170,13 +170,45 @@ int ol_search_many(ol_ctx *ctx, const ol_doc *doc, const ol_dict *dict, size_t m
 int ol_search_many_device(ol_ctx *ctx, const ol_doc *doc, const ol_dict *dict, uint64_t *d_offsets,
                           uint32_t *d_ids, size_t cap, uint64_t *total);
 
+/* ---- batches (no counterpart in the reference, which handles one document
+ * per call): parse, tree and dictionary search of many documents as one
+ * launch sequence over one HBM text, replayable as a captured hipGraph.
+ * Results per document are those of the single-document calls. */
+typedef struct ol_batch ol_batch;
+/* Uploads `count` documents back to back (empty ones allowed). */
+int ol_batch_create(ol_ctx *ctx, const uint8_t *const *docs, const size_t *lens, size_t count, ol_batch **out);
+void ol_batch_free(ol_batch *batch);
+size_t ol_batch_doc_count(const ol_batch *batch);
+/* Sum of the document lengths. */
+size_t ol_batch_text_bytes(const ol_batch *batch);
+/* Parse + tree of every document, then (dict non-NULL) its dictionary search.
+ * The first call for a given dictionary and launch tuning sizes the outputs
+ * (synchronous); later calls launch the same sequence asynchronously, as a
+ * captured hipGraph when `graph` is non-zero.  ol_ctx_synchronize waits. */
+int ol_batch_run(ol_ctx *ctx, ol_batch *batch, const ol_dict *dict, int graph);
+/* Total lines; per document its first line (in the batch's line order) and
+ * line count (either pointer may be NULL; arrays of ol_batch_doc_count). */
+int ol_batch_lines(ol_ctx *ctx, const ol_batch *batch, uint64_t *total, uint32_t *doc_line_start,
+                   uint32_t *doc_line_count);
+/* Per-line arrays of document `doc` (its line count entries each; any pointer
+ * may be NULL): offsets relative to the document, classes (NodeKind as u8),
+ * and tree links local to the document. */
+int ol_batch_download_doc(ol_ctx *ctx, const ol_batch *batch, size_t doc, uint32_t *offsets, uint8_t *classes,
+                          uint32_t *parent, uint32_t *subtree_end, uint32_t *first_child, uint32_t *next_sibling);
+/* Dictionary matches of document `doc` sorted by (offset, id), offsets
+ * relative to the document: *count = all of them, the first `cap` copied. */
+int ol_batch_matches(ol_ctx *ctx, ol_batch *batch, size_t doc, uint64_t *count, uint32_t *offsets, uint32_t *ids,
+                     size_t cap);
+
 /* ---- launch tuning (no counterpart in the reference: its tile sizes are
  * compile-time constants, e.g. crates/oneline-gpu/src/scan.rs:11).  The
  * autotuner (oneline_amd/autotune.py) sets these from measurements:
  *   "scan_bpt"      bytes per lane of the scan kernel: 64, 128 or 256
  *                   (tile = 256 lanes x bpt bytes);
  *   "scan_grid"     persistent workgroups of the scan kernel (0 = as many as
- *                   are co-resident on the device, the default). */
+ *                   are co-resident on the device, the default);
+ *   "tree_rounds"   rounds of 64 lines per tree wave: 1, 4 or 16 (tile =
+ *                   256 x rounds lines; 16 by default). */
 int ol_tune_set(ol_ctx *ctx, const char *name, int value);
 int ol_tune_get(ol_ctx *ctx, const char *name, int *value);
 

@@ -101,15 +101,23 @@ def load_library() -> ctypes.CDLL:
         "ol_tune_set": [P, ctypes.c_char_p, ctypes.c_int],
         "ol_tune_get": [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
         "ol_profile_take": [P, P, S, P, P, S, ctypes.POINTER(S)],
+        "ol_batch_create": [P, P, P, S, ctypes.POINTER(P)],
+        "ol_batch_run": [P, P, P, ctypes.c_int],
+        "ol_batch_lines": [P, P, U64P, P, P],
+        "ol_batch_download_doc": [P, P, S, P, P, P, P, P, P],
+        "ol_batch_matches": [P, P, S, U64P, P, P, S],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
-    for name in ("ol_doc_line_count", "ol_doc_text_len", "ol_tree_line_count", "ol_dict_len"):
+    for name in ("ol_doc_line_count", "ol_doc_text_len", "ol_tree_line_count", "ol_dict_len",
+                 "ol_batch_doc_count", "ol_batch_text_bytes"):
         getattr(lib, name).restype = ctypes.c_size_t
-    for name in ("ol_ctx_destroy", "ol_free", "ol_doc_free", "ol_tree_free", "ol_dict_free"):
+        getattr(lib, name).argtypes = [P]
+    for name in ("ol_ctx_destroy", "ol_free", "ol_doc_free", "ol_tree_free", "ol_dict_free", "ol_batch_free"):
         getattr(lib, name).restype = None
+        getattr(lib, name).argtypes = [P]
     lib.ol_last_error.argtypes = []
     lib.ol_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -280,6 +288,68 @@ class GpuPatternSet:
             pass
 
 
+class GpuBatch:
+    """Many documents parsed, tree-linked and dictionary-searched by one launch
+    sequence (ol_batch_*); per-document results equal the single-document calls."""
+
+    def __init__(self, parser: "GpuParser", handle: ctypes.c_void_p, keep):
+        self._p = parser
+        self._h = handle
+        self._keep = keep
+
+    def __len__(self) -> int:
+        return int(_lib.ol_batch_doc_count(self._h))
+
+    def text_bytes(self) -> int:
+        return int(_lib.ol_batch_text_bytes(self._h))
+
+    def run(self, dictionary: Optional["GpuPatternSet"] = None, graph: bool = False) -> None:
+        """Runs (the first time: sizes, synchronously) parse + tree (+ search)."""
+        _check(_lib.ol_batch_run(self._p._ctx, self._h, dictionary._h if dictionary is not None else None,
+                                 1 if graph else 0))
+
+    def lines(self):
+        """(total lines, per-document first line, per-document line count)."""
+        n = len(self)
+        st = np.zeros(max(1, n), np.uint32)
+        cnt = np.zeros(max(1, n), np.uint32)
+        tot = ctypes.c_uint64()
+        _check(_lib.ol_batch_lines(self._p._ctx, self._h, ctypes.byref(tot), _ptr(st), _ptr(cnt)))
+        return tot.value, st[:n], cnt[:n]
+
+    def document(self, i: int):
+        """(offsets, classes, TreeLinks) of document i, local to it."""
+        _, st, cnt = self.lines()
+        n = int(cnt[i])
+        off = np.zeros(n, np.uint32)
+        cls = np.zeros(n, np.uint8)
+        par, se, fc, ns = (np.zeros(n, np.uint32) for _ in range(4))
+        _check(_lib.ol_batch_download_doc(self._p._ctx, self._h, i, _ptr(off), _ptr(cls), _ptr(par), _ptr(se),
+                                          _ptr(fc), _ptr(ns)))
+        return off, cls, TreeLinks(par, se, fc, ns)
+
+    def matches(self, i: int):
+        """(offsets, ids) of document i's dictionary matches, (offset, id) order."""
+        cnt = ctypes.c_uint64()
+        _check(_lib.ol_batch_matches(self._p._ctx, self._h, i, ctypes.byref(cnt), None, None, 0))
+        off = np.zeros(cnt.value, np.uint32)
+        ids = np.zeros(cnt.value, np.uint32)
+        _check(_lib.ol_batch_matches(self._p._ctx, self._h, i, ctypes.byref(cnt), _ptr(off), _ptr(ids),
+                                     cnt.value))
+        return off, ids
+
+    def close(self) -> None:
+        if self._h:
+            _lib.ol_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class GpuParser:
     """GpuParser (crates/oneline-gpu/src/parser.rs:8-57) over HIP device 0."""
 
@@ -414,6 +484,16 @@ class GpuParser:
         return v.value
 
     # -- profiling ----------------------------------------------------------------------
+    def batch(self, documents: Sequence) -> GpuBatch:
+        """Uploads the documents back to back (ol_batch_create)."""
+        arrs = [as_bytes_array(d) for d in documents]
+        ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data if a.size else None for a in arrs])
+        lens = (ctypes.c_size_t * max(1, len(arrs)))(*[a.size for a in arrs])
+        h = ctypes.c_void_p()
+        _check(_lib.ol_batch_create(self._ctx, ctypes.cast(ptrs, ctypes.c_void_p), ctypes.cast(lens, ctypes.c_void_p),
+                                    len(arrs), ctypes.byref(h)))
+        return GpuBatch(self, h, arrs)
+
     def profile_enable(self, on: bool = True) -> None:
         _check(_lib.ol_profile_enable(self._ctx, 1 if on else 0))
 

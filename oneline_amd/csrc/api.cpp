@@ -203,25 +203,19 @@ int alloc_doc_lines(ol_doc *d, uint64_t cap)
     return OL_OK;
 }
 
-int launch_scan(ol_ctx *c, ol_doc *d)
+// Scan + classify of a text into the per-line arrays set in `a` (text, n,
+// cap, outputs and, for batches, the document fields); fills in the rest.
+int launch_scan_args(ol_ctx *c, ScanArgs &a, bool cooperative = true)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->n, (uint64_t)SCAN_THREADS * c->scan_bpt);
+    uint32_t tiles = (uint32_t)ceil_div(a.n, (uint64_t)SCAN_THREADS * c->scan_bpt);
     HIPCHK(ensure_scratch(c, tiles));
     HIPCHK(clear_scratch(c, tiles));
-    ScanArgs a{};
-    a.text = d->text;
-    a.n = (uint32_t)d->n;
     a.num_tiles = tiles;
-    a.cap = d->cap;
-    a.offsets = d->offsets;
-    a.classes = d->classes;
-    a.keys = d->keys;
-    a.nodes = d->nodes;
     a.status = c->status();
     a.meta = c->meta();
     a.lb_err = c->ctr() + 4;
     // a line needing the fixup pass is longer than the scan halo: at most n / 256 of them
-    a.fix_cap = (uint32_t)(d->n / SCAN_HALO + 64);
+    a.fix_cap = (uint32_t)(a.n / SCAN_HALO + 64);
     HIPCHK(c->fix.ensure((size_t)a.fix_cap * 16));
     a.fix_list = (uint4 *)c->fix.p;
     a.fix_count = c->ctr() + 2;
@@ -237,11 +231,24 @@ int launch_scan(ol_ctx *c, ol_doc *d)
         grid = tiles;
     {
         Prof p(c, "scan_classify");
-        HIPCHK(ol_launch_scan(a, c->scan_bpt, grid, c->stream));
+        HIPCHK(ol_launch_scan(a, c->scan_bpt, grid, cooperative, c->stream));
     }
     Prof p(c, "scan_fixup");
     HIPCHK(ol_launch_scan_fixup(a, c->stream));
     return OL_OK;
+}
+
+int launch_scan(ol_ctx *c, ol_doc *d)
+{
+    ScanArgs a{};
+    a.text = d->text;
+    a.n = (uint32_t)d->n;
+    a.cap = d->cap;
+    a.offsets = d->offsets;
+    a.classes = d->classes;
+    a.keys = d->keys;
+    a.nodes = d->nodes;
+    return launch_scan_args(c, a);
 }
 
 void sample_histogram(const uint8_t *t, size_t n, uint32_t *freq)
@@ -1196,6 +1203,529 @@ int ol_profile_take(ol_ctx *c, char *names, size_t name_stride, uint64_t *counts
         if (ms) ms[i] = sum[i];
     }
     *n = keys.size();
+    return OL_OK;
+}
+
+}  // extern "C"
+
+// ---- batches ------------------------------------------------------------------------
+// Many documents in one launch sequence (no counterpart in the reference,
+// whose every call handles one document).  The documents lie back to back in
+// one HBM text, each followed by a '\n' unless it already ends in one (an
+// unterminated last line keeps its start, class and key: the trailing '\n' is
+// trimmed like any line end); empty documents occupy nothing and have no
+// lines.  The scan runs over the whole text and writes document-relative
+// offsets plus each document's first line; the tree runs one segment per
+// document (tiles never cross documents); the dictionary search rejects
+// matches that run past their document.  The launch sequence is sized once
+// (host round trips) and then replayed, optionally as a captured hipGraph.
+
+struct ol_batch {
+    size_t count = 0;                     // documents, empty ones included
+    std::vector<uint64_t> h_start, h_len; // per document: byte start in the batch text, length
+    std::vector<uint32_t> h_dev;          // per document: index among the non-empty ones (or ~0u)
+    uint32_t nd = 0;                      // non-empty documents
+    uint64_t n = 0;                       // batch text bytes (documents and separators)
+    uint64_t payload = 0;                 // document bytes
+    uint8_t *text = nullptr;
+    DevBuf d_starts, d_ends, d_line_start;   // [nd]
+    DevBuf scan_tile_doc, search_tile_doc;
+    uint32_t scan_tile_doc_bpt = 0;
+    // per-line outputs (global line order)
+    uint64_t cap = 0, lines = 0;
+    DevBuf offsets, classes, keys, parent, fc, ns, se;
+    std::vector<uint32_t> h_line_start, h_line_count;   // [nd]
+    // tree segments
+    DevBuf seg_line0, seg_lines, seg_tile0, tile_seg, tree_sum;
+    uint32_t tree_tiles = 0;
+    // dictionary matches, sorted by (global offset, id)
+    DevBuf m_off, m_id;
+    uint64_t m_cap = 0, m_total = 0;
+    std::vector<uint64_t> h_moff;
+    std::vector<uint32_t> h_mid;
+    bool h_matches = false;
+    // the sized launch sequence
+    bool prepared = false;
+    uint32_t prep_bpt = 0, prep_rounds = 0;
+    const ol_dict *prep_dict = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+namespace {
+
+// scan + fixup of the batch text (outputs sized by b->cap)
+int batch_scan(ol_ctx *c, ol_batch *b, bool cooperative)
+{
+    ScanArgs a{};
+    a.text = b->text;
+    a.n = (uint32_t)b->n;
+    a.cap = b->cap;
+    a.offsets = (uint32_t *)b->offsets.p;
+    a.classes = (uint8_t *)b->classes.p;
+    a.keys = (uint8_t *)b->keys.p;
+    a.ndocs = b->nd;
+    a.doc_starts = (const uint32_t *)b->d_starts.p;
+    a.tile_doc = (const uint32_t *)b->scan_tile_doc.p;
+    a.doc_line_start = (uint32_t *)b->d_line_start.p;
+    return launch_scan_args(c, a, cooperative);
+}
+
+int batch_tree(ol_ctx *c, ol_batch *b)
+{
+    const size_t part = (size_t)b->tree_tiles * TREE_D;
+    HIPCHK(ensure_scratch(c, 0));
+    HIPCHK(b->tree_sum.ensure(part * 16));
+    HIPCHK(clear_scratch(c, 0));
+    uint32_t *base = (uint32_t *)b->tree_sum.p;
+    TreeArgs a{};
+    a.keys = (const uint8_t *)b->keys.p;
+    a.lines = b->lines;
+    a.num_tiles = b->tree_tiles;
+    a.parent = (uint32_t *)b->parent.p;
+    a.first_child = (uint32_t *)b->fc.p;
+    a.next_sibling = (uint32_t *)b->ns.p;
+    a.subtree_end = (uint32_t *)b->se.p;
+    a.sum_last = base;
+    a.sum_first = base + part;
+    a.in_last = base + 2 * part;
+    a.out_first = base + 3 * part;
+    a.err = c->ctr() + 1;
+    a.nsegs = b->nd;
+    a.seg_line0 = (const uint32_t *)b->seg_line0.p;
+    a.seg_lines = (const uint32_t *)b->seg_lines.p;
+    a.seg_tile0 = (const uint32_t *)b->seg_tile0.p;
+    a.tile_seg = (const uint32_t *)b->tile_seg.p;
+    Prof p(c, "tree");
+    HIPCHK(ol_launch_tree(a, c->tree_rounds, c->stream));
+    return OL_OK;
+}
+
+int batch_many(ol_ctx *c, ol_batch *b, const ol_dict *s, uint64_t cap)
+{
+    const uint32_t tiles = (uint32_t)ceil_div(b->n, SEARCH_TILE);
+    HIPCHK(ensure_scratch(c, tiles));
+    HIPCHK(clear_scratch(c, tiles));
+    ManyArgs a{};
+    a.text = b->text;
+    a.n = (uint32_t)b->n;
+    a.num_tiles = tiles;
+    a.d = s->d;
+    a.cap = cap;
+    a.out_off = cap ? (uint64_t *)b->m_off.p : nullptr;
+    a.out_id = cap ? (uint32_t *)b->m_id.p : nullptr;
+    a.status = c->status();
+    a.meta = c->meta();
+    a.lb_err = c->ctr() + 4;
+    a.tile_ctr = c->ctr() + 3;
+    a.ndocs = b->nd;
+    a.doc_starts = (const uint32_t *)b->d_starts.p;
+    a.doc_ends = (const uint32_t *)b->d_ends.p;
+    a.tile_doc = (const uint32_t *)b->search_tile_doc.p;
+    Prof p(c, "many");
+    HIPCHK(ol_launch_many(a, c->stream));
+    return OL_OK;
+}
+
+int alloc_batch_lines(ol_batch *b, uint64_t cap)
+{
+    const uint64_t capa = round_up(cap ? cap : 1, LINE_ALIGN);
+    HIPCHK(b->offsets.ensure(capa * 4));
+    HIPCHK(b->classes.ensure(capa));
+    HIPCHK(b->keys.ensure(capa));
+    HIPCHK(b->parent.ensure(capa * 4));
+    HIPCHK(b->fc.ensure(capa * 4));
+    HIPCHK(b->ns.ensure(capa * 4));
+    HIPCHK(b->se.ensure(capa * 4));
+    b->cap = capa;
+    return OL_OK;
+}
+
+// Sizes and runs the whole sequence once, with host round trips.
+int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict)
+{
+    b->drop_graph();
+    b->prepared = false;
+    b->h_matches = false;
+    if (b->nd == 0) {
+        b->lines = 0;
+        b->m_total = 0;
+        b->prepared = true;
+        b->prep_bpt = c->scan_bpt;
+        b->prep_rounds = c->tree_rounds;
+        b->prep_dict = dict;
+        return OL_OK;
+    }
+    // scan tiles -> first document starting at or after each tile
+    const uint64_t stile = (uint64_t)SCAN_THREADS * c->scan_bpt;
+    if (b->scan_tile_doc_bpt != c->scan_bpt) {
+        const uint32_t T = (uint32_t)ceil_div(b->n, stile);
+        std::vector<uint32_t> td(T + 1);
+        uint32_t j = 0;
+        // starts of the non-empty documents, ascending
+        std::vector<uint32_t> st;
+        st.reserve(b->nd);
+        for (size_t i = 0; i < b->count; i++)
+            if (b->h_dev[i] != ~0u)
+                st.push_back((uint32_t)b->h_start[i]);
+        j = 0;
+        for (uint32_t t = 0; t <= T; t++) {
+            const uint64_t pos = (uint64_t)t * stile;
+            while (j < b->nd && st[j] < pos)
+                j++;
+            td[t] = j;
+        }
+        HIPCHK(b->scan_tile_doc.ensure(td.size() * 4));
+        HIPCHK(hipMemcpy(b->scan_tile_doc.p, td.data(), td.size() * 4, hipMemcpyHostToDevice));
+        b->scan_tile_doc_bpt = c->scan_bpt;
+    }
+    // scan (re-run with a larger capacity if the estimate was short)
+    for (int attempt = 0; attempt < 2; attempt++) {
+        int rc = batch_scan(c, b, true);
+        if (rc)
+            return rc;
+        uint64_t lines = 0;
+        rc = read_meta(c, &lines);
+        if (rc)
+            return rc;
+        b->lines = lines;
+        if (lines <= b->cap)
+            break;
+        rc = alloc_batch_lines(b, lines);
+        if (rc)
+            return rc;
+    }
+    if (b->lines > b->cap)
+        return fail(OL_ERR_DEVICE, "batch line count exceeds the sized capacity");
+    // per-document line ranges -> tree segments
+    b->h_line_start.assign(b->nd, 0);
+    HIPCHK(hipMemcpy(b->h_line_start.data(), b->d_line_start.p, (size_t)b->nd * 4, hipMemcpyDeviceToHost));
+    b->h_line_count.assign(b->nd, 0);
+    const uint64_t TL = ol_tree_tile_lines(c->tree_rounds);
+    std::vector<uint32_t> seg_tile0(b->nd), tile_seg;
+    uint64_t tiles = 0;
+    for (uint32_t j = 0; j < b->nd; j++) {
+        const uint64_t end = (j + 1 < b->nd) ? b->h_line_start[j + 1] : b->lines;
+        b->h_line_count[j] = (uint32_t)(end - b->h_line_start[j]);
+        seg_tile0[j] = (uint32_t)tiles;
+        const uint64_t k = ceil_div(b->h_line_count[j] ? b->h_line_count[j] : 1, TL);
+        for (uint64_t q = 0; q < k; q++)
+            tile_seg.push_back(j);
+        tiles += k;
+    }
+    b->tree_tiles = (uint32_t)tiles;
+    HIPCHK(b->seg_line0.ensure((size_t)b->nd * 4));
+    HIPCHK(b->seg_lines.ensure((size_t)b->nd * 4));
+    HIPCHK(b->seg_tile0.ensure((size_t)b->nd * 4));
+    HIPCHK(b->tile_seg.ensure(tile_seg.size() * 4));
+    HIPCHK(hipMemcpy(b->seg_line0.p, b->h_line_start.data(), (size_t)b->nd * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->seg_lines.p, b->h_line_count.data(), (size_t)b->nd * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->seg_tile0.p, seg_tile0.data(), (size_t)b->nd * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->tile_seg.p, tile_seg.data(), tile_seg.size() * 4, hipMemcpyHostToDevice));
+    int rc = batch_tree(c, b);
+    if (rc)
+        return rc;
+    rc = check_tree_error(c);
+    if (rc)
+        return rc;
+    // dictionary: count, size, write
+    b->m_total = 0;
+    if (dict) {
+        rc = batch_many(c, b, dict, 0);
+        if (rc)
+            return rc;
+        uint64_t total = 0;
+        rc = read_meta(c, &total);
+        if (rc)
+            return rc;
+        HIPCHK(b->m_off.ensure((total ? total : 1) * 8));
+        HIPCHK(b->m_id.ensure((total ? total : 1) * 4));
+        b->m_cap = total;
+        rc = batch_many(c, b, dict, total ? total : 1);
+        if (rc)
+            return rc;
+        rc = read_meta(c, &total);
+        if (rc)
+            return rc;
+        b->m_total = total;
+    }
+    b->prepared = true;
+    b->prep_bpt = c->scan_bpt;
+    b->prep_rounds = c->tree_rounds;
+    b->prep_dict = dict;
+    return OL_OK;
+}
+
+// The sized sequence, launched asynchronously (no host round trip).
+int batch_launch(ol_ctx *c, ol_batch *b, bool cooperative)
+{
+    if (b->nd == 0)
+        return OL_OK;
+    int rc = batch_scan(c, b, cooperative);
+    if (!rc)
+        rc = batch_tree(c, b);
+    if (!rc && b->prep_dict)
+        rc = batch_many(c, b, b->prep_dict, b->m_cap ? b->m_cap : 1);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, size_t count, ol_batch **out)
+{
+    if (!c || !out || (count && (!docs || !lens)))
+        return fail(OL_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> g(c->mu);
+    ol_batch *b = new (std::nothrow) ol_batch();
+    if (!b)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    b->count = count;
+    b->h_start.resize(count);
+    b->h_len.resize(count);
+    b->h_dev.resize(count);
+    uint64_t pos = 0;
+    for (size_t i = 0; i < count; i++) {
+        if (lens[i] && !docs[i]) {
+            delete b;
+            return fail(OL_ERR_INVALID, "document " + std::to_string(i) + " is null");
+        }
+        b->h_start[i] = pos;
+        b->h_len[i] = lens[i];
+        b->payload += lens[i];
+        if (lens[i]) {
+            b->h_dev[i] = b->nd++;
+            pos += lens[i] + (docs[i][lens[i] - 1] != '\n' ? 1 : 0);
+        } else {
+            b->h_dev[i] = ~0u;
+        }
+    }
+    b->n = pos;
+    if (b->n > OL_MAX_GPU_INPUT_LEN) {
+        delete b;
+        return fail(OL_ERR_TOO_LARGE, "batch text is " + std::to_string(b->n) + " bytes; at most " +
+                                          std::to_string(OL_MAX_GPU_INPUT_LEN) + " (32-bit offsets)");
+    }
+    auto bail = [&](int rc) {
+        if (b->text) (void)hipFree(b->text);
+        delete b;
+        return rc;
+    };
+    const uint64_t padded = round_up(b->n ? b->n : 1, TEXT_ALIGN) + TEXT_SLACK;
+    if (hipMalloc(&b->text, padded) != hipSuccess) {
+        b->text = nullptr;
+        return bail(fail(OL_ERR_NOMEM, "device allocation of the batch text failed"));
+    }
+    // copy through a pinned staging buffer
+    const size_t STAGE = 64u << 20;
+    uint8_t *stage = nullptr;
+    if (hipHostMalloc((void **)&stage, STAGE, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(OL_ERR_NOMEM, "pinned staging allocation failed"));
+    size_t fill = 0;
+    uint64_t dst = 0;
+    int rc = OL_OK;
+    auto flush = [&]() -> bool {
+        if (!fill)
+            return true;
+        if (hipMemcpy(b->text + dst, stage, fill, hipMemcpyHostToDevice) != hipSuccess)
+            return false;
+        dst += fill;
+        fill = 0;
+        return true;
+    };
+    for (size_t i = 0; i < count && rc == OL_OK; i++) {
+        size_t done = 0;
+        while (done < lens[i]) {
+            const size_t k = std::min(lens[i] - done, STAGE - fill);
+            std::memcpy(stage + fill, docs[i] + done, k);
+            fill += k;
+            done += k;
+            if (fill == STAGE && !flush())
+                rc = fail(OL_ERR_DEVICE, "batch upload failed");
+        }
+        if (lens[i] && docs[i][lens[i] - 1] != '\n') {
+            if (fill == STAGE && !flush())
+                rc = fail(OL_ERR_DEVICE, "batch upload failed");
+            stage[fill++] = '\n';
+        }
+    }
+    if (rc == OL_OK && !flush())
+        rc = fail(OL_ERR_DEVICE, "batch upload failed");
+    (void)hipHostFree(stage);
+    if (rc)
+        return bail(rc);
+    if (hipMemset(b->text + b->n, 0, padded - b->n) != hipSuccess)
+        return bail(fail(OL_ERR_DEVICE, "batch upload failed"));
+    // document bounds of the non-empty documents; search tiles -> document
+    std::vector<uint32_t> st, en;
+    st.reserve(b->nd);
+    en.reserve(b->nd);
+    for (size_t i = 0; i < count; i++)
+        if (b->h_dev[i] != ~0u) {
+            st.push_back((uint32_t)b->h_start[i]);
+            en.push_back((uint32_t)(b->h_start[i] + b->h_len[i]));
+        }
+    const uint32_t TS = (uint32_t)ceil_div(b->n ? b->n : 1, SEARCH_TILE);
+    std::vector<uint32_t> std_(TS);
+    uint32_t j = 0;
+    for (uint32_t t = 0; t < TS; t++) {
+        const uint64_t p = (uint64_t)t * SEARCH_TILE;
+        while (j + 1 < b->nd && st[j + 1] <= p)
+            j++;
+        std_[t] = j;
+    }
+    if (b->d_starts.ensure((size_t)(b->nd ? b->nd : 1) * 4) != hipSuccess ||
+        b->d_ends.ensure((size_t)(b->nd ? b->nd : 1) * 4) != hipSuccess ||
+        b->d_line_start.ensure((size_t)(b->nd ? b->nd : 1) * 4) != hipSuccess ||
+        b->search_tile_doc.ensure((size_t)TS * 4) != hipSuccess)
+        return bail(fail(OL_ERR_NOMEM, "batch index allocation failed"));
+    if ((b->nd && (hipMemcpy(b->d_starts.p, st.data(), st.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(b->d_ends.p, en.data(), en.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) ||
+        hipMemcpy(b->search_tile_doc.p, std_.data(), std_.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(OL_ERR_DEVICE, "batch index upload failed"));
+    rc = alloc_batch_lines(b, std::min<uint64_t>(b->n, b->n / 16 + b->nd + 4096));
+    if (rc)
+        return bail(rc);
+    *out = b;
+    return OL_OK;
+}
+
+void ol_batch_free(ol_batch *b)
+{
+    if (!b)
+        return;
+    b->drop_graph();
+    if (b->text)
+        (void)hipFree(b->text);
+    for (DevBuf *d : {&b->d_starts, &b->d_ends, &b->d_line_start, &b->scan_tile_doc, &b->search_tile_doc,
+                      &b->offsets, &b->classes, &b->keys, &b->parent, &b->fc, &b->ns, &b->se, &b->seg_line0,
+                      &b->seg_lines, &b->seg_tile0, &b->tile_seg, &b->tree_sum, &b->m_off, &b->m_id})
+        d->release();
+    delete b;
+}
+
+size_t ol_batch_doc_count(const ol_batch *b) { return b ? b->count : 0; }
+size_t ol_batch_text_bytes(const ol_batch *b) { return b ? b->payload : 0; }
+
+int ol_batch_run(ol_ctx *c, ol_batch *b, const ol_dict *dict, int graph)
+{
+    if (!c || !b)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!b->prepared || b->prep_bpt != c->scan_bpt || b->prep_rounds != c->tree_rounds || b->prep_dict != dict)
+        return batch_prepare(c, b, dict);
+    b->h_matches = false;
+    if (!graph)
+        return batch_launch(c, b, true);
+    if (!b->exec) {
+        const bool prof = c->profiling;
+        c->profiling = false;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = batch_launch(c, b, false);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        c->profiling = prof;
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return fail(OL_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        b->graph = gr;
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->exec, c->stream));
+    return OL_OK;
+}
+
+int ol_batch_lines(ol_ctx *c, const ol_batch *b, uint64_t *total, uint32_t *doc_line_start, uint32_t *doc_line_count)
+{
+    if (!c || !b)
+        return fail(OL_ERR_INVALID, "null argument");
+    if (!b->prepared)
+        return fail(OL_ERR_INVALID, "the batch has not been run");
+    if (total)
+        *total = b->lines;
+    // empty documents: no lines, starting where the next document's lines start
+    uint64_t next = b->lines;
+    for (size_t i = b->count; i-- > 0;) {
+        const uint32_t j = b->h_dev[i];
+        const uint64_t st = (j == ~0u) ? next : b->h_line_start[j];
+        if (doc_line_start) doc_line_start[i] = (uint32_t)st;
+        if (doc_line_count) doc_line_count[i] = (j == ~0u) ? 0u : b->h_line_count[j];
+        next = st;
+    }
+    return OL_OK;
+}
+
+int ol_batch_download_doc(ol_ctx *c, const ol_batch *b, size_t doc, uint32_t *offsets, uint8_t *classes,
+                          uint32_t *parent, uint32_t *subtree_end, uint32_t *first_child, uint32_t *next_sibling)
+{
+    if (!c || !b)
+        return fail(OL_ERR_INVALID, "null argument");
+    if (!b->prepared)
+        return fail(OL_ERR_INVALID, "the batch has not been run");
+    if (doc >= b->count)
+        return fail(OL_ERR_INVALID, "document index out of range");
+    std::lock_guard<std::mutex> g(c->mu);
+    const uint32_t j = b->h_dev[doc];
+    if (j == ~0u)
+        return OL_OK;
+    const size_t l0 = b->h_line_start[j], nl = b->h_line_count[j];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto get = [&](void *dst, const DevBuf &src, size_t width) -> hipError_t {
+        if (!dst || !nl)
+            return hipSuccess;
+        return hipMemcpy(dst, (const uint8_t *)src.p + l0 * width, nl * width, hipMemcpyDeviceToHost);
+    };
+    HIPCHK(get(offsets, b->offsets, 4));
+    HIPCHK(get(classes, b->classes, 1));
+    HIPCHK(get(parent, b->parent, 4));
+    HIPCHK(get(subtree_end, b->se, 4));
+    HIPCHK(get(first_child, b->fc, 4));
+    HIPCHK(get(next_sibling, b->ns, 4));
+    return OL_OK;
+}
+
+int ol_batch_matches(ol_ctx *c, ol_batch *b, size_t doc, uint64_t *count, uint32_t *offsets, uint32_t *ids,
+                     size_t cap)
+{
+    if (!c || !b || !count)
+        return fail(OL_ERR_INVALID, "null argument");
+    if (!b->prepared)
+        return fail(OL_ERR_INVALID, "the batch has not been run");
+    if (doc >= b->count)
+        return fail(OL_ERR_INVALID, "document index out of range");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!b->h_matches) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        b->h_moff.resize(b->m_total);
+        b->h_mid.resize(b->m_total);
+        if (b->m_total) {
+            HIPCHK(hipMemcpy(b->h_moff.data(), b->m_off.p, b->m_total * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(b->h_mid.data(), b->m_id.p, b->m_total * 4, hipMemcpyDeviceToHost));
+        }
+        b->h_matches = true;
+    }
+    const uint64_t s = b->h_start[doc], e = s + b->h_len[doc];
+    const size_t lo = std::lower_bound(b->h_moff.begin(), b->h_moff.end(), s) - b->h_moff.begin();
+    const size_t hi = std::lower_bound(b->h_moff.begin(), b->h_moff.end(), e) - b->h_moff.begin();
+    *count = hi - lo;
+    const size_t k = std::min(cap, hi - lo);
+    for (size_t i = 0; i < k; i++) {
+        if (offsets) offsets[i] = (uint32_t)(b->h_moff[lo + i] - s);
+        if (ids) ids[i] = b->h_mid[lo + i];
+    }
     return OL_OK;
 }
 

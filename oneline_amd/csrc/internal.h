@@ -241,7 +241,7 @@ hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s);
 // Persistent scan: `grid` co-resident workgroups (cooperative launch), at most
 // ol_scan_max_grid(bpt); bpt (bytes per lane per tile) is 64, 128 or 256, i.e.
 // tiles of 1, 2 or 4 LDS-staged quarters of 256 x 64 bytes.
-hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, hipStream_t s);
+hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, bool cooperative, hipStream_t s);
 uint32_t ol_scan_max_grid(uint32_t bpt);
 hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s);   // reads the line count from meta[0]
 // summary + scan + walk; `rounds` of 64 lines per wave: 1, 4 or 16 (tile = 256 x rounds lines)

@@ -733,14 +733,17 @@ uint32_t ol_scan_max_grid(uint32_t bpt)
     return f ? ol_coresident(f, SCAN_THREADS, 0) : 0u;
 }
 
-hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, hipStream_t s)
+hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, bool cooperative, hipStream_t s)
 {
     const void *f = scan_fn_bpt(bpt);
     if (!f || grid == 0)
         return hipErrorInvalidValue;
     ScanArgs args = a;
     void *params[] = {&args};
-    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SCAN_THREADS), params, 0, s);
+    if (cooperative)
+        return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SCAN_THREADS), params, 0, s);
+    // inside a captured graph: an ordinary launch of the same co-resident grid
+    return hipLaunchKernel(f, dim3(grid), dim3(SCAN_THREADS), params, 0, s);
 }
 
 // ---- parse with caller-given line offsets ------------------------------------
