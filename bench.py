@@ -46,6 +46,9 @@ def main():
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload (10,000 documents)")
+    ap.add_argument("--no-tune", action="store_true", help="use the default launch geometry")
+    ap.add_argument("--batch-docs", type=int, default=10000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, copy 1 GiB device-to-device twice (counter calibration)")
     args = ap.parse_args()
@@ -95,6 +98,12 @@ def main():
     def step():
         ol._check(lib.ol_doc_reparse(p._ctx, doc._h))
         ol._check(lib.ol_tree_rebuild_async(p._ctx, doc._h, tree._h))
+
+    tuning = None
+    if not args.no_tune:
+        from oneline_amd import autotune
+        tuning = autotune.tune(p, "single", step, reps=2)
+        log(f"[bench] autotune: scan_bpt {tuning['scan_bpt']}, tree_rounds {tuning['tree_rounds']}")
 
     for _ in range(args.warmup):
         step()
@@ -160,6 +169,9 @@ def main():
     extra = {}
     if not args.no_search:
         extra = search_benchmarks(p, doc, text, args)
+    del doc, tree, text
+    if not args.no_batch:
+        extra["batch"] = batch_benchmark(p, args)
 
     out = {
         "metric": "parse & dict-search GB/s (frac of HBM peak); served p50/p99 at fixed QPS",
@@ -184,6 +196,7 @@ def main():
             "verified_bit_exact": verified,
             "upload_s_pcie_inclusive": round(upload_s, 3),
             "kernels": kernels,
+            "tuning": tuning and {"scan_bpt": tuning["scan_bpt"], "tree_rounds": tuning["tree_rounds"]},
         },
         "roofline": {
             "bound": "hbm",
@@ -232,6 +245,128 @@ def copy_calibration(nbytes):
     hip.hipFree(a)
     hip.hipFree(b)
     log(f"[bench] copy calibration: 2 x {nbytes} bytes device-to-device")
+
+
+def batch_benchmark(p, args):
+    """Batched workload: documents with log-normal sizes (median 16 KiB, sigma
+    2.0, 64 B .. 64 MiB, seed 1), parse + tree + a 4,096-pattern dictionary
+    search as one launch sequence replayed as a hipGraph, against the plain
+    per-document path (reparse, tree, dictionary search per document)."""
+    import ctypes
+
+    import oneline_amd as ol
+    from oneline_amd import autotune, datagen
+    from oracle import pyoracle as orc
+
+    lib = ol.load_library()
+    hip = ctypes.CDLL("libamdhip64.so")
+    count = args.batch_docs
+    sizes = datagen.lognormal_sizes(count, 1, 16384.0, 2.0, 64, 64 << 20)
+    t0 = time.time()
+    docs = [datagen.document(int(sz), seed=datagen.mix(1, i)) for i, sz in enumerate(sizes)]
+    nbytes = int(sum(int(x) for x in sizes))
+    log(f"[bench] batch: {count} documents, {nbytes} bytes, generated in {time.time() - t0:.1f}s")
+    pats = datagen.patterns(4096, seed=0xC3B)
+    pset = p.compile_patterns(None, pats)
+    b = p.batch(docs)
+
+    tuning = None
+    if not args.no_tune:
+        tuning = autotune.tune(p, "batch", lambda: b.run(pset), reps=2)
+        log(f"[bench] batch autotune: scan_bpt {tuning['scan_bpt']}, tree_rounds {tuning['tree_rounds']}")
+    b.run(pset)                          # sized for the chosen geometry
+    total_lines, _, _ = b.lines()
+
+    # parity on a sample of documents (the oracle, document alone)
+    rng = np.random.default_rng(11)
+    sample = rng.choice(count, size=min(count, 60), replace=False)
+    for i in sample:
+        d = docs[int(i)]
+        off, cls, links = b.document(int(i))
+        w_off, nodes = orc.parse(d)
+        assert np.array_equal(off, w_off), f"batch doc {i}: offsets"
+        assert np.array_equal(cls, nodes["kind"].astype(np.uint8)), f"batch doc {i}: classes"
+        w = orc.links(nodes)
+        for g, ww in zip((links.parent, links.subtree_end, links.first_child, links.next_sibling), w):
+            assert np.array_equal(g, ww), f"batch doc {i}: links"
+        mo, mi = b.matches(int(i))
+        wo, wi = orc.many_find_all(d, pats)
+        assert np.array_equal(mo, wo.astype(np.uint32)) and np.array_equal(mi, wi.astype(np.uint32)), \
+            f"batch doc {i}: matches"
+    log(f"[bench] batch: {len(sample)} sampled documents bit-exact against the oracle")
+
+    # total matches (all documents)
+    cnt = ctypes.c_uint64()
+    matches = 0
+    for i in range(count):
+        ol._check(lib.ol_batch_matches(p._ctx, b._h, i, ctypes.byref(cnt), None, None, 0))
+        matches += cnt.value
+    alg = nbytes + 17 * total_lines + 12 * matches
+
+    def timed(fn, reps):
+        fn()
+        p.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        p.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    reps = 10
+    t_graph = timed(lambda: b.run(pset, graph=True), reps)
+    t_eager = timed(lambda: b.run(pset, graph=False), reps)
+    p.profile_take()
+    p.profile_enable(True)
+    b.run(pset, graph=False)
+    p.synchronize()
+    p.profile_enable(False)
+    kern = {k: round(ms, 4) for k, (c, ms) in p.profile_take().items()}
+    log(f"[bench] batch: graph {t_graph * 1e3:.3f} ms, eager {t_eager * 1e3:.3f} ms, kernels {kern}")
+    b.close()
+
+    # plain per-document path on the same documents and dictionary
+    pdocs = [p.upload(d, nodes=False) for d in docs]
+    ptrees = [p.build_tree(d) for d in pdocs]
+    caps = []
+    for d in pdocs:
+        ol._check(lib.ol_search_many_device(p._ctx, d._h, pset._h, None, None, 0, ctypes.byref(cnt)))
+        caps.append(cnt.value)
+    bufs = []
+    for c in caps:
+        o, q = ctypes.c_void_p(), ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(o), ctypes.c_size_t(max(1, c) * 8)) == 0
+        assert hip.hipMalloc(ctypes.byref(q), ctypes.c_size_t(max(1, c) * 4)) == 0
+        bufs.append((o, q))
+
+    def per_doc():
+        for d, tr, (o, q), c in zip(pdocs, ptrees, bufs, caps):
+            ol._check(lib.ol_doc_reparse(p._ctx, d._h))
+            ol._check(lib.ol_tree_rebuild_async(p._ctx, d._h, tr._h))
+            ol._check(lib.ol_search_many_device(p._ctx, d._h, pset._h, o, q, max(1, c), ctypes.byref(cnt)))
+
+    t_loop = timed(per_doc, 1)
+    log(f"[bench] batch: per-document loop {t_loop * 1e3:.1f} ms")
+    for o, q in bufs:
+        hip.hipFree(o)
+        hip.hipFree(q)
+    for d in pdocs:
+        d.close()
+    for tr in ptrees:
+        tr.close()
+    return {
+        "workload": f"{count} documents, log-normal sizes (median 16 KiB, sigma 2.0, 64 B..64 MiB, seed 1), "
+                    "parse + tree + 4,096-pattern dictionary search",
+        "documents": count, "text_bytes": nbytes, "lines": total_lines, "matches": matches,
+        "algorithmic_bytes": alg,
+        "graph_ms": round(t_graph * 1e3, 4), "graph_gbs": round(alg / t_graph / 1e9, 2),
+        "eager_ms": round(t_eager * 1e3, 4), "eager_gbs": round(alg / t_eager / 1e9, 2),
+        "per_document_loop_ms": round(t_loop * 1e3, 3), "per_document_loop_gbs": round(alg / t_loop / 1e9, 2),
+        "speedup_vs_per_document": round(t_loop / t_graph, 2),
+        "kernel_ms": kern,
+        "tuning": tuning and {"scan_bpt": tuning["scan_bpt"], "tree_rounds": tuning["tree_rounds"],
+                              "sweep_kernel_ms": tuning["sweep"]},
+        "verified_sample": int(len(sample)),
+    }
 
 
 def search_benchmarks(p, doc, text, args):
