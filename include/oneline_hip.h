@@ -182,10 +182,14 @@ size_t ol_batch_doc_count(const ol_batch *batch);
 /* Sum of the document lengths. */
 size_t ol_batch_text_bytes(const ol_batch *batch);
 /* Parse + tree of every document, then (dict non-NULL) its dictionary search.
- * The first call for a given dictionary and launch tuning sizes the outputs
- * (synchronous); later calls launch the same sequence asynchronously, as a
- * captured hipGraph when `graph` is non-zero.  ol_ctx_synchronize waits. */
-int ol_batch_run(ol_ctx *ctx, ol_batch *batch, const ol_dict *dict, int graph);
+ * The first call for a given dictionary, flags and launch tuning sizes the
+ * outputs (synchronous); later calls launch the same sequence asynchronously,
+ * as a captured hipGraph with OL_BATCH_GRAPH.  OL_BATCH_SEARCH_ONLY skips the
+ * parse and tree (a dictionary is then required, and only ol_batch_matches
+ * has results).  ol_ctx_synchronize waits. */
+#define OL_BATCH_GRAPH 1
+#define OL_BATCH_SEARCH_ONLY 2
+int ol_batch_run(ol_ctx *ctx, ol_batch *batch, const ol_dict *dict, int flags);
 /* Total lines; per document its first line (in the batch's line order) and
  * line count (either pointer may be NULL; arrays of ol_batch_doc_count). */
 int ol_batch_lines(ol_ctx *ctx, const ol_batch *batch, uint64_t *total, uint32_t *doc_line_start,
@@ -200,6 +204,56 @@ int ol_batch_download_doc(ol_ctx *ctx, const ol_batch *batch, size_t doc, uint32
 int ol_batch_matches(ol_ctx *ctx, ol_batch *batch, size_t doc, uint64_t *count, uint32_t *offsets, uint32_t *ids,
                      size_t cap);
 
+/* ---- serving (no counterpart in the reference): a continuous-batching
+ * request loop.  Worker slots (each a context with its own HIP stream) take
+ * the oldest pending requests that belong together (parse requests, or
+ * searches with the same dictionary), as far as the HBM budget and batch
+ * limits allow, run them as one batch and complete them; requests arriving
+ * meanwhile join the next batch a slot forms, so batches overlap on the
+ * device.  Dictionaries stay resident.  Results equal the single-document
+ * calls.  Documents passed to ol_server_submit must stay valid until the
+ * request completes. */
+typedef struct ol_server ol_server;
+typedef struct ol_server_config {
+    size_t hbm_budget;        /* device bytes of requests in flight (0: 32 GiB) */
+    uint32_t slots;           /* concurrent batches (0: 4, at most 16) */
+    uint32_t max_batch_docs;  /* requests per batch (0: 256) */
+    size_t max_batch_bytes;   /* document bytes per batch (0: 256 MiB) */
+} ol_server_config;
+#define OL_REQ_PARSE 0        /* offsets, classes and tree links */
+#define OL_REQ_SEARCH 1       /* dictionary matches */
+typedef struct ol_serve_stats {
+    uint64_t requests, parse_requests, search_requests, rejected, bytes, batches, batched_requests;
+    double seconds, qps_offered, qps_achieved, gbps, p50_ms, p90_ms, p99_ms, max_ms, mean_ms;
+} ol_serve_stats;
+int ol_server_create(const ol_server_config *cfg, ol_server **out);
+/* Finishes the pending requests, then releases everything. */
+void ol_server_destroy(ol_server *server);
+/* Message of the calling thread's last failed ol_server_* call. */
+const char *ol_server_last_error(void);
+int ol_server_add_dict(ol_server *server, const uint8_t *const *patterns, const size_t *lens, size_t count,
+                       uint32_t *dict_id);
+/* kind OL_REQ_PARSE or OL_REQ_SEARCH (dict_id from ol_server_add_dict).  A
+ * request needing more HBM than the whole budget fails with OL_ERR_NOMEM. */
+int ol_server_submit(ol_server *server, int kind, uint32_t dict_id, const uint8_t *doc, size_t len,
+                     int keep_results, uint64_t *req_id);
+/* Waits for a request (timeout_s <= 0: no limit); its status. */
+int ol_server_wait(ol_server *server, uint64_t req_id, double timeout_s);
+/* Results of a completed request (two calls: sizes with NULL arrays, then data). */
+int ol_server_result_lines(ol_server *server, uint64_t req_id, size_t *lines, uint32_t *offsets, uint8_t *classes,
+                           uint32_t *parent, uint32_t *subtree_end, uint32_t *first_child, uint32_t *next_sibling);
+int ol_server_result_matches(ol_server *server, uint64_t req_id, size_t *count, uint32_t *offsets, uint32_t *ids);
+/* Submission, batch start and completion times (seconds, steady clock). */
+int ol_server_timing(ol_server *server, uint64_t req_id, double *submit, double *start, double *done);
+int ol_server_release(ol_server *server, uint64_t req_id);
+/* Open-loop load: `nreq` requests with exponential inter-arrival times at
+ * `qps` (seeded), documents taken round-robin from `docs`, a search with
+ * probability search_fraction against a uniformly chosen dictionary; waits
+ * for all and reports latency (completion - scheduled arrival) percentiles
+ * and served GB/s ((document + result bytes) / wall time). */
+int ol_server_load(ol_server *server, const uint8_t *const *docs, const size_t *lens, size_t ndocs, double qps,
+                   size_t nreq, double search_fraction, uint64_t seed, ol_serve_stats *out);
+
 /* ---- launch tuning (no counterpart in the reference: its tile sizes are
  * compile-time constants, e.g. crates/oneline-gpu/src/scan.rs:11).  The
  * autotuner (oneline_amd/autotune.py) sets these from measurements:
@@ -208,7 +262,10 @@ int ol_batch_matches(ol_ctx *ctx, ol_batch *batch, size_t doc, uint64_t *count, 
  *   "scan_grid"     persistent workgroups of the scan kernel (0 = as many as
  *                   are co-resident on the device, the default);
  *   "tree_rounds"   rounds of 64 lines per tree wave: 1, 4 or 16 (tile =
- *                   256 x rounds lines; 16 by default). */
+ *                   256 x rounds lines; 16 by default);
+ *   "scan_ticket"   1: scan tiles are claimed from an atomic ticket by an
+ *                   ordinary launch (safe beside other work on the device),
+ *                   0: static tiles under a cooperative launch (default). */
 int ol_tune_set(ol_ctx *ctx, const char *name, int value);
 int ol_tune_get(ol_ctx *ctx, const char *name, int *value);
 

@@ -106,6 +106,15 @@ def load_library() -> ctypes.CDLL:
         "ol_batch_lines": [P, P, U64P, P, P],
         "ol_batch_download_doc": [P, P, S, P, P, P, P, P, P],
         "ol_batch_matches": [P, P, S, U64P, P, P, S],
+        "ol_server_create": [P, ctypes.POINTER(P)],
+        "ol_server_add_dict": [P, P, P, S, ctypes.POINTER(ctypes.c_uint32)],
+        "ol_server_submit": [P, ctypes.c_int, ctypes.c_uint32, P, S, ctypes.c_int, U64P],
+        "ol_server_wait": [P, ctypes.c_uint64, ctypes.c_double],
+        "ol_server_result_lines": [P, ctypes.c_uint64, ctypes.POINTER(S), P, P, P, P, P, P],
+        "ol_server_result_matches": [P, ctypes.c_uint64, ctypes.POINTER(S), P, P],
+        "ol_server_timing": [P, ctypes.c_uint64, P, P, P],
+        "ol_server_release": [P, ctypes.c_uint64],
+        "ol_server_load": [P, P, P, S, ctypes.c_double, S, ctypes.c_double, ctypes.c_uint64, P],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -115,7 +124,10 @@ def load_library() -> ctypes.CDLL:
                  "ol_batch_doc_count", "ol_batch_text_bytes"):
         getattr(lib, name).restype = ctypes.c_size_t
         getattr(lib, name).argtypes = [P]
-    for name in ("ol_ctx_destroy", "ol_free", "ol_doc_free", "ol_tree_free", "ol_dict_free", "ol_batch_free"):
+    lib.ol_server_last_error.argtypes = []
+    lib.ol_server_last_error.restype = ctypes.c_char_p
+    for name in ("ol_ctx_destroy", "ol_free", "ol_doc_free", "ol_tree_free", "ol_dict_free", "ol_batch_free",
+                 "ol_server_destroy"):
         getattr(lib, name).restype = None
         getattr(lib, name).argtypes = [P]
     lib.ol_last_error.argtypes = []
@@ -303,10 +315,11 @@ class GpuBatch:
     def text_bytes(self) -> int:
         return int(_lib.ol_batch_text_bytes(self._h))
 
-    def run(self, dictionary: Optional["GpuPatternSet"] = None, graph: bool = False) -> None:
+    def run(self, dictionary: Optional["GpuPatternSet"] = None, graph: bool = False,
+            search_only: bool = False) -> None:
         """Runs (the first time: sizes, synchronously) parse + tree (+ search)."""
-        _check(_lib.ol_batch_run(self._p._ctx, self._h, dictionary._h if dictionary is not None else None,
-                                 1 if graph else 0))
+        flags = (1 if graph else 0) | (2 if search_only else 0)
+        _check(_lib.ol_batch_run(self._p._ctx, self._h, dictionary._h if dictionary is not None else None, flags))
 
     def lines(self):
         """(total lines, per-document first line, per-document line count)."""
@@ -511,3 +524,112 @@ class GpuParser:
             nm = raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode()
             out[nm] = (int(counts[i]), float(ms[i]))
         return out
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_uint32),
+                ("max_batch_docs", ctypes.c_uint32), ("max_batch_bytes", ctypes.c_size_t)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("requests", "parse_requests", "search_requests", "rejected", "bytes",
+                                               "batches", "batched_requests")] + \
+               [(n, ctypes.c_double) for n in ("seconds", "qps_offered", "qps_achieved", "gbps", "p50_ms", "p90_ms",
+                                               "p99_ms", "max_ms", "mean_ms")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+PARSE, SEARCH = 0, 1
+
+
+def _scheck(rc: int) -> None:
+    if rc != 0:
+        raise OnelineError(rc, _lib.ol_server_last_error().decode(errors="replace"))
+
+
+class GpuServer:
+    """Continuous-batching request loop (ol_server_*): parse and dictionary
+    search requests batched across documents, several batches in flight."""
+
+    def __init__(self, hbm_budget: int = 0, slots: int = 0, max_batch_docs: int = 0, max_batch_bytes: int = 0):
+        load_library()
+        cfg = ServerConfig(hbm_budget, slots, max_batch_docs, max_batch_bytes)
+        h = ctypes.c_void_p()
+        _scheck(_lib.ol_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._docs = {}      # request id -> document bytes (kept alive until completion)
+        self._dict_keep = []
+
+    def add_dictionary(self, patterns: Sequence[bytes]) -> int:
+        arrs = [np.concatenate([as_bytes_array(p), np.zeros(1, np.uint8)]) for p in patterns]
+        ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+        lens = (ctypes.c_size_t * max(1, len(arrs)))(*[a.size - 1 for a in arrs])
+        out = ctypes.c_uint32()
+        _scheck(_lib.ol_server_add_dict(self._h, ctypes.cast(ptrs, ctypes.c_void_p),
+                                        ctypes.cast(lens, ctypes.c_void_p), len(arrs), ctypes.byref(out)))
+        return out.value
+
+    def submit(self, doc, kind: int = PARSE, dictionary: int = 0, keep: bool = True) -> int:
+        a = as_bytes_array(doc)
+        rid = ctypes.c_uint64()
+        _scheck(_lib.ol_server_submit(self._h, kind, dictionary, _ptr(a) if a.size else None, a.size,
+                                      1 if keep else 0, ctypes.byref(rid)))
+        self._docs[rid.value] = a
+        return rid.value
+
+    def wait(self, rid: int, timeout: float = 120.0) -> None:
+        _scheck(_lib.ol_server_wait(self._h, rid, timeout))
+
+    def parse_result(self, rid: int):
+        """(offsets, classes, TreeLinks) of a completed parse request."""
+        n = ctypes.c_size_t()
+        _scheck(_lib.ol_server_result_lines(self._h, rid, ctypes.byref(n), None, None, None, None, None, None))
+        off = np.zeros(n.value, np.uint32)
+        cls = np.zeros(n.value, np.uint8)
+        par, se, fc, ns = (np.zeros(n.value, np.uint32) for _ in range(4))
+        _scheck(_lib.ol_server_result_lines(self._h, rid, ctypes.byref(n), _ptr(off), _ptr(cls), _ptr(par),
+                                            _ptr(se), _ptr(fc), _ptr(ns)))
+        return off, cls, TreeLinks(par, se, fc, ns)
+
+    def search_result(self, rid: int):
+        """(offsets, ids) of a completed search request, (offset, id) order."""
+        n = ctypes.c_size_t()
+        _scheck(_lib.ol_server_result_matches(self._h, rid, ctypes.byref(n), None, None))
+        off = np.zeros(n.value, np.uint32)
+        ids = np.zeros(n.value, np.uint32)
+        _scheck(_lib.ol_server_result_matches(self._h, rid, ctypes.byref(n), _ptr(off), _ptr(ids)))
+        return off, ids
+
+    def timing(self, rid: int):
+        v = [ctypes.c_double() for _ in range(3)]
+        _scheck(_lib.ol_server_timing(self._h, rid, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def release(self, rid: int) -> None:
+        _scheck(_lib.ol_server_release(self._h, rid))
+        self._docs.pop(rid, None)
+
+    def load(self, docs: Sequence, qps: float, requests: int, search_fraction: float = 0.3,
+             seed: int = 5) -> dict:
+        """Open-loop Poisson load over `docs` (ol_server_load); latency and GB/s."""
+        arrs = [as_bytes_array(d) for d in docs]
+        ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        lens = (ctypes.c_size_t * len(arrs))(*[a.size for a in arrs])
+        st = ServeStats()
+        _scheck(_lib.ol_server_load(self._h, ctypes.cast(ptrs, ctypes.c_void_p), ctypes.cast(lens, ctypes.c_void_p),
+                                    len(arrs), qps, requests, search_fraction, seed, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            _lib.ol_server_destroy(self._h)
+            self._h = None
+            self._docs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

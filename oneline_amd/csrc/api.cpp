@@ -77,6 +77,8 @@ struct ol_ctx {
     uint32_t scan_grid = 0;                // persistent workgroups of scan_classify (0 = all co-resident)
     uint32_t scan_max_grid[3] = {0, 0, 0}; // co-resident limit per bpt (64, 128, 256), from the occupancy API
     uint32_t tree_rounds = 16;             // rounds of 64 lines per tree wave (tile = 256 x rounds lines)
+    bool scan_ticket = false;              // scan tiles from an atomic ticket (ordinary launch)
+    uint8_t *stage = nullptr;              // pinned host staging for batch uploads (lazily allocated)
     // profiling
     bool profiling = false;
     struct Rec {
@@ -229,9 +231,12 @@ int launch_scan_args(ol_ctx *c, ScanArgs &a, bool cooperative = true)
         grid = c->scan_grid;
     if (grid > tiles)
         grid = tiles;
+    // tiles from a ticket unless the cooperative launch guarantees residency
+    const bool coop = cooperative && !c->scan_ticket;
+    a.tile_ctr = coop ? nullptr : c->ctr() + 5;
     {
         Prof p(c, "scan_classify");
-        HIPCHK(ol_launch_scan(a, c->scan_bpt, grid, cooperative, c->stream));
+        HIPCHK(ol_launch_scan(a, c->scan_bpt, grid, coop, c->stream));
     }
     Prof p(c, "scan_fixup");
     HIPCHK(ol_launch_scan_fixup(a, c->stream));
@@ -551,6 +556,8 @@ void ol_ctx_destroy(ol_ctx *c)
     c->scratch.release();
     c->tree_sum.release();
     c->fix.release();
+    if (c->stage)
+        (void)hipHostFree(c->stage);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1130,6 +1137,8 @@ int ol_tune_set(ol_ctx *c, const char *name, int value)
         if (value < 0)
             return fail(OL_ERR_INVALID, "scan_grid must be >= 0 (0 = all co-resident workgroups)");
         c->scan_grid = (uint32_t)value;
+    } else if (n == "scan_ticket") {
+        c->scan_ticket = value != 0;
     } else if (n == "tree_rounds") {
         if (value != 1 && value != 4 && value != 16)
             return fail(OL_ERR_INVALID, "tree_rounds must be 1, 4 or 16");
@@ -1152,6 +1161,8 @@ int ol_tune_get(ol_ctx *c, const char *name, int *value)
         *value = (int)c->scan_grid;
     else if (n == "tree_rounds")
         *value = (int)c->tree_rounds;
+    else if (n == "scan_ticket")
+        *value = c->scan_ticket ? 1 : 0;
     else
         return fail(OL_ERR_INVALID, "unknown tuning parameter " + n);
     return OL_OK;
@@ -1245,7 +1256,7 @@ struct ol_batch {
     std::vector<uint32_t> h_mid;
     bool h_matches = false;
     // the sized launch sequence
-    bool prepared = false;
+    bool prepared = false, prep_search_only = false;
     uint32_t prep_bpt = 0, prep_rounds = 0;
     const ol_dict *prep_dict = nullptr;
     hipGraph_t graph = nullptr;
@@ -1350,20 +1361,9 @@ int alloc_batch_lines(ol_batch *b, uint64_t cap)
 }
 
 // Sizes and runs the whole sequence once, with host round trips.
-int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict)
+// Parse + tree part of the sizing run.
+int batch_prepare_parse(ol_ctx *c, ol_batch *b)
 {
-    b->drop_graph();
-    b->prepared = false;
-    b->h_matches = false;
-    if (b->nd == 0) {
-        b->lines = 0;
-        b->m_total = 0;
-        b->prepared = true;
-        b->prep_bpt = c->scan_bpt;
-        b->prep_rounds = c->tree_rounds;
-        b->prep_dict = dict;
-        return OL_OK;
-    }
     // scan tiles -> first document starting at or after each tile
     const uint64_t stile = (uint64_t)SCAN_THREADS * c->scan_bpt;
     if (b->scan_tile_doc_bpt != c->scan_bpt) {
@@ -1436,9 +1436,33 @@ int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict)
     rc = check_tree_error(c);
     if (rc)
         return rc;
+    return OL_OK;
+}
+
+int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict, bool search_only)
+{
+    b->drop_graph();
+    b->prepared = false;
+    b->h_matches = false;
+    b->prep_search_only = search_only;
+    if (b->nd == 0) {
+        b->lines = 0;
+        b->m_total = 0;
+        b->prepared = true;
+        b->prep_bpt = c->scan_bpt;
+        b->prep_rounds = c->tree_rounds;
+        b->prep_dict = dict;
+        return OL_OK;
+    }
+    if (!search_only) {
+        const int prc = batch_prepare_parse(c, b);
+        if (prc)
+            return prc;
+    }
     // dictionary: count, size, write
     b->m_total = 0;
     if (dict) {
+        int rc;
         rc = batch_many(c, b, dict, 0);
         if (rc)
             return rc;
@@ -1469,9 +1493,12 @@ int batch_launch(ol_ctx *c, ol_batch *b, bool cooperative)
 {
     if (b->nd == 0)
         return OL_OK;
-    int rc = batch_scan(c, b, cooperative);
-    if (!rc)
-        rc = batch_tree(c, b);
+    int rc = OL_OK;
+    if (!b->prep_search_only) {
+        rc = batch_scan(c, b, cooperative);
+        if (!rc)
+            rc = batch_tree(c, b);
+    }
     if (!rc && b->prep_dict)
         rc = batch_many(c, b, b->prep_dict, b->m_cap ? b->m_cap : 1);
     return rc;
@@ -1526,11 +1553,13 @@ int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, s
         b->text = nullptr;
         return bail(fail(OL_ERR_NOMEM, "device allocation of the batch text failed"));
     }
-    // copy through a pinned staging buffer
-    const size_t STAGE = 64u << 20;
-    uint8_t *stage = nullptr;
-    if (hipHostMalloc((void **)&stage, STAGE, hipHostMallocDefault) != hipSuccess)
+    // copy through the context's pinned staging buffer
+    const size_t STAGE = 32u << 20;
+    if (!c->stage && hipHostMalloc((void **)&c->stage, STAGE, hipHostMallocDefault) != hipSuccess) {
+        c->stage = nullptr;
         return bail(fail(OL_ERR_NOMEM, "pinned staging allocation failed"));
+    }
+    uint8_t *stage = c->stage;
     size_t fill = 0;
     uint64_t dst = 0;
     int rc = OL_OK;
@@ -1561,7 +1590,6 @@ int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, s
     }
     if (rc == OL_OK && !flush())
         rc = fail(OL_ERR_DEVICE, "batch upload failed");
-    (void)hipHostFree(stage);
     if (rc)
         return bail(rc);
     if (hipMemset(b->text + b->n, 0, padded - b->n) != hipSuccess)
@@ -1617,13 +1645,18 @@ void ol_batch_free(ol_batch *b)
 size_t ol_batch_doc_count(const ol_batch *b) { return b ? b->count : 0; }
 size_t ol_batch_text_bytes(const ol_batch *b) { return b ? b->payload : 0; }
 
-int ol_batch_run(ol_ctx *c, ol_batch *b, const ol_dict *dict, int graph)
+int ol_batch_run(ol_ctx *c, ol_batch *b, const ol_dict *dict, int flags)
 {
     if (!c || !b)
         return fail(OL_ERR_INVALID, "null argument");
+    const bool graph = (flags & OL_BATCH_GRAPH) != 0;
+    const bool search_only = (flags & OL_BATCH_SEARCH_ONLY) != 0;
+    if (search_only && !dict)
+        return fail(OL_ERR_INVALID, "a search-only run needs a dictionary");
     std::lock_guard<std::mutex> g(c->mu);
-    if (!b->prepared || b->prep_bpt != c->scan_bpt || b->prep_rounds != c->tree_rounds || b->prep_dict != dict)
-        return batch_prepare(c, b, dict);
+    if (!b->prepared || b->prep_bpt != c->scan_bpt || b->prep_rounds != c->tree_rounds || b->prep_dict != dict ||
+        b->prep_search_only != search_only)
+        return batch_prepare(c, b, dict, search_only);
     b->h_matches = false;
     if (!graph)
         return batch_launch(c, b, true);
@@ -1654,6 +1687,8 @@ int ol_batch_lines(ol_ctx *c, const ol_batch *b, uint64_t *total, uint32_t *doc_
         return fail(OL_ERR_INVALID, "null argument");
     if (!b->prepared)
         return fail(OL_ERR_INVALID, "the batch has not been run");
+    if (b->prep_search_only)
+        return fail(OL_ERR_INVALID, "the batch was run for its dictionary search only");
     if (total)
         *total = b->lines;
     // empty documents: no lines, starting where the next document's lines start
@@ -1677,6 +1712,8 @@ int ol_batch_download_doc(ol_ctx *c, const ol_batch *b, size_t doc, uint32_t *of
         return fail(OL_ERR_INVALID, "the batch has not been run");
     if (doc >= b->count)
         return fail(OL_ERR_INVALID, "document index out of range");
+    if (b->prep_search_only)
+        return fail(OL_ERR_INVALID, "the batch was run for its dictionary search only");
     std::lock_guard<std::mutex> g(c->mu);
     const uint32_t j = b->h_dev[doc];
     if (j == ~0u)

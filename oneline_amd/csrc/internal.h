@@ -150,6 +150,7 @@ struct ScanArgs {
     uint64_t *status;        // look-back words                    [num_tiles]
     uint64_t *meta;          // meta[0] <- line count
     uint32_t *lb_err;        // set when a look-back wait gave up
+    uint32_t *tile_ctr;      // non-null: tiles claimed from this ticket (zero at launch)
     uint4 *fix_list;         // {line, start, end, 0} of lines whose head was not in LDS (fixup pass)
     uint32_t *fix_count;     // entries appended to fix_list
     uint32_t fix_cap;        // capacity of fix_list (>= lines longer than the halo)

@@ -461,13 +461,31 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     bool pending = false;
     uint32_t p_t = 0, p_total = 0;
 
-    uint32_t t = blockIdx.x;
+    // Tile order: static (t = blockIdx + k * grid: needs every workgroup
+    // resident, the cooperative launch), or claimed from a ticket (tile_ctr:
+    // a tile's look-back then only waits on tiles held by running workgroups,
+    // whatever shares the device).  A workgroup knows its next tile one
+    // iteration ahead, for the prefetch.
+    __shared__ uint32_t s_claim[2];
+    const bool ticket = a.tile_ctr != nullptr;
+    uint32_t t, tn;
+    if (ticket) {
+        if (tid == 0) {
+            s_claim[0] = atomicAdd(a.tile_ctr, 1u);
+            s_claim[1] = atomicAdd(a.tile_ctr, 1u);
+        }
+        __syncthreads();
+        t = s_claim[0];
+        tn = s_claim[1];
+    } else {
+        t = blockIdx.x;
+        tn = t + G;
+    }
     int b = 0;
     if (t < T)
         load_tile(t);
-    for (; t < T; t += G, b ^= 1) {
+    while (t < T) {
         const uint32_t base = t * TILE;
-        const uint32_t tn = t + G;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             s_nm[k * THREADS + tid] = (uint16_t)nl_mask16(v[k]);
@@ -676,7 +694,12 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         pending = staged;
         p_t = t;
         p_total = total;
+        if (ticket && tid == 0)
+            s_claim[b] = atomicAdd(a.tile_ctr, 1u);
         __syncthreads();   // the next tile overwrites LDS
+        t = tn;
+        tn = ticket ? s_claim[b] : tn + G;
+        b ^= 1;
     }
     if (pending) {
         if (wid == 0) {

@@ -155,3 +155,15 @@ def test_newline_bytes_at_every_vector_lane_and_neighbours(parser):
     near = np.array([0x0B, 0x0E, 0x08, 0x8A, 0x2A, 0x09, 0x0A], np.uint8)
     text = rng.choice(near, size=3 << 20).astype(np.uint8)
     assert_parse_matches(parser, text, "near-newline bytes")
+
+
+def test_ticket_ordered_scan(parser):
+    """Scan tiles claimed from a ticket (ordinary launch) give the same results."""
+    parser.tune_set("scan_ticket", 1)
+    try:
+        for bpt in (64, 128, 256):
+            parser.tune_set("scan_bpt", bpt)
+            assert_parse_matches(parser, datagen.document(24 << 20, seed=21, crlf=10, empty=5), f"ticket bpt={bpt}")
+    finally:
+        parser.tune_set("scan_ticket", 0)
+        parser.tune_set("scan_bpt", 128)
