@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload (10,000 documents)")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch geometry")
     ap.add_argument("--batch-docs", type=int, default=10000)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--serve-qps", type=str, default="250,1000,2000,4000,8000")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, copy 1 GiB device-to-device twice (counter calibration)")
     args = ap.parse_args()
@@ -172,6 +175,8 @@ def main():
     del doc, tree, text
     if not args.no_batch:
         extra["batch"] = batch_benchmark(p, args)
+    if not args.no_serve:
+        extra["serve"] = serve_benchmark(args)
 
     out = {
         "metric": "parse & dict-search GB/s (frac of HBM peak); served p50/p99 at fixed QPS",
@@ -366,6 +371,40 @@ def batch_benchmark(p, args):
         "tuning": tuning and {"scan_bpt": tuning["scan_bpt"], "tree_rounds": tuning["tree_rounds"],
                               "sweep_kernel_ms": tuning["sweep"]},
         "verified_sample": int(len(sample)),
+    }
+
+
+def serve_benchmark(args):
+    """Serving: open-loop Poisson arrivals at each QPS of a sweep, 70% parse and
+    30% dictionary-search requests; document sizes log-normal (median 64 KiB,
+    sigma 2.0, 64 B .. 256 MiB, seed 2); each search uses one of 8 resident
+    dictionaries of 1,024 .. 65,536 patterns.  p50/p99 latency (completion -
+    scheduled arrival) and served GB/s ((document + result bytes) / wall)."""
+    import oneline_amd as ol
+    from oneline_amd import datagen
+
+    pool = 400
+    sizes = datagen.lognormal_sizes(pool, 2, 65536.0, 2.0, 64, 256 << 20)
+    docs = [datagen.document(int(sz), seed=datagen.mix(2, i)) for i, sz in enumerate(sizes)]
+    log(f"[bench] serve: {pool} documents, {int(sum(int(x) for x in sizes))} bytes in the pool")
+    srv = ol.GpuServer(hbm_budget=64 << 30, slots=4, max_batch_docs=256, max_batch_bytes=256 << 20)
+    dict_sizes = (1024, 2048, 4096, 8192, 16384, 32768, 65536, 1024)
+    for k, cnt in enumerate(dict_sizes):
+        srv.add_dictionary(datagen.patterns(cnt, seed=0xC3B + 17 * k))
+    srv.load(docs[:32], qps=500.0, requests=200, search_fraction=0.3, seed=1)   # warm up
+    sweep = []
+    for q in [float(x) for x in args.serve_qps.split(",")]:
+        n = max(200, int(q * args.serve_seconds))
+        st = srv.load(docs, qps=q, requests=n, search_fraction=0.3, seed=int(q))
+        rec = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+        sweep.append(rec)
+        log(f"[bench] serve qps {q:.0f}: achieved {st['qps_achieved']:.0f}, p50 {st['p50_ms']:.2f} ms, "
+            f"p99 {st['p99_ms']:.2f} ms, {st['gbps']:.2f} GB/s, batches {st['batches']}")
+    srv.close()
+    return {
+        "workload": "open-loop Poisson arrivals, 70% parse / 30% dictionary search, sizes log-normal (median "
+                    "64 KiB, sigma 2.0, max 256 MiB, seed 2), 8 resident dictionaries of 1k..64k patterns",
+        "slots": 4, "hbm_budget_bytes": 64 << 30, "sweep": sweep,
     }
 
 

@@ -428,6 +428,7 @@ int ol_server_load(ol_server *s, const uint8_t *const *docs, const size_t *lens,
         std::lock_guard<std::mutex> g(s->mu);
         ndict = (uint32_t)s->dicts.size();
     }
+    const uint64_t batches0 = s->batches.load(), batched0 = s->batched_requests.load();
     std::mt19937_64 rng(seed);
     std::exponential_distribution<double> gap(qps);
     std::uniform_real_distribution<double> u(0.0, 1.0);
@@ -509,8 +510,8 @@ int ol_server_load(ol_server *s, const uint8_t *const *docs, const size_t *lens,
     for (double x : lat)
         sum += x;
     out->mean_ms = lat.empty() ? 0 : sum / lat.size() * 1e3;
-    out->batches = s->batches.load();
-    out->batched_requests = s->batched_requests.load();
+    out->batches = s->batches.load() - batches0;
+    out->batched_requests = s->batched_requests.load() - batched0;
     return OL_OK;
 }
 
