@@ -905,24 +905,31 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
                                             "..=" + std::to_string(OL_MAX_SET_PATTERN_LEN));
     }
     std::lock_guard<std::mutex> g(c->mu);
-    // Key each pattern by its first min(len, 4) bytes.
-    std::vector<uint32_t> cls(count), key(count), order(count);
+    // Class and key of each pattern: class = length for 1..3, 4 for 4..7,
+    // 5 for >= 8; key = its first min(len, 3), 4 or 8 bytes (lo, hi words).
+    std::vector<uint32_t> cls(count), klo(count), khi(count), order(count);
     size_t pool_bytes = 0;
     uint32_t max_len = 0;
-    for (size_t i = 0; i < count; i++) {
-        uint32_t k = (uint32_t)std::min<size_t>(lens[i], 4);
+    auto word = [&](const uint8_t *p, size_t from, size_t n) {
         uint32_t v = 0;
-        for (uint32_t b = 0; b < k; b++)
-            v |= (uint32_t)patterns[i][b] << (8 * b);
+        for (size_t b = 0; b < n; b++)
+            v |= (uint32_t)p[from + b] << (8 * b);
+        return v;
+    };
+    for (size_t i = 0; i < count; i++) {
+        const size_t L = lens[i];
+        const uint32_t k = L <= 3 ? (uint32_t)L : (L < 8 ? 4u : 5u);
         cls[i] = k;
-        key[i] = v;
+        klo[i] = word(patterns[i], 0, std::min<size_t>(L, 4));
+        khi[i] = k == 5 ? word(patterns[i], 4, 4) : 0u;
         order[i] = (uint32_t)i;
-        pool_bytes += lens[i];
-        max_len = std::max<uint32_t>(max_len, (uint32_t)lens[i]);
+        pool_bytes += L;
+        max_len = std::max<uint32_t>(max_len, (uint32_t)L);
     }
     std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         if (cls[a] != cls[b]) return cls[a] < cls[b];
-        if (key[a] != key[b]) return key[a] < key[b];
+        if (klo[a] != klo[b]) return klo[a] < klo[b];
+        if (khi[a] != khi[b]) return khi[a] < khi[b];
         return a < b;
     });
     // Entries (in sorted order) and the pattern byte pool.
@@ -939,53 +946,68 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     }
     // Per class: distinct keys with their entry ranges.
     struct Bucket {
-        uint32_t key, begin, end;
+        uint32_t lo, hi, begin, end;
     };
-    std::vector<Bucket> buckets[5];
+    std::vector<Bucket> buckets[6];
     for (size_t e = 0; e < count;) {
         uint32_t i = order[e];
         size_t f = e + 1;
-        while (f < count && cls[order[f]] == cls[i] && key[order[f]] == key[i])
+        while (f < count && cls[order[f]] == cls[i] && klo[order[f]] == klo[i] && khi[order[f]] == khi[i])
             f++;
-        buckets[cls[i]].push_back({key[i], (uint32_t)e, (uint32_t)f});
+        buckets[cls[i]].push_back({klo[i], khi[i], (uint32_t)e, (uint32_t)f});
         e = f;
     }
     DictDev dd{};
     dd.max_len = max_len;
+    // filters: 1 and 2 direct, 3 hashed (2^16 bits), 4 = 4-byte prefixes of classes 4 and 5
     std::vector<std::vector<uint32_t>> filt(5);
-    std::vector<std::vector<uint32_t>> tab(5);   // 4 words per slot
     uint32_t fwords = 0;
-    for (uint32_t k = 1; k <= 4; k++) {
-        size_t nb = buckets[k].size();
+    std::vector<uint32_t> pref4;
+    for (uint32_t k = 4; k <= 5; k++)
+        for (const Bucket &b : buckets[k])
+            pref4.push_back(b.lo);
+    std::sort(pref4.begin(), pref4.end());
+    pref4.erase(std::unique(pref4.begin(), pref4.end()), pref4.end());
+    for (uint32_t f = 1; f <= 4; f++) {
+        const bool used = f <= 3 ? !buckets[f].empty() : !pref4.empty();
+        if (!used)
+            continue;
+        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : f == 3 ? 16 : (pref4.size() <= 4096 ? 16 : pref4.size() <= 16384 ? 17 : 18);
+        dd.fbits[f] = fb;
+        dd.foff[f] = fwords;
+        const uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
+        filt[f].assign(words, 0);
+        fwords += words;
+        auto setbit = [&](uint32_t fi) { filt[f][fi >> 5] |= 1u << (fi & 31); };
+        if (f <= 2)
+            for (const Bucket &b : buckets[f]) setbit(b.lo);
+        else if (f == 3)
+            for (const Bucket &b : buckets[3]) setbit(dict_fhash3(b.lo, fb));
+        else
+            for (uint32_t p4 : pref4) setbit(dict_fhash4(p4, fb));
+    }
+    dd.fwords = fwords;
+    // tables
+    std::vector<std::vector<uint32_t>> tab(6);   // 4 words per slot
+    for (uint32_t k = 1; k <= 5; k++) {
+        const size_t nb = buckets[k].size();
         if (!nb)
             continue;
         dd.class_mask |= 1u << k;
-        uint32_t fb;
-        if (k == 1) fb = 8;
-        else if (k == 2) fb = 16;
-        else if (k == 3) fb = std::min<uint32_t>(16, std::max<uint32_t>(10, ceil_log2(nb) + 3));
-        else fb = std::min<uint32_t>(17, std::max<uint32_t>(10, ceil_log2(nb) + 2));
-        dd.fbits[k] = fb;
-        dd.foff[k] = fwords;
-        uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
-        filt[k].assign(words, 0);
-        fwords += words;
-        uint32_t tb = std::max<uint32_t>(4, ceil_log2(nb * 2));
+        const uint32_t tb = std::max<uint32_t>(4, ceil_log2(nb * 2));
         dd.tbits[k] = tb;
         tab[k].assign((size_t)4 << tb, 0);
-        uint32_t tmask = (1u << tb) - 1;
+        const uint32_t tmask = (1u << tb) - 1;
         for (const Bucket &b : buckets[k]) {
-            uint32_t fi = (k <= 2) ? b.key : (uint32_t)((b.key * 0x9E3779B1u) >> (32 - fb));
-            filt[k][fi >> 5] |= 1u << (fi & 31);
-            uint32_t h = (uint32_t)(((b.key ^ (b.key >> 15)) * 0x85EBCA6Bu) >> (32 - tb));
+            uint32_t h = dict_thash(b.lo, b.hi, tb);
             while (tab[k][(size_t)h * 4 + 2] != 0)
                 h = (h + 1) & tmask;
-            tab[k][(size_t)h * 4 + 0] = b.key;
+            tab[k][(size_t)h * 4 + 0] = b.lo;
             tab[k][(size_t)h * 4 + 1] = b.begin;
             tab[k][(size_t)h * 4 + 2] = b.end;
+            tab[k][(size_t)h * 4 + 3] = b.hi;
         }
     }
-    dd.fwords = fwords;
     // One device allocation: [filter][tables][ent_id][ent_len][ent_off][pool]
     size_t bytes = 0;
     auto place = [&](size_t n) {
@@ -993,9 +1015,9 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         bytes = o + n;
         return o;
     };
-    size_t o_f = place((size_t)fwords * 4);
-    size_t o_t[5] = {0, 0, 0, 0, 0};
-    for (uint32_t k = 1; k <= 4; k++)
+    size_t o_f = place((size_t)std::max<uint32_t>(fwords, 1) * 4);
+    size_t o_t[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t k = 1; k <= 5; k++)
         if (!tab[k].empty())
             o_t[k] = place(tab[k].size() * 4);
     size_t o_id = place(count * 4), o_len = place(count * 4), o_off = place(count * 4);
@@ -1009,12 +1031,12 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     }
     uint8_t *base = (uint8_t *)s->dev;
     std::vector<uint8_t> host(bytes, 0);
-    for (uint32_t k = 1; k <= 4; k++) {
-        if (!filt[k].empty())
-            std::memcpy(host.data() + o_f + (size_t)dd.foff[k] * 4, filt[k].data(), filt[k].size() * 4);
+    for (uint32_t f = 1; f <= 4; f++)
+        if (!filt[f].empty())
+            std::memcpy(host.data() + o_f + (size_t)dd.foff[f] * 4, filt[f].data(), filt[f].size() * 4);
+    for (uint32_t k = 1; k <= 5; k++)
         if (!tab[k].empty())
             std::memcpy(host.data() + o_t[k], tab[k].data(), tab[k].size() * 4);
-    }
     std::memcpy(host.data() + o_id, ent_id.data(), count * 4);
     std::memcpy(host.data() + o_len, ent_len.data(), count * 4);
     std::memcpy(host.data() + o_off, ent_off.data(), count * 4);
@@ -1025,7 +1047,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         return fail(OL_ERR_DEVICE, "dictionary upload failed");
     }
     dd.filter = (const uint32_t *)(base + o_f);
-    for (uint32_t k = 1; k <= 4; k++)
+    for (uint32_t k = 1; k <= 5; k++)
         dd.table[k] = tab[k].empty() ? nullptr : (const uint4 *)(base + o_t[k]);
     dd.ent_id = (const uint32_t *)(base + o_id);
     dd.ent_len = (const uint32_t *)(base + o_len);

@@ -202,20 +202,48 @@ struct SearchArgs {
 // LDS by every workgroup) and an open-addressing table of 16-byte slots
 // {key, begin, end, 0} giving the range of that key's entries.  Entries are
 // sorted by (class, key, pattern id).
+// Compiled dictionary.  Patterns are keyed by class: 1, 2 and 3 for patterns
+// of that length (key = all their bytes), 4 for lengths 4..7 (key = first 4
+// bytes), 5 for lengths >= 8 (key = first 8 bytes).  Per class an
+// open-addressing table of {key_lo, begin, end, key_hi} slots (end == 0:
+// empty) maps a key to its run of entries, sorted by pattern id.  LDS filters:
+// class 1 and 2 direct bitmaps, class 3 a hashed bitmap of the 3-byte key, and
+// one hashed bitmap of the 4-byte prefix shared by classes 4 and 5 (index 4).
 struct DictDev {
-    uint32_t class_mask;     // bit k set if class k (1..4) is non-empty
-    uint32_t fbits[5];       // log2 filter bits per class
-    uint32_t foff[5];        // word offset of each class's filter in `filter`
+    uint32_t class_mask;     // bit k set if class k (1..5) is non-empty
+    uint32_t fbits[6];       // log2 bits of the filters 1..4
+    uint32_t foff[6];        // word offset of each filter in `filter`
     uint32_t fwords;         // total filter words
-    uint32_t tbits[5];       // log2 table slots per class
+    uint32_t tbits[6];       // log2 table slots per class 1..5
     const uint32_t *filter;
-    const uint4 *table[5];
+    const uint4 *table[6];
     const uint32_t *ent_id;
     const uint32_t *ent_len;
     const uint32_t *ent_off;
     const uint8_t *pool;
     uint32_t max_len;
 };
+
+// Hashes shared by the compiler (host) and the search kernel (device).  The
+// filter hashes use 24-bit multiplies (full rate on the device).
+__host__ __device__ __forceinline__ uint32_t dict_mul24(uint32_t a, uint32_t b)
+{
+    return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+}
+__host__ __device__ __forceinline__ uint32_t dict_fhash3(uint32_t key3, uint32_t bits)
+{
+    return dict_mul24(key3, 0x9E3779u) >> (32 - bits);
+}
+__host__ __device__ __forceinline__ uint32_t dict_fhash4(uint32_t key4, uint32_t bits)
+{
+    return (dict_mul24(key4, 0x9E3779u) ^ dict_mul24(key4 >> 8, 0x85EBCAu)) >> (32 - bits);
+}
+__host__ __device__ __forceinline__ uint32_t dict_thash(uint32_t lo, uint32_t hi, uint32_t bits)
+{
+    uint32_t x = lo ^ (hi * 0x9E3779B1u);
+    x ^= x >> 15;
+    return (x * 0x85EBCA6Bu) >> (32 - bits);
+}
 
 struct ManyArgs {
     const uint8_t *text;

@@ -192,48 +192,55 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 }
 
 // ---- dictionary -----------------------------------------------------------------
-__device__ __forceinline__ uint32_t filter_index(uint32_t k, uint32_t key, uint32_t bits)
-{
-    if (k <= 2)
-        return key;   // direct: 2^8 / 2^16 bits
-    return (key * 0x9E3779B1u) >> (32 - bits);
-}
+// Per tile (256 lanes x 64 positions, staged in LDS with 128 bytes after it):
+//   1. filter: each lane tests its 64 positions against the LDS bitmaps of the
+//      dictionary's key classes (straight-line code, 4-byte windows from
+//      registers) and gets a 64-bit candidate mask;
+//   2. the candidates are compacted into an LDS list in position order;
+//   3. count: thread i probes candidates i, i + 256, ...: the class tables
+//      give, per class, the run of patterns sharing the key (sorted by id);
+//      the runs are merged by id and each pattern's remaining bytes compared;
+//   4. an exclusive scan of the per-candidate counts places every
+//      candidate's matches, the look-back places the tile, and (when output
+//      is wanted) the candidates are probed again and their matches written.
+// A tile with more candidates than the list holds is handled per lane.
 
-__device__ __forceinline__ uint32_t table_hash(uint32_t key, uint32_t bits)
-{
-    return ((key ^ (key >> 15)) * 0x85EBCA6Bu) >> (32 - bits);
-}
+constexpr int MANY_HALO = 128;      // bytes staged after the tile (8-byte keys, 64-byte patterns)
+constexpr int MANY_LCAP = 2048;     // candidates per tile taken through the list
 
-// Matches starting at local position lp (global gp).  When `emit`, writes them
-// at out[r..] (up to cap) in pattern-id order.  Returns the number of matches.
-// `lim` (> gp) bounds the match end: the text end, or the end of gp's document.
-__device__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
-                                   uint32_t lp, uint32_t gp, uint32_t lim, uint32_t w, bool emit,
-                                   uint64_t *out_off, uint32_t *out_id, uint64_t &r, uint64_t cap)
+// Matches starting at tile position lp (global gp) with at most `rem` bytes
+// (1..64) before the text or document end; written from out[r] in pattern-id
+// order (up to cap) when `emit`.  Returns their number.
+__device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8_t *s_text, uint32_t lp, uint64_t gp,
+                                                   uint32_t rem, bool emit, uint64_t *out_off, uint32_t *out_id,
+                                                   uint64_t r, uint64_t cap)
 {
-    const uint32_t rem = lim - gp;
-    uint32_t beg[4], end[4], kk[4];
+    const uint32_t *wp = (const uint32_t *)(s_text + (lp & ~3u));
+    const uint32_t sh = lp & 3u;
+    const uint32_t a0 = wp[0], a1 = wp[1], a2 = wp[2];
+    const uint32_t w0 = sh ? __builtin_amdgcn_alignbyte(a1, a0, sh) : a0;
+    const uint32_t w1 = sh ? __builtin_amdgcn_alignbyte(a2, a1, sh) : a1;
+    uint32_t beg[5], end[5], kl[5];
     int nr = 0;
 #pragma unroll
-    for (uint32_t k = 1; k <= 4; k++) {
-        if (!((d.class_mask >> k) & 1u) || k > rem)
+    for (uint32_t k = 1; k <= 5; k++) {
+        const uint32_t klen = k <= 4 ? k : 8u;   // bytes the key covers
+        if (!((d.class_mask >> k) & 1u) || klen > rem)
             continue;
-        uint32_t key = (k == 4) ? w : (w & ((1u << (8 * k)) - 1u));
-        uint32_t fi = filter_index(k, key, d.fbits[k]);
-        if (!((s_filter[d.foff[k] + (fi >> 5)] >> (fi & 31)) & 1u))
-            continue;
+        const uint32_t lo = k == 1 ? (w0 & 0xFFu) : k == 2 ? (w0 & 0xFFFFu) : k == 3 ? (w0 & 0xFFFFFFu) : w0;
+        const uint32_t hi = k == 5 ? w1 : 0u;
         const uint32_t tb = d.tbits[k];
         const uint32_t tmask = (1u << tb) - 1u;
-        uint32_t h = table_hash(key, tb);
         const uint4 *tab = d.table[k];
+        uint32_t h = dict_thash(lo, hi, tb);
         while (true) {
-            uint4 sl = tab[h];
+            const uint4 sl = tab[h];
             if (sl.z == 0)
                 break;
-            if (sl.x == key) {
+            if (sl.x == lo && sl.w == hi) {
                 beg[nr] = sl.y;
                 end[nr] = sl.z;
-                kk[nr] = k;
+                kl[nr] = k == 4 ? 4u : klen;
                 nr++;
                 break;
             }
@@ -248,7 +255,7 @@ __device__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, c
         uint32_t bid = 0xFFFFFFFFu;
         for (int i = 0; i < nr; i++) {
             if (beg[i] < end[i]) {
-                uint32_t id = d.ent_id[beg[i]];
+                const uint32_t id = d.ent_id[beg[i]];
                 if (id < bid) {
                     bid = id;
                     best = i;
@@ -263,7 +270,7 @@ __device__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, c
             continue;
         const uint8_t *pb = d.pool + d.ent_off[e];
         bool ok = true;
-        for (uint32_t i = kk[best]; i < len; i++) {
+        for (uint32_t i = kl[best]; i < len; i++) {
             if (pb[i] != s_text[lp + i]) {
                 ok = false;
                 break;
@@ -271,35 +278,40 @@ __device__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, c
         }
         if (!ok)
             continue;
-        count++;
-        if (emit) {
-            if (r < cap) {
-                if (out_off) out_off[r] = gp;
-                if (out_id) out_id[r] = bid;
-            }
-            r++;
+        if (emit && r + count < cap) {
+            if (out_off) out_off[r + count] = gp;
+            if (out_id) out_id[r + count] = bid;
         }
+        count++;
     }
     return count;
 }
 
-template <int THREADS, int BPT>
+__device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i) { return (f[i >> 5] >> (i & 31)) & 1u; }
+
+template <int THREADS>
 __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
 {
-    constexpr int TILE = THREADS * BPT;
-    static_assert(BPT == 64, "64-bit lane masks");
+    constexpr int TILE = THREADS * 64;
     extern __shared__ __align__(16) uint32_t s_dyn[];
+    __shared__ uint16_t s_cand[MANY_LCAP];
+    __shared__ uint8_t s_rem[MANY_LCAP];
+    __shared__ uint32_t s_cnt[MANY_LCAP];
     __shared__ uint32_t s_w[THREADS / 64];
     __shared__ uint64_t s_prefix;
-
-    uint8_t *s_text = (uint8_t *)s_dyn;                       // TILE + HALO_AFTER bytes
-    uint32_t *s_filter = s_dyn + (TILE + HALO_AFTER) / 4;   // fwords
-
     __shared__ uint32_t s_tile;
 
+    uint8_t *s_text = (uint8_t *)s_dyn;                      // TILE + MANY_HALO bytes
+    uint32_t *s_filter = s_dyn + (TILE + MANY_HALO) / 4;    // fwords
+    const DictDev &d = a.d;
     const int tid = threadIdx.x;
-    for (uint32_t i = tid; i < a.d.fwords; i += THREADS)
-        s_filter[i] = a.d.filter[i];
+    for (uint32_t i = tid; i < d.fwords; i += THREADS)
+        s_filter[i] = d.filter[i];
+    const uint32_t cm = d.class_mask;
+    const uint32_t *f1 = s_filter + d.foff[1], *f2 = s_filter + d.foff[2], *f3 = s_filter + d.foff[3],
+                   *f4 = s_filter + d.foff[4];
+    const uint32_t fb4 = d.fbits[4];
+
     // Tiles are claimed in order from a ticket: a tile's look-back then only
     // waits on tiles already claimed by running workgroups, whatever the
     // residency of the launch (its LDS depends on the dictionary).
@@ -311,20 +323,24 @@ __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
         if (t >= a.num_tiles)
             break;
         const uint32_t base = t * TILE;
-        load_tile<THREADS, BPT>(s_text, a.text, base);
+        {
+            const uint4 *src = (const uint4 *)(a.text + base);
+            uint4 *dst = (uint4 *)s_text;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                dst[k * THREADS + tid] = src[k * THREADS + tid];
+            if (tid < MANY_HALO / 16)
+                dst[TILE / 16 + tid] = src[TILE / 16 + tid];
+        }
         __syncthreads();
 
-        const uint32_t lo = tid * BPT;
+        const uint32_t lo = tid * 64;
         const uint32_t g0 = base + lo;
-        uint32_t lim = (g0 < a.n) ? a.n - g0 : 0;
-        if (lim > BPT)
-            lim = BPT;
+        const uint32_t lim = (g0 < a.n) ? ((a.n - g0) < 64u ? a.n - g0 : 64u) : 0u;
 
-        const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
         // match-end bound per position: the text end, or (batches) the end of
         // the position's document; positions between documents match nothing
-        uint32_t dlim = a.n, dnext = 0xFFFFFFFFu;
-        uint32_t di = 0;
+        uint32_t dlim = a.n, dnext = 0xFFFFFFFFu, di = 0;
         if (a.ndocs) {
             di = a.tile_doc[t];   // then the last document starting at or before g0
             while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= g0)
@@ -332,48 +348,120 @@ __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
             dlim = a.doc_ends[di];
             dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
         }
-        auto bound = [&](uint32_t gp) -> uint32_t {   // 0: no match may start at gp
+        const uint32_t di0 = di, dlim0 = dlim, dnext0 = dnext;
+        auto rem_at = [&](uint32_t gp) -> uint32_t {   // 0: no match may start at gp
             while (gp >= dnext) {
                 di++;
                 dlim = a.doc_ends[di];
                 dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
             }
-            return gp < dlim ? dlim : 0u;
+            const uint32_t rm = gp < dlim ? dlim - gp : 0u;
+            return rm < 64u ? rm : 64u;
         };
-        const uint32_t di0 = di, dlim0 = dlim, dnext0 = dnext;
-        uint64_t has = 0;
-        uint32_t count = 0;
-        uint64_t dummy = 0;
-        for (uint32_t p = 0; p < lim; p++) {
-            const uint32_t bl = bound(g0 + p);
-            if (!bl)
-                continue;
-            uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
-            uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
-            uint32_t c = probe_position(a.d, s_filter, s_text, lo + p, g0 + p, bl, win, false,
-                                        nullptr, nullptr, dummy, 0);
-            if (c) {
-                has |= 1ull << p;
-                count += c;
+
+        // 1. filter
+        uint64_t cand = 0;
+        {
+            uint32_t W[17];
+            const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
+#pragma unroll
+            for (int j = 0; j < 17; j++)
+                W[j] = wsrc[j];
+#pragma unroll
+            for (int p = 0; p < 64; p++) {
+                const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3) : W[p >> 2];
+                uint32_t hit = 0;
+                if (cm & 0x02u) hit |= filter_bit(f1, w & 0xFFu);
+                if (cm & 0x04u) hit |= filter_bit(f2, w & 0xFFFFu);
+                if (cm & 0x08u) hit |= filter_bit(f3, dict_fhash3(w, 16));
+                if (cm & 0x30u) hit |= filter_bit(f4, dict_fhash4(w, fb4));
+                cand |= (uint64_t)hit << p;
+            }
+            if (lim < 64)
+                cand &= lim ? ((1ull << lim) - 1ull) : 0ull;
+        }
+        // positions outside documents (batches) are no candidates; remaining bytes per candidate
+        uint32_t nc = (uint32_t)__popcll(cand);
+        uint32_t total_c;
+        const uint32_t cex = block_excl_sum<THREADS>(nc, s_w, total_c);
+        const bool listed = total_c <= (uint32_t)MANY_LCAP;
+        if (listed) {
+            uint32_t at = cex;
+            uint64_t m = cand;
+            while (m) {
+                const uint32_t p = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                s_cand[at] = (uint16_t)(lo + p);
+                s_rem[at] = (uint8_t)rem_at(g0 + p);
+                at++;
             }
         }
-        uint32_t total;
-        const uint32_t excl = block_excl_sum<THREADS>(count, s_w, total);
+        __syncthreads();
+
+        // 2-3. count
+        uint32_t own = 0;   // per-lane count (unlisted tiles)
+        if (listed) {
+            for (uint32_t i = tid; i < total_c; i += THREADS) {
+                const uint32_t lp = s_cand[i], rm = s_rem[i];
+                s_cnt[i] = rm ? probe_position(d, s_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0)
+                              : 0u;
+            }
+        } else {
+            di = di0;
+            dlim = dlim0;
+            dnext = dnext0;
+            uint64_t m = cand;
+            while (m) {
+                const uint32_t p = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t rm = rem_at(g0 + p);
+                if (rm)
+                    own += probe_position(d, s_text, lo + p, (uint64_t)g0 + p, rm, false, nullptr, nullptr, 0, 0);
+            }
+        }
+        __syncthreads();
+        // 4. place: per-candidate exclusive offsets (listed) or per-lane ones
+        uint32_t total, lane_ex = 0;
+        if (listed) {
+            const uint32_t K = (total_c + THREADS - 1) / THREADS;
+            const uint32_t i0 = min(total_c, tid * K), i1 = min(total_c, i0 + K);
+            uint32_t loc = 0;
+            for (uint32_t i = i0; i < i1; i++)
+                loc += s_cnt[i];
+            uint32_t run = block_excl_sum<THREADS>(loc, s_w, total);
+            for (uint32_t i = i0; i < i1; i++) {
+                const uint32_t c = s_cnt[i];
+                s_cnt[i] = run;
+                run += c;
+            }
+        } else {
+            lane_ex = block_excl_sum<THREADS>(own, s_w, total);
+        }
         const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix, a.lb_err);
         if (tid == 0 && t == a.num_tiles - 1)
             a.meta[0] = prefix + total;
         if (a.out_off || a.out_id) {
-            uint64_t r = prefix + excl;
-            di = di0;
-            dlim = dlim0;
-            dnext = dnext0;
-            while (has && r < a.cap) {
-                uint32_t p = (uint32_t)__builtin_ctzll(has);
-                has &= has - 1;
-                uint32_t w0 = wsrc[p >> 2], w1 = wsrc[(p >> 2) + 1];
-                uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w1, w0, p & 3) : w0;
-                probe_position(a.d, s_filter, s_text, lo + p, g0 + p, bound(g0 + p), win, true, a.out_off,
-                               a.out_id, r, a.cap);
+            if (listed) {
+                for (uint32_t i = tid; i < total_c; i += THREADS) {
+                    const uint32_t lp = s_cand[i], rm = s_rem[i];
+                    const uint64_t r = prefix + s_cnt[i];
+                    if (rm && r < a.cap)
+                        probe_position(d, s_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r, a.cap);
+                }
+            } else {
+                di = di0;
+                dlim = dlim0;
+                dnext = dnext0;
+                uint64_t r = prefix + lane_ex;
+                uint64_t m = cand;
+                while (m && r < a.cap) {
+                    const uint32_t p = (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1;
+                    const uint32_t rm = rem_at(g0 + p);
+                    if (rm)
+                        r += probe_position(d, s_text, lo + p, (uint64_t)g0 + p, rm, true, a.out_off, a.out_id, r,
+                                            a.cap);
+                }
             }
         }
         __syncthreads();   // LDS is reused by the next tile
@@ -382,17 +470,19 @@ __global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
 
 size_t ol_many_lds_bytes(uint32_t fwords)
 {
-    return (size_t)SEARCH_TILE + HALO_AFTER + (size_t)fwords * 4;
+    return (size_t)SEARCH_TILE + MANY_HALO + (size_t)fwords * 4;
 }
 
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
 {
     const size_t lds = ol_many_lds_bytes(a.d.fwords);
-    const void *f = (const void *)many_kernel<SEARCH_THREADS, SEARCH_BPT>;
+    const void *f = (const void *)many_kernel<SEARCH_THREADS>;
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     uint32_t cap = max_grid(f, SEARCH_THREADS, lds);
     if (!cap)
         return hipErrorInvalidValue;
     uint32_t grid = a.num_tiles < cap ? a.num_tiles : cap;
-    hipLaunchKernelGGL((many_kernel<SEARCH_THREADS, SEARCH_BPT>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
+    hipLaunchKernelGGL((many_kernel<SEARCH_THREADS>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
     return hipGetLastError();
 }
