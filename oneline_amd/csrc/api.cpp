@@ -932,18 +932,25 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         if (khi[a] != khi[b]) return khi[a] < khi[b];
         return a < b;
     });
-    // Entries (in sorted order) and the pattern byte pool.
-    std::vector<uint32_t> ent_id(count), ent_len(count), ent_off(count);
-    std::vector<uint8_t> pool(pool_bytes ? pool_bytes : 1);
-    size_t at = 0;
+    // Entries (in sorted order) and the word-aligned pattern pool.
+    (void)pool_bytes;
+    std::vector<uint32_t> ents((size_t)count * 4);
+    std::vector<uint32_t> pool;
     for (size_t e = 0; e < count; e++) {
-        uint32_t i = order[e];
-        ent_id[e] = i;
-        ent_len[e] = (uint32_t)lens[i];
-        ent_off[e] = (uint32_t)at;
-        std::memcpy(pool.data() + at, patterns[i], lens[i]);
-        at += lens[i];
+        const uint32_t i = order[e];
+        const size_t L = lens[i];
+        const uint32_t kb = cls[i] <= 3 ? cls[i] : (cls[i] == 4 ? 4u : 8u);   // key bytes
+        uint32_t chk = 0;
+        for (size_t b = kb; b < L && b < kb + 4; b++)
+            chk |= (uint32_t)patterns[i][b] << (8 * (b - kb));
+        ents[e * 4 + 0] = i;
+        ents[e * 4 + 1] = (uint32_t)L;
+        ents[e * 4 + 2] = (uint32_t)pool.size();
+        ents[e * 4 + 3] = chk;
+        for (size_t b = 0; b < L; b += 4)
+            pool.push_back(word(patterns[i], b, std::min<size_t>(4, L - b)));
     }
+    pool.resize(pool.size() + 16, 0);
     // Per class: distinct keys with their entry ranges.
     struct Bucket {
         uint32_t lo, hi, begin, end;
@@ -1020,8 +1027,8 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     for (uint32_t k = 1; k <= 5; k++)
         if (!tab[k].empty())
             o_t[k] = place(tab[k].size() * 4);
-    size_t o_id = place(count * 4), o_len = place(count * 4), o_off = place(count * 4);
-    size_t o_pool = place(pool.size() + 64);
+    size_t o_ent = place(ents.size() * 4);
+    size_t o_pool = place(pool.size() * 4);
     ol_dict *s = new (std::nothrow) ol_dict();
     if (!s)
         return fail(OL_ERR_NOMEM, "out of host memory");
@@ -1037,10 +1044,8 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     for (uint32_t k = 1; k <= 5; k++)
         if (!tab[k].empty())
             std::memcpy(host.data() + o_t[k], tab[k].data(), tab[k].size() * 4);
-    std::memcpy(host.data() + o_id, ent_id.data(), count * 4);
-    std::memcpy(host.data() + o_len, ent_len.data(), count * 4);
-    std::memcpy(host.data() + o_off, ent_off.data(), count * 4);
-    std::memcpy(host.data() + o_pool, pool.data(), pool.size());
+    std::memcpy(host.data() + o_ent, ents.data(), ents.size() * 4);
+    std::memcpy(host.data() + o_pool, pool.data(), pool.size() * 4);
     if (hipMemcpy(base, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(s->dev);
         delete s;
@@ -1049,10 +1054,8 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     dd.filter = (const uint32_t *)(base + o_f);
     for (uint32_t k = 1; k <= 5; k++)
         dd.table[k] = tab[k].empty() ? nullptr : (const uint4 *)(base + o_t[k]);
-    dd.ent_id = (const uint32_t *)(base + o_id);
-    dd.ent_len = (const uint32_t *)(base + o_len);
-    dd.ent_off = (const uint32_t *)(base + o_off);
-    dd.pool = base + o_pool;
+    dd.ents = (const uint4 *)(base + o_ent);
+    dd.pool = (const uint32_t *)(base + o_pool);
     s->d = dd;
     s->count = (uint32_t)count;
     *out = s;

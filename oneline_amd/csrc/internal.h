@@ -217,10 +217,9 @@ struct DictDev {
     uint32_t tbits[6];       // log2 table slots per class 1..5
     const uint32_t *filter;
     const uint4 *table[6];
-    const uint32_t *ent_id;
-    const uint32_t *ent_len;
-    const uint32_t *ent_off;
-    const uint8_t *pool;
+    const uint4 *ents;       // per entry {pattern id, length, word offset in pool, check word}: the
+                             // check word holds pattern bytes key..key+3 (zero past the end)
+    const uint32_t *pool;    // pattern bytes, each pattern starting on a word, zero padded
     uint32_t max_len;
 };
 

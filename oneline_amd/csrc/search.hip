@@ -249,32 +249,52 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8
     }
     if (nr == 0)
         return 0;
+    // text word at tile position x (any alignment)
+    auto tword = [&](uint32_t x) -> uint32_t {
+        const uint32_t *q = (const uint32_t *)(s_text + (x & ~3u));
+        const uint32_t sx = x & 3u;
+        return sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
+    };
+    auto tail_mask = [](uint32_t nb) -> uint32_t { return nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u); };
+    uint4 cur[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if (i < nr && beg[i] < end[i])
+            cur[i] = d.ents[beg[i]];
     uint32_t count = 0;
     while (true) {
         int best = -1;
         uint32_t bid = 0xFFFFFFFFu;
-        for (int i = 0; i < nr; i++) {
-            if (beg[i] < end[i]) {
-                const uint32_t id = d.ent_id[beg[i]];
-                if (id < bid) {
-                    bid = id;
-                    best = i;
-                }
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            if (i < nr && beg[i] < end[i] && cur[i].x < bid) {
+                bid = cur[i].x;
+                best = i;
             }
         }
         if (best < 0)
             break;
-        const uint32_t e = beg[best]++;
-        const uint32_t len = d.ent_len[e];
+        uint4 en = cur[0];
+        uint32_t kb = kl[0];
+#pragma unroll
+        for (int i = 1; i < 5; i++)
+            if (i == best) {
+                en = cur[i];
+                kb = kl[i];
+            }
+        // advance the chosen run
+#pragma unroll
+        for (int i = 0; i < 5; i++)
+            if (i == best && ++beg[i] < end[i])
+                cur[i] = d.ents[beg[i]];
+        const uint32_t len = en.y;
         if (len > rem)
             continue;
-        const uint8_t *pb = d.pool + d.ent_off[e];
         bool ok = true;
-        for (uint32_t i = kl[best]; i < len; i++) {
-            if (pb[i] != s_text[lp + i]) {
-                ok = false;
-                break;
-            }
+        if (len > kb) {
+            ok = ((tword(lp + kb) ^ en.w) & tail_mask(len - kb)) == 0;
+            for (uint32_t j = kb + 4; ok && j < len; j += 4)
+                ok = ((tword(lp + j) ^ d.pool[en.z + (j >> 2)]) & tail_mask(len - j)) == 0;
         }
         if (!ok)
             continue;
