@@ -487,10 +487,9 @@ int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, ui
     a.cap = cap;
     a.out_off = out_off;
     a.out_id = out_id;
-    a.status = c->status();
     a.meta = c->meta();
-    a.lb_err = c->ctr() + 4;
-    a.tile_ctr = c->ctr() + 3;
+    a.tile_count = (uint32_t *)c->status();
+    a.tile_off = c->status() + tiles;
     {
         Prof p(c, "many");
         HIPCHK(ol_launch_many(a, c->stream));
@@ -979,7 +978,9 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         const bool used = f <= 3 ? !buckets[f].empty() : !pref4.empty();
         if (!used)
             continue;
-        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : f == 3 ? 16 : (pref4.size() <= 4096 ? 16 : pref4.size() <= 16384 ? 17 : 18);
+        // single-bit filters: 2^16 bits for 3-byte keys, 2^17 (2^18 beyond 16k
+        // prefixes) for 4-byte prefixes
+        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : f == 3 ? 16 : (pref4.size() <= 16384 ? 17 : 18);
         dd.fbits[f] = fb;
         dd.foff[f] = fwords;
         const uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
@@ -989,9 +990,9 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         if (f <= 2)
             for (const Bucket &b : buckets[f]) setbit(b.lo);
         else if (f == 3)
-            for (const Bucket &b : buckets[3]) setbit(dict_fhash3(b.lo, fb));
+            for (const Bucket &b : buckets[3]) setbit(dict_fh3(b.lo) >> (32 - fb));
         else
-            for (uint32_t p4 : pref4) setbit(dict_fhash4(p4, fb));
+            for (uint32_t p4 : pref4) setbit(dict_fh4(p4) >> (32 - fb));
     }
     dd.fwords = fwords;
     // tables
@@ -1358,10 +1359,9 @@ int batch_many(ol_ctx *c, ol_batch *b, const ol_dict *s, uint64_t cap)
     a.cap = cap;
     a.out_off = cap ? (uint64_t *)b->m_off.p : nullptr;
     a.out_id = cap ? (uint32_t *)b->m_id.p : nullptr;
-    a.status = c->status();
     a.meta = c->meta();
-    a.lb_err = c->ctr() + 4;
-    a.tile_ctr = c->ctr() + 3;
+    a.tile_count = (uint32_t *)c->status();
+    a.tile_off = c->status() + tiles;
     a.ndocs = b->nd;
     a.doc_starts = (const uint32_t *)b->d_starts.p;
     a.doc_ends = (const uint32_t *)b->d_ends.p;

@@ -229,13 +229,12 @@ __host__ __device__ __forceinline__ uint32_t dict_mul24(uint32_t a, uint32_t b)
 {
     return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
 }
-__host__ __device__ __forceinline__ uint32_t dict_fhash3(uint32_t key3, uint32_t bits)
+// Filter hashes of a 3-byte key and of a 4-byte prefix; a filter of 2^b
+// bits uses the top b bits of the hash.
+__host__ __device__ __forceinline__ uint32_t dict_fh3(uint32_t key3) { return dict_mul24(key3, 0x9E3779u); }
+__host__ __device__ __forceinline__ uint32_t dict_fh4(uint32_t key4)
 {
-    return dict_mul24(key3, 0x9E3779u) >> (32 - bits);
-}
-__host__ __device__ __forceinline__ uint32_t dict_fhash4(uint32_t key4, uint32_t bits)
-{
-    return (dict_mul24(key4, 0x9E3779u) ^ dict_mul24(key4 >> 8, 0x85EBCAu)) >> (32 - bits);
+    return dict_mul24(key4, 0x9E3779u) ^ dict_mul24(key4 >> 8, 0x85EBCAu);
 }
 __host__ __device__ __forceinline__ uint32_t dict_thash(uint32_t lo, uint32_t hi, uint32_t bits)
 {
@@ -252,10 +251,9 @@ struct ManyArgs {
     uint64_t cap;
     uint64_t *out_off;       // u64 offsets (nullable)
     uint32_t *out_id;        // pattern ids (nullable)
-    uint64_t *status;
-    uint64_t *meta;          // meta[0] <- total matches
-    uint32_t *tile_ctr;      // tile ticket, zero at launch
-    uint32_t *lb_err;        // set when a look-back wait gave up
+    uint64_t *meta;          // meta[0] <- total matches (zero at launch)
+    uint32_t *tile_count;    // [num_tiles] matches per tile
+    uint64_t *tile_off;      // [num_tiles] first output index of each tile
     // Batches (ndocs > 0): a match must end inside the document it starts in
     // (the bytes between documents belong to none).
     uint32_t ndocs;

@@ -192,20 +192,21 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 }
 
 // ---- dictionary -----------------------------------------------------------------
-// Per tile (256 lanes x 64 positions, staged in LDS with 128 bytes after it):
-//   1. filter: each lane tests its 64 positions against the LDS bitmaps of the
-//      dictionary's key classes (straight-line code, 4-byte windows from
-//      registers) and gets a 64-bit candidate mask;
-//   2. the candidates are compacted into an LDS list in position order;
-//   3. count: thread i probes candidates i, i + 256, ...: the class tables
-//      give, per class, the run of patterns sharing the key (sorted by id);
-//      the runs are merged by id and each pattern's remaining bytes compared;
-//   4. an exclusive scan of the per-candidate counts places every
-//      candidate's matches, the look-back places the tile, and (when output
-//      is wanted) the candidates are probed again and their matches written.
-// A tile with more candidates than the list holds is handled per lane.
+// Three launches with no waits between workgroups:
+//   many_count_kernel  per tile (256 lanes x 64 positions): filter, candidate
+//                      list, per-candidate match counts -> tile_count[t];
+//   many_scan_kernel   exclusive scan of tile_count -> tile_off, total;
+//   many_write_kernel  per tile again, each candidate's matches written at
+//                      tile_off[t] + (matches of the earlier candidates).
+// Filter: each lane tests its 64 positions against the LDS bitmaps of the
+// dictionary's key classes (straight-line code, 4-byte windows from
+// registers).  Probe: the class tables give, per class, the run of patterns
+// sharing the key (sorted by id); the runs are merged by id and each pattern's
+// remaining bytes compared word by word.  A tile with more candidates than
+// the LDS list holds is probed per lane.  The text is read from global memory
+// (the probes hit L2), so the LDS holds only the filters and the list.
 
-constexpr int MANY_HALO = 128;      // bytes staged after the tile (8-byte keys, 64-byte patterns)
+constexpr int MANY_HALO = 128;      // bytes readable past a tile (8-byte keys, 64-byte patterns)
 constexpr int MANY_LCAP = 2048;     // candidates per tile taken through the list
 
 // Matches starting at tile position lp (global gp) with at most `rem` bytes
@@ -215,39 +216,39 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8
                                                    uint32_t rem, bool emit, uint64_t *out_off, uint32_t *out_id,
                                                    uint64_t r, uint64_t cap)
 {
+    // s_text: the tile's first byte in global memory (word aligned)
     const uint32_t *wp = (const uint32_t *)(s_text + (lp & ~3u));
     const uint32_t sh = lp & 3u;
     const uint32_t a0 = wp[0], a1 = wp[1], a2 = wp[2];
     const uint32_t w0 = sh ? __builtin_amdgcn_alignbyte(a1, a0, sh) : a0;
     const uint32_t w1 = sh ? __builtin_amdgcn_alignbyte(a2, a1, sh) : a1;
-    uint32_t beg[5], end[5], kl[5];
-    int nr = 0;
-#pragma unroll
-    for (uint32_t k = 1; k <= 5; k++) {
-        const uint32_t klen = k <= 4 ? k : 8u;   // bytes the key covers
-        if (!((d.class_mask >> k) & 1u) || klen > rem)
-            continue;
-        const uint32_t lo = k == 1 ? (w0 & 0xFFu) : k == 2 ? (w0 & 0xFFFFu) : k == 3 ? (w0 & 0xFFFFFFu) : w0;
-        const uint32_t hi = k == 5 ? w1 : 0u;
-        const uint32_t tb = d.tbits[k];
-        const uint32_t tmask = (1u << tb) - 1u;
-        const uint4 *tab = d.table[k];
-        uint32_t h = dict_thash(lo, hi, tb);
-        while (true) {
-            const uint4 sl = tab[h];
-            if (sl.z == 0)
-                break;
-            if (sl.x == lo && sl.w == hi) {
-                beg[nr] = sl.y;
-                end[nr] = sl.z;
-                kl[nr] = k == 4 ? 4u : klen;
-                nr++;
-                break;
-            }
-            h = (h + 1) & tmask;
-        }
-    }
-    if (nr == 0)
+    // One run per class (1..5): the patterns whose key matches the text here.
+    // Named variables per class (no arrays: nothing may be indexed at run time).
+    // First the table slots of every class are loaded together, then resolved.
+#define OL_KEY(K, LO)                                                                       \
+    const uint32_t klo##K = (LO), khi##K = (K) == 5 ? w1 : 0u;                              \
+    const bool use##K = ((d.class_mask >> (K)) & 1u) && ((K) <= 4 ? (K) : 8u) <= rem;      \
+    uint32_t h##K = use##K ? dict_thash(klo##K, khi##K, d.tbits[K]) : 0u;                  \
+    uint4 v##K = use##K ? d.table[K][h##K] : make_uint4(0, 0, 0, 0);
+    OL_KEY(1, w0 & 0xFFu)
+    OL_KEY(2, w0 & 0xFFFFu)
+    OL_KEY(3, w0 & 0xFFFFFFu)
+    OL_KEY(4, w0)
+    OL_KEY(5, w0)
+#undef OL_KEY
+#define OL_RUN(K)                                                                           \
+    while (v##K.z != 0 && !(v##K.x == klo##K && v##K.w == khi##K)) {                        \
+        h##K = (h##K + 1) & ((1u << d.tbits[K]) - 1u);                                       \
+        v##K = d.table[K][h##K];                                                             \
+    }                                                                                        \
+    uint32_t b##K = v##K.y, e##K = v##K.z != 0 ? v##K.z : v##K.y;
+    OL_RUN(1)
+    OL_RUN(2)
+    OL_RUN(3)
+    OL_RUN(4)
+    OL_RUN(5)
+#undef OL_RUN
+    if (b1 >= e1 && b2 >= e2 && b3 >= e3 && b4 >= e4 && b5 >= e5)
         return 0;
     // text word at tile position x (any alignment)
     auto tword = [&](uint32_t x) -> uint32_t {
@@ -256,37 +257,33 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8
         return sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
     };
     auto tail_mask = [](uint32_t nb) -> uint32_t { return nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u); };
-    uint4 cur[5];
-#pragma unroll
-    for (int i = 0; i < 5; i++)
-        if (i < nr && beg[i] < end[i])
-            cur[i] = d.ents[beg[i]];
+    const uint4 none = make_uint4(0xFFFFFFFFu, 0, 0, 0);
+    uint4 c1 = b1 < e1 ? d.ents[b1] : none, c2 = b2 < e2 ? d.ents[b2] : none, c3 = b3 < e3 ? d.ents[b3] : none,
+          c4 = b4 < e4 ? d.ents[b4] : none, c5 = b5 < e5 ? d.ents[b5] : none;
     uint32_t count = 0;
     while (true) {
-        int best = -1;
-        uint32_t bid = 0xFFFFFFFFu;
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-            if (i < nr && beg[i] < end[i] && cur[i].x < bid) {
-                bid = cur[i].x;
-                best = i;
-            }
+        // the run whose next entry has the smallest pattern id (runs are sorted by id)
+        uint32_t bid = 0xFFFFFFFFu, bk = 0;
+#define OL_PICK(K)                                                                          \
+        if (b##K < e##K && c##K.x < bid) {                                                   \
+            bid = c##K.x;                                                                    \
+            bk = (K);                                                                        \
         }
-        if (best < 0)
+        OL_PICK(1) OL_PICK(2) OL_PICK(3) OL_PICK(4) OL_PICK(5)
+#undef OL_PICK
+        if (!bk)
             break;
-        uint4 en = cur[0];
-        uint32_t kb = kl[0];
-#pragma unroll
-        for (int i = 1; i < 5; i++)
-            if (i == best) {
-                en = cur[i];
-                kb = kl[i];
-            }
-        // advance the chosen run
-#pragma unroll
-        for (int i = 0; i < 5; i++)
-            if (i == best && ++beg[i] < end[i])
-                cur[i] = d.ents[beg[i]];
+        uint4 en = none;
+        uint32_t kb = 0;
+#define OL_ADV(K)                                                                           \
+        if (bk == (K)) {                                                                     \
+            en = c##K;                                                                       \
+            kb = (K) <= 4 ? (K) : 8u;                                                        \
+            if (++b##K < e##K)                                                               \
+                c##K = d.ents[b##K];                                                         \
+        }
+        OL_ADV(1) OL_ADV(2) OL_ADV(3) OL_ADV(4) OL_ADV(5)
+#undef OL_ADV
         const uint32_t len = en.y;
         if (len > rem)
             continue;
@@ -309,200 +306,261 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8
 
 __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i) { return (f[i >> 5] >> (i & 31)) & 1u; }
 
-template <int THREADS>
-__global__ __launch_bounds__(THREADS) void many_kernel(ManyArgs a)
+// One tile of the dictionary search: filter, candidate list, counts; with
+// WRITE the matches are also written from `out0` (the tile's first output
+// index).  Returns the tile's match count (in every thread).
+template <int THREADS, bool WRITE>
+__device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
+                                              uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt, uint32_t *s_w,
+                                              uint64_t out0)
 {
     constexpr int TILE = THREADS * 64;
-    extern __shared__ __align__(16) uint32_t s_dyn[];
-    __shared__ uint16_t s_cand[MANY_LCAP];
-    __shared__ uint8_t s_rem[MANY_LCAP];
-    __shared__ uint32_t s_cnt[MANY_LCAP];
-    __shared__ uint32_t s_w[THREADS / 64];
-    __shared__ uint64_t s_prefix;
-    __shared__ uint32_t s_tile;
-
-    uint8_t *s_text = (uint8_t *)s_dyn;                      // TILE + MANY_HALO bytes
-    uint32_t *s_filter = s_dyn + (TILE + MANY_HALO) / 4;    // fwords
     const DictDev &d = a.d;
     const int tid = threadIdx.x;
-    for (uint32_t i = tid; i < d.fwords; i += THREADS)
-        s_filter[i] = d.filter[i];
     const uint32_t cm = d.class_mask;
     const uint32_t *f1 = s_filter + d.foff[1], *f2 = s_filter + d.foff[2], *f3 = s_filter + d.foff[3],
                    *f4 = s_filter + d.foff[4];
-    const uint32_t fb4 = d.fbits[4];
+    const uint32_t sh4 = 32 - d.fbits[4];   // filter 4 index = hash >> sh4
+    const uint32_t base = t * TILE;
+    const uint8_t *g_text = a.text + base;
+    const uint32_t lo = tid * 64;
+    const uint32_t g0 = base + lo;
+    const uint32_t lim = (g0 < a.n) ? ((a.n - g0) < 64u ? a.n - g0 : 64u) : 0u;
 
-    // Tiles are claimed in order from a ticket: a tile's look-back then only
-    // waits on tiles already claimed by running workgroups, whatever the
-    // residency of the launch (its LDS depends on the dictionary).
-    while (true) {
-        if (tid == 0)
-            s_tile = atomicAdd(a.tile_ctr, 1u);
-        __syncthreads();
-        const uint32_t t = s_tile;
-        if (t >= a.num_tiles)
-            break;
-        const uint32_t base = t * TILE;
-        {
-            const uint4 *src = (const uint4 *)(a.text + base);
-            uint4 *dst = (uint4 *)s_text;
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                dst[k * THREADS + tid] = src[k * THREADS + tid];
-            if (tid < MANY_HALO / 16)
-                dst[TILE / 16 + tid] = src[TILE / 16 + tid];
-        }
-        __syncthreads();
-
-        const uint32_t lo = tid * 64;
-        const uint32_t g0 = base + lo;
-        const uint32_t lim = (g0 < a.n) ? ((a.n - g0) < 64u ? a.n - g0 : 64u) : 0u;
-
-        // match-end bound per position: the text end, or (batches) the end of
-        // the position's document; positions between documents match nothing
-        uint32_t dlim = a.n, dnext = 0xFFFFFFFFu, di = 0;
-        if (a.ndocs) {
-            di = a.tile_doc[t];   // then the last document starting at or before g0
-            while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= g0)
-                di++;
+    // match-end bound per position: the text end, or (batches) the end of
+    // the position's document; positions between documents match nothing
+    uint32_t dlim = a.n, dnext = 0xFFFFFFFFu, di = 0;
+    if (a.ndocs) {
+        di = a.tile_doc[t];   // then the last document starting at or before g0
+        while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= g0)
+            di++;
+        dlim = a.doc_ends[di];
+        dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
+    }
+    const uint32_t di0 = di, dlim0 = dlim, dnext0 = dnext;
+    auto rem_at = [&](uint32_t gp) -> uint32_t {   // 0: no match may start at gp
+        while (gp >= dnext) {
+            di++;
             dlim = a.doc_ends[di];
             dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
         }
-        const uint32_t di0 = di, dlim0 = dlim, dnext0 = dnext;
-        auto rem_at = [&](uint32_t gp) -> uint32_t {   // 0: no match may start at gp
-            while (gp >= dnext) {
-                di++;
-                dlim = a.doc_ends[di];
-                dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
-            }
-            const uint32_t rm = gp < dlim ? dlim - gp : 0u;
-            return rm < 64u ? rm : 64u;
-        };
+        const uint32_t rm = gp < dlim ? dlim - gp : 0u;
+        return rm < 64u ? rm : 64u;
+    };
 
-        // 1. filter
-        uint64_t cand = 0;
-        {
-            uint32_t W[17];
-            const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
+    // 1. filter
+    uint64_t cand = 0;
+    if (lim) {
+        uint32_t W[20];
+        const uint4 *vsrc = (const uint4 *)(g_text + lo);
 #pragma unroll
-            for (int j = 0; j < 17; j++)
-                W[j] = wsrc[j];
+        for (int j = 0; j < 5; j++) {
+            const uint4 v = vsrc[j];
+            W[4 * j] = v.x;
+            W[4 * j + 1] = v.y;
+            W[4 * j + 2] = v.z;
+            W[4 * j + 3] = v.w;
+        }
 #pragma unroll
-            for (int p = 0; p < 64; p++) {
-                const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3) : W[p >> 2];
-                uint32_t hit = 0;
-                if (cm & 0x02u) hit |= filter_bit(f1, w & 0xFFu);
-                if (cm & 0x04u) hit |= filter_bit(f2, w & 0xFFFFu);
-                if (cm & 0x08u) hit |= filter_bit(f3, dict_fhash3(w, 16));
-                if (cm & 0x30u) hit |= filter_bit(f4, dict_fhash4(w, fb4));
-                cand |= (uint64_t)hit << p;
-            }
-            if (lim < 64)
-                cand &= lim ? ((1ull << lim) - 1ull) : 0ull;
+        for (int p = 0; p < 64; p++) {
+            const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3) : W[p >> 2];
+            uint32_t hit = 0;
+            if (cm & 0x02u) hit |= filter_bit(f1, w & 0xFFu);
+            if (cm & 0x04u) hit |= filter_bit(f2, w & 0xFFFFu);
+            if (cm & 0x08u) hit |= filter_bit(f3, dict_fh3(w) >> 16);
+            if (cm & 0x30u) hit |= filter_bit(f4, dict_fh4(w) >> sh4);
+            cand |= (uint64_t)hit << p;
         }
-        // positions outside documents (batches) are no candidates; remaining bytes per candidate
-        uint32_t nc = (uint32_t)__popcll(cand);
-        uint32_t total_c;
-        const uint32_t cex = block_excl_sum<THREADS>(nc, s_w, total_c);
-        const bool listed = total_c <= (uint32_t)MANY_LCAP;
-        if (listed) {
-            uint32_t at = cex;
-            uint64_t m = cand;
-            while (m) {
-                const uint32_t p = (uint32_t)__builtin_ctzll(m);
-                m &= m - 1;
-                s_cand[at] = (uint16_t)(lo + p);
-                s_rem[at] = (uint8_t)rem_at(g0 + p);
-                at++;
-            }
+        if (lim < 64)
+            cand &= (1ull << lim) - 1ull;
+    }
+    // 2. candidate list (position, remaining bytes)
+    uint32_t total_c;
+    const uint32_t cex = block_excl_sum<THREADS>((uint32_t)__popcll(cand), s_w, total_c);
+    const bool listed = total_c <= (uint32_t)MANY_LCAP;
+    if (listed) {
+        uint32_t at = cex;
+        uint64_t m = cand;
+        while (m) {
+            const uint32_t p = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            s_cand[at] = (uint16_t)(lo + p);
+            s_rem[at] = (uint8_t)rem_at(g0 + p);
+            at++;
         }
-        __syncthreads();
-
-        // 2-3. count
-        uint32_t own = 0;   // per-lane count (unlisted tiles)
-        if (listed) {
-            for (uint32_t i = tid; i < total_c; i += THREADS) {
-                const uint32_t lp = s_cand[i], rm = s_rem[i];
-                s_cnt[i] = rm ? probe_position(d, s_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0)
-                              : 0u;
-            }
-        } else {
-            di = di0;
-            dlim = dlim0;
-            dnext = dnext0;
-            uint64_t m = cand;
-            while (m) {
-                const uint32_t p = (uint32_t)__builtin_ctzll(m);
-                m &= m - 1;
-                const uint32_t rm = rem_at(g0 + p);
-                if (rm)
-                    own += probe_position(d, s_text, lo + p, (uint64_t)g0 + p, rm, false, nullptr, nullptr, 0, 0);
-            }
+    }
+    __syncthreads();
+    // 3. counts
+    uint32_t own = 0;
+    if (listed) {
+        for (uint32_t i = tid; i < total_c; i += THREADS) {
+            const uint32_t lp = s_cand[i], rm = s_rem[i];
+            s_cnt[i] = rm ? probe_position(d, g_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0) : 0u;
         }
-        __syncthreads();
-        // 4. place: per-candidate exclusive offsets (listed) or per-lane ones
-        uint32_t total, lane_ex = 0;
-        if (listed) {
-            const uint32_t K = (total_c + THREADS - 1) / THREADS;
-            const uint32_t i0 = min(total_c, tid * K), i1 = min(total_c, i0 + K);
-            uint32_t loc = 0;
-            for (uint32_t i = i0; i < i1; i++)
-                loc += s_cnt[i];
-            uint32_t run = block_excl_sum<THREADS>(loc, s_w, total);
+    } else {
+        di = di0;
+        dlim = dlim0;
+        dnext = dnext0;
+        uint64_t m = cand;
+        while (m) {
+            const uint32_t p = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t rm = rem_at(g0 + p);
+            if (rm)
+                own += probe_position(d, g_text, lo + p, (uint64_t)g0 + p, rm, false, nullptr, nullptr, 0, 0);
+        }
+    }
+    __syncthreads();
+    uint32_t total;
+    if (listed) {
+        // per-candidate exclusive offsets: contiguous runs of the list per thread
+        const uint32_t K = (total_c + THREADS - 1) / THREADS;
+        const uint32_t i0 = min(total_c, tid * K), i1 = min(total_c, i0 + K);
+        uint32_t loc = 0;
+        for (uint32_t i = i0; i < i1; i++)
+            loc += s_cnt[i];
+        uint32_t run = block_excl_sum<THREADS>(loc, s_w, total);
+        if (WRITE) {
             for (uint32_t i = i0; i < i1; i++) {
                 const uint32_t c = s_cnt[i];
                 s_cnt[i] = run;
                 run += c;
             }
-        } else {
-            lane_ex = block_excl_sum<THREADS>(own, s_w, total);
-        }
-        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix, a.lb_err);
-        if (tid == 0 && t == a.num_tiles - 1)
-            a.meta[0] = prefix + total;
-        if (a.out_off || a.out_id) {
-            if (listed) {
-                for (uint32_t i = tid; i < total_c; i += THREADS) {
-                    const uint32_t lp = s_cand[i], rm = s_rem[i];
-                    const uint64_t r = prefix + s_cnt[i];
-                    if (rm && r < a.cap)
-                        probe_position(d, s_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r, a.cap);
-                }
-            } else {
-                di = di0;
-                dlim = dlim0;
-                dnext = dnext0;
-                uint64_t r = prefix + lane_ex;
-                uint64_t m = cand;
-                while (m && r < a.cap) {
-                    const uint32_t p = (uint32_t)__builtin_ctzll(m);
-                    m &= m - 1;
-                    const uint32_t rm = rem_at(g0 + p);
-                    if (rm)
-                        r += probe_position(d, s_text, lo + p, (uint64_t)g0 + p, rm, true, a.out_off, a.out_id, r,
-                                            a.cap);
-                }
+            __syncthreads();
+            for (uint32_t i = tid; i < total_c; i += THREADS) {
+                const uint32_t lp = s_cand[i], rm = s_rem[i];
+                const uint64_t r = out0 + s_cnt[i];
+                if (rm && r < a.cap)
+                    probe_position(d, g_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r, a.cap);
             }
         }
-        __syncthreads();   // LDS is reused by the next tile
+    } else {
+        const uint32_t lane_ex = block_excl_sum<THREADS>(own, s_w, total);
+        if (WRITE) {
+            di = di0;
+            dlim = dlim0;
+            dnext = dnext0;
+            uint64_t r = out0 + lane_ex;
+            uint64_t m = cand;
+            while (m && r < a.cap) {
+                const uint32_t p = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t rm = rem_at(g0 + p);
+                if (rm)
+                    r += probe_position(d, g_text, lo + p, (uint64_t)g0 + p, rm, true, a.out_off, a.out_id, r, a.cap);
+            }
+        }
     }
+    __syncthreads();   // the list is reused by the next tile
+    return total;
+}
+
+template <int THREADS, bool WRITE>
+__global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
+{
+    extern __shared__ __align__(16) uint32_t s_filter[];
+    __shared__ uint16_t s_cand[MANY_LCAP];
+    __shared__ uint8_t s_rem[MANY_LCAP];
+    __shared__ uint32_t s_cnt[MANY_LCAP];
+    __shared__ uint32_t s_w[THREADS / 64];
+    for (uint32_t i = threadIdx.x; i < a.d.fwords; i += THREADS)
+        s_filter[i] = a.d.filter[i];
+    __syncthreads();
+    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
+        const uint32_t total = many_tile<THREADS, WRITE>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w,
+                                                         WRITE ? a.tile_off[t] : 0ull);
+        if (!WRITE && threadIdx.x == 0) {
+            a.tile_count[t] = total;
+            if (total)
+                atomicAdd((unsigned long long *)a.meta, (unsigned long long)total);
+        }
+    }
+}
+
+// Exclusive scan of tile_count into tile_off (one workgroup, chunks through LDS).
+template <int ST>
+__global__ __launch_bounds__(ST) void many_scan_kernel(ManyArgs a)
+{
+    constexpr int K = 8;
+    constexpr int CH = ST * K;
+    constexpr int NWS = ST / 64;
+    __shared__ uint32_t s_v[CH];
+    __shared__ uint64_t s_wt[NWS];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t T = a.num_tiles;
+    uint64_t carry = 0;
+    for (uint32_t c0 = 0; c0 < T; c0 += CH) {
+        const uint32_t n = (T - c0) < (uint32_t)CH ? T - c0 : (uint32_t)CH;
+        for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
+            s_v[i] = i < n ? a.tile_count[c0 + i] : 0u;
+        __syncthreads();
+        uint32_t loc[K];
+        uint64_t m = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            loc[k] = s_v[tid * K + k];
+            m += loc[k];
+        }
+        uint64_t inc = m;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint64_t y = __shfl_up(inc, o, 64);
+            if (lane >= o)
+                inc += y;
+        }
+        if (lane == 63)
+            s_wt[w] = inc;
+        __syncthreads();
+        uint64_t run = carry + inc - m, tot = 0;
+        for (int k = 0; k < NWS; k++) {
+            if (k < w)
+                run += s_wt[k];
+            tot += s_wt[k];
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t i = tid * K + k;
+            if (i < n)
+                a.tile_off[c0 + i] = run;
+            run += loc[k];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0)
+        a.meta[0] = carry;
 }
 
 size_t ol_many_lds_bytes(uint32_t fwords)
 {
-    return (size_t)SEARCH_TILE + MANY_HALO + (size_t)fwords * 4;
+    return (size_t)fwords * 4;
 }
 
-hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
+template <bool WRITE>
+static hipError_t launch_many_tiles(const ManyArgs &a, hipStream_t s)
 {
     const size_t lds = ol_many_lds_bytes(a.d.fwords);
-    const void *f = (const void *)many_kernel<SEARCH_THREADS>;
+    const void *f = (const void *)many_tiles_kernel<SEARCH_THREADS, WRITE>;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     uint32_t cap = max_grid(f, SEARCH_THREADS, lds);
     if (!cap)
         return hipErrorInvalidValue;
-    uint32_t grid = a.num_tiles < cap ? a.num_tiles : cap;
-    hipLaunchKernelGGL((many_kernel<SEARCH_THREADS>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
+    const uint32_t grid = a.num_tiles < 4 * cap ? a.num_tiles : 4 * cap;
+    hipLaunchKernelGGL((many_tiles_kernel<SEARCH_THREADS, WRITE>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
     return hipGetLastError();
+}
+
+// Counts (total in meta[0]); with outputs also the scan and the writes.
+hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
+{
+    if (a.num_tiles == 0)
+        return hipSuccess;
+    hipError_t e = launch_many_tiles<false>(a, s);
+    if (e != hipSuccess || (!a.out_off && !a.out_id))
+        return e;
+    hipLaunchKernelGGL((many_scan_kernel<1024>), dim3(1), dim3(1024), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    return launch_many_tiles<true>(a, s);
 }
