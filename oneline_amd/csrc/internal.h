@@ -50,6 +50,51 @@ __device__ __forceinline__ void st_relaxed_u64(uint64_t *p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---- wave-wide scans on DPP lane moves (no LDS round trip) -----------------
+// DPP controls of the GFX9 encoding: row_shr:n = 0x110 + n (within rows of 16
+// lanes), row_bcast:15 = 0x142 (lane 15 of each row to the next row, written
+// in the rows of row_mask), row_bcast:31 = 0x143 (lane 31 to rows 2 and 3),
+// wave_shr:1 = 0x138.  A lane whose source is outside its row, or whose row is
+// masked off, gets `old`.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Inclusive prefix sum over the 64 lanes (Hillis-Steele in rows, then the row
+// totals carried with two broadcasts).
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t x)
+{
+    x += dpp_move<0x111>(0u, x);
+    x += dpp_move<0x112>(0u, x);
+    x += dpp_move<0x114>(0u, x);
+    x += dpp_move<0x118>(0u, x);
+    x += dpp_move<0x142, 0xA>(0u, x);
+    x += dpp_move<0x143, 0xC>(0u, x);
+    return x;
+}
+
+// Inclusive prefix maximum over the 64 lanes (signed; -1 is the identity for
+// values >= -1).
+__device__ __forceinline__ int32_t wave_incl_max(int32_t x)
+{
+    auto mx = [](int32_t a, uint32_t b) { return a > (int32_t)b ? a : (int32_t)b; };
+    x = mx(x, dpp_move<0x111>(~0u, (uint32_t)x));
+    x = mx(x, dpp_move<0x112>(~0u, (uint32_t)x));
+    x = mx(x, dpp_move<0x114>(~0u, (uint32_t)x));
+    x = mx(x, dpp_move<0x118>(~0u, (uint32_t)x));
+    x = mx(x, dpp_move<0x142, 0xA>(~0u, (uint32_t)x));
+    x = mx(x, dpp_move<0x143, 0xC>(~0u, (uint32_t)x));
+    return x;
+}
+
+// The value of the next lower lane (lane 0 gets `first`).
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t x, uint32_t first)
+{
+    return dpp_move<0x138>(first, x);
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 {
 #pragma unroll

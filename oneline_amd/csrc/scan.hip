@@ -296,15 +296,36 @@ __device__ __forceinline__ bool decode_head(const uint8_t *s_text, uint32_t rs, 
     return true;
 }
 
-// Class and tree key of the line text[s ..= e] (e its terminator) staged in
-// LDS at byte rs, for the common shape: a prefix of "| " pairs and then at
-// least three content bytes before any '\r'/'\n', or an empty line ("\n",
-// "\r\n", possibly after a "| " prefix).  Only these need no trailing trim:
-// classify (crates/oneline-simd/src/line.rs:59-81) reads the first three
-// content bytes, and when none of them is '\r'/'\n' the content reaches past
-// them.  Returns false for every other line (the general decoder handles it).
-__device__ __forceinline__ bool decode_common(const uint8_t *s_text, uint32_t rs, uint32_t s, uint32_t e,
-                                              uint32_t &rc)
+// classify_line (crates/oneline-simd/src/line.rs:59-81) on the raw bytes at the
+// content start, x = c0 | c1 << 8 | c2 << 16, as selects (no branches).  The
+// bytes may run into the line's trailing '\r'/'\n' (classify_line would see
+// "no byte" there): neither is '/', '.', ' ' or '\t', so the class is the
+// same.  c1 is looked at only after c0 in {'/', '-'} and c2 only after
+// "-.", so neither lies past the line's terminator when it matters.
+__device__ __forceinline__ uint32_t classify_bytes(uint32_t x)
+{
+    const uint32_t c0 = x & 0xFFu, c01 = x & 0xFFFFu, c012 = x & 0xFFFFFFu;
+    uint32_t k = OLK_TEXT;
+    k = (c0 == '=') ? (uint32_t)OLK_HEADING : k;
+    k = (c0 == ':') ? (uint32_t)OLK_CONNECT : k;
+    k = (c0 == '@') ? (uint32_t)OLK_ELEMENT : k;
+    k = (c01 == 0x2F2Fu) ? (uint32_t)OLK_COMMENT : k;                          // "//"
+    k = ((c01 == 0x202Du) | (c01 == 0x092Du)) ? (uint32_t)OLK_LIST : k;         // "- ", "-\t"
+    k = ((c012 == 0x202E2Du) | (c012 == 0x092E2Du)) ? (uint32_t)OLK_ORDERED : k; // "-. ", "-.\t"
+    return k;
+}
+
+constexpr uint32_t RC_SLOW = 0x10000u;   // decode_fast: not a class | key << 8
+
+// Class | key << 8 of the line staged in LDS at byte rs (its terminator in
+// LDS), for the common shape: a prefix of "| " pairs (at most 19 of them), then
+// content whose first byte is not '|', ' ' or '\t' -- or no content: the line
+// is "\n" or "\r\n" after the prefix.  Those lines need no trailing trim:
+// the first content byte is '\n' exactly when the content is empty, and
+// classify_bytes reads the raw bytes.  Returns RC_SLOW for every other line
+// (the general decoder handles it).  Straight-line code: the prefix words are
+// compared all at once, the content bytes come from one more read.
+__device__ __forceinline__ uint32_t decode_fast(const uint8_t *s_text, uint32_t rs)
 {
     uint32_t A[10];
     load_words<10>(s_text, rs, A);
@@ -313,23 +334,16 @@ __device__ __forceinline__ bool decode_common(const uint8_t *s_text, uint32_t rs
     for (int j = 0; j < 10; j++)
         eq |= (uint32_t)(A[j] == 0x207C207Cu) << j;   // "| | "
     const uint32_t k = (uint32_t)__builtin_ctz(~eq);   // whole "| | " words (a '\n' ends the run)
-    if (k >= 10)
-        return false;
     uint32_t T[2];
-    load_words<2>(s_text, rs + 4 * k, T);
+    load_words<2>(s_text, rs + 4 * (k < 10 ? k : 9u), T);
     const bool half = (T[0] & 0xFFFFu) == 0x207Cu;     // one more "| "
     const uint32_t x = half ? __builtin_amdgcn_alignbyte(T[1], T[0], 2) : T[0];
-    const uint32_t cs = s + 4 * k + (half ? 2u : 0u);
-    const uint32_t x0 = x & 0xFFu, x1 = (x >> 8) & 0xFFu, x2 = (x >> 16) & 0xFFu;
-    if ((x0 == '\n' && cs == e) || (x0 == '\r' && x1 == '\n' && cs + 1 == e)) {
-        rc = OLK_BLANK | (uint32_t)KEY_BLANK << 8;
-        return true;
-    }
-    if (cs + 2 > e || x0 == '|' || x0 == ' ' || x0 == '\t' || x0 == '\r' || x0 == '\n' || x1 == '\r' ||
-        x1 == '\n' || x2 == '\r' || x2 == '\n')
-        return false;
-    rc = classify(x0, x1, x2) | (2 * k + (half ? 1u : 0u)) << 8;   // depth <= 19: the key is the depth
-    return true;
+    const uint32_t x0 = x & 0xFFu, x01 = x & 0xFFFFu;
+    const bool slow = (k >= 10) | (x0 == '|') | (x0 == ' ') | (x0 == '\t') | (x01 == 0x0D0Du);
+    const bool blank = (x0 == '\n') | (x01 == 0x0A0Du);
+    const uint32_t rc = blank ? (OLK_BLANK | (uint32_t)KEY_BLANK << 8)
+                              : (classify_bytes(x) | (2 * k + (half ? 1u : 0u)) << 8);   // depth <= 19: the key is the depth
+    return slow ? RC_SLOW : rc;
 }
 
 // One tile = THREADS lanes x 64*Q contiguous bytes.  The tile arrives in
@@ -513,20 +527,9 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             if (mk[q])
                 il = (int32_t)(tid * LB + 64 * q + 63 - __clzll(mk[q]));
         }
-        uint32_t ic = cnt;
-        int32_t ilm = il;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t cc = __shfl_up(ic, o, 64);
-            const int32_t l = __shfl_up(ilm, o, 64);
-            if (lane >= o) {
-                ic += cc;
-                ilm = ilm > l ? ilm : l;
-            }
-        }
-        int32_t prev_l = __shfl_up(ilm, 1, 64);
-        if (lane == 0)
-            prev_l = -1;
+        const uint32_t ic = wave_incl_add(cnt);
+        const int32_t ilm = wave_incl_max(il);
+        const int32_t prev_l = (int32_t)wave_shr1((uint32_t)ilm, ~0u);
         if (lane == 63) {
             s_wcnt[wid] = ic;
             s_wlast[wid] = ilm;
@@ -651,9 +654,13 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
                 const uint32_t e = base + s_le[b][i];
                 const uint32_t s = i ? base + s_le[b][i - 1] + 1u : first_w1;
                 uint32_t rc = FIX;
-                if (s >= lds_lo && !decode_common(s_text, s + (uint32_t)HALO - base, s, e, rc)) {
-                    const ol_node_dev nd = decode(s, e);
-                    rc = nd.kind | (uint32_t)key_of(nd) << 8;
+                if (s >= lds_lo) {
+                    // (the text's last byte ends a line without being a '\n')
+                    rc = (e == last) ? RC_SLOW : decode_fast(s_text, s + (uint32_t)HALO - base);
+                    if (rc == RC_SLOW) {
+                        const ol_node_dev nd = decode(s, e);
+                        rc = nd.kind | (uint32_t)key_of(nd) << 8;
+                    }
                 }
                 s_rc[b][i] = (uint16_t)rc;
             }
