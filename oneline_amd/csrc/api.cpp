@@ -156,15 +156,16 @@ struct Prof {
     }
 };
 
-hipError_t ensure_scratch(ol_ctx *c, size_t tiles)
+// Scratch: meta and counters, then `words` u64 status words per tile.
+hipError_t ensure_scratch(ol_ctx *c, size_t tiles, size_t words = LB_STRIDE)
 {
-    return c->scratch.ensure((SCR_STATUS + tiles * LB_STRIDE + 64) * 8);
+    return c->scratch.ensure((SCR_STATUS + tiles * words + 64) * 8);
 }
 
-hipError_t clear_scratch(ol_ctx *c, size_t tiles)
+hipError_t clear_scratch(ol_ctx *c, size_t tiles, size_t words = LB_STRIDE)
 {
     Prof p(c, "clear");
-    return ol_launch_clear_u64((uint64_t *)c->scratch.p, SCR_STATUS + tiles * LB_STRIDE, c->stream);
+    return ol_launch_clear_u64((uint64_t *)c->scratch.p, SCR_STATUS + tiles * words, c->stream);
 }
 
 // Waits for the stream and reads meta[0]; fails if a look-back wait gave up.
@@ -175,7 +176,8 @@ int read_meta(ol_ctx *c, uint64_t *out)
     HIPCHK(hipStreamSynchronize(c->stream));
     const uint32_t *ctr = (const uint32_t *)(head + SCR_CTR);
     if (ctr[4])
-        return fail(OL_ERR_DEVICE, "a look-back wait gave up: not every workgroup of a persistent launch was resident");
+        return fail(OL_ERR_DEVICE, ctr[4] & 2u ? "the scan's prefix pass gave up waiting for a tile count"
+                                               : "a look-back wait gave up: not every workgroup of a persistent launch was resident");
     *out = head[SCR_META];
     return OL_OK;
 }
@@ -210,8 +212,11 @@ int alloc_doc_lines(ol_doc *d, uint64_t cap)
 int launch_scan_args(ol_ctx *c, ScanArgs &a, bool cooperative = true)
 {
     uint32_t tiles = (uint32_t)ceil_div(a.n, (uint64_t)SCAN_THREADS * c->scan_bpt);
-    HIPCHK(ensure_scratch(c, tiles));
-    HIPCHK(clear_scratch(c, tiles));
+    // tiles from a ticket unless the cooperative launch guarantees residency
+    const bool coop = cooperative && !c->scan_ticket;
+    const size_t words = coop ? SCAN_STATUS_WORDS : LB_STRIDE;
+    HIPCHK(ensure_scratch(c, tiles, words));
+    HIPCHK(clear_scratch(c, tiles, words));
     a.num_tiles = tiles;
     a.status = c->status();
     a.meta = c->meta();
@@ -226,13 +231,15 @@ int launch_scan_args(ol_ctx *c, ScanArgs &a, bool cooperative = true)
         c->scan_max_grid[vi] = ol_scan_max_grid(c->scan_bpt);
     if (!c->scan_max_grid[vi])
         return fail(OL_ERR_DEVICE, "occupancy query for the scan kernel failed");
+    // cooperative: workgroups taking tiles plus the one that turns tile counts
+    // into prefixes; ticket order: workgroups taking tiles
     uint32_t grid = c->scan_max_grid[vi];
     if (c->scan_grid && c->scan_grid < grid)
         grid = c->scan_grid;
-    if (grid > tiles)
-        grid = tiles;
-    // tiles from a ticket unless the cooperative launch guarantees residency
-    const bool coop = cooperative && !c->scan_ticket;
+    if (grid > tiles + (coop ? 1 : 0))
+        grid = tiles + (coop ? 1 : 0);
+    if (coop && grid < 2)
+        return fail(OL_ERR_DEVICE, "the scan needs two resident workgroups");
     a.tile_ctr = coop ? nullptr : c->ctr() + 5;
     {
         Prof p(c, "scan_classify");

@@ -50,6 +50,20 @@ __device__ __forceinline__ void st_relaxed_u64(uint64_t *p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Read-modify-write forms of the same: performed where agent-scope atomics are
+// resolved for the whole device, so a poll sees another agent's update even
+// when its own cache holds the line (a plain relaxed load may be served from
+// the local L2 of this XCD, which does not see other XCDs' stores until the
+// line leaves it).  Used for every word one agent waits on another to write.
+__device__ __forceinline__ uint64_t ld_coherent_u64(uint64_t *p)
+{
+    return __hip_atomic_fetch_or(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent_u64(uint64_t *p, uint64_t v)
+{
+    (void)__hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---- wave-wide scans on DPP lane moves (no LDS round trip) -----------------
 // DPP controls of the GFX9 encoding: row_shr:n = 0x110 + n (within rows of 16
 // lanes), row_bcast:15 = 0x142 (lane 15 of each row to the next row, written
@@ -151,7 +165,8 @@ __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_
                 top -= 64 * k + fn;
                 uint32_t spins = 0;
                 if (lane == 0) {
-                    while ((ld_relaxed_u64(status + top * LB_STRIDE) >> LB_SHIFT) == 0 && ++spins < LB_SPIN_LIMIT)
+                    while ((ld_coherent_u64(const_cast<uint64_t *>(status) + top * LB_STRIDE) >> LB_SHIFT) == 0 &&
+                           ++spins < LB_SPIN_LIMIT)
                         __builtin_amdgcn_s_sleep(8);
                     if (spins >= LB_SPIN_LIMIT)
                         atomicOr(err, 1u);
@@ -326,8 +341,13 @@ constexpr uint32_t SCAN_THREADS = 256;
 constexpr uint32_t SCAN_BPT = 128;                          // default bytes per lane (32 KiB tiles)
 constexpr uint32_t SCAN_TILE_MAX = SCAN_THREADS * 256;
 constexpr uint32_t SCAN_HALO = 256;
-#ifndef SCAN_LB_WINDOWS
-#define SCAN_LB_WINDOWS 1                                  // look-back windows of 64 tiles read per step
+// Scan status per tile: its line count (written by the tile's workgroup, read
+// by the prefix scanner) and its inclusive prefix (written by the scanner, read
+// by the tile's workgroup), all accessed with the read-modify-write atomics.
+constexpr uint32_t SCAN_STATUS_WORDS = 2;                  // u64 words per tile
+constexpr uint32_t SCAN_PREFIX_WORD = 1;                   // offset of the prefix word
+#ifndef SCAN_PREFIX_WINDOWS
+#define SCAN_PREFIX_WINDOWS 8                              // 64-tile windows the prefix scanner reads per poll
 #endif
 constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;

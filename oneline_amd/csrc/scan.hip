@@ -9,8 +9,10 @@
 //      with coalesced 16-byte loads;
 //   2. each lane turns its 64 contiguous bytes into a 64-bit terminator mask
 //      (SWAR '\n' test), counts it, and the workgroup scans the counts;
-//   3. wave 0 publishes the tile's line count and resolves its prefix by a
-//      decoupled look-back over the per-tile status words;
+//   3. wave 0 publishes the tile's line count in the tile's status word; one
+//      workgroup of the grid does no tiles but turns the counts, in tile
+//      order, into inclusive prefixes in the same words (scan_prefixes), and
+//      a tile's wave 0 reads its prefix back from there;
 //   4. each lane classifies the lines that END in its bytes (line = previous
 //      terminator + 1 .. terminator) from the LDS copy and writes the line
 //      offset, the kind byte, the tree key byte and optionally the 16-byte
@@ -346,6 +348,54 @@ __device__ __forceinline__ uint32_t decode_fast(const uint8_t *s_text, uint32_t 
     return slow ? RC_SLOW : rc;
 }
 
+// The prefix scanner (one wave of the one workgroup that takes no tiles): the
+// count word of tile t holds its line count (flag AGG) once the tile is
+// counted; the scanner reads the count words in tile order, NWS windows of 64
+// per poll, and writes every complete window's inclusive prefixes (flag INCL)
+// to the tiles' prefix words, carrying the running total from window to
+// window.  A window with a tile not counted yet is polled again.  Tiles wait
+// only for their own prefix word, never for a chain of their neighbours'
+// look-backs.
+template <int NWS>
+__device__ __forceinline__ void scan_prefixes(uint64_t *status, uint32_t T, uint32_t *err)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t carry = 0;
+    uint32_t t0 = 0, spins = 0;
+    while (t0 < T) {
+        uint64_t st[NWS];
+#pragma unroll
+        for (int k = 0; k < NWS; k++) {
+            const uint32_t idx = t0 + 64 * k + lane;
+            st[k] = idx < T ? ld_coherent_u64(status + (size_t)idx * SCAN_STATUS_WORDS) : (LB_FLAG_AGG << LB_SHIFT);
+        }
+        uint32_t done = 0;
+#pragma unroll
+        for (int k = 0; k < NWS; k++) {
+            if (done != (uint32_t)k || __ballot((st[k] >> LB_SHIFT) == 0))
+                break;
+            const uint32_t inc = wave_incl_add((uint32_t)(st[k] & LB_MASK));   // a tile has < 2^17 lines
+            const uint32_t idx = t0 + 64 * k + lane;
+            if (idx < T)
+                st_coherent_u64(status + (size_t)idx * SCAN_STATUS_WORDS + SCAN_PREFIX_WORD,
+                                (LB_FLAG_INCL << LB_SHIFT) | (carry + inc));
+            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+            done++;
+        }
+        t0 += 64 * done;
+        if (done == 0) {
+            if (++spins >= LB_SPIN_LIMIT) {
+                if (lane == 0)
+                    atomicOr(err, 2u);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        } else {
+            spins = 0;
+        }
+    }
+}
+
 // One tile = THREADS lanes x 64*Q contiguous bytes.  The tile arrives in
 // registers (coalesced 16-byte vectors), is staged whole in LDS behind a
 // 256-byte halo, and the registers are then refilled with the workgroup's next
@@ -386,7 +436,7 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     __shared__ uint32_t s_dj[2][2];                      // their index range [j0, j1)
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const uint32_t G = gridDim.x;
+    const uint32_t G = gridDim.x - 1;   // workgroups that take tiles (static order)
     const uint32_t T = a.num_tiles;
     const uint32_t last = a.n - 1;   // always a terminator, '\n' or not
     constexpr uint16_t FIX = 0xFFFF;  // never a class | key << 8 (classes are < 8)
@@ -403,15 +453,39 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         if (tt > 0 && tid < HALO / 16)
             hv = ((const uint4 *)(a.text + b - HALO))[tid];
     };
-    // wave 0: the tile's exclusive prefix (look-back), then its inclusive status
-    auto resolve = [&](uint32_t tt, uint32_t tot) -> uint64_t {
-        uint64_t prefix = 0;
-        if (tt > 0) {
-            prefix = lookback_sum<SCAN_LB_WINDOWS>(a.status, tt, a.lb_err);
-            if (lane == 0)
-                st_relaxed_u64(a.status + (size_t)tt * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + tot));
+    // wave 0: the tile's exclusive prefix.  Static order: lane 0 waits for the
+    // scanner's inclusive prefix of the tile (bounded wait: see LB_SPIN_LIMIT).
+    // Ticket order: a decoupled look-back over the tiles' status words, and
+    // the tile's own inclusive status after it.
+    const bool ticket = a.tile_ctr != nullptr;
+    // static order: the prefix once the scanner has written the tile's prefix
+    // word (`w`: a first read of it, 0 if none)
+    auto finish_prefix = [&](uint32_t tt, uint32_t tot, uint64_t w) -> uint64_t {
+        if (lane == 0) {
+            uint32_t spins = 0;
+            while ((w >> LB_SHIFT) != LB_FLAG_INCL && spins < LB_SPIN_LIMIT) {
+                if (spins++)
+                    __builtin_amdgcn_s_sleep(8);
+                w = ld_coherent_u64(a.status + (size_t)tt * SCAN_STATUS_WORDS + SCAN_PREFIX_WORD);
+            }
+            if ((w >> LB_SHIFT) != LB_FLAG_INCL) {
+                atomicOr(a.lb_err, 1u);
+                w = (LB_FLAG_INCL << LB_SHIFT) | tot;
+            }
         }
-        return prefix;
+        return (w & LB_MASK) - tot;
+    };
+    auto resolve = [&](uint32_t tt, uint32_t tot) -> uint64_t {
+        if (ticket) {
+            uint64_t prefix = 0;
+            if (tt > 0) {
+                prefix = lookback_sum<1>(a.status, tt, a.lb_err);
+                if (lane == 0)
+                    st_relaxed_u64(a.status + (size_t)tt * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + tot));
+            }
+            return prefix;
+        }
+        return finish_prefix(tt, tot, 0);
     };
     auto add_fix = [&](uint64_t r, uint32_t s, uint32_t e) {
         const uint32_t slot = atomicAdd(a.fix_count, 1u);
@@ -451,22 +525,41 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         if (s == ds)
             a.doc_line_start[d] = (uint32_t)r;
     };
-    // writes a staged tile: lines i < tot of buffer pb, first line at pfirst
+    // writes a staged tile: lines i < tot of buffer pb, first line at pfirst.
+    // Line indices fit 32 bits (a text has fewer than 2^32 lines).
+    uint32_t *const offsets = a.offsets;
+    uint8_t *const classes = a.classes;
+    uint8_t *const keys = a.keys;
     auto write_staged = [&](int pb, uint32_t pbase, uint32_t pfirst, uint32_t tot, uint64_t running, bool is_last) {
         if (tid == 0 && is_last)
             a.meta[0] = running + tot;
-        for (uint32_t i = tid; i < tot; i += THREADS) {
-            const uint64_t r = running + i;
-            if (r >= a.cap)
-                break;
+        const uint32_t nw = (running + tot <= a.cap) ? tot : (running < a.cap ? (uint32_t)(a.cap - running) : 0u);
+        const uint32_t r0 = (uint32_t)running;
+        if (a.ndocs == 0) {
+            for (uint32_t i = tid; i < nw; i += THREADS) {
+                const uint32_t r = r0 + i;
+                const uint32_t ls = i ? pbase + s_le[pb][i - 1] + 1u : pfirst;
+                const uint32_t rc = s_rc[pb][i];
+                offsets[r] = ls;
+                if (rc != FIX) {
+                    classes[r] = (uint8_t)(rc & 0xFFu);
+                    keys[r] = (uint8_t)(rc >> 8);
+                } else {
+                    add_fix(r, ls, pbase + s_le[pb][i]);
+                }
+            }
+            return;
+        }
+        for (uint32_t i = tid; i < nw; i += THREADS) {
+            const uint32_t r = r0 + i;
             const uint32_t ls = i ? pbase + s_le[pb][i - 1] + 1u : pfirst;
             put_offset(pb, r, ls);
             const uint32_t rc = s_rc[pb][i];
             if (rc == FIX) {
                 add_fix(r, ls, pbase + s_le[pb][i]);
             } else {
-                a.classes[r] = (uint8_t)(rc & 0xFFu);
-                a.keys[r] = (uint8_t)(rc >> 8);
+                classes[r] = (uint8_t)(rc & 0xFFu);
+                keys[r] = (uint8_t)(rc >> 8);
             }
         }
     };
@@ -475,13 +568,13 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     bool pending = false;
     uint32_t p_t = 0, p_total = 0;
 
-    // Tile order: static (t = blockIdx + k * grid: needs every workgroup
-    // resident, the cooperative launch), or claimed from a ticket (tile_ctr:
-    // a tile's look-back then only waits on tiles held by running workgroups,
-    // whatever shares the device).  A workgroup knows its next tile one
-    // iteration ahead, for the prefetch.
+    // Tile order: static (t = blockIdx + k * workers: needs every workgroup
+    // resident, the cooperative launch; the last workgroup is the prefix
+    // scanner), or claimed from a ticket (tile_ctr: a tile's look-back then
+    // only waits on tiles held by running workgroups, whatever shares the
+    // device; every workgroup takes tiles).  A workgroup knows its next tile
+    // one iteration ahead, for the prefetch.
     __shared__ uint32_t s_claim[2];
-    const bool ticket = a.tile_ctr != nullptr;
     uint32_t t, tn;
     if (ticket) {
         if (tid == 0) {
@@ -492,6 +585,11 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         t = s_claim[0];
         tn = s_claim[1];
     } else {
+        if (blockIdx.x == G) {
+            if (wid == 0)
+                scan_prefixes<SCAN_PREFIX_WINDOWS>(a.status, T, a.lb_err);
+            return;
+        }
         t = blockIdx.x;
         tn = t + G;
     }
@@ -553,9 +651,12 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         const bool staged = listed && !a.nodes;   // written one iteration later
 
         // Publish the tile's count at once, then compact the line ends.
-        if (wid == 0 && lane == 0)
-            st_relaxed_u64(a.status + (size_t)t * LB_STRIDE,
-                           ((t == 0 ? LB_FLAG_INCL : LB_FLAG_AGG) << LB_SHIFT) | total);
+        if (wid == 0 && lane == 0) {
+            if (ticket)
+                st_relaxed_u64(a.status + (size_t)t * LB_STRIDE, ((t == 0 ? LB_FLAG_INCL : LB_FLAG_AGG) << LB_SHIFT) | total);
+            else
+                st_coherent_u64(a.status + (size_t)t * SCAN_STATUS_WORDS, (LB_FLAG_AGG << LB_SHIFT) | total);
+        }
         if (listed) {
             uint32_t at = excl_c;
 #pragma unroll
