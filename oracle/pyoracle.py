@@ -1,4 +1,4 @@
-"""ctypes front of the CPU oracle (oracle/oracle.c).
+"""ctypes front of the CPU oracle (oracle/oracle.c, oracle/oracle_par.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg as the checker; the product never imports it.
@@ -46,6 +46,10 @@ def lib():
         o.orc_find_all.restype = S
         o.orc_many_find_all.argtypes = [P, S, P, P, P, S, P, P, S]
         o.orc_many_find_all.restype = S
+        o.orc_parse_par.argtypes = [P, S, P, P, ctypes.c_int]
+        o.orc_parse_par.restype = S
+        o.orc_links_par.argtypes = [P, S, P, P, P, P, ctypes.c_int]
+        o.orc_links_par.restype = ctypes.c_int
         _o = o
     return _o
 
@@ -105,6 +109,37 @@ def links_from(depth: np.ndarray, kind: np.ndarray):
     kind = np.ascontiguousarray(kind, np.uint32)
     lib().orc_links_sequential(_p(depth), _p(kind), n, _p(parent), _p(se))
     lib().orc_links_derived(_p(kind), _p(parent), _p(se), n, _p(fc), _p(ns))
+    return parent, se, fc, ns
+
+
+def line_count(text) -> int:
+    a = _arr(text)
+    return int(lib().orc_line_count(_p(a), a.size))
+
+
+def parse_par(text, threads: int, out=None):
+    """parse() on `threads` threads (the two-pass chunked parse); `out` may
+    hold (offsets, nodes) arrays of line_count(text) entries to fill."""
+    a = _arr(text)
+    if out is None:
+        n = lib().orc_line_count(_p(a), a.size)
+        out = (np.empty(n, np.uint32), np.empty(n, NODE))
+    off, nodes = out
+    got = lib().orc_parse_par(_p(a), a.size, _p(off), _p(nodes), threads)
+    assert got == off.size == nodes.size
+    return off, nodes
+
+
+def links_par(nodes: np.ndarray, threads: int, out=None):
+    """links() on `threads` threads (three-phase nearest-smaller-values);
+    `out` may hold the four u32 arrays to fill."""
+    n = nodes.size
+    nodes = np.ascontiguousarray(nodes)
+    if out is None:
+        out = tuple(np.empty(n, np.uint32) for _ in range(4))
+    parent, se, fc, ns = out
+    if lib().orc_links_par(_p(nodes), n, _p(parent), _p(se), _p(fc), _p(ns), threads):
+        raise MemoryError("orc_links_par")
     return parent, se, fc, ns
 
 

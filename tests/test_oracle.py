@@ -263,3 +263,29 @@ def test_many_each_pattern_equals_single_search():
     oo, ii = orc.many_find_all(text, pats)
     for i, p in enumerate(pats):
         assert oo[ii == i].tolist() == orc.find_all(text, p).tolist()
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_parallel_restatement_equals_sequential(threads, known):
+    """The multi-threaded CPU baseline (chunked parse, three-phase tree) gives
+    the sequential oracle's outputs, on edge texts, generated documents split
+    into several blocks, and random texts over the markup alphabet."""
+    import numpy as np
+    from oneline_amd import datagen
+    cases = [b"", b"x", b"\n", b"| | a", b"a\nb\n", b"| x\n|| y\n||| z\n| w", b"\n\n\n", b"\r\n\r", b"|" * 40 + b"\n"]
+    cases.append(known["tree_doc"].encode() if "tree_doc" in known else b"|=[ A ]\n|| x\n||- y\n|||- z\n|=[ B ]\n|| w\n")
+    for i, size in enumerate((5000, 70000, 3 << 20)):
+        cases.append(datagen.document(size, seed=40 + i, crlf=10, empty=60).tobytes())
+    rng = random.Random(threads)
+    for _ in range(6):
+        cases.append(bytes(rng.choice(b"| \t\n\r-=.") for _ in range(rng.randrange(1, 200000))))
+    # deep and flat runs across block boundaries (blocks are 32768 lines)
+    cases.append(b"".join(b"|" * (i % 40) + b" x\n" for i in range(70000)))
+    cases.append(b"x\n" * 70000 + b"| y\n" * 3)
+    for c in cases:
+        o1, n1 = orc.parse(c)
+        o2, n2 = orc.parse_par(c, threads)
+        assert np.array_equal(o1, o2) and np.array_equal(n1, n2), len(c)
+        if n1.size:
+            for a, b in zip(orc.links(n1), orc.links_par(n1, threads)):
+                assert np.array_equal(a, b), len(c)
