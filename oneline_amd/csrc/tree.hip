@@ -379,7 +379,13 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
 
     const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t above = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
-    uint32_t *const P = a.parent, *const FC = a.first_child, *const NS = a.next_sibling, *const SE = a.subtree_end;
+    // the wave's first output line: outputs are stored at a per-wave base
+    // (uniform) plus a small per-lane offset
+    const uint64_t wg = ts.g0 + wbase;
+    uint32_t *const P = a.parent ? a.parent + wg : nullptr;
+    uint32_t *const FC = a.first_child ? a.first_child + wg : nullptr;
+    uint32_t *const NS = a.next_sibling ? a.next_sibling + wg : nullptr;
+    uint32_t *const SE = a.subtree_end ? a.subtree_end + wg : nullptr;
 
 #pragma unroll
     for (int r = 0; r < TR; r++) {
@@ -399,24 +405,22 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
         const uint32_t nv = s_h[w][r][D - 1], nl = s_hl[w][r][D - 1];   // first content line after the round
 
         const uint32_t g = rb + lane;
-        uint32_t par, se, fc, ns;
-        if (!valid) {   // blank (and too deep: reported by the error flag)
-            par = OL_NONE;
-            se = g + 1;
-            fc = OL_NONE;
-            ns = OL_NONE;
-        } else {
-            par = pm ? rb + jp : (gv ? gv - 1 : OL_NONE);
-            const uint32_t plv = pm ? kp : gl;
-            se = sm ? rb + js : (hv != OL_NONE ? hv : (uint32_t)L);
-            const uint32_t slv = sm ? ks : hl;
-            const uint32_t fi = fm ? rb + jf : nv;
-            const uint32_t fl = fm ? kf : nl;
-            fc = (fi != OL_NONE && fl > x) ? fi : OL_NONE;
-            ns = (se < L && (par == OL_NONE || plv < slv)) ? se : OL_NONE;
-        }
+        // content line (blank and too deep lines: below; too deep is reported
+        // by the error flag)
+        const uint32_t cpar = pm ? rb + jp : (gv ? gv - 1 : OL_NONE);
+        const uint32_t plv = pm ? kp : gl;
+        const uint32_t cse = sm ? rb + js : (hv != OL_NONE ? hv : (uint32_t)L);
+        const uint32_t slv = sm ? ks : hl;
+        const uint32_t fi = fm ? rb + jf : nv;
+        const uint32_t fl = fm ? kf : nl;
+        const uint32_t cfc = (fi != OL_NONE && fl > x) ? fi : OL_NONE;
+        const uint32_t cns = (cse < L && (cpar == OL_NONE || plv < slv)) ? cse : OL_NONE;
+        const uint32_t par = valid ? cpar : OL_NONE;
+        const uint32_t se = valid ? cse : g + 1;
+        const uint32_t fc = valid ? cfc : OL_NONE;
+        const uint32_t ns = valid ? cns : OL_NONE;
         if (g < L) {
-            const uint64_t o = ts.g0 + g;
+            const uint32_t o = (uint32_t)(r * 64 + lane);
             if (P) P[o] = par;
             if (SE) SE[o] = se;
             if (FC) FC[o] = fc;
