@@ -972,8 +972,10 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     }
     DictDev dd{};
     dd.max_len = max_len;
-    // filters: 1 and 2 direct, 3 hashed (2^16 bits), 4 = 4-byte prefixes of classes 4 and 5
-    std::vector<std::vector<uint32_t>> filt(5);
+    // filters: 1 and 2 direct, 3 hashed (2^16 bits), 4 = 4-byte prefixes of
+    // classes 4 and 5 (2^17 bits, 2^18 beyond 16k prefixes).  Layout: filter 4
+    // first, then 1, 2, 3, then a zero word that absent filters point at.
+    std::vector<std::vector<uint32_t>> filt(6);
     uint32_t fwords = 0;
     std::vector<uint32_t> pref4;
     for (uint32_t k = 4; k <= 5; k++)
@@ -981,13 +983,12 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
             pref4.push_back(b.lo);
     std::sort(pref4.begin(), pref4.end());
     pref4.erase(std::unique(pref4.begin(), pref4.end()), pref4.end());
-    for (uint32_t f = 1; f <= 4; f++) {
+    const bool big4 = pref4.size() > 16384;
+    for (uint32_t f : {4u, 1u, 2u, 3u}) {
         const bool used = f <= 3 ? !buckets[f].empty() : !pref4.empty();
         if (!used)
             continue;
-        // single-bit filters: 2^16 bits for 3-byte keys, 2^17 (2^18 beyond 16k
-        // prefixes) for 4-byte prefixes
-        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : f == 3 ? 16 : (pref4.size() <= 16384 ? 17 : 18);
+        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : f == 3 ? 16 : (big4 ? 18 : 17);
         dd.fbits[f] = fb;
         dd.foff[f] = fwords;
         const uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
@@ -1001,6 +1002,10 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         else
             for (uint32_t p4 : pref4) setbit(dict_fh4(p4) >> (32 - fb));
     }
+    for (uint32_t f = 1; f <= 4; f++)
+        if (!dd.fbits[f])
+            dd.foff[f] = fwords;   // the zero word
+    fwords += 1;
     dd.fwords = fwords;
     // tables
     std::vector<std::vector<uint32_t>> tab(6);   // 4 words per slot

@@ -206,15 +206,20 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 // the LDS list holds is probed per lane.  The text is read from global memory
 // (the probes hit L2), so the LDS holds only the filters and the list.
 
+__device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
+{
+    return __builtin_amdgcn_ubfe(f[i >> 5], i & 31, 1);
+}
+
 constexpr int MANY_HALO = 128;      // bytes readable past a tile (8-byte keys, 64-byte patterns)
 constexpr int MANY_LCAP = 2048;     // candidates per tile taken through the list
 
 // Matches starting at tile position lp (global gp) with at most `rem` bytes
 // (1..64) before the text or document end; written from out[r] in pattern-id
 // order (up to cap) when `emit`.  Returns their number.
-__device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8_t *s_text, uint32_t lp, uint64_t gp,
-                                                   uint32_t rem, bool emit, uint64_t *out_off, uint32_t *out_id,
-                                                   uint64_t r, uint64_t cap)
+__device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
+                                                   uint32_t lp, uint64_t gp, uint32_t rem, bool emit, uint64_t *out_off,
+                                                   uint32_t *out_id, uint64_t r, uint64_t cap)
 {
     // s_text: the tile's first byte in global memory (word aligned)
     const uint32_t *wp = (const uint32_t *)(s_text + (lp & ~3u));
@@ -225,9 +230,15 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8
     // One run per class (1..5): the patterns whose key matches the text here.
     // Named variables per class (no arrays: nothing may be indexed at run time).
     // First the table slots of every class are loaded together, then resolved.
+    // A class's table is read only where its filter passes: the 3-byte keys'
+    // filter for class 3, the 4-byte prefix filter for classes 4 and 5 (a
+    // candidate passed at least one of them).
+    const bool pass3 = !d.fbits[3] || filter_bit(s_filter + d.foff[3], dict_fh3(w0) >> 16);
+    const bool pass45 = !d.fbits[4] || filter_bit(s_filter, dict_fh4(w0) >> (32 - d.fbits[4]));
 #define OL_KEY(K, LO)                                                                       \
     const uint32_t klo##K = (LO), khi##K = (K) == 5 ? w1 : 0u;                              \
-    const bool use##K = ((d.class_mask >> (K)) & 1u) && ((K) <= 4 ? (K) : 8u) <= rem;      \
+    const bool use##K = ((d.class_mask >> (K)) & 1u) && ((K) <= 4 ? (K) : 8u) <= rem &&    \
+                        ((K) == 3 ? pass3 : (K) >= 4 ? pass45 : true);                       \
     uint32_t h##K = use##K ? dict_thash(klo##K, khi##K, d.tbits[K]) : 0u;                  \
     uint4 v##K = use##K ? d.table[K][h##K] : make_uint4(0, 0, 0, 0);
     OL_KEY(1, w0 & 0xFFu)
@@ -304,12 +315,15 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint8
     return count;
 }
 
-__device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i) { return (f[i >> 5] >> (i & 31)) & 1u; }
 
 // One tile of the dictionary search: filter, candidate list, counts; with
 // WRITE the matches are also written from `out0` (the tile's first output
 // index).  Returns the tile's match count (in every thread).
-template <int THREADS, bool WRITE>
+// FS: the filters the dictionary has, which the kernel looks up (bit 1: 1- and
+// 2-byte keys, bit 3: 3-byte keys, bit 4: 4-byte prefixes); compile-time, so
+// the 64 lookups of a lane are straight-line code whose LDS reads the
+// compiler can overlap.
+template <int THREADS, bool WRITE, int FS>
 __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
                                               uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt, uint32_t *s_w,
                                               uint64_t out0)
@@ -317,9 +331,12 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
     constexpr int TILE = THREADS * 64;
     const DictDev &d = a.d;
     const int tid = threadIdx.x;
-    const uint32_t cm = d.class_mask;
-    const uint32_t *f1 = s_filter + d.foff[1], *f2 = s_filter + d.foff[2], *f3 = s_filter + d.foff[3],
-                   *f4 = s_filter + d.foff[4];
+    // Filter 4, when the dictionary has it, is first in LDS.  The 1- and
+    // 2-byte filters are looked up together: an absent one reads the zero
+    // word (index mask 0).
+    const uint32_t *f1 = s_filter + d.foff[1], *f2 = s_filter + d.foff[2], *f3 = s_filter + d.foff[3];
+    const uint32_t *f4 = s_filter;
+    const uint32_t m1 = d.fbits[1] ? 0xFFu : 0u, m2 = d.fbits[2] ? 0xFFFFu : 0u;
     const uint32_t sh4 = 32 - d.fbits[4];   // filter 4 index = hash >> sh4
     const uint32_t base = t * TILE;
     const uint8_t *g_text = a.text + base;
@@ -361,16 +378,20 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
             W[4 * j + 2] = v.z;
             W[4 * j + 3] = v.w;
         }
+        uint32_t clo = 0, chi = 0;
 #pragma unroll
         for (int p = 0; p < 64; p++) {
             const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3) : W[p >> 2];
             uint32_t hit = 0;
-            if (cm & 0x02u) hit |= filter_bit(f1, w & 0xFFu);
-            if (cm & 0x04u) hit |= filter_bit(f2, w & 0xFFFFu);
-            if (cm & 0x08u) hit |= filter_bit(f3, dict_fh3(w) >> 16);
-            if (cm & 0x30u) hit |= filter_bit(f4, dict_fh4(w) >> sh4);
-            cand |= (uint64_t)hit << p;
+            if (FS & 0x10) hit |= filter_bit(f4, dict_fh4(w) >> sh4);
+            if (FS & 0x08) hit |= filter_bit(f3, dict_fh3(w) >> 16);
+            if (FS & 0x02) hit |= filter_bit(f1, w & m1) | filter_bit(f2, w & m2);
+            if (p < 32)
+                clo |= hit << p;
+            else
+                chi |= hit << (p - 32);
         }
+        cand = (uint64_t)chi << 32 | clo;
         if (lim < 64)
             cand &= (1ull << lim) - 1ull;
     }
@@ -395,7 +416,7 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
     if (listed) {
         for (uint32_t i = tid; i < total_c; i += THREADS) {
             const uint32_t lp = s_cand[i], rm = s_rem[i];
-            s_cnt[i] = rm ? probe_position(d, g_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0) : 0u;
+            s_cnt[i] = rm ? probe_position(d, s_filter, g_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0) : 0u;
         }
     } else {
         di = di0;
@@ -407,7 +428,7 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
             m &= m - 1;
             const uint32_t rm = rem_at(g0 + p);
             if (rm)
-                own += probe_position(d, g_text, lo + p, (uint64_t)g0 + p, rm, false, nullptr, nullptr, 0, 0);
+                own += probe_position(d, s_filter, g_text, lo + p, (uint64_t)g0 + p, rm, false, nullptr, nullptr, 0, 0);
         }
     }
     __syncthreads();
@@ -431,7 +452,7 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
                 const uint32_t lp = s_cand[i], rm = s_rem[i];
                 const uint64_t r = out0 + s_cnt[i];
                 if (rm && r < a.cap)
-                    probe_position(d, g_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r, a.cap);
+                    probe_position(d, s_filter, g_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r, a.cap);
             }
         }
     } else {
@@ -447,7 +468,7 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
                 m &= m - 1;
                 const uint32_t rm = rem_at(g0 + p);
                 if (rm)
-                    r += probe_position(d, g_text, lo + p, (uint64_t)g0 + p, rm, true, a.out_off, a.out_id, r, a.cap);
+                    r += probe_position(d, s_filter, g_text, lo + p, (uint64_t)g0 + p, rm, true, a.out_off, a.out_id, r, a.cap);
             }
         }
     }
@@ -455,7 +476,7 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
     return total;
 }
 
-template <int THREADS, bool WRITE>
+template <int THREADS, bool WRITE, int FS>
 __global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
 {
     extern __shared__ __align__(16) uint32_t s_filter[];
@@ -467,7 +488,7 @@ __global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
         s_filter[i] = a.d.filter[i];
     __syncthreads();
     for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
-        const uint32_t total = many_tile<THREADS, WRITE>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w,
+        const uint32_t total = many_tile<THREADS, WRITE, FS>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w,
                                                          WRITE ? a.tile_off[t] : 0ull);
         if (!WRITE && threadIdx.x == 0) {
             a.tile_count[t] = total;
@@ -536,19 +557,37 @@ size_t ol_many_lds_bytes(uint32_t fwords)
     return (size_t)fwords * 4;
 }
 
-template <bool WRITE>
-static hipError_t launch_many_tiles(const ManyArgs &a, hipStream_t s)
+template <bool WRITE, int FS>
+static hipError_t launch_many_tiles_fs(const ManyArgs &a, hipStream_t s)
 {
     const size_t lds = ol_many_lds_bytes(a.d.fwords);
-    const void *f = (const void *)many_tiles_kernel<SEARCH_THREADS, WRITE>;
+    const void *f = (const void *)many_tiles_kernel<SEARCH_THREADS, WRITE, FS>;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     uint32_t cap = max_grid(f, SEARCH_THREADS, lds);
     if (!cap)
         return hipErrorInvalidValue;
     const uint32_t grid = a.num_tiles < 4 * cap ? a.num_tiles : 4 * cap;
-    hipLaunchKernelGGL((many_tiles_kernel<SEARCH_THREADS, WRITE>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
+    hipLaunchKernelGGL((many_tiles_kernel<SEARCH_THREADS, WRITE, FS>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
     return hipGetLastError();
+}
+
+// The kernel for the filters the dictionary has: 4-byte prefixes (with the
+// 3-byte keys folded in, the usual case), plus 3-byte keys, plus 1/2-byte keys.
+template <bool WRITE>
+static hipError_t launch_many_tiles(const ManyArgs &a, hipStream_t s)
+{
+    const int fs = (a.d.fbits[4] ? 0x10 : 0) | (a.d.fbits[3] ? 0x08 : 0) | (a.d.fbits[1] || a.d.fbits[2] ? 0x02 : 0);
+    switch (fs) {
+    case 0x10: return launch_many_tiles_fs<WRITE, 0x10>(a, s);
+    case 0x18: return launch_many_tiles_fs<WRITE, 0x18>(a, s);
+    case 0x12: return launch_many_tiles_fs<WRITE, 0x12>(a, s);
+    case 0x1A: return launch_many_tiles_fs<WRITE, 0x1A>(a, s);
+    case 0x08: return launch_many_tiles_fs<WRITE, 0x08>(a, s);
+    case 0x0A: return launch_many_tiles_fs<WRITE, 0x0A>(a, s);
+    case 0x02: return launch_many_tiles_fs<WRITE, 0x02>(a, s);
+    default: return hipErrorInvalidValue;   // a dictionary has at least one filter
+    }
 }
 
 // Counts (total in meta[0]); with outputs also the scan and the writes.
