@@ -72,6 +72,7 @@ struct ol_ctx {
     DevBuf scratch;
     DevBuf tree_sum;                       // tree per-tile level states (4 arrays of tiles x TREE_D)
     DevBuf fix;                            // list of lines decoded by the scan fixup pass
+    DevBuf cands;                          // dictionary search: candidates kept for the write pass
     // launch geometry picked by the autotuner (ol_tune_set)
     uint32_t scan_bpt = SCAN_BPT;          // bytes per lane of scan_classify (tile = 256 lanes x bpt)
     uint32_t scan_grid = 0;                // persistent workgroups of scan_classify (0 = all co-resident)
@@ -480,6 +481,27 @@ int search_doc_host(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, si
     return OL_OK;
 }
 
+// The dictionary search's candidate store (when matches are written): room
+// for one candidate in eight text positions; tiles beyond it are filtered and
+// counted again by the write pass.  tile_list follows tile_off in the scratch
+// status words, the store's counter is meta[1] (both cleared per launch).
+hipError_t set_candidate_store(ol_ctx *c, ManyArgs &a, uint32_t tiles, bool writes)
+{
+    a.tile_list = c->status() + 2 * (size_t)tiles;
+    a.cand_ctr = c->meta() + 1;
+    a.cands = nullptr;
+    a.cand_cap = 0;
+    if (!writes)
+        return hipSuccess;
+    const uint64_t cap = (uint64_t)a.n / 8 + 4096;
+    const hipError_t e = c->cands.ensure(cap * 4);
+    if (e != hipSuccess)
+        return e;
+    a.cands = (uint32_t *)c->cands.p;
+    a.cand_cap = cap;
+    return hipSuccess;
+}
+
 int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, uint32_t *out_id,
              uint64_t cap, uint64_t *total)
 {
@@ -497,6 +519,7 @@ int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, ui
     a.meta = c->meta();
     a.tile_count = (uint32_t *)c->status();
     a.tile_off = c->status() + tiles;
+    HIPCHK(set_candidate_store(c, a, tiles, out_off || out_id));
     {
         Prof p(c, "many");
         HIPCHK(ol_launch_many(a, c->stream));
@@ -1374,6 +1397,7 @@ int batch_many(ol_ctx *c, ol_batch *b, const ol_dict *s, uint64_t cap)
     a.meta = c->meta();
     a.tile_count = (uint32_t *)c->status();
     a.tile_off = c->status() + tiles;
+    HIPCHK(set_candidate_store(c, a, tiles, cap != 0));
     a.ndocs = b->nd;
     a.doc_starts = (const uint32_t *)b->d_starts.p;
     a.doc_ends = (const uint32_t *)b->d_ends.p;

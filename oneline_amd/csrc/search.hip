@@ -193,18 +193,22 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 
 // ---- dictionary -----------------------------------------------------------------
 // Three launches with no waits between workgroups:
-//   many_count_kernel  per tile (256 lanes x 64 positions): filter, candidate
-//                      list, per-candidate match counts -> tile_count[t];
-//   many_scan_kernel   exclusive scan of tile_count -> tile_off, total;
-//   many_write_kernel  per tile again, each candidate's matches written at
-//                      tile_off[t] + (matches of the earlier candidates).
+//   count pass   per tile (256 lanes x 64 positions): filter, candidate list,
+//                per-candidate match counts -> tile_count[t]; the candidates
+//                and their counts are stored (4 bytes each) while the store
+//                has room;
+//   scan         exclusive scan of tile_count -> tile_off, total;
+//   write pass   per tile: its stored candidates (or filter and count again),
+//                each candidate's matches written at tile_off[t] + (matches
+//                of the earlier candidates).
 // Filter: each lane tests its 64 positions against the LDS bitmaps of the
 // dictionary's key classes (straight-line code, 4-byte windows from
 // registers).  Probe: the class tables give, per class, the run of patterns
 // sharing the key (sorted by id); the runs are merged by id and each pattern's
-// remaining bytes compared word by word.  A tile with more candidates than
-// the LDS list holds is probed per lane.  The text is read from global memory
-// (the probes hit L2), so the LDS holds only the filters and the list.
+// remaining bytes compared word by word.  The candidate list goes through LDS
+// in chunks of MANY_LCAP, one candidate per thread.  The text is read from
+// global memory (the probes hit L2), so the LDS holds only the filters and
+// the list.
 
 __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
 {
@@ -212,7 +216,7 @@ __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
 }
 
 constexpr int MANY_HALO = 128;      // bytes readable past a tile (8-byte keys, 64-byte patterns)
-constexpr int MANY_LCAP = 2048;     // candidates per tile taken through the list
+constexpr int MANY_LCAP = 2048;     // candidates of a tile per list chunk
 
 // Matches starting at tile position lp (global gp) with at most `rem` bytes
 // (1..64) before the text or document end; written from out[r] in pattern-id
@@ -316,21 +320,55 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
 }
 
 
-// One tile of the dictionary search: filter, candidate list, counts; with
-// WRITE the matches are also written from `out0` (the tile's first output
-// index).  Returns the tile's match count (in every thread).
-// FS: the filters the dictionary has, which the kernel looks up (bit 1: 1- and
-// 2-byte keys, bit 3: 3-byte keys, bit 4: 4-byte prefixes); compile-time, so
-// the 64 lookups of a lane are straight-line code whose LDS reads the
-// compiler can overlap.
-template <int THREADS, bool WRITE, int FS>
-__device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
-                                              uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt, uint32_t *s_w,
-                                              uint64_t out0)
+// Stored candidate: tile position (14 bits) | bytes before the text or
+// document end (7 bits) << 14 | match count (11 bits, CAND_RECOUNT: count
+// again) << 21.
+constexpr uint32_t CAND_RECOUNT = 2047u;
+constexpr uint64_t TILE_LIST_NONE = ~0ull;   // tile_list: candidates not stored
+
+// Per-tile state of the dictionary search: the tile's text, and the lanes'
+// document bounds (batches) for the end bound of a match.
+struct ManyTile {
+    const uint8_t *g_text;   // the tile's first byte (global memory)
+    uint32_t base, lo, g0, lim;
+    uint32_t di0, dlim0, dnext0;
+};
+
+template <int THREADS>
+__device__ __forceinline__ ManyTile many_tile_setup(const ManyArgs &a, uint32_t t)
 {
     constexpr int TILE = THREADS * 64;
-    const DictDev &d = a.d;
-    const int tid = threadIdx.x;
+    ManyTile m;
+    m.base = t * TILE;
+    m.g_text = a.text + m.base;
+    m.lo = threadIdx.x * 64;
+    m.g0 = m.base + m.lo;
+    m.lim = (m.g0 < a.n) ? ((a.n - m.g0) < 64u ? a.n - m.g0 : 64u) : 0u;
+    // match-end bound per position: the text end, or (batches) the end of
+    // the position's document; positions between documents match nothing
+    uint32_t dlim = a.n, dnext = 0xFFFFFFFFu, di = 0;
+    if (a.ndocs) {
+        di = a.tile_doc[t];   // then the last document starting at or before g0
+        while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= m.g0)
+            di++;
+        dlim = a.doc_ends[di];
+        dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
+    }
+    m.di0 = di;
+    m.dlim0 = dlim;
+    m.dnext0 = dnext;
+    return m;
+}
+
+// The lane's 64 positions through the filters: bit p set when a pattern may
+// start at g0 + p.  FS: the filters the dictionary has (bit 1: 1- and 2-byte
+// keys, bit 3: 3-byte keys, bit 4: 4-byte prefixes); compile-time, so the 64
+// lookups are straight-line code whose LDS reads the compiler can overlap.
+template <int FS>
+__device__ __forceinline__ uint64_t many_filter(const DictDev &d, const uint32_t *s_filter, const ManyTile &m)
+{
+    if (!m.lim)
+        return 0;
     // Filter 4, when the dictionary has it, is first in LDS.  The 1- and
     // 2-byte filters are looked up together: an absent one reads the zero
     // word (index mask 0).
@@ -338,142 +376,188 @@ __device__ __forceinline__ uint32_t many_tile(const ManyArgs &a, uint32_t t, con
     const uint32_t *f4 = s_filter;
     const uint32_t m1 = d.fbits[1] ? 0xFFu : 0u, m2 = d.fbits[2] ? 0xFFFFu : 0u;
     const uint32_t sh4 = 32 - d.fbits[4];   // filter 4 index = hash >> sh4
-    const uint32_t base = t * TILE;
-    const uint8_t *g_text = a.text + base;
-    const uint32_t lo = tid * 64;
-    const uint32_t g0 = base + lo;
-    const uint32_t lim = (g0 < a.n) ? ((a.n - g0) < 64u ? a.n - g0 : 64u) : 0u;
-
-    // match-end bound per position: the text end, or (batches) the end of
-    // the position's document; positions between documents match nothing
-    uint32_t dlim = a.n, dnext = 0xFFFFFFFFu, di = 0;
-    if (a.ndocs) {
-        di = a.tile_doc[t];   // then the last document starting at or before g0
-        while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= g0)
-            di++;
-        dlim = a.doc_ends[di];
-        dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
-    }
-    const uint32_t di0 = di, dlim0 = dlim, dnext0 = dnext;
-    auto rem_at = [&](uint32_t gp) -> uint32_t {   // 0: no match may start at gp
-        while (gp >= dnext) {
-            di++;
-            dlim = a.doc_ends[di];
-            dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
-        }
-        const uint32_t rm = gp < dlim ? dlim - gp : 0u;
-        return rm < 64u ? rm : 64u;
-    };
-
-    // 1. filter
-    uint64_t cand = 0;
-    if (lim) {
-        uint32_t W[20];
-        const uint4 *vsrc = (const uint4 *)(g_text + lo);
+    uint32_t W[20];
+    const uint4 *vsrc = (const uint4 *)(m.g_text + m.lo);
 #pragma unroll
-        for (int j = 0; j < 5; j++) {
-            const uint4 v = vsrc[j];
-            W[4 * j] = v.x;
-            W[4 * j + 1] = v.y;
-            W[4 * j + 2] = v.z;
-            W[4 * j + 3] = v.w;
-        }
-        uint32_t clo = 0, chi = 0;
-#pragma unroll
-        for (int p = 0; p < 64; p++) {
-            const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3) : W[p >> 2];
-            uint32_t hit = 0;
-            if (FS & 0x10) hit |= filter_bit(f4, dict_fh4(w) >> sh4);
-            if (FS & 0x08) hit |= filter_bit(f3, dict_fh3(w) >> 16);
-            if (FS & 0x02) hit |= filter_bit(f1, w & m1) | filter_bit(f2, w & m2);
-            if (p < 32)
-                clo |= hit << p;
-            else
-                chi |= hit << (p - 32);
-        }
-        cand = (uint64_t)chi << 32 | clo;
-        if (lim < 64)
-            cand &= (1ull << lim) - 1ull;
+    for (int j = 0; j < 5; j++) {
+        const uint4 v = vsrc[j];
+        W[4 * j] = v.x;
+        W[4 * j + 1] = v.y;
+        W[4 * j + 2] = v.z;
+        W[4 * j + 3] = v.w;
     }
-    // 2. candidate list (position, remaining bytes)
+    uint32_t clo = 0, chi = 0;
+#pragma unroll
+    for (int p = 0; p < 64; p++) {
+        const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3) : W[p >> 2];
+        uint32_t hit = 0;
+        if (FS & 0x10) hit |= filter_bit(f4, dict_fh4(w) >> sh4);
+        if (FS & 0x08) hit |= filter_bit(f3, dict_fh3(w) >> 16);
+        if (FS & 0x02) hit |= filter_bit(f1, w & m1) | filter_bit(f2, w & m2);
+        if (p < 32)
+            clo |= hit << p;
+        else
+            chi |= hit << (p - 32);
+    }
+    uint64_t cand = (uint64_t)chi << 32 | clo;
+    if (m.lim < 64)
+        cand &= (1ull << m.lim) - 1ull;
+    return cand;
+}
+
+// Candidates [c0, c0 + LCAP) of the tile (in position order; the lane's own
+// start at index cex) into the LDS list: tile position and end bound.
+__device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyTile &m, uint64_t cand, uint32_t cex,
+                                                uint32_t c0, uint16_t *s_cand, uint8_t *s_rem)
+{
+    uint32_t di = m.di0, dlim = m.dlim0, dnext = m.dnext0;
+    uint32_t at = cex;
+    uint64_t q = cand;
+    while (q && at < c0 + (uint32_t)MANY_LCAP) {
+        const uint32_t p = (uint32_t)__builtin_ctzll(q);
+        q &= q - 1;
+        if (at >= c0) {
+            const uint32_t gp = m.g0 + p;
+            while (gp >= dnext) {
+                di++;
+                dlim = a.doc_ends[di];
+                dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
+            }
+            const uint32_t rm = gp < dlim ? dlim - gp : 0u;   // 0: no match may start here
+            s_cand[at - c0] = (uint16_t)(m.lo + p);
+            s_rem[at - c0] = (uint8_t)(rm < 64u ? rm : 64u);
+        }
+        at++;
+    }
+}
+
+// Per-candidate exclusive offsets of s_cnt[0 .. n) (contiguous runs per
+// thread), in place; returns the chunk's total (every thread).
+template <int THREADS>
+__device__ __forceinline__ uint32_t many_chunk_offsets(uint32_t *s_cnt, uint32_t n, uint32_t *s_w)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t K = (n + THREADS - 1) / THREADS;
+    const uint32_t i0 = min(n, tid * K), i1 = min(n, i0 + K);
+    uint32_t loc = 0;
+    for (uint32_t i = i0; i < i1; i++)
+        loc += s_cnt[i];
+    uint32_t total;
+    uint32_t run = block_excl_sum<THREADS>(loc, s_w, total);
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t c = s_cnt[i];
+        s_cnt[i] = run;
+        run += c;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Count pass over one tile: filter, candidate list in chunks of LCAP,
+// per-candidate match counts.  The candidates (with their counts) are stored
+// for the write pass when the store has room.  Returns the tile's match count.
+template <int THREADS, int FS>
+__device__ __forceinline__ uint32_t many_count_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
+                                                    uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_w,
+                                                    uint64_t *s_store)
+{
+    const DictDev &d = a.d;
+    const ManyTile m = many_tile_setup<THREADS>(a, t);
+    const uint64_t cand = many_filter<FS>(d, s_filter, m);
     uint32_t total_c;
     const uint32_t cex = block_excl_sum<THREADS>((uint32_t)__popcll(cand), s_w, total_c);
-    const bool listed = total_c <= (uint32_t)MANY_LCAP;
-    if (listed) {
-        uint32_t at = cex;
-        uint64_t m = cand;
-        while (m) {
-            const uint32_t p = (uint32_t)__builtin_ctzll(m);
-            m &= m - 1;
-            s_cand[at] = (uint16_t)(lo + p);
-            s_rem[at] = (uint8_t)rem_at(g0 + p);
-            at++;
+    if (threadIdx.x == 0) {
+        uint64_t off = TILE_LIST_NONE;
+        if (total_c && a.cands) {
+            off = atomicAdd((unsigned long long *)a.cand_ctr, (unsigned long long)total_c);
+            if (off + total_c > a.cand_cap)
+                off = TILE_LIST_NONE;
         }
+        *s_store = off;
     }
-    __syncthreads();
-    // 3. counts
     uint32_t own = 0;
-    if (listed) {
-        for (uint32_t i = tid; i < total_c; i += THREADS) {
+    for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_LCAP) {
+        const uint32_t n = min((uint32_t)MANY_LCAP, total_c - c0);
+        many_list_chunk(a, m, cand, cex, c0, s_cand, s_rem);
+        __syncthreads();
+        const uint64_t off = *s_store;
+        for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
             const uint32_t lp = s_cand[i], rm = s_rem[i];
-            s_cnt[i] = rm ? probe_position(d, s_filter, g_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0) : 0u;
+            const uint32_t c =
+                rm ? probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, false, nullptr, nullptr, 0, 0)
+                   : 0u;
+            own += c;
+            if (off != TILE_LIST_NONE)
+                a.cands[off + c0 + i] = lp | rm << 14 | (c < CAND_RECOUNT ? c : CAND_RECOUNT) << 21;
         }
-    } else {
-        di = di0;
-        dlim = dlim0;
-        dnext = dnext0;
-        uint64_t m = cand;
-        while (m) {
-            const uint32_t p = (uint32_t)__builtin_ctzll(m);
-            m &= m - 1;
-            const uint32_t rm = rem_at(g0 + p);
-            if (rm)
-                own += probe_position(d, s_filter, g_text, lo + p, (uint64_t)g0 + p, rm, false, nullptr, nullptr, 0, 0);
-        }
+        __syncthreads();   // the list is rewritten by the next chunk
     }
-    __syncthreads();
+    __syncthreads();   // s_w is reused
     uint32_t total;
-    if (listed) {
-        // per-candidate exclusive offsets: contiguous runs of the list per thread
-        const uint32_t K = (total_c + THREADS - 1) / THREADS;
-        const uint32_t i0 = min(total_c, tid * K), i1 = min(total_c, i0 + K);
-        uint32_t loc = 0;
-        for (uint32_t i = i0; i < i1; i++)
-            loc += s_cnt[i];
-        uint32_t run = block_excl_sum<THREADS>(loc, s_w, total);
-        if (WRITE) {
-            for (uint32_t i = i0; i < i1; i++) {
-                const uint32_t c = s_cnt[i];
-                s_cnt[i] = run;
-                run += c;
-            }
-            __syncthreads();
-            for (uint32_t i = tid; i < total_c; i += THREADS) {
-                const uint32_t lp = s_cand[i], rm = s_rem[i];
-                const uint64_t r = out0 + s_cnt[i];
-                if (rm && r < a.cap)
-                    probe_position(d, s_filter, g_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r, a.cap);
-            }
-        }
-    } else {
-        const uint32_t lane_ex = block_excl_sum<THREADS>(own, s_w, total);
-        if (WRITE) {
-            di = di0;
-            dlim = dlim0;
-            dnext = dnext0;
-            uint64_t r = out0 + lane_ex;
-            uint64_t m = cand;
-            while (m && r < a.cap) {
-                const uint32_t p = (uint32_t)__builtin_ctzll(m);
-                m &= m - 1;
-                const uint32_t rm = rem_at(g0 + p);
-                if (rm)
-                    r += probe_position(d, s_filter, g_text, lo + p, (uint64_t)g0 + p, rm, true, a.out_off, a.out_id, r, a.cap);
-            }
-        }
-    }
-    __syncthreads();   // the list is reused by the next tile
+    (void)block_excl_sum<THREADS>(own, s_w, total);
+    if (threadIdx.x == 0)
+        a.tile_list[t] = *s_store == TILE_LIST_NONE ? TILE_LIST_NONE : (*s_store << 24 | total_c);
+    __syncthreads();
     return total;
+}
+
+// Write pass over one tile: its matches from out0 on, in position then
+// pattern-id order.  With stored candidates: read them, offsets from their
+// counts, one probe each.  Without: filter and count again first.
+template <int THREADS, int FS>
+__device__ __forceinline__ void many_write_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
+                                                uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt, uint32_t *s_w,
+                                                uint64_t out0)
+{
+    const DictDev &d = a.d;
+    const ManyTile m = many_tile_setup<THREADS>(a, t);
+    const uint64_t stored = a.tile_list[t];
+    uint64_t cand = 0;
+    uint32_t total_c, cex = 0;
+    if (stored != TILE_LIST_NONE) {
+        total_c = (uint32_t)(stored & 0xFFFFFFu);
+    } else {
+        cand = many_filter<FS>(d, s_filter, m);
+        cex = block_excl_sum<THREADS>((uint32_t)__popcll(cand), s_w, total_c);
+    }
+    uint64_t run = out0;
+    for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_LCAP) {
+        const uint32_t n = min((uint32_t)MANY_LCAP, total_c - c0);
+        if (stored != TILE_LIST_NONE) {
+            const uint32_t *src = a.cands + (stored >> 24) + c0;
+            for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
+                const uint32_t e = src[i];
+                const uint32_t lp = e & 0x3FFFu, rm = (e >> 14) & 0x7Fu;
+                uint32_t c = e >> 21;
+                if (c == CAND_RECOUNT)
+                    c = probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, false, nullptr, nullptr, 0, 0);
+                s_cand[i] = (uint16_t)lp;
+                s_rem[i] = (uint8_t)rm;
+                s_cnt[i] = c;
+            }
+        } else {
+            many_list_chunk(a, m, cand, cex, c0, s_cand, s_rem);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
+                const uint32_t lp = s_cand[i], rm = s_rem[i];
+                s_cnt[i] = rm ? probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, false, nullptr,
+                                               nullptr, 0, 0)
+                              : 0u;
+            }
+        }
+        __syncthreads();
+        const uint32_t ctot = many_chunk_offsets<THREADS>(s_cnt, n, s_w);
+        for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
+            const uint32_t lp = s_cand[i], rm = s_rem[i];
+            const uint64_t r = run + s_cnt[i];
+            if (rm && r < a.cap)
+                probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, true, a.out_off, a.out_id, r,
+                               a.cap);
+        }
+        run += ctot;
+        __syncthreads();   // the list is rewritten by the next chunk
+    }
+    __syncthreads();   // s_w is reused by the next tile
 }
 
 template <int THREADS, bool WRITE, int FS>
@@ -484,16 +568,20 @@ __global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
     __shared__ uint8_t s_rem[MANY_LCAP];
     __shared__ uint32_t s_cnt[MANY_LCAP];
     __shared__ uint32_t s_w[THREADS / 64];
+    __shared__ uint64_t s_store;
     for (uint32_t i = threadIdx.x; i < a.d.fwords; i += THREADS)
         s_filter[i] = a.d.filter[i];
     __syncthreads();
     for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
-        const uint32_t total = many_tile<THREADS, WRITE, FS>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w,
-                                                         WRITE ? a.tile_off[t] : 0ull);
-        if (!WRITE && threadIdx.x == 0) {
-            a.tile_count[t] = total;
-            if (total)
-                atomicAdd((unsigned long long *)a.meta, (unsigned long long)total);
+        if (WRITE) {
+            many_write_tile<THREADS, FS>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w, a.tile_off[t]);
+        } else {
+            const uint32_t total = many_count_tile<THREADS, FS>(a, t, s_filter, s_cand, s_rem, s_w, &s_store);
+            if (threadIdx.x == 0) {
+                a.tile_count[t] = total;
+                if (total)
+                    atomicAdd((unsigned long long *)a.meta, (unsigned long long)total);
+            }
         }
     }
 }
