@@ -481,24 +481,24 @@ int search_doc_host(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, si
     return OL_OK;
 }
 
-// The dictionary search's candidate store (when matches are written): room
-// for one candidate in eight text positions; tiles beyond it are filtered and
-// counted again by the write pass.  tile_list follows tile_off in the scratch
-// status words, the store's counter is meta[1] (both cleared per launch).
+// The dictionary search's candidate store (when matches are written): per
+// tile room for one candidate in four text positions (4 bytes each: the text
+// size in total); a tile with more is filtered and counted again by the write
+// pass.  tile_list follows tile_off in
+// the scratch status words.
 hipError_t set_candidate_store(ol_ctx *c, ManyArgs &a, uint32_t tiles, bool writes)
 {
     a.tile_list = c->status() + 2 * (size_t)tiles;
-    a.cand_ctr = c->meta() + 1;
     a.cands = nullptr;
-    a.cand_cap = 0;
+    a.cand_per_tile = 0;
     if (!writes)
         return hipSuccess;
-    const uint64_t cap = (uint64_t)a.n / 8 + 4096;
-    const hipError_t e = c->cands.ensure(cap * 4);
+    const uint32_t per_tile = SEARCH_TILE / 4;
+    const hipError_t e = c->cands.ensure((size_t)tiles * per_tile * 4);
     if (e != hipSuccess)
         return e;
     a.cands = (uint32_t *)c->cands.p;
-    a.cand_cap = cap;
+    a.cand_per_tile = per_tile;
     return hipSuccess;
 }
 

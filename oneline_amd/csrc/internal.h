@@ -314,12 +314,12 @@ struct ManyArgs {
     uint64_t *meta;          // meta[0] <- total matches (zero at launch)
     uint32_t *tile_count;    // [num_tiles] matches per tile
     uint64_t *tile_off;      // [num_tiles] first output index of each tile
-    // Candidate store (nullable): the count pass appends each tile's
-    // candidates at cand_ctr (zero at launch) while they fit in cand_cap and
-    // records (offset << 24 | count) in tile_list[t], or ~0 when they did not.
+    // Candidate store (nullable): tile t's candidates go to
+    // cands[t * cand_per_tile ..] when there are at most cand_per_tile of
+    // them; tile_list[t] records (offset << 24 | count), or ~0 when they were
+    // not stored.
     uint32_t *cands;
-    uint64_t cand_cap;
-    uint64_t *cand_ctr;
+    uint32_t cand_per_tile;
     uint64_t *tile_list;     // [num_tiles]
     // Batches (ndocs > 0): a match must end inside the document it starts in
     // (the bytes between documents belong to none).

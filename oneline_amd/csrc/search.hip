@@ -195,8 +195,9 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 // Three launches with no waits between workgroups:
 //   count pass   per tile (256 lanes x 64 positions): filter, candidate list,
 //                per-candidate match counts -> tile_count[t]; the candidates
-//                and their counts are stored (4 bytes each) while the store
-//                has room;
+//                that have matches are stored with their counts (4 bytes
+//                each) in the tile's own region of the store when the tile's
+//                candidates fit it;
 //   scan         exclusive scan of tile_count -> tile_off, total;
 //   write pass   per tile: its stored candidates (or filter and count again),
 //                each candidate's matches written at tile_off[t] + (matches
@@ -215,7 +216,6 @@ __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
     return __builtin_amdgcn_ubfe(f[i >> 5], i & 31, 1);
 }
 
-constexpr int MANY_HALO = 128;      // bytes readable past a tile (8-byte keys, 64-byte patterns)
 constexpr int MANY_LCAP = 2048;     // candidates of a tile per list chunk
 
 // Matches starting at tile position lp (global gp) with at most `rem` bytes
@@ -458,37 +458,47 @@ __device__ __forceinline__ uint32_t many_chunk_offsets(uint32_t *s_cnt, uint32_t
 // for the write pass when the store has room.  Returns the tile's match count.
 template <int THREADS, int FS>
 __device__ __forceinline__ uint32_t many_count_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
-                                                    uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_w,
-                                                    uint64_t *s_store)
+                                                    uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt,
+                                                    uint32_t *s_w)
 {
     const DictDev &d = a.d;
     const ManyTile m = many_tile_setup<THREADS>(a, t);
     const uint64_t cand = many_filter<FS>(d, s_filter, m);
     uint32_t total_c;
     const uint32_t cex = block_excl_sum<THREADS>((uint32_t)__popcll(cand), s_w, total_c);
-    if (threadIdx.x == 0) {
-        uint64_t off = TILE_LIST_NONE;
-        if (total_c && a.cands) {
-            off = atomicAdd((unsigned long long *)a.cand_ctr, (unsigned long long)total_c);
-            if (off + total_c > a.cand_cap)
-                off = TILE_LIST_NONE;
-        }
-        *s_store = off;
-    }
-    uint32_t own = 0;
+    uint32_t own = 0, kept = 0;   // kept: candidates with matches stored so far (every thread)
+    const uint64_t off = (total_c && a.cands && total_c <= a.cand_per_tile) ? (uint64_t)t * a.cand_per_tile
+                                                                            : TILE_LIST_NONE;
     for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_LCAP) {
         const uint32_t n = min((uint32_t)MANY_LCAP, total_c - c0);
         many_list_chunk(a, m, cand, cex, c0, s_cand, s_rem);
         __syncthreads();
-        const uint64_t off = *s_store;
         for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
             const uint32_t lp = s_cand[i], rm = s_rem[i];
             const uint32_t c =
                 rm ? probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, false, nullptr, nullptr, 0, 0)
                    : 0u;
             own += c;
-            if (off != TILE_LIST_NONE)
-                a.cands[off + c0 + i] = lp | rm << 14 | (c < CAND_RECOUNT ? c : CAND_RECOUNT) << 21;
+            s_cnt[i] = c;
+        }
+        __syncthreads();
+        if (off != TILE_LIST_NONE) {
+            // store the candidates that have matches, in position order: each
+            // thread takes a contiguous run of the chunk
+            const uint32_t K = (n + THREADS - 1) / THREADS;
+            const uint32_t i0 = min(n, threadIdx.x * K), i1 = min(n, i0 + K);
+            uint32_t nz = 0;
+            for (uint32_t i = i0; i < i1; i++)
+                nz += s_cnt[i] != 0;
+            uint32_t chunk_kept;
+            uint32_t at = kept + block_excl_sum<THREADS>(nz, s_w, chunk_kept);
+            for (uint32_t i = i0; i < i1; i++) {
+                const uint32_t c = s_cnt[i];
+                if (c)
+                    a.cands[off + at++] = s_cand[i] | (uint32_t)s_rem[i] << 14 | (c < CAND_RECOUNT ? c : CAND_RECOUNT)
+                                                                                    << 21;
+            }
+            kept += chunk_kept;
         }
         __syncthreads();   // the list is rewritten by the next chunk
     }
@@ -496,14 +506,15 @@ __device__ __forceinline__ uint32_t many_count_tile(const ManyArgs &a, uint32_t 
     uint32_t total;
     (void)block_excl_sum<THREADS>(own, s_w, total);
     if (threadIdx.x == 0)
-        a.tile_list[t] = *s_store == TILE_LIST_NONE ? TILE_LIST_NONE : (*s_store << 24 | total_c);
+        a.tile_list[t] = off == TILE_LIST_NONE ? TILE_LIST_NONE : (off << 24 | kept);
     __syncthreads();
     return total;
 }
 
 // Write pass over one tile: its matches from out0 on, in position then
-// pattern-id order.  With stored candidates: read them, offsets from their
-// counts, one probe each.  Without: filter and count again first.
+// pattern-id order.  With stored candidates (those with matches): read them,
+// offsets from their counts, one probe each.  Without: filter and count again
+// first.
 template <int THREADS, int FS>
 __device__ __forceinline__ void many_write_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
                                                 uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt, uint32_t *s_w,
@@ -568,7 +579,6 @@ __global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
     __shared__ uint8_t s_rem[MANY_LCAP];
     __shared__ uint32_t s_cnt[MANY_LCAP];
     __shared__ uint32_t s_w[THREADS / 64];
-    __shared__ uint64_t s_store;
     for (uint32_t i = threadIdx.x; i < a.d.fwords; i += THREADS)
         s_filter[i] = a.d.filter[i];
     __syncthreads();
@@ -576,7 +586,7 @@ __global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
         if (WRITE) {
             many_write_tile<THREADS, FS>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w, a.tile_off[t]);
         } else {
-            const uint32_t total = many_count_tile<THREADS, FS>(a, t, s_filter, s_cand, s_rem, s_w, &s_store);
+            const uint32_t total = many_count_tile<THREADS, FS>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w);
             if (threadIdx.x == 0) {
                 a.tile_count[t] = total;
                 if (total)
@@ -586,58 +596,74 @@ __global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
     }
 }
 
-// Exclusive scan of tile_count into tile_off (one workgroup, chunks through LDS).
+// Exclusive scan of tile_count into tile_off: one workgroup per chunk of
+// ST * K tiles.  Each publishes its chunk's total, then waits for the totals
+// of the chunks before it (lower workgroups, dispatched first, which wait on
+// nothing later) and adds them up; the last chunk's workgroup writes the
+// grand total to meta[0].  Chunk totals live in `sums` (zero at launch).
 template <int ST>
-__global__ __launch_bounds__(ST) void many_scan_kernel(ManyArgs a)
+__global__ __launch_bounds__(ST) void many_scan_kernel(ManyArgs a, uint64_t *sums)
 {
     constexpr int K = 8;
     constexpr int CH = ST * K;
     constexpr int NWS = ST / 64;
-    __shared__ uint32_t s_v[CH];
     __shared__ uint64_t s_wt[NWS];
+    __shared__ uint64_t s_carry;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t T = a.num_tiles;
-    uint64_t carry = 0;
-    for (uint32_t c0 = 0; c0 < T; c0 += CH) {
-        const uint32_t n = (T - c0) < (uint32_t)CH ? T - c0 : (uint32_t)CH;
-        for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
-            s_v[i] = i < n ? a.tile_count[c0 + i] : 0u;
-        __syncthreads();
-        uint32_t loc[K];
-        uint64_t m = 0;
+    const uint32_t c0 = blockIdx.x * CH;
+    uint32_t loc[K];
+    uint64_t m = 0;
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            loc[k] = s_v[tid * K + k];
-            m += loc[k];
-        }
-        uint64_t inc = m;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint64_t y = __shfl_up(inc, o, 64);
-            if (lane >= o)
-                inc += y;
-        }
-        if (lane == 63)
-            s_wt[w] = inc;
-        __syncthreads();
-        uint64_t run = carry + inc - m, tot = 0;
-        for (int k = 0; k < NWS; k++) {
-            if (k < w)
-                run += s_wt[k];
-            tot += s_wt[k];
-        }
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const uint32_t i = tid * K + k;
-            if (i < n)
-                a.tile_off[c0 + i] = run;
-            run += loc[k];
-        }
-        carry += tot;
-        __syncthreads();
+    for (int k = 0; k < K; k++) {
+        const uint32_t i = c0 + tid * K + k;
+        loc[k] = i < T ? a.tile_count[i] : 0u;
+        m += loc[k];
     }
+    uint64_t inc = m;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o)
+            inc += y;
+    }
+    if (lane == 63)
+        s_wt[w] = inc;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+    for (int k = 0; k < NWS; k++) {
+        if (k < w)
+            before += s_wt[k];
+        tot += s_wt[k];
+    }
+    // publish the chunk total, then the earlier chunks' (wave 0, one per lane)
     if (tid == 0)
-        a.meta[0] = carry;
+        st_coherent_u64(sums + blockIdx.x, (LB_FLAG_AGG << LB_SHIFT) | tot);
+    if (w == 0) {
+        uint64_t carry = 0;
+        for (uint32_t b0 = 0; b0 < blockIdx.x; b0 += 64) {
+            uint64_t v = 0;
+            if (b0 + lane < blockIdx.x) {
+                uint32_t spins = 0;
+                while (((v = ld_coherent_u64(sums + b0 + lane)) >> LB_SHIFT) == 0 && ++spins < LB_SPIN_LIMIT)
+                    __builtin_amdgcn_s_sleep(4);
+            }
+            carry += wave_sum_u64(v & LB_MASK);
+        }
+        if (lane == 0)
+            s_carry = carry;
+    }
+    __syncthreads();
+    uint64_t run = s_carry + before + inc - m;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint32_t i = c0 + tid * K + k;
+        if (i < T)
+            a.tile_off[i] = run;
+        run += loc[k];
+    }
+    if (tid == 0 && blockIdx.x == gridDim.x - 1)
+        a.meta[0] = s_carry + tot;
 }
 
 size_t ol_many_lds_bytes(uint32_t fwords)
@@ -686,7 +712,8 @@ hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
     hipError_t e = launch_many_tiles<false>(a, s);
     if (e != hipSuccess || (!a.out_off && !a.out_id))
         return e;
-    hipLaunchKernelGGL((many_scan_kernel<1024>), dim3(1), dim3(1024), 0, s, a);
+    const uint32_t chunks = (a.num_tiles + 8 * 1024 - 1) / (8 * 1024);
+    hipLaunchKernelGGL((many_scan_kernel<1024>), dim3(chunks), dim3(1024), 0, s, a, a.tile_list + a.num_tiles);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     return launch_many_tiles<true>(a, s);
