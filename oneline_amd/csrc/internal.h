@@ -190,6 +190,65 @@ __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_
     return prefix;
 }
 
+// Status per tile of the prefix scanner's kernels: the tile's count (written
+// by the tile's workgroup, read by the scanner) and its inclusive prefix
+// (written by the scanner, read by the tile's workgroup), all accessed with
+// the read-modify-write atomics.
+constexpr uint32_t SCAN_STATUS_WORDS = 2;                  // u64 words per tile
+constexpr uint32_t SCAN_PREFIX_WORD = 1;                   // offset of the prefix word
+#ifndef SCAN_PREFIX_WINDOWS
+#define SCAN_PREFIX_WINDOWS 8                              // 64-tile windows the prefix scanner reads per poll
+#endif
+
+// The prefix scanner of the persistent tile kernels (one wave of the one
+// workgroup that takes no tiles; the scan and the literal search use it): the
+// count word of tile t holds its line count (flag AGG) once the tile is
+// counted; the scanner reads the count words in tile order, NWS windows of 64
+// per poll, and writes every complete window's inclusive prefixes (flag INCL)
+// to the tiles' prefix words, carrying the running total from window to
+// window.  A window with a tile not counted yet is polled again.  Tiles wait
+// only for their own prefix word, never for a chain of their neighbours'
+// look-backs.
+template <int NWS>
+__device__ __forceinline__ void scan_prefixes(uint64_t *status, uint32_t T, uint32_t *err)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t carry = 0;
+    uint32_t t0 = 0, spins = 0;
+    while (t0 < T) {
+        uint64_t st[NWS];
+#pragma unroll
+        for (int k = 0; k < NWS; k++) {
+            const uint32_t idx = t0 + 64 * k + lane;
+            st[k] = idx < T ? ld_coherent_u64(status + (size_t)idx * SCAN_STATUS_WORDS) : (LB_FLAG_AGG << LB_SHIFT);
+        }
+        uint32_t done = 0;
+#pragma unroll
+        for (int k = 0; k < NWS; k++) {
+            if (done != (uint32_t)k || __ballot((st[k] >> LB_SHIFT) == 0))
+                break;
+            const uint32_t inc = wave_incl_add((uint32_t)(st[k] & LB_MASK));   // a tile has < 2^17 lines
+            const uint32_t idx = t0 + 64 * k + lane;
+            if (idx < T)
+                st_coherent_u64(status + (size_t)idx * SCAN_STATUS_WORDS + SCAN_PREFIX_WORD,
+                                (LB_FLAG_INCL << LB_SHIFT) | (carry + inc));
+            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+            done++;
+        }
+        t0 += 64 * done;
+        if (done == 0) {
+            if (++spins >= LB_SPIN_LIMIT) {
+                if (lane == 0)
+                    atomicOr(err, 2u);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        } else {
+            spins = 0;
+        }
+    }
+}
+
 // Workgroups of kernel f that can be resident at once on the device, for the
 // persistent launches whose look-backs need every workgroup resident: the
 // occupancy API's count per CU, capped by the CU's LDS divided by the
@@ -348,14 +407,6 @@ constexpr uint32_t SCAN_THREADS = 256;
 constexpr uint32_t SCAN_BPT = 128;                          // default bytes per lane (32 KiB tiles)
 constexpr uint32_t SCAN_TILE_MAX = SCAN_THREADS * 256;
 constexpr uint32_t SCAN_HALO = 256;
-// Scan status per tile: its line count (written by the tile's workgroup, read
-// by the prefix scanner) and its inclusive prefix (written by the scanner, read
-// by the tile's workgroup), all accessed with the read-modify-write atomics.
-constexpr uint32_t SCAN_STATUS_WORDS = 2;                  // u64 words per tile
-constexpr uint32_t SCAN_PREFIX_WORD = 1;                   // offset of the prefix word
-#ifndef SCAN_PREFIX_WINDOWS
-#define SCAN_PREFIX_WINDOWS 8                              // 64-tile windows the prefix scanner reads per poll
-#endif
 constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks

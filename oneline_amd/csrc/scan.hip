@@ -348,54 +348,6 @@ __device__ __forceinline__ uint32_t decode_fast(const uint8_t *s_text, uint32_t 
     return slow ? RC_SLOW : rc;
 }
 
-// The prefix scanner (one wave of the one workgroup that takes no tiles): the
-// count word of tile t holds its line count (flag AGG) once the tile is
-// counted; the scanner reads the count words in tile order, NWS windows of 64
-// per poll, and writes every complete window's inclusive prefixes (flag INCL)
-// to the tiles' prefix words, carrying the running total from window to
-// window.  A window with a tile not counted yet is polled again.  Tiles wait
-// only for their own prefix word, never for a chain of their neighbours'
-// look-backs.
-template <int NWS>
-__device__ __forceinline__ void scan_prefixes(uint64_t *status, uint32_t T, uint32_t *err)
-{
-    const uint32_t lane = threadIdx.x & 63;
-    uint64_t carry = 0;
-    uint32_t t0 = 0, spins = 0;
-    while (t0 < T) {
-        uint64_t st[NWS];
-#pragma unroll
-        for (int k = 0; k < NWS; k++) {
-            const uint32_t idx = t0 + 64 * k + lane;
-            st[k] = idx < T ? ld_coherent_u64(status + (size_t)idx * SCAN_STATUS_WORDS) : (LB_FLAG_AGG << LB_SHIFT);
-        }
-        uint32_t done = 0;
-#pragma unroll
-        for (int k = 0; k < NWS; k++) {
-            if (done != (uint32_t)k || __ballot((st[k] >> LB_SHIFT) == 0))
-                break;
-            const uint32_t inc = wave_incl_add((uint32_t)(st[k] & LB_MASK));   // a tile has < 2^17 lines
-            const uint32_t idx = t0 + 64 * k + lane;
-            if (idx < T)
-                st_coherent_u64(status + (size_t)idx * SCAN_STATUS_WORDS + SCAN_PREFIX_WORD,
-                                (LB_FLAG_INCL << LB_SHIFT) | (carry + inc));
-            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-            done++;
-        }
-        t0 += 64 * done;
-        if (done == 0) {
-            if (++spins >= LB_SPIN_LIMIT) {
-                if (lane == 0)
-                    atomicOr(err, 2u);
-                return;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        } else {
-            spins = 0;
-        }
-    }
-}
-
 // One tile = THREADS lanes x 64*Q contiguous bytes.  The tile arrives in
 // registers (coalesced 16-byte vectors), is staged whole in LDS behind a
 // 256-byte halo, and the registers are then refilled with the workgroup's next

@@ -1,9 +1,7 @@
 // search.hip -- single-literal and dictionary search with ordered output.
 //
-// Both kernels share one shape: a workgroup claims the next text tile, stages
-// it (plus a 80-byte halo for patterns running past the tile end) in LDS,
-// finds the matches that START in each lane's 64 bytes, counts them, resolves
-// the tile's output position with a decoupled look-back, and writes its matches
+// Both find the matches that START in each tile, count them, get the tile's
+// output position from the tile counts before it, and write the matches
 // there.  Output is therefore ascending by offset (then by pattern id) without
 // any sort, and a capacity smaller than the total keeps the first matches.
 //
@@ -15,20 +13,6 @@
 // first min(len, 4) bytes; a filter bitmap per key length sits in LDS, the
 // hash tables, entries and pattern bytes are read through L2.
 #include "internal.h"
-
-constexpr int HALO_AFTER = 80;
-
-template <int THREADS, int BPT>
-__device__ __forceinline__ void load_tile(uint8_t *s_text, const uint8_t *text, uint32_t base)
-{
-    const uint4 *src = (const uint4 *)(text + base);
-    uint4 *dst = (uint4 *)s_text;
-#pragma unroll
-    for (int k = 0; k < BPT / 16; k++)
-        dst[k * THREADS + threadIdx.x] = src[k * THREADS + threadIdx.x];
-    if (threadIdx.x < HALO_AFTER / 16)
-        dst[(THREADS * BPT) / 16 + threadIdx.x] = src[(THREADS * BPT) / 16 + threadIdx.x];
-}
 
 // Block-wide exclusive sum of `v` (THREADS/64 waves); also returns the total.
 template <int THREADS>
@@ -57,30 +41,6 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *s_w, ui
     return wb + inc - v;
 }
 
-// Tile prefix via look-back; wave 0 computes it, everyone reads it.
-__device__ __forceinline__ uint64_t tile_prefix(uint64_t *status, uint32_t t, uint64_t total,
-                                                uint64_t *s_prefix, uint32_t *err)
-{
-    if ((threadIdx.x >> 6) == 0) {
-        const int lane = threadIdx.x & 63;
-        uint64_t prefix = 0;
-        if (t == 0) {
-            if (lane == 0)
-                st_relaxed_u64(status, (LB_FLAG_INCL << LB_SHIFT) | total);
-        } else {
-            if (lane == 0)
-                st_relaxed_u64(status + (size_t)t * LB_STRIDE, (LB_FLAG_AGG << LB_SHIFT) | total);
-            prefix = lookback_sum(status, t, err);
-            if (lane == 0)
-                st_relaxed_u64(status + (size_t)t * LB_STRIDE, (LB_FLAG_INCL << LB_SHIFT) | (prefix + total));
-        }
-        if (lane == 0)
-            *s_prefix = prefix;
-    }
-    __syncthreads();
-    return *s_prefix;
-}
-
 uint32_t ol_coresident(const void *f, int threads, size_t dyn_lds)
 {
     int dev = 0, ncu = 0, per_cu = 0, lds_cu = 0;
@@ -103,89 +63,206 @@ uint32_t ol_coresident(const void *f, int threads, size_t dyn_lds)
 static uint32_t max_grid(const void *f, int threads, size_t lds) { return ol_coresident(f, threads, lds); }
 
 // ---- single literal -----------------------------------------------------------
-// Persistent: tile t = blockIdx + k * gridDim (all workgroups co-resident, each
-// walking its tiles in increasing order, so look-backs always make progress).
-template <int THREADS, int BPT>
+// Persistent, cooperative launch: the last workgroup is the prefix scanner
+// (scan_prefixes); the others take tiles t = blockIdx + k * workers in
+// increasing order.  No LDS staging: lane l holds the tile's 16-byte vectors
+// k * 256 + l (k < 4, coalesced loads) plus the first word of the vector after
+// each, so every 4-byte window of its 64 positions is an alignbyte of its own
+// registers.  Windows that equal the pattern's first (up to) four bytes are
+// verified against the rest from global memory (an L2 hit).  Matches are
+// ordered by vector (k major), counted per vector in 16-bit fields, and kept
+// as one 64-bit mask per lane until the next iteration, when the scanner has
+// (almost always) produced the tile's prefix; then they are written.  The next
+// tile is loaded while the current one is processed.
+template <int THREADS>
 __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
 {
-    constexpr int TILE = THREADS * BPT;
-    static_assert(BPT == 64, "64-bit lane masks");
-    __shared__ __align__(16) uint8_t s_text[TILE + HALO_AFTER];
-    __shared__ uint32_t s_w[THREADS / 64];
+    constexpr int NV = 4;                     // vectors per lane
+    constexpr int TILE = THREADS * NV * 16;
+    constexpr int NW = THREADS / 64;
+    __shared__ uint32_t s_w[2][NW];
     __shared__ uint64_t s_prefix;
 
-    const int tid = threadIdx.x;
-    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
-        const uint32_t base = t * TILE;
-        load_tile<THREADS, BPT>(s_text, a.text, base);
-        __syncthreads();
-
-        const uint32_t lo = tid * BPT;
-        const uint32_t g0 = base + lo;
-        // positions p with g0 + p + m <= n
-        uint32_t lim = 0;
-        if ((uint64_t)g0 + a.m <= a.n)
-            lim = a.n - a.m - g0 + 1;
-        if (lim > BPT)
-            lim = BPT;
-        uint64_t match = 0;
-        if (lim) {
-            const uint32_t *wsrc = (const uint32_t *)(s_text + lo);
-            uint32_t w[BPT / 4 + 1];
-#pragma unroll
-            for (int i = 0; i < BPT / 4 + 1; i++)
-                w[i] = wsrc[i];
-            uint64_t cand = 0;
-#pragma unroll
-            for (int p = 0; p < BPT; p++) {
-                uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(w[(p >> 2) + 1], w[p >> 2], p & 3)
-                                       : w[p >> 2];
-                cand |= (uint64_t)((win & a.win_mask) == a.win) << p;
-            }
-            if (lim < BPT)
-                cand &= (1ull << lim) - 1;
-            while (cand) {
-                int p = __builtin_ctzll(cand);
-                cand &= cand - 1;
-                bool ok = true;
-                for (uint32_t i = 4; i < a.m; i++) {
-                    if (s_text[lo + p + i] != a.pat[i]) {
-                        ok = false;
-                        break;
-                    }
-                }
-                if (ok)
-                    match |= 1ull << p;
-            }
-        }
-        uint32_t total;
-        const uint32_t excl = block_excl_sum<THREADS>((uint32_t)__popcll(match), s_w, total);
-        const uint64_t prefix = tile_prefix(a.status, t, total, &s_prefix, a.lb_err);
-        if (tid == 0 && t == a.num_tiles - 1)
-            a.meta[0] = prefix + total;
-        uint64_t r = prefix + excl;
-        while (match) {
-            int p = __builtin_ctzll(match);
-            match &= match - 1;
-            if (r < a.cap) {
-                if (a.out64) a.out64[r] = g0 + p;
-                if (a.out32) a.out32[r] = g0 + p;
-            }
-            r++;
-        }
-        __syncthreads();   // LDS is reused by the next tile
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t G = gridDim.x - 1;   // workgroups that take tiles
+    const uint32_t T = a.num_tiles;
+    if (blockIdx.x == G) {
+        if (wid == 0)
+            scan_prefixes<SCAN_PREFIX_WINDOWS>(a.status, T, a.lb_err);
+        return;
     }
+    uint4 v[NV];
+    uint32_t nx[NV];
+    auto load = [&](uint32_t tt) {
+        const uint8_t *src = a.text + (size_t)tt * TILE;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            v[k] = ((const uint4 *)src)[k * THREADS + tid];
+            nx[k] = *(const uint32_t *)(src + (size_t)(k * THREADS + tid + 1) * 16);   // text is padded
+        }
+    };
+    const uint32_t mlen = a.m;
+    const uint32_t pat4 = a.win, mask4 = a.win_mask;
+    // the rest of a candidate at text position gp: pattern bytes 4 .. m
+    auto verify = [&](uint32_t gp) -> bool {
+        for (uint32_t i = 4; i < mlen; i += 4) {
+            const uint32_t x = gp + i;
+            const uint32_t *q = (const uint32_t *)(a.text + (x & ~3u));
+            const uint32_t sx = x & 3u;
+            const uint32_t w = sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
+            uint32_t pw = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                pw |= (uint32_t)a.pat[(i + b) & 63] << (8 * b);
+            const uint32_t nb = mlen - i;
+            const uint32_t msk = nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u);
+            if ((w ^ pw) & msk)
+                return false;
+        }
+        return true;
+    };
+    // the previous tile: its matches, waiting for its prefix
+    bool pend = false;
+    uint32_t p_t = 0, p_total = 0;
+    uint32_t p_off[NV];
+    uint64_t p_match = 0;
+    auto flush = [&]() {   // writes the previous tile's matches (all threads)
+        if (wid == 0 && lane == 0) {
+            uint64_t w = 0;
+            uint32_t spins = 0;
+            while (((w = ld_coherent_u64(a.status + (size_t)p_t * SCAN_STATUS_WORDS + SCAN_PREFIX_WORD)) >> LB_SHIFT) !=
+                       LB_FLAG_INCL &&
+                   ++spins < LB_SPIN_LIMIT)
+                __builtin_amdgcn_s_sleep(8);
+            if (spins >= LB_SPIN_LIMIT) {
+                atomicOr(a.lb_err, 1u);
+                w = (LB_FLAG_INCL << LB_SHIFT) | p_total;
+            }
+            s_prefix = (w & LB_MASK) - p_total;
+        }
+        __syncthreads();
+        const uint64_t prefix = s_prefix;
+        if (tid == 0 && p_t == T - 1)
+            a.meta[0] = prefix + p_total;
+        const uint32_t pbase = p_t * TILE;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            uint32_t m = (uint32_t)(p_match >> (16 * k)) & 0xFFFFu;
+            uint64_t r = prefix + p_off[k];
+            while (m) {
+                const uint32_t bit = (uint32_t)__builtin_ctz(m);
+                m &= m - 1;
+                const uint32_t gp = pbase + (uint32_t)(k * THREADS + tid) * 16 + bit;
+                if (r < a.cap) {
+                    if (a.out64) a.out64[r] = gp;
+                    if (a.out32) a.out32[r] = gp;
+                }
+                r++;
+            }
+        }
+    };
+
+    uint32_t t = blockIdx.x;
+    if (t < T)
+        load(t);
+    for (; t < T; t += G) {
+        const uint32_t base = t * TILE;
+        uint4 cv[NV];
+        uint32_t cn[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            cv[k] = v[k];
+            cn[k] = nx[k];
+        }
+        if (t + G < T)
+            load(t + G);
+        // positions gp with gp + m <= n may match
+        const uint32_t last_start = a.n >= mlen ? a.n - mlen : 0u;   // inclusive, when n >= m
+        uint64_t match = 0;
+        uint32_t cnt01 = 0, cnt23 = 0;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const uint32_t W[5] = {cv[k].x, cv[k].y, cv[k].z, cv[k].w, cn[k]};
+            const uint32_t g = base + (uint32_t)(k * THREADS + tid) * 16;
+            uint32_t hits = 0;
+#pragma unroll
+            for (int p = 0; p < 16; p++) {
+                const uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3)
+                                             : W[p >> 2];
+                hits |= (uint32_t)((win & mask4) == pat4) << p;
+            }
+            // positions past the last possible start
+            if (a.n < mlen || g > last_start)
+                hits = 0;
+            else if (last_start - g < 15)
+                hits &= (2u << (last_start - g)) - 1u;
+            if (hits && mlen > 4) {
+                uint32_t q = hits;
+                while (q) {
+                    const uint32_t bit = (uint32_t)__builtin_ctz(q);
+                    q &= q - 1;
+                    if (!verify(g + bit))
+                        hits &= ~(1u << bit);
+                }
+            }
+            match |= (uint64_t)hits << (16 * k);
+            const uint32_t c = (uint32_t)__builtin_popcount(hits);
+            if (k == 0) cnt01 |= c;
+            if (k == 1) cnt01 |= c << 16;
+            if (k == 2) cnt23 |= c;
+            if (k == 3) cnt23 |= c << 16;
+        }
+        // offsets of each vector's matches in the tile (vector order k major):
+        // two 16-bit counts per word, block-scanned together
+        const uint32_t i01 = wave_incl_add(cnt01), i23 = wave_incl_add(cnt23);
+        if (lane == 63) {
+            s_w[0][wid] = i01;
+            s_w[1][wid] = i23;
+        }
+        __syncthreads();
+        uint32_t b01 = 0, b23 = 0, t01 = 0, t23 = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            if (w < wid) {
+                b01 += s_w[0][w];
+                b23 += s_w[1][w];
+            }
+            t01 += s_w[0][w];
+            t23 += s_w[1][w];
+        }
+        const uint32_t e01 = b01 + i01 - cnt01, e23 = b23 + i23 - cnt23;
+        const uint32_t T0 = t01 & 0xFFFFu, T1 = t01 >> 16, T2 = t23 & 0xFFFFu, T3 = t23 >> 16;
+        const uint32_t total = T0 + T1 + T2 + T3;
+        if (tid == 0)
+            st_coherent_u64(a.status + (size_t)t * SCAN_STATUS_WORDS, (LB_FLAG_AGG << LB_SHIFT) | total);
+        if (pend)
+            flush();
+        pend = true;
+        p_t = t;
+        p_total = total;
+        p_off[0] = e01 & 0xFFFFu;
+        p_off[1] = T0 + (e01 >> 16);
+        p_off[2] = T0 + T1 + (e23 & 0xFFFFu);
+        p_off[3] = T0 + T1 + T2 + (e23 >> 16);
+        p_match = match;
+        __syncthreads();   // s_w and s_prefix are reused
+    }
+    if (pend)
+        flush();
 }
 
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 {
     static uint32_t cap = 0;   // co-resident limit (same device for the process)
-    const void *f = (const void *)search_kernel<SEARCH_THREADS, SEARCH_BPT>;
+    const void *f = (const void *)search_kernel<SEARCH_THREADS>;
     if (!cap)
         cap = max_grid(f, SEARCH_THREADS, 0);
     if (!cap)
         return hipErrorInvalidValue;
-    uint32_t grid = a.num_tiles < cap ? a.num_tiles : cap;
+    // workgroups taking tiles plus the prefix scanner
+    uint32_t grid = a.num_tiles + 1 < cap ? a.num_tiles + 1 : cap;
+    if (grid < 2)
+        return hipErrorInvalidValue;
     SearchArgs args = a;
     void *params[] = {&args};
     return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SEARCH_THREADS), params, 0, s);
