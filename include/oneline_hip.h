@@ -151,6 +151,30 @@ int ol_search_raw(ol_ctx *ctx, const uint8_t *text, size_t text_len, const uint8
 int ol_search_device(ol_ctx *ctx, const ol_doc *doc, const uint8_t *pattern, size_t len,
                      uint64_t *d_offsets, size_t cap, uint64_t *total);
 
+/* ---- kind / depth queries ------------------------------------------------------------ */
+/* The line filter of oneline-gpu (filter_nodes, crates/oneline-gpu/src/shader.wgsl:152-179):
+ * a line matches when (kind == OL_QUERY_ANY_KIND or its NodeKind == kind) and
+ * (depth == OL_QUERY_ANY_DEPTH or its FlatNode depth == depth).  As in the
+ * reference, kind 0 means any kind (NodeKind::Blank is 0, so blank lines cannot
+ * be selected by kind alone).  total = every matching line; indices = the first
+ * min(total, max_matches) of them in ascending order (library-allocated, release
+ * with ol_free).  The reference returns an arbitrary subset when truncating; this
+ * one is the first lines. */
+#define OL_QUERY_ANY_KIND 0u
+#define OL_QUERY_ANY_DEPTH 0xFFFFFFFFu
+/* GpuParser::query_document (crates/oneline-gpu/src/document.rs:94). */
+int ol_query_document(ol_ctx *ctx, const ol_doc *doc, uint32_t kind, uint32_t depth, size_t max_matches,
+                      uint32_t *total, uint32_t **indices, size_t *n_indices);
+/* GpuParser::query_subtree (crates/oneline-gpu/src/tree.rs:295): lines
+ * root .. subtree_end[root]; OL_ERR_INVALID when root is out of range or the
+ * tree does not belong to the document. */
+int ol_query_subtree(ol_ctx *ctx, const ol_doc *doc, const ol_tree *tree, size_t root, uint32_t kind,
+                     uint32_t depth, size_t max_matches, uint32_t *total, uint32_t **indices,
+                     size_t *n_indices);
+/* GpuParser::query_raw (crates/oneline-gpu/src/query.rs:232): uploads, scans, queries. */
+int ol_query_raw(ol_ctx *ctx, const uint8_t *text, size_t len, uint32_t kind, uint32_t depth,
+                 size_t max_matches, uint32_t *total, uint32_t **indices, size_t *n_indices);
+
 /* ---- dictionary search ------------------------------------------------------------ */
 /* GpuParser::compile_patterns (crates/oneline-gpu/src/many.rs:158).  `doc` is
  * accepted for signature parity and may be NULL (the set is not bound to it). */

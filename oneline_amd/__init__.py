@@ -92,6 +92,10 @@ def load_library() -> ctypes.CDLL:
         "ol_search_document": [P, P, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_raw": [P, P, S, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_device": [P, P, P, S, P, S, U64P],
+        "ol_query_document": [P, P, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_query_subtree": [P, P, P, S, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P),
+                             ctypes.POINTER(S)],
+        "ol_query_raw": [P, P, S, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_dict_compile": [P, P, P, P, S, ctypes.POINTER(P)],
         "ol_dict_free": [P],
         "ol_dict_len": [P],
@@ -205,6 +209,18 @@ class TreeLinks:
 class SearchResult:
     total_matches: int
     offsets: np.ndarray
+
+
+@dataclass
+class QueryResult:
+    """QueryResult (crates/oneline-gpu/src/query.rs:11): every matching line
+    counted; the first min(total, max_matches) line indices, ascending."""
+    total_matches: int
+    indices: np.ndarray
+
+
+QUERY_ANY_KIND = 0             # kind 0 matches every kind (NodeKind.Blank is 0)
+QUERY_ANY_DEPTH = 0xFFFFFFFF
 
 
 @dataclass
@@ -465,6 +481,40 @@ class GpuParser:
         _check(_lib.ol_search_raw(self._ctx, _ptr(t), t.size, _ptr(pat), pat.size, max_matches,
                                   ctypes.byref(tot), ctypes.byref(p), ctypes.byref(n)))
         return SearchResult(tot.value, _take(p, n.value, np.uint32))
+
+    # -- kind / depth queries (GpuParser::query_document / query_subtree / query_raw) ---
+    @staticmethod
+    def _query_args(kind, depth):
+        k = QUERY_ANY_KIND if kind is None else int(kind)
+        dp = QUERY_ANY_DEPTH if depth is None else int(depth)
+        return k, dp
+
+    def query_document(self, doc: GpuDocument, kind=None, depth=None, max_matches: int = 1 << 20) -> QueryResult:
+        """Lines of `doc` with NodeKind `kind` and FlatNode depth `depth` (None:
+        any); total plus the first min(total, max_matches) indices, ascending."""
+        k, dp = self._query_args(kind, depth)
+        tot, p, n = ctypes.c_uint32(), ctypes.c_void_p(), ctypes.c_size_t()
+        _check(_lib.ol_query_document(self._ctx, doc._h, k, dp, max_matches, ctypes.byref(tot), ctypes.byref(p),
+                                      ctypes.byref(n)))
+        return QueryResult(tot.value, _take(p, n.value, np.uint32))
+
+    def query_subtree(self, doc: GpuDocument, tree: "GpuTree", root: int, kind=None, depth=None,
+                      max_matches: int = 1 << 20) -> QueryResult:
+        """query_document restricted to the lines root .. subtree_end[root]."""
+        k, dp = self._query_args(kind, depth)
+        tot, p, n = ctypes.c_uint32(), ctypes.c_void_p(), ctypes.c_size_t()
+        _check(_lib.ol_query_subtree(self._ctx, doc._h, tree._h, root, k, dp, max_matches, ctypes.byref(tot),
+                                     ctypes.byref(p), ctypes.byref(n)))
+        return QueryResult(tot.value, _take(p, n.value, np.uint32))
+
+    def query_raw(self, text, kind=None, depth=None, max_matches: int = 1 << 20) -> QueryResult:
+        """Upload, scan and query in one call."""
+        t = as_bytes_array(text)
+        k, dp = self._query_args(kind, depth)
+        tot, p, n = ctypes.c_uint32(), ctypes.c_void_p(), ctypes.c_size_t()
+        _check(_lib.ol_query_raw(self._ctx, _ptr(t), t.size, k, dp, max_matches, ctypes.byref(tot),
+                                 ctypes.byref(p), ctypes.byref(n)))
+        return QueryResult(tot.value, _take(p, n.value, np.uint32))
 
     # -- dictionary search --------------------------------------------------------------
     def compile_patterns(self, doc: Optional[GpuDocument], patterns: Sequence[bytes]) -> GpuPatternSet:

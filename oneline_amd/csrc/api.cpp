@@ -502,6 +502,74 @@ hipError_t set_candidate_store(ol_ctx *c, ManyArgs &a, uint32_t tiles, bool writ
     return hipSuccess;
 }
 
+// Kind / depth query of lines [r0, r1) of a resident document: the first
+// min(total, max_matches) matching line indices, ascending, library-allocated.
+int run_query(ol_ctx *c, const ol_doc *d, uint32_t r0, uint32_t r1, uint32_t kind, uint32_t depth,
+              size_t max_matches, uint32_t *total, uint32_t **indices, size_t *n_indices)
+{
+    if (!total || !indices || !n_indices)
+        return fail(OL_ERR_INVALID, "null argument");
+    if (max_matches > 0xFFFFFFFFull)
+        return fail(OL_ERR_TOO_MANY, "max_matches " + std::to_string(max_matches) + " does not fit in u32");
+    *total = 0;
+    *indices = nullptr;
+    *n_indices = 0;
+    if (r1 <= r0)
+        return OL_OK;
+    const uint32_t blocks = ol_query_blocks(r1 - r0);
+    HIPCHK(ensure_scratch(c, blocks));
+    HIPCHK(clear_scratch(c, blocks));
+    const uint64_t cap = std::min<uint64_t>(max_matches, r1 - r0);
+    DevBuf out;
+    HIPCHK(out.ensure((cap ? cap : 1) * 4));
+    QueryArgs a{};
+    a.text = d->text;
+    a.n = (uint32_t)d->n;
+    a.lines = d->lines;
+    a.offsets = d->offsets;
+    a.classes = d->classes;
+    a.keys = d->keys;
+    a.nodes = d->nodes;
+    a.kind = kind;
+    a.depth = depth;
+    a.r0 = r0;
+    a.r1 = r1;
+    a.block_count = (uint32_t *)c->status();
+    a.block_off = c->status() + blocks;
+    a.out = cap ? (uint32_t *)out.p : nullptr;
+    a.cap = cap;
+    {
+        Prof p(c, "query");
+        const hipError_t e = ol_launch_query(a, c->meta(), c->stream);
+        if (e != hipSuccess) {
+            out.release();
+            return fail(OL_ERR_DEVICE, std::string("query: ") + hipGetErrorString(e));
+        }
+    }
+    uint64_t tot = 0;
+    int rc = read_meta(c, &tot);
+    if (rc) {
+        out.release();
+        return rc;
+    }
+    const uint64_t keep = std::min<uint64_t>(tot, cap);
+    uint32_t *h = (uint32_t *)std::malloc((keep ? keep : 1) * 4);
+    if (!h) {
+        out.release();
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    }
+    if (keep && hipMemcpy(h, out.p, keep * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        std::free(h);
+        out.release();
+        return fail(OL_ERR_DEVICE, "query download failed");
+    }
+    out.release();
+    *total = (uint32_t)tot;
+    *indices = h;
+    *n_indices = (size_t)keep;
+    return OL_OK;
+}
+
 int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, uint32_t *out_id,
              uint64_t cap, uint64_t *total)
 {
@@ -900,6 +968,50 @@ int ol_search_raw(ol_ctx *c, const uint8_t *text, size_t text_len, const uint8_t
     (void)hipFree(tmp.text);
     tmp.text = nullptr;
     return rc;
+}
+
+int ol_query_document(ol_ctx *c, const ol_doc *d, uint32_t kind, uint32_t depth, size_t max_matches,
+                      uint32_t *total, uint32_t **indices, size_t *n_indices)
+{
+    if (!c || !d)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return run_query(c, d, 0, (uint32_t)d->lines, kind, depth, max_matches, total, indices, n_indices);
+}
+
+int ol_query_subtree(ol_ctx *c, const ol_doc *d, const ol_tree *t, size_t root, uint32_t kind, uint32_t depth,
+                     size_t max_matches, uint32_t *total, uint32_t **indices, size_t *n_indices)
+{
+    if (!c || !d || !t)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (root >= t->lines || t->lines != d->lines)
+        return fail(OL_ERR_INVALID, "line " + std::to_string(root) + " is out of range (document has " +
+                                        std::to_string(d->lines) + " lines, tree has " + std::to_string(t->lines) +
+                                        ")");
+    uint32_t end = 0;
+    HIPCHK(hipMemcpyAsync(&end, t->se + root, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return run_query(c, d, (uint32_t)root, end, kind, depth, max_matches, total, indices, n_indices);
+}
+
+int ol_query_raw(ol_ctx *c, const uint8_t *text, size_t len, uint32_t kind, uint32_t depth, size_t max_matches,
+                 uint32_t *total, uint32_t **indices, size_t *n_indices)
+{
+    if (!c || (!text && len))
+        return fail(OL_ERR_INVALID, "null argument");
+    if (!total || !indices || !n_indices)
+        return fail(OL_ERR_INVALID, "null argument");
+    ol_doc *d = nullptr;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        int rc = upload_doc(c, text, len, false, &d);
+        if (rc)
+            return rc;
+        rc = run_query(c, d, 0, (uint32_t)d->lines, kind, depth, max_matches, total, indices, n_indices);
+        ol_doc_free(d);
+        return rc;
+    }
 }
 
 int ol_search_device(ol_ctx *c, const ol_doc *d, const uint8_t *pattern, size_t len, uint64_t *d_offsets,

@@ -411,6 +411,22 @@ constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks
 constexpr uint32_t SEARCH_TILE = SEARCH_THREADS * SEARCH_BPT;
+struct QueryArgs {
+    const uint8_t *text;
+    uint32_t n;
+    uint64_t lines;
+    const uint32_t *offsets;
+    const uint8_t *classes, *keys;
+    const ol_node_dev *nodes;   // nullable
+    uint32_t kind, depth;
+    uint32_t r0, r1;            // line range
+    uint32_t *block_count;      // [blocks]
+    uint64_t *block_off;        // [blocks]
+    uint32_t *out;              // [cap] (nullable)
+    uint64_t cap;
+};
+hipError_t ol_launch_query(const QueryArgs &a, uint64_t *total, hipStream_t s);
+uint32_t ol_query_blocks(uint32_t lines);   // count blocks of a query over `lines` lines
 hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t *offsets,
                                  uint64_t lines, ol_node_dev *nodes, hipStream_t s);
 size_t ol_many_lds_bytes(uint32_t fwords);

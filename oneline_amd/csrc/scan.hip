@@ -701,9 +701,10 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
                 a.nodes[r] = nd;
         };
         if (staged) {
-            // Decode into LDS.  Line i goes to thread (i + 64) mod THREADS, so
-            // line 0 is wave 1's, which has just found its start.
-            for (uint32_t i = (tid + THREADS - 64) % THREADS; i < total; i += THREADS) {
+            // Decode into LDS, by waves 1.. (wave 0 resolves the prefix): line i
+            // goes to thread 64 + i mod (THREADS - 64), so line 0 is wave 1's,
+            // which has just found its start.
+            for (uint32_t i = (uint32_t)tid - 64u; tid >= 64 && i < total; i += THREADS - 64) {
                 const uint32_t e = base + s_le[b][i];
                 const uint32_t s = i ? base + s_le[b][i - 1] + 1u : first_w1;
                 uint32_t rc = FIX;
@@ -862,3 +863,137 @@ hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t
                        offsets, lines, nodes);
     return hipGetLastError();
 }
+
+// ---- kind / depth queries over a resident document ---------------------------
+// GpuParser::query_document / query_subtree (crates/oneline-gpu/src/document.rs:94,
+// crates/oneline-gpu/src/tree.rs:295) with the predicate of filter_nodes
+// (crates/oneline-gpu/src/shader.wgsl:152-179): kind 0 matches every kind,
+// depth 0xFFFFFFFF every depth; otherwise node.kind == kind and node.depth ==
+// depth.  The matching line indices of [r0, r1) are written in ascending
+// order: a count pass (per block of QUERY_BLOCK lines), an exclusive scan of
+// the block counts, and a write pass that recomputes the predicate.  A line's
+// depth is its tree key for content lines; blank lines (key 255) and lines at
+// depth >= 254 are parsed again from the text when a depth is asked for (or
+// read from the FlatNodes when the document keeps them).
+constexpr uint32_t QUERY_THREADS = 256, QUERY_PER_THREAD = 16;
+constexpr uint32_t QUERY_BLOCK = QUERY_THREADS * QUERY_PER_THREAD;
+
+__device__ __forceinline__ bool query_match(const QueryArgs &a, uint32_t i)
+{
+    if (a.kind != 0 && a.classes[i] != a.kind)
+        return false;
+    if (a.depth == 0xFFFFFFFFu)
+        return true;
+    uint32_t depth;
+    if (a.nodes) {
+        depth = a.nodes[i].depth;
+    } else {
+        const uint32_t key = a.keys[i];
+        if (key < KEY_DEEP) {
+            depth = key;
+        } else {   // blank or very deep: the line's own depth
+            const uint32_t s = a.offsets[i];
+            const uint32_t e = (i + 1 < a.lines) ? a.offsets[i + 1] - 1 : a.n - 1;
+            depth = parse_span(GlobalSrc{a.text}, s, e).depth;
+        }
+    }
+    return depth == a.depth;
+}
+
+// The thread's QUERY_PER_THREAD consecutive lines of its block: match bits.
+__device__ __forceinline__ uint32_t query_bits(const QueryArgs &a, uint32_t blk)
+{
+    const uint32_t i0 = a.r0 + blk * QUERY_BLOCK + threadIdx.x * QUERY_PER_THREAD;
+    uint32_t bits = 0;
+    for (uint32_t k = 0; k < QUERY_PER_THREAD; k++) {
+        const uint32_t i = i0 + k;
+        if (i < a.r1 && query_match(a, i))
+            bits |= 1u << k;
+    }
+    return bits;
+}
+
+__device__ __forceinline__ uint32_t query_block_excl(uint32_t v, uint32_t &total)
+{
+    __shared__ uint32_t s_w[QUERY_THREADS / 64];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_add(v);
+    if (lane == 63)
+        s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (uint32_t k = 0; k < QUERY_THREADS / 64; k++) {
+        before += k < w ? s_w[k] : 0u;
+        total += s_w[k];
+    }
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(QUERY_THREADS) void query_count_kernel(QueryArgs a)
+{
+    const uint32_t bits = query_bits(a, blockIdx.x);
+    uint32_t total;
+    (void)query_block_excl((uint32_t)__builtin_popcount(bits), total);
+    if (threadIdx.x == 0)
+        a.block_count[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(QUERY_THREADS) void query_write_kernel(QueryArgs a)
+{
+    const uint32_t bits = query_bits(a, blockIdx.x);
+    uint32_t total;
+    const uint32_t ex = query_block_excl((uint32_t)__builtin_popcount(bits), total);
+    uint64_t r = a.block_off[blockIdx.x] + ex;
+    const uint32_t i0 = a.r0 + blockIdx.x * QUERY_BLOCK + threadIdx.x * QUERY_PER_THREAD;
+    uint32_t m = bits;
+    while (m) {
+        const uint32_t k = (uint32_t)__builtin_ctz(m);
+        m &= m - 1;
+        if (r < a.cap)
+            a.out[r] = i0 + k;
+        r++;
+    }
+}
+
+// Exclusive scan of the block counts (one workgroup, chunks of 1024).
+__global__ __launch_bounds__(1024) void query_scan_kernel(QueryArgs a, uint32_t blocks, uint64_t *total)
+{
+    __shared__ uint32_t s_w[16];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t carry = 0;
+    for (uint32_t c0 = 0; c0 < blocks; c0 += 1024) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < blocks ? a.block_count[i] : 0u;
+        const uint32_t inc = wave_incl_add(v);
+        if (lane == 63)
+            s_w[w] = inc;
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (uint32_t k = 0; k < 16; k++) {
+            before += k < w ? s_w[k] : 0u;
+            tot += s_w[k];
+        }
+        if (i < blocks)
+            a.block_off[i] = carry + before + inc - v;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *total = carry;
+}
+
+// Count, scan and (when `a.out`) write; *total (device) gets the match count.
+hipError_t ol_launch_query(const QueryArgs &a, uint64_t *total, hipStream_t s)
+{
+    if (a.r1 <= a.r0)
+        return hipSuccess;
+    const uint32_t blocks = (a.r1 - a.r0 + QUERY_BLOCK - 1) / QUERY_BLOCK;
+    hipLaunchKernelGGL(query_count_kernel, dim3(blocks), dim3(QUERY_THREADS), 0, s, a);
+    hipLaunchKernelGGL(query_scan_kernel, dim3(1), dim3(1024), 0, s, a, blocks, total);
+    if (a.out && a.cap)
+        hipLaunchKernelGGL(query_write_kernel, dim3(blocks), dim3(QUERY_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+uint32_t ol_query_blocks(uint32_t lines) { return (lines + QUERY_BLOCK - 1) / QUERY_BLOCK; }

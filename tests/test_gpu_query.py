@@ -1,0 +1,84 @@
+"""Kind / depth queries (ol_query_document, ol_query_subtree, ol_query_raw)
+against the oracle's nodes: the predicate of filter_nodes
+(crates/oneline-gpu/src/shader.wgsl:152-179), lines in ascending order, the
+true total kept when the indices are truncated."""
+import numpy as np
+import pytest
+
+import oneline_amd as ol
+from oneline_amd import datagen
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def want(nodes, kind, depth, r0=0, r1=None):
+    r1 = nodes.size if r1 is None else r1
+    sel = np.ones(nodes.size, bool)
+    if kind:
+        sel &= nodes["kind"] == kind
+    if depth is not None:
+        sel &= nodes["depth"] == depth
+    idx = np.flatnonzero(sel)
+    return idx[(idx >= r0) & (idx < r1)].astype(np.uint32)
+
+
+def mixed_text():
+    body = datagen.document(1 << 20, seed=61, crlf=20, empty=30).tobytes()
+    extra = (b"| | \n|\t|  \n| | | \r\n" + b"|" * 300 + b" deep\n" + b"|" * 260 + b" \n"
+             + b"= h\n:c\n@e\n- l\n-. o\n// c\n-x\n/x\n")
+    return extra + body + extra
+
+
+@pytest.mark.parametrize("nodes_kept", [True, False])
+def test_query_document_against_oracle(parser, nodes_kept):
+    text = mixed_text()
+    _, nodes = orc.parse(np.frombuffer(text, np.uint8))
+    doc = parser.upload(text, nodes=nodes_kept)
+    for kind in (None, 1, 2, 4, 6, 7):
+        for depth in (None, 0, 1, 2, 3, 7, 300, 260):
+            got = parser.query_document(doc, kind, depth, max_matches=1 << 22)
+            w = want(nodes, kind, depth)
+            assert got.total_matches == w.size, (kind, depth)
+            assert np.array_equal(got.indices, w), (kind, depth)
+
+
+def test_query_truncation_keeps_total(parser):
+    text = mixed_text()
+    _, nodes = orc.parse(np.frombuffer(text, np.uint8))
+    doc = parser.upload(text, nodes=False)
+    w = want(nodes, 2, None)
+    for m in (0, 1, 7, w.size - 1, w.size, w.size + 5):
+        got = parser.query_document(doc, 2, None, max_matches=m)
+        assert got.total_matches == w.size
+        assert np.array_equal(got.indices, w[: min(m, w.size)])
+
+
+def test_query_subtree(parser):
+    text = datagen.document(256 << 10, seed=62, crlf=10, empty=40).tobytes()
+    _, nodes = orc.parse(np.frombuffer(text, np.uint8))
+    parent, se, _, _ = orc.links(nodes)
+    doc = parser.upload(text, nodes=False)
+    tree = parser.build_tree(doc)
+    rng = np.random.default_rng(3)
+    roots = list(rng.integers(0, nodes.size, 40)) + [0, nodes.size - 1]
+    roots += [int(i) for i in np.flatnonzero(nodes["kind"] == 0)[:3]]   # blank roots: the line alone
+    for root in roots:
+        for kind, depth in ((None, None), (4, None), (None, int(nodes["depth"][root]) + 1)):
+            got = parser.query_subtree(doc, tree, int(root), kind, depth)
+            w = want(nodes, kind, depth, int(root), int(se[root]))
+            assert got.total_matches == w.size and np.array_equal(got.indices, w), (root, kind, depth)
+    with pytest.raises(ol.OnelineError):
+        parser.query_subtree(doc, tree, nodes.size, None, None)
+
+
+def test_query_raw_and_empty(parser):
+    text = mixed_text()
+    doc = parser.upload(text, nodes=True)
+    a = parser.query_document(doc, 4, 2)
+    b = parser.query_raw(text, 4, 2)
+    assert a.total_matches == b.total_matches and np.array_equal(a.indices, b.indices)
+    e = parser.query_raw(b"", None, None)
+    assert e.total_matches == 0 and e.indices.size == 0
+    one = parser.query_raw(b"| | x", None, 2)
+    assert one.total_matches == 1 and list(one.indices) == [0]
