@@ -151,6 +151,18 @@ int ol_search_raw(ol_ctx *ctx, const uint8_t *text, size_t text_len, const uint8
 int ol_search_device(ol_ctx *ctx, const ol_doc *doc, const uint8_t *pattern, size_t len,
                      uint64_t *d_offsets, size_t cap, uint64_t *total);
 
+/* ---- .olb files ------------------------------------------------------------------------ */
+/* A parsed document on disk, the ".olb" container the reference specifies
+ * (.agents/tasks/phase4-zero-copy-binary-serialization.md:13-36; not built there):
+ * a 64-byte header (magic "ONELINE\0", version 1, flags, text_len, line_count,
+ * section offsets, CRC-32 of the sections), the text, the 16-byte FlatNodes and
+ * the u32 line offsets, each on a 64-byte boundary.  Loading maps the file and
+ * uploads text, offsets and nodes without scanning again.  No counterpart
+ * entry point in the reference. */
+int ol_doc_save(ol_ctx *ctx, const ol_doc *doc, const char *path);
+/* verify != 0: check the CRC (OL_ERR_INVALID on mismatch, bad magic, version or sizes). */
+int ol_doc_load(ol_ctx *ctx, const char *path, int verify, ol_doc **out);
+
 /* ---- kind / depth queries ------------------------------------------------------------ */
 /* The line filter of oneline-gpu (filter_nodes, crates/oneline-gpu/src/shader.wgsl:152-179):
  * a line matches when (kind == OL_QUERY_ANY_KIND or its NodeKind == kind) and

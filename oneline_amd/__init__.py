@@ -92,6 +92,8 @@ def load_library() -> ctypes.CDLL:
         "ol_search_document": [P, P, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_raw": [P, P, S, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_device": [P, P, P, S, P, S, U64P],
+        "ol_doc_save": [P, P, ctypes.c_char_p],
+        "ol_doc_load": [P, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(P)],
         "ol_query_document": [P, P, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_query_subtree": [P, P, P, S, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P),
                              ctypes.POINTER(S)],
@@ -442,6 +444,17 @@ class GpuParser:
         h = ctypes.c_void_p()
         fn = _lib.ol_doc_upload if nodes else _lib.ol_doc_upload_lean
         _check(fn(self._ctx, _ptr(t), t.size, ctypes.byref(h)))
+        return GpuDocument(self, h)
+
+    def save(self, doc: GpuDocument, path: str) -> None:
+        """Writes the parsed document as a .olb file (header, text, FlatNodes,
+        line offsets, CRC-32)."""
+        _check(_lib.ol_doc_save(self._ctx, doc._h, os.fsencode(path)))
+
+    def load(self, path: str, verify: bool = True) -> GpuDocument:
+        """A .olb file as a resident document, without scanning the text again."""
+        h = ctypes.c_void_p()
+        _check(_lib.ol_doc_load(self._ctx, os.fsencode(path), 1 if verify else 0, ctypes.byref(h)))
         return GpuDocument(self, h)
 
     # -- tree -----------------------------------------------------------------------

@@ -7,6 +7,10 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -570,6 +574,195 @@ int run_query(ol_ctx *c, const ol_doc *d, uint32_t r0, uint32_t r1, uint32_t kin
     return OL_OK;
 }
 
+// ---- .olb files: a parsed document on disk ------------------------------------
+// Layout (.agents/tasks/phase4-zero-copy-binary-serialization.md:13-36 of the
+// reference, a specification it never built): a 64-byte header
+//   0x00 magic "ONELINE\0" | 0x08 version u32 = 1 | 0x0C flags u32
+//   0x10 text_len u64 | 0x18 line_count u64
+//   0x20 text_offset u64 | 0x28 nodes_offset u64 | 0x30 line_offsets_offset u64
+//   0x38 checksum u64
+// then the text, the FlatNode array (16 bytes per line) and the u32 line
+// offsets, each section starting on a 64-byte boundary.  The checksum (low 32
+// bits) is CRC-32 (IEEE 802.3, reflected, polynomial 0xEDB88320) of every byte
+// from text_offset to the end of the file; flag bit 0 says it is present.
+constexpr uint32_t OLB_VERSION = 1, OLB_FLAG_CRC = 1;
+constexpr size_t OLB_HEADER = 64;
+
+uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n)
+{
+    static uint32_t table[8][256];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; k++)
+                c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[0][i] = c;
+        }
+        for (uint32_t i = 0; i < 256; i++)
+            for (int t = 1; t < 8; t++)
+                table[t][i] = (table[t - 1][i] >> 8) ^ table[0][table[t - 1][i] & 0xFF];
+    });
+    crc = ~crc;
+    while (n >= 8) {   // slicing by 8
+        uint32_t lo, hi;
+        std::memcpy(&lo, p, 4);
+        std::memcpy(&hi, p + 4, 4);
+        lo ^= crc;
+        crc = table[7][lo & 0xFF] ^ table[6][(lo >> 8) & 0xFF] ^ table[5][(lo >> 16) & 0xFF] ^ table[4][lo >> 24] ^
+              table[3][hi & 0xFF] ^ table[2][(hi >> 8) & 0xFF] ^ table[1][(hi >> 16) & 0xFF] ^ table[0][hi >> 24];
+        p += 8;
+        n -= 8;
+    }
+    while (n--)
+        crc = table[0][(crc ^ *p++) & 0xFF] ^ (crc >> 8);
+    return ~crc;
+}
+
+struct OlbLayout {
+    uint64_t text_off, nodes_off, offs_off, size;
+};
+
+OlbLayout olb_layout(uint64_t n, uint64_t lines)
+{
+    OlbLayout l;
+    l.text_off = OLB_HEADER;
+    l.nodes_off = round_up(l.text_off + n, 64);
+    l.offs_off = round_up(l.nodes_off + lines * 16, 64);
+    l.size = l.offs_off + lines * 4;
+    return l;
+}
+
+int save_olb(ol_ctx *c, const ol_doc *d, const char *path)
+{
+    const OlbLayout l = olb_layout(d->n, d->lines);
+    std::vector<uint8_t> buf;
+    try {
+        buf.assign(l.size, 0);
+    } catch (...) {
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    }
+    if (d->n)
+        HIPCHK(hipMemcpyAsync(buf.data() + l.text_off, d->text, d->n, hipMemcpyDeviceToHost, c->stream));
+    if (d->lines) {
+        HIPCHK(hipMemcpyAsync(buf.data() + l.offs_off, d->offsets, d->lines * 4, hipMemcpyDeviceToHost, c->stream));
+        if (d->nodes) {
+            HIPCHK(hipMemcpyAsync(buf.data() + l.nodes_off, d->nodes, d->lines * 16, hipMemcpyDeviceToHost, c->stream));
+        } else {   // a lean document: parse its lines again into a temporary node array
+            DevBuf nodes;
+            HIPCHK(nodes.ensure(d->lines * 16));
+            HIPCHK(ol_launch_parse_given(d->text, (uint32_t)d->n, d->offsets, d->lines, (ol_node_dev *)nodes.p,
+                                         c->stream));
+            HIPCHK(hipMemcpyAsync(buf.data() + l.nodes_off, nodes.p, d->lines * 16, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            nodes.release();
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint8_t *h = buf.data();
+    std::memcpy(h, "ONELINE\0", 8);
+    const uint32_t ver = OLB_VERSION, flags = OLB_FLAG_CRC;
+    std::memcpy(h + 0x08, &ver, 4);
+    std::memcpy(h + 0x0C, &flags, 4);
+    std::memcpy(h + 0x10, &d->n, 8);
+    std::memcpy(h + 0x18, &d->lines, 8);
+    std::memcpy(h + 0x20, &l.text_off, 8);
+    std::memcpy(h + 0x28, &l.nodes_off, 8);
+    std::memcpy(h + 0x30, &l.offs_off, 8);
+    const uint64_t crc = crc32_update(0, h + l.text_off, l.size - l.text_off);
+    std::memcpy(h + 0x38, &crc, 8);
+    FILE *f = std::fopen(path, "wb");
+    if (!f)
+        return fail(OL_ERR_INVALID, std::string("cannot open ") + path + " for writing");
+    const bool ok = std::fwrite(h, 1, l.size, f) == l.size;
+    if (std::fclose(f) != 0 || !ok)
+        return fail(OL_ERR_DEVICE, std::string("write to ") + path + " failed");
+    return OL_OK;
+}
+
+int load_olb(ol_ctx *c, const char *path, bool verify, ol_doc **out)
+{
+    const int fd = ::open(path, O_RDONLY);
+    if (fd < 0)
+        return fail(OL_ERR_INVALID, std::string("cannot open ") + path);
+    struct stat st {};
+    if (::fstat(fd, &st) != 0 || (size_t)st.st_size < OLB_HEADER) {
+        ::close(fd);
+        return fail(OL_ERR_INVALID, std::string(path) + " is not a .olb file (too short)");
+    }
+    const size_t size = (size_t)st.st_size;
+    void *map = ::mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+    ::close(fd);
+    if (map == MAP_FAILED)
+        return fail(OL_ERR_INVALID, std::string("cannot map ") + path);
+    const uint8_t *h = (const uint8_t *)map;
+    auto bad = [&](const std::string &why) {
+        ::munmap(map, size);
+        return fail(OL_ERR_INVALID, std::string(path) + ": " + why);
+    };
+    uint32_t ver, flags;
+    uint64_t n, lines, toff, noff, ooff, crc;
+    std::memcpy(&ver, h + 0x08, 4);
+    std::memcpy(&flags, h + 0x0C, 4);
+    std::memcpy(&n, h + 0x10, 8);
+    std::memcpy(&lines, h + 0x18, 8);
+    std::memcpy(&toff, h + 0x20, 8);
+    std::memcpy(&noff, h + 0x28, 8);
+    std::memcpy(&ooff, h + 0x30, 8);
+    std::memcpy(&crc, h + 0x38, 8);
+    if (std::memcmp(h, "ONELINE\0", 8) != 0)
+        return bad("bad magic");
+    if (ver != OLB_VERSION)
+        return bad("unsupported version " + std::to_string(ver));
+    if (n > OL_MAX_GPU_INPUT_LEN)
+        return bad("text of " + std::to_string(n) + " bytes is too large for the GPU path");
+    if (lines > n || toff < OLB_HEADER || toff > size || n > size - toff || noff > size || lines * 16 > size - noff ||
+        ooff > size || lines * 4 > size - ooff)
+        return bad("sections do not fit the file");
+    if (verify && (flags & OLB_FLAG_CRC) && crc32_update(0, h + toff, size - toff) != (uint32_t)crc)
+        return bad("checksum mismatch");
+    ol_doc *d = new (std::nothrow) ol_doc();
+    if (!d)
+        return bad("out of host memory");
+    d->n = n;
+    d->with_nodes = true;
+    std::memset(d->byte_freq, 0, sizeof d->byte_freq);
+    auto fail_doc = [&](int rc) {
+        free_doc_lines(d);
+        if (d->text)
+            (void)hipFree(d->text);
+        delete d;
+        ::munmap(map, size);
+        return rc;
+    };
+    if (n) {
+        const uint64_t padded = round_up(n, TEXT_ALIGN) + TEXT_SLACK;
+        if (hipMalloc(&d->text, padded) != hipSuccess) {
+            d->text = nullptr;
+            return fail_doc(fail(OL_ERR_NOMEM, "device allocation of the text failed"));
+        }
+        if (hipMemcpyAsync(d->text, h + toff, n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemsetAsync(d->text + n, 0, padded - n, c->stream) != hipSuccess)
+            return fail_doc(fail(OL_ERR_DEVICE, "text upload failed"));
+        int rc = alloc_doc_lines(d, lines);
+        if (rc)
+            return fail_doc(rc);
+        if (lines) {
+            if (hipMemcpyAsync(d->offsets, h + ooff, lines * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                hipMemcpyAsync(d->nodes, h + noff, lines * 16, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                ol_launch_nodes_to_lines(d->nodes, lines, d->classes, d->keys, c->stream) != hipSuccess)
+                return fail_doc(fail(OL_ERR_DEVICE, "upload of the parsed lines failed"));
+        }
+        d->lines = lines;
+        sample_histogram(h + toff, n, d->byte_freq);
+        if (hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail_doc(fail(OL_ERR_DEVICE, "upload failed"));
+    }
+    ::munmap(map, size);
+    *out = d;
+    return OL_OK;
+}
+
 int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, uint32_t *out_id,
              uint64_t cap, uint64_t *total)
 {
@@ -968,6 +1161,22 @@ int ol_search_raw(ol_ctx *c, const uint8_t *text, size_t text_len, const uint8_t
     (void)hipFree(tmp.text);
     tmp.text = nullptr;
     return rc;
+}
+
+int ol_doc_save(ol_ctx *c, const ol_doc *d, const char *path)
+{
+    if (!c || !d || !path)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return save_olb(c, d, path);
+}
+
+int ol_doc_load(ol_ctx *c, const char *path, int verify, ol_doc **out)
+{
+    if (!c || !path || !out)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    return load_olb(c, path, verify != 0, out);
 }
 
 int ol_query_document(ol_ctx *c, const ol_doc *d, uint32_t kind, uint32_t depth, size_t max_matches,

@@ -997,3 +997,27 @@ hipError_t ol_launch_query(const QueryArgs &a, uint64_t *total, hipStream_t s)
 }
 
 uint32_t ol_query_blocks(uint32_t lines) { return (lines + QUERY_BLOCK - 1) / QUERY_BLOCK; }
+
+// Classes and tree keys from FlatNodes (documents loaded from a .olb file,
+// whose nodes were parsed when it was written).
+__global__ void nodes_to_lines_kernel(const ol_node_dev *nodes, uint64_t lines, uint8_t *classes, uint8_t *keys)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < lines; i += stride) {
+        const ol_node_dev nd = nodes[i];
+        classes[i] = (uint8_t)nd.kind;
+        keys[i] = key_of(nd);
+    }
+}
+
+hipError_t ol_launch_nodes_to_lines(const ol_node_dev *nodes, uint64_t lines, uint8_t *classes, uint8_t *keys,
+                                    hipStream_t s)
+{
+    if (lines == 0)
+        return hipSuccess;
+    uint64_t blocks = (lines + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    hipLaunchKernelGGL(nodes_to_lines_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nodes, lines, classes, keys);
+    return hipGetLastError();
+}
