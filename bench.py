@@ -208,6 +208,8 @@ def main():
     extra = {}
     if not args.no_search:
         extra = search_benchmarks(p, doc, text, args)
+        extra["query"] = query_benchmark(p, doc, tree, L)
+        extra["olb"] = olb_benchmark(p, text)
     del doc, tree, text
     if not args.no_batch:
         extra["batch"] = batch_benchmark(p, args)
@@ -442,6 +444,61 @@ def serve_benchmark(args):
                     "64 KiB, sigma 2.0, max 256 MiB, seed 2), 8 resident dictionaries of 1k..64k patterns",
         "slots": 4, "hbm_budget_bytes": 64 << 30, "sweep": sweep,
     }
+
+
+def query_benchmark(p, doc, tree, L):
+    """Kind / depth queries over the resident document and one of its
+    subtrees (the next step after the path: the tree's callers)."""
+    res = []
+    for kind, depth in ((4, None), (None, 3), (2, 5)):
+        p.query_document(doc, kind, depth, max_matches=1 << 26)   # warm
+        p.profile_enable(True)
+        t0 = time.perf_counter()
+        q = p.query_document(doc, kind, depth, max_matches=1 << 26)
+        dt = time.perf_counter() - t0
+        p.profile_enable(False)
+        c, ms = p.profile_take().get("query", (1, 0.0))
+        kms = ms / max(1, c)
+        # bytes: classes + tree keys read twice (count, write) + 4 B per match written
+        b = 4 * L + 4 * q.total_matches
+        res.append({"kind": kind, "depth": depth, "matches": q.total_matches, "call_ms": round(dt * 1e3, 3),
+                    "kernel_ms": round(kms, 4), "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1)})
+        log(f"[bench] query {res[-1]}")
+    return res
+
+
+def olb_benchmark(p, text, size=256 << 20):
+    """A parsed document saved as .olb and loaded back (map, upload text,
+    offsets and nodes, no scan) against uploading and scanning its text."""
+    import tempfile
+    sample = text[:size]
+    path = os.path.join(tempfile.gettempdir(), f"oneline_bench_{os.getpid()}.olb")
+    try:
+        t0 = time.perf_counter()
+        d0 = p.upload(sample, nodes=True)
+        p.synchronize()
+        up_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p.save(d0, path)
+        save_s = time.perf_counter() - t0
+        p.load(path, verify=False).close()   # warm: page cache, first-use allocations
+        t0 = time.perf_counter()
+        d1 = p.load(path, verify=False)
+        p.synchronize()
+        load_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p.load(path, verify=True).close()
+        p.synchronize()
+        verify_s = time.perf_counter() - t0
+        ok = d1.line_count() == d0.line_count()
+        res = {"bytes": int(sample.size), "file_bytes": os.path.getsize(path), "upload_scan_ms": round(up_s * 1e3, 2),
+               "save_ms": round(save_s * 1e3, 2), "load_ms": round(load_s * 1e3, 2),
+               "load_crc_ms": round(verify_s * 1e3, 2), "lines_equal": ok}
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    log(f"[bench] olb {res}")
+    return res
 
 
 def search_benchmarks(p, doc, text, args):
