@@ -668,8 +668,9 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
 
         const uint32_t lds_lo = (base >= (uint32_t)HALO) ? base - HALO : 0u;
         auto lds_byte = [&](uint32_t pos) -> uint32_t { return s_text[pos + (uint32_t)HALO - base]; };
-        // parse_line of text[s ..= e] from LDS (s >= lds_lo)
-        auto decode = [&](uint32_t s, uint32_t e) -> ol_node_dev {
+        // parse_line of text[s ..= e] from LDS (s >= lds_lo).  Inlined: as a call it
+        // spilled the registers live across it to scratch on every tile.
+        auto decode = [&](uint32_t s, uint32_t e) __attribute__((always_inline)) -> ol_node_dev {
             uint32_t eff = e + 1;
             while (eff > s) {
                 const uint32_t c = lds_byte(eff - 1);
