@@ -485,24 +485,35 @@ int search_doc_host(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, si
     return OL_OK;
 }
 
-// The dictionary search's candidate store (when matches are written): per
-// tile room for one candidate in four text positions (4 bytes each: the text
-// size in total); a tile with more is filtered and counted again by the write
-// pass.  tile_list follows tile_off in
-// the scratch status words.
-hipError_t set_candidate_store(ol_ctx *c, ManyArgs &a, uint32_t tiles, bool writes)
+// The dictionary search's per-wave-tile state in the scratch status words:
+// the scan's chunk totals first (the only words cleared), then tile_count
+// (u32), tile_off and tile_list.  The candidate store (when matches are
+// written) holds one candidate in four text positions per wave tile (4 bytes
+// each: the text size in total); a wave tile with more candidates that have
+// matches is filtered and counted again by the write pass.
+hipError_t setup_many(ol_ctx *c, ManyArgs &a, uint64_t n, bool writes)
 {
-    a.tile_list = c->status() + 2 * (size_t)tiles;
+    const uint32_t tiles = (uint32_t)ceil_div(n, SEARCH_TILE);
+    const uint32_t wtiles = tiles * (SEARCH_TILE / MANY_WT_BYTES);
+    const size_t nsums = wtiles / MANY_SCAN_CHUNK + 1;
+    const size_t words = nsums + (wtiles + 1) / 2 + 2 * (size_t)wtiles;
+    hipError_t e = ensure_scratch(c, words, 1);
+    if (e != hipSuccess || (e = clear_scratch(c, nsums, 1)) != hipSuccess)
+        return e;
+    uint64_t *st = c->status();
+    a.num_tiles = tiles;
+    a.num_wtiles = wtiles;
+    a.scan_sums = st;
+    a.tile_count = (uint32_t *)(st + nsums);
+    a.tile_off = st + nsums + (wtiles + 1) / 2;
+    a.tile_list = a.tile_off + wtiles;
+    a.meta = c->meta();
     a.cands = nullptr;
-    a.cand_per_tile = 0;
     if (!writes)
         return hipSuccess;
-    const uint32_t per_tile = SEARCH_TILE / 4;
-    const hipError_t e = c->cands.ensure((size_t)tiles * per_tile * 4);
-    if (e != hipSuccess)
+    if ((e = c->cands.ensure((size_t)wtiles * MANY_WT_BYTES)) != hipSuccess)
         return e;
     a.cands = (uint32_t *)c->cands.p;
-    a.cand_per_tile = per_tile;
     return hipSuccess;
 }
 
@@ -766,21 +777,14 @@ int load_olb(ol_ctx *c, const char *path, bool verify, ol_doc **out)
 int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, uint32_t *out_id,
              uint64_t cap, uint64_t *total)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->n, SEARCH_TILE);
-    HIPCHK(ensure_scratch(c, tiles));
-    HIPCHK(clear_scratch(c, tiles));
     ManyArgs a{};
     a.text = d->text;
     a.n = (uint32_t)d->n;
-    a.num_tiles = tiles;
     a.d = s->d;
     a.cap = cap;
     a.out_off = out_off;
     a.out_id = out_id;
-    a.meta = c->meta();
-    a.tile_count = (uint32_t *)c->status();
-    a.tile_off = c->status() + tiles;
-    HIPCHK(set_candidate_store(c, a, tiles, out_off || out_id));
+    HIPCHK(setup_many(c, a, d->n, out_off || out_id));
     {
         Prof p(c, "many");
         HIPCHK(ol_launch_many(a, c->stream));
@@ -1316,35 +1320,34 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     }
     DictDev dd{};
     dd.max_len = max_len;
-    // filters: 1 and 2 direct, 3 hashed (2^16 bits), 4 = 4-byte prefixes of
-    // classes 4 and 5 (2^17 bits, 2^18 beyond 16k prefixes).  Layout: filter 4
-    // first, then 1, 2, 3, then a zero word that absent filters point at.
+    // filters: 1 and 2 direct; 4 = the prefix filter of classes 3..5 (4-byte
+    // keys, a 3-byte key with each of the 32 values of the fourth byte's low
+    // 5 bits).  Layout: filter 4 first, then 1, 2, then a zero word that
+    // absent filters point at.
     std::vector<std::vector<uint32_t>> filt(6);
     uint32_t fwords = 0;
-    std::vector<uint32_t> pref4;
-    for (uint32_t k = 4; k <= 5; k++)
-        for (const Bucket &b : buckets[k])
-            pref4.push_back(b.lo);
-    std::sort(pref4.begin(), pref4.end());
-    pref4.erase(std::unique(pref4.begin(), pref4.end()), pref4.end());
-    const bool big4 = pref4.size() > 16384;
-    for (uint32_t f : {4u, 1u, 2u, 3u}) {
-        const bool used = f <= 3 ? !buckets[f].empty() : !pref4.empty();
+    for (uint32_t f : {4u, 1u, 2u}) {
+        const bool used = f <= 2 ? !buckets[f].empty() : !(buckets[3].empty() && buckets[4].empty() && buckets[5].empty());
         if (!used)
             continue;
-        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : f == 3 ? 16 : (big4 ? 18 : 17);
+        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : DICT_PBITS;
         dd.fbits[f] = fb;
         dd.foff[f] = fwords;
         const uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
         filt[f].assign(words, 0);
         fwords += words;
-        auto setbit = [&](uint32_t fi) { filt[f][fi >> 5] |= 1u << (fi & 31); };
-        if (f <= 2)
-            for (const Bucket &b : buckets[f]) setbit(b.lo);
-        else if (f == 3)
-            for (const Bucket &b : buckets[3]) setbit(dict_fh3(b.lo) >> (32 - fb));
-        else
-            for (uint32_t p4 : pref4) setbit(dict_fh4(p4) >> (32 - fb));
+        if (f <= 2) {
+            for (const Bucket &b : buckets[f]) filt[f][b.lo >> 5] |= 1u << (b.lo & 31);
+            continue;
+        }
+        auto setkey = [&](uint32_t w) {
+            const uint32_t h = dict_ph(w);
+            filt[f][dict_ph_word(h)] |= 1u << (h & 31);
+        };
+        for (const Bucket &b : buckets[3])
+            for (uint32_t c = 0; c < 32; c++) setkey(b.lo | c << 24);
+        for (uint32_t k = 4; k <= 5; k++)
+            for (const Bucket &b : buckets[k]) setkey(b.lo);
     }
     for (uint32_t f = 1; f <= 4; f++)
         if (!dd.fbits[f])
@@ -1704,21 +1707,14 @@ int batch_tree(ol_ctx *c, ol_batch *b)
 
 int batch_many(ol_ctx *c, ol_batch *b, const ol_dict *s, uint64_t cap)
 {
-    const uint32_t tiles = (uint32_t)ceil_div(b->n, SEARCH_TILE);
-    HIPCHK(ensure_scratch(c, tiles));
-    HIPCHK(clear_scratch(c, tiles));
     ManyArgs a{};
     a.text = b->text;
     a.n = (uint32_t)b->n;
-    a.num_tiles = tiles;
     a.d = s->d;
     a.cap = cap;
     a.out_off = cap ? (uint64_t *)b->m_off.p : nullptr;
     a.out_id = cap ? (uint32_t *)b->m_id.p : nullptr;
-    a.meta = c->meta();
-    a.tile_count = (uint32_t *)c->status();
-    a.tile_off = c->status() + tiles;
-    HIPCHK(set_candidate_store(c, a, tiles, cap != 0));
+    HIPCHK(setup_many(c, a, b->n, cap != 0));
     a.ndocs = b->nd;
     a.doc_starts = (const uint32_t *)b->d_starts.p;
     a.doc_ends = (const uint32_t *)b->d_ends.p;

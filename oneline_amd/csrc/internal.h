@@ -316,21 +316,17 @@ struct SearchArgs {
     uint64_t *meta;          // meta[0] <- total matches
 };
 
-// Compiled dictionary in device memory.  Patterns are keyed by their first
-// k = min(len, 4) bytes ("class" k).  Per class: a filter bitmap (copied to
-// LDS by every workgroup) and an open-addressing table of 16-byte slots
-// {key, begin, end, 0} giving the range of that key's entries.  Entries are
-// sorted by (class, key, pattern id).
 // Compiled dictionary.  Patterns are keyed by class: 1, 2 and 3 for patterns
 // of that length (key = all their bytes), 4 for lengths 4..7 (key = first 4
 // bytes), 5 for lengths >= 8 (key = first 8 bytes).  Per class an
 // open-addressing table of {key_lo, begin, end, key_hi} slots (end == 0:
 // empty) maps a key to its run of entries, sorted by pattern id.  LDS filters:
-// class 1 and 2 direct bitmaps, class 3 a hashed bitmap of the 3-byte key, and
-// one hashed bitmap of the 4-byte prefix shared by classes 4 and 5 (index 4).
+// class 1 and 2 direct bitmaps (indices 1, 2), and one prefix bitmap for
+// classes 3, 4 and 5 (index 4): the 4-byte key with the fourth byte cut to its
+// low 5 bits, hashed (dict_ph); a class-3 key sets all 32 of its fourth bytes.
 struct DictDev {
     uint32_t class_mask;     // bit k set if class k (1..5) is non-empty
-    uint32_t fbits[6];       // log2 bits of the filters 1..4
+    uint32_t fbits[6];       // log2 bits of the filters 1, 2 and 4 (0: absent)
     uint32_t foff[6];        // word offset of each filter in `filter`
     uint32_t fwords;         // total filter words
     uint32_t tbits[6];       // log2 table slots per class 1..5
@@ -342,19 +338,19 @@ struct DictDev {
     uint32_t max_len;
 };
 
-// Hashes shared by the compiler (host) and the search kernel (device).  The
-// filter hashes use 24-bit multiplies (full rate on the device).
+// Hashes shared by the compiler (host) and the search kernel (device).
 __host__ __device__ __forceinline__ uint32_t dict_mul24(uint32_t a, uint32_t b)
 {
     return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
 }
-// Filter hashes of a 3-byte key and of a 4-byte prefix; a filter of 2^b
-// bits uses the top b bits of the hash.
-__host__ __device__ __forceinline__ uint32_t dict_fh3(uint32_t key3) { return dict_mul24(key3, 0x9E3779u); }
-__host__ __device__ __forceinline__ uint32_t dict_fh4(uint32_t key4)
-{
-    return dict_mul24(key4, 0x9E3779u) ^ dict_mul24(key4 >> 8, 0x85EBCAu);
-}
+// Prefix filter: 2^DICT_PBITS bits, 64 KiB of LDS.  The hash of a 4-byte
+// window w (first byte lowest) depends on bytes 0..2 and the low 5 bits of
+// byte 3 only (w << 3 drops byte 3's top 3 bits); two full-rate operations on
+// the device (24-bit multiply, shift-add).  Word = top DICT_PBITS - 5 bits,
+// bit = low 5 bits.
+constexpr uint32_t DICT_PBITS = 19;
+__host__ __device__ __forceinline__ uint32_t dict_ph(uint32_t w) { return dict_mul24(w, 0x9E3779u) + (w << 3); }
+__host__ __device__ __forceinline__ uint32_t dict_ph_word(uint32_t h) { return h >> (32 - (DICT_PBITS - 5)); }
 __host__ __device__ __forceinline__ uint32_t dict_thash(uint32_t lo, uint32_t hi, uint32_t bits)
 {
     uint32_t x = lo ^ (hi * 0x9E3779B1u);
@@ -365,21 +361,21 @@ __host__ __device__ __forceinline__ uint32_t dict_thash(uint32_t lo, uint32_t hi
 struct ManyArgs {
     const uint8_t *text;
     uint32_t n;
-    uint32_t num_tiles;
+    uint32_t num_tiles;      // SEARCH_TILE tiles (tile_doc)
+    uint32_t num_wtiles;     // wave tiles of MANY_WT_BYTES: num_tiles * SEARCH_TILE / MANY_WT_BYTES
     DictDev d;
     uint64_t cap;
     uint64_t *out_off;       // u64 offsets (nullable)
     uint32_t *out_id;        // pattern ids (nullable)
     uint64_t *meta;          // meta[0] <- total matches (zero at launch)
-    uint32_t *tile_count;    // [num_tiles] matches per tile
-    uint64_t *tile_off;      // [num_tiles] first output index of each tile
-    // Candidate store (nullable): tile t's candidates go to
-    // cands[t * cand_per_tile ..] when there are at most cand_per_tile of
-    // them; tile_list[t] records (offset << 24 | count), or ~0 when they were
-    // not stored.
+    uint32_t *tile_count;    // [num_wtiles] matches per wave tile
+    uint64_t *tile_off;      // [num_wtiles] first output index of each wave tile
+    uint64_t *scan_sums;     // [num_wtiles / MANY_SCAN_CHUNK + 1] chunk totals of the scan (zero at launch)
+    // Candidate store (nullable): the candidates of wave tile wt that have
+    // matches go to cands[wt * MANY_WT_BYTES / 4 ..] when they fit;
+    // tile_list[wt] records their number, or ~0 when they were not stored.
     uint32_t *cands;
-    uint32_t cand_per_tile;
-    uint64_t *tile_list;     // [num_tiles]
+    uint64_t *tile_list;     // [num_wtiles]
     // Batches (ndocs > 0): a match must end inside the document it starts in
     // (the bytes between documents belong to none).
     uint32_t ndocs;
@@ -411,6 +407,12 @@ constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks
 constexpr uint32_t SEARCH_TILE = SEARCH_THREADS * SEARCH_BPT;
+constexpr uint32_t MANY_THREADS = 512;     // dictionary search: 8 independent waves per workgroup
+#ifndef OL_MANY_WT_BYTES
+#define OL_MANY_WT_BYTES 8192
+#endif
+constexpr uint32_t MANY_WT_BYTES = OL_MANY_WT_BYTES;   // dictionary wave tile: 4 parts of 64 lanes x 32 positions
+constexpr uint32_t MANY_SCAN_CHUNK = 8192; // wave tiles per workgroup of the dictionary's tile scan
 struct QueryArgs {
     const uint8_t *text;
     uint32_t n;
@@ -431,4 +433,4 @@ hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t
                                  uint64_t lines, ol_node_dev *nodes, hipStream_t s);
 hipError_t ol_launch_nodes_to_lines(const ol_node_dev *nodes, uint64_t lines, uint8_t *classes, uint8_t *keys,
                                     hipStream_t s);
-size_t ol_many_lds_bytes(uint32_t fwords);
+size_t ol_many_lds_bytes(const DictDev &d);

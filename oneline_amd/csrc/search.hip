@@ -269,35 +269,38 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 }
 
 // ---- dictionary -----------------------------------------------------------------
-// Three launches with no waits between workgroups:
-//   count pass   per tile (256 lanes x 64 positions): filter, candidate list,
-//                per-candidate match counts -> tile_count[t]; the candidates
-//                that have matches are stored with their counts (4 bytes
-//                each) in the tile's own region of the store when the tile's
-//                candidates fit it;
+// Three launches with no waits between workgroups, all working on wave tiles
+// (64 lanes x 32 positions = 2 KiB of text; every wave independent, no
+// workgroup barriers after the filter load):
+//   count pass   per wave tile: filter, candidate list (per-wave LDS chunk),
+//                per-candidate match counts, one candidate per lane ->
+//                tile_count[wt]; the candidates that have matches are stored
+//                with their counts (4 bytes each) in the wave tile's region of
+//                the store when they fit it;
 //   scan         exclusive scan of tile_count -> tile_off, total;
-//   write pass   per tile: its stored candidates (or filter and count again),
-//                each candidate's matches written at tile_off[t] + (matches
-//                of the earlier candidates).
-// Filter: each lane tests its 64 positions against the LDS bitmaps of the
-// dictionary's key classes (straight-line code, 4-byte windows from
-// registers).  Probe: the class tables give, per class, the run of patterns
-// sharing the key (sorted by id); the runs are merged by id and each pattern's
-// remaining bytes compared word by word.  The candidate list goes through LDS
-// in chunks of MANY_LCAP, one candidate per thread.  The text is read from
-// global memory (the probes hit L2), so the LDS holds only the filters and
-// the list.
+//   write pass   per wave tile: its stored candidates (or filter and count
+//                again), each candidate's matches written at tile_off[wt] +
+//                (matches of the earlier candidates).
+// Filter: each lane tests its 32 positions against the LDS bitmaps
+// (straight-line code, 4-byte windows from registers): one lookup per
+// position in the 64 KiB prefix filter of classes 3..5, plus the direct 1- and
+// 2-byte bitmaps when the dictionary has such patterns.  Two workgroups of 8
+// waves share a CU.  Probe: the class tables give, per class, the run of
+// patterns sharing the key (sorted by id); the runs are merged by id and each
+// pattern's remaining bytes compared word by word.  The text is read from
+// global memory (the probes hit the cache lines the filter just read).
 
 __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
 {
     return __builtin_amdgcn_ubfe(f[i >> 5], i & 31, 1);
 }
 
-constexpr int MANY_LCAP = 2048;     // candidates of a tile per list chunk
-
 // Matches starting at tile position lp (global gp) with at most `rem` bytes
 // (1..64) before the text or document end; written from out[r] in pattern-id
-// order (up to cap) when `emit`.  Returns their number.
+// order (up to cap) when `emit`.  Returns their number.  GATE: read the
+// tables of classes 3..5 only where the prefix filter (s_filter) passes; a
+// candidate that passed it needs no gate.
+template <bool GATE>
 __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
                                                    uint32_t lp, uint64_t gp, uint32_t rem, bool emit, uint64_t *out_off,
                                                    uint32_t *out_id, uint64_t r, uint64_t cap)
@@ -311,15 +314,15 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
     // One run per class (1..5): the patterns whose key matches the text here.
     // Named variables per class (no arrays: nothing may be indexed at run time).
     // First the table slots of every class are loaded together, then resolved.
-    // A class's table is read only where its filter passes: the 3-byte keys'
-    // filter for class 3, the 4-byte prefix filter for classes 4 and 5 (a
-    // candidate passed at least one of them).
-    const bool pass3 = !d.fbits[3] || filter_bit(s_filter + d.foff[3], dict_fh3(w0) >> 16);
-    const bool pass45 = !d.fbits[4] || filter_bit(s_filter, dict_fh4(w0) >> (32 - d.fbits[4]));
+    bool pass345 = true;
+    if (GATE && d.fbits[4]) {
+        const uint32_t ph = dict_ph(w0);
+        pass345 = __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
+    }
 #define OL_KEY(K, LO)                                                                       \
     const uint32_t klo##K = (LO), khi##K = (K) == 5 ? w1 : 0u;                              \
     const bool use##K = ((d.class_mask >> (K)) & 1u) && ((K) <= 4 ? (K) : 8u) <= rem &&    \
-                        ((K) == 3 ? pass3 : (K) >= 4 ? pass45 : true);                       \
+                        ((K) >= 3 ? pass345 : true);                                         \
     uint32_t h##K = use##K ? dict_thash(klo##K, khi##K, d.tbits[K]) : 0u;                  \
     uint4 v##K = use##K ? d.table[K][h##K] : make_uint4(0, 0, 0, 0);
     OL_KEY(1, w0 & 0xFFu)
@@ -397,36 +400,50 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
 }
 
 
-// Stored candidate: tile position (14 bits) | bytes before the text or
-// document end (7 bits) << 14 | match count (11 bits, CAND_RECOUNT: count
-// again) << 21.
-constexpr uint32_t CAND_RECOUNT = 2047u;
-constexpr uint64_t TILE_LIST_NONE = ~0ull;   // tile_list: candidates not stored
+// Wave tiles: MANY_H parts of 64 lanes x MANY_LB positions (a lane's
+// positions are contiguous within a part; parts follow each other).
+constexpr uint32_t MANY_LB = 32;
+constexpr uint32_t MANY_PART = 64 * MANY_LB;                // 2 KiB
+constexpr uint32_t MANY_WT = MANY_WT_BYTES;
+constexpr uint32_t MANY_H = MANY_WT / MANY_PART;            // parts per wave tile
+constexpr uint32_t MANY_WPT = SEARCH_TILE / MANY_WT;        // wave tiles per search tile
+constexpr uint32_t MANY_WCAP = MANY_WT / 4;                 // stored candidates per wave tile
+constexpr uint32_t MANY_WLIST = 512;                        // per-wave LDS list chunk
+constexpr uint32_t MANY_PWORDS = 1u << (DICT_PBITS - 5);    // prefix filter words
+constexpr uint32_t MANY_JOB = MANY_WT <= 4096 ? 64 : 64 * 4096 / MANY_WT;   // wave tiles per write job
+static_assert(MANY_WPT * MANY_WT == SEARCH_TILE && MANY_H * MANY_PART == MANY_WT, "wave tiles");
 
-// Per-tile state of the dictionary search: the tile's text, and the lanes'
-// document bounds (batches) for the end bound of a match.
-struct ManyTile {
-    const uint8_t *g_text;   // the tile's first byte (global memory)
-    uint32_t base, lo, g0, lim;
+// Stored candidate: wave-tile position (CAND_POS_BITS) | bytes before the
+// text or document end (7 bits) | match count (the rest; CAND_RECOUNT: count
+// again).
+constexpr uint32_t CAND_POS_BITS = __builtin_ctz(MANY_WT);
+constexpr uint32_t CAND_CNT_SHIFT = CAND_POS_BITS + 7;
+constexpr uint32_t CAND_RECOUNT = (1u << (32 - CAND_CNT_SHIFT)) - 1u;
+constexpr uint64_t TILE_LIST_NONE = ~0ull;   // tile_list: candidates not stored
+static_assert(32 - CAND_CNT_SHIFT >= 10, "count bits");
+
+// A lane's share of wave tile wt (its positions lo + h * MANY_PART + p of the
+// tile, p < MANY_LB, in part h): the text and the document bounds (batches)
+// for the end bound of a match at its first position.
+struct ManyLane {
+    const uint8_t *g_text;   // the wave tile's first byte (global memory)
+    uint32_t base, lo;
     uint32_t di0, dlim0, dnext0;
 };
 
-template <int THREADS>
-__device__ __forceinline__ ManyTile many_tile_setup(const ManyArgs &a, uint32_t t)
+__device__ __forceinline__ ManyLane many_lane_setup(const ManyArgs &a, uint32_t wt)
 {
-    constexpr int TILE = THREADS * 64;
-    ManyTile m;
-    m.base = t * TILE;
+    ManyLane m;
+    m.base = wt * MANY_WT;
     m.g_text = a.text + m.base;
-    m.lo = threadIdx.x * 64;
-    m.g0 = m.base + m.lo;
-    m.lim = (m.g0 < a.n) ? ((a.n - m.g0) < 64u ? a.n - m.g0 : 64u) : 0u;
+    m.lo = (threadIdx.x & 63u) * MANY_LB;
+    const uint32_t g0 = m.base + m.lo;
     // match-end bound per position: the text end, or (batches) the end of
     // the position's document; positions between documents match nothing
     uint32_t dlim = a.n, dnext = 0xFFFFFFFFu, di = 0;
     if (a.ndocs) {
-        di = a.tile_doc[t];   // then the last document starting at or before g0
-        while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= m.g0)
+        di = a.tile_doc[wt / MANY_WPT];   // then the last document starting at or before g0
+        while (di + 1 < a.ndocs && a.doc_starts[di + 1] <= g0)
             di++;
         dlim = a.doc_ends[di];
         dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
@@ -437,238 +454,322 @@ __device__ __forceinline__ ManyTile many_tile_setup(const ManyArgs &a, uint32_t 
     return m;
 }
 
-// The lane's 64 positions through the filters: bit p set when a pattern may
-// start at g0 + p.  FS: the filters the dictionary has (bit 1: 1- and 2-byte
-// keys, bit 3: 3-byte keys, bit 4: 4-byte prefixes); compile-time, so the 64
-// lookups are straight-line code whose LDS reads the compiler can overlap.
-template <int FS>
-__device__ __forceinline__ uint64_t many_filter(const DictDev &d, const uint32_t *s_filter, const ManyTile &m)
+// The lane's text words in part h: MANY_LB bytes and the next word (the
+// padded text covers it).
+struct ManyWords {
+    uint32_t w[MANY_LB / 4 + 1];
+};
+__device__ __forceinline__ ManyWords many_load(const ManyArgs &a, uint32_t wt, uint32_t h)
 {
-    if (!m.lim)
-        return 0;
-    // Filter 4, when the dictionary has it, is first in LDS.  The 1- and
-    // 2-byte filters are looked up together: an absent one reads the zero
-    // word (index mask 0).
-    const uint32_t *f1 = s_filter + d.foff[1], *f2 = s_filter + d.foff[2], *f3 = s_filter + d.foff[3];
-    const uint32_t *f4 = s_filter;
-    const uint32_t m1 = d.fbits[1] ? 0xFFu : 0u, m2 = d.fbits[2] ? 0xFFFFu : 0u;
-    const uint32_t sh4 = 32 - d.fbits[4];   // filter 4 index = hash >> sh4
-    uint32_t W[20];
-    const uint4 *vsrc = (const uint4 *)(m.g_text + m.lo);
+    const uint8_t *p = a.text + (size_t)wt * MANY_WT + h * MANY_PART + (threadIdx.x & 63u) * MANY_LB;
+    ManyWords W;
 #pragma unroll
-    for (int j = 0; j < 5; j++) {
-        const uint4 v = vsrc[j];
-        W[4 * j] = v.x;
-        W[4 * j + 1] = v.y;
-        W[4 * j + 2] = v.z;
-        W[4 * j + 3] = v.w;
+    for (int j = 0; j < (int)MANY_LB / 16; j++) {
+        const uint4 v = ((const uint4 *)p)[j];
+        W.w[4 * j] = v.x;
+        W.w[4 * j + 1] = v.y;
+        W.w[4 * j + 2] = v.z;
+        W.w[4 * j + 3] = v.w;
     }
-    uint32_t clo = 0, chi = 0;
-#pragma unroll
-    for (int p = 0; p < 64; p++) {
-        const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3) : W[p >> 2];
-        uint32_t hit = 0;
-        if (FS & 0x10) hit |= filter_bit(f4, dict_fh4(w) >> sh4);
-        if (FS & 0x08) hit |= filter_bit(f3, dict_fh3(w) >> 16);
-        if (FS & 0x02) hit |= filter_bit(f1, w & m1) | filter_bit(f2, w & m2);
-        if (p < 32)
-            clo |= hit << p;
-        else
-            chi |= hit << (p - 32);
-    }
-    uint64_t cand = (uint64_t)chi << 32 | clo;
-    if (m.lim < 64)
-        cand &= (1ull << m.lim) - 1ull;
-    return cand;
+    W.w[MANY_LB / 4] = *(const uint32_t *)(p + MANY_LB);
+    return W;
 }
 
-// Candidates [c0, c0 + LCAP) of the tile (in position order; the lane's own
-// start at index cex) into the LDS list: tile position and end bound.
-__device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyTile &m, uint64_t cand, uint32_t cex,
-                                                uint32_t c0, uint16_t *s_cand, uint8_t *s_rem)
+// One part's positions through the filters: bit p set when a pattern may
+// start at the lane's position p.  FS: the filters the dictionary has (bit 4:
+// the prefix filter at pf, bit 1: the 1- and 2-byte filters at their word
+// offsets from small); compile-time, so the lookups are straight-line code
+// whose reads the compiler can overlap.  A position of the prefix filter
+// costs a byte align, a shift, a 24-bit multiply-add, the word address, one
+// read and the bit extract.
+template <int FS>
+__device__ __forceinline__ uint32_t many_filter(const DictDev &d, const uint32_t *pf, const uint32_t *small,
+                                                const ManyWords &W, uint32_t lim)
+{
+    const uint32_t *f1 = small + d.foff[1], *f2 = small + d.foff[2];
+    const uint32_t m1 = d.fbits[1] ? 0xFFu : 0u, m2 = d.fbits[2] ? 0xFFFFu : 0u;
+    uint32_t cand = 0;
+#pragma unroll
+    for (int p = 0; p < (int)MANY_LB; p++) {
+        const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W.w[(p >> 2) + 1], W.w[p >> 2], p & 3) : W.w[p >> 2];
+        uint32_t hit = 0;
+        if (FS & 0x10) {
+            const uint32_t h = dict_ph(w);
+            hit |= __builtin_amdgcn_ubfe(pf[dict_ph_word(h)], h, 1);   // offset: h's low 5 bits
+        }
+        if (FS & 0x02) hit |= filter_bit(f1, w & m1) | filter_bit(f2, w & m2);
+        cand |= hit << p;
+    }
+    return lim >= MANY_LB ? cand : cand & ((1u << lim) - 1u);
+}
+
+// The wave tile's candidates: per part the lane's bits and its first index
+// in the tile's list (position order: part, then lane, then position).
+struct ManyCand {
+    uint32_t q[MANY_H];
+    uint32_t ex[MANY_H];
+    uint32_t total;   // the tile's candidates (every lane)
+};
+
+// Filters wave tile wt part by part from the text words in W, loading each
+// next part (and at the last part the first part of wave tile `next`, when
+// there is one) into W before filtering, so the loads overlap the lookups and
+// the probes that follow.
+template <int FS>
+__device__ __forceinline__ ManyCand many_candidates(const ManyArgs &a, uint32_t wt, uint32_t next, const ManyLane &m,
+                                                    const uint32_t *pf, const uint32_t *small, ManyWords &W)
+{
+    ManyCand c;
+    uint32_t run = 0;
+#pragma unroll
+    for (uint32_t h = 0; h < MANY_H; h++) {
+        const ManyWords cur = W;
+        if (h + 1 < MANY_H)
+            W = many_load(a, wt, h + 1);
+        else if (next < a.num_wtiles)
+            W = many_load(a, next, 0);
+        const uint32_t g0 = m.base + h * MANY_PART + m.lo;
+        const uint32_t lim = g0 < a.n ? min(a.n - g0, MANY_LB) : 0u;
+        c.q[h] = lim ? many_filter<FS>(a.d, pf, small, cur, lim) : 0u;
+        const uint32_t nc = (uint32_t)__popc(c.q[h]);
+        const uint32_t inc = wave_incl_add(nc);
+        c.ex[h] = run + inc - nc;
+        run += __shfl(inc, 63, 64);
+    }
+    c.total = run;
+    return c;
+}
+
+// Candidates [c0, c0 + MANY_WLIST) of the wave tile into the wave's LDS list:
+// wave-tile position and end bound (0: no match may start there).
+__device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyLane &m, const ManyCand &cand,
+                                                uint32_t c0, uint16_t *l_pos, uint8_t *l_rem)
 {
     uint32_t di = m.di0, dlim = m.dlim0, dnext = m.dnext0;
-    uint32_t at = cex;
-    uint64_t q = cand;
-    while (q && at < c0 + (uint32_t)MANY_LCAP) {
-        const uint32_t p = (uint32_t)__builtin_ctzll(q);
-        q &= q - 1;
-        if (at >= c0) {
-            const uint32_t gp = m.g0 + p;
-            while (gp >= dnext) {
-                di++;
-                dlim = a.doc_ends[di];
-                dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t h = 0; h < MANY_H; h++) {
+        uint32_t at = cand.ex[h];
+        uint32_t q = cand.q[h];
+        while (q && at < c0 + MANY_WLIST) {
+            const uint32_t lp = h * MANY_PART + m.lo + (uint32_t)__builtin_ctz(q);
+            q &= q - 1;
+            if (at >= c0) {
+                const uint32_t gp = m.base + lp;
+                while (gp >= dnext) {
+                    di++;
+                    dlim = a.doc_ends[di];
+                    dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
+                }
+                const uint32_t rm = gp < dlim ? dlim - gp : 0u;
+                l_pos[at - c0] = (uint16_t)lp;
+                l_rem[at - c0] = (uint8_t)(rm < 64u ? rm : 64u);
             }
-            const uint32_t rm = gp < dlim ? dlim - gp : 0u;   // 0: no match may start here
-            s_cand[at - c0] = (uint16_t)(m.lo + p);
-            s_rem[at - c0] = (uint8_t)(rm < 64u ? rm : 64u);
+            at++;
         }
-        at++;
     }
 }
 
-// Per-candidate exclusive offsets of s_cnt[0 .. n) (contiguous runs per
-// thread), in place; returns the chunk's total (every thread).
-template <int THREADS>
-__device__ __forceinline__ uint32_t many_chunk_offsets(uint32_t *s_cnt, uint32_t n, uint32_t *s_w)
+// Orders the wave's LDS accesses around a point (a wave's lists are its own;
+// a wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_lds_sync()
 {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t K = (n + THREADS - 1) / THREADS;
-    const uint32_t i0 = min(n, tid * K), i1 = min(n, i0 + K);
-    uint32_t loc = 0;
-    for (uint32_t i = i0; i < i1; i++)
-        loc += s_cnt[i];
-    uint32_t total;
-    uint32_t run = block_excl_sum<THREADS>(loc, s_w, total);
-    for (uint32_t i = i0; i < i1; i++) {
-        const uint32_t c = s_cnt[i];
-        s_cnt[i] = run;
-        run += c;
-    }
-    __syncthreads();
-    return total;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-// Count pass over one tile: filter, candidate list in chunks of LCAP,
-// per-candidate match counts.  The candidates (with their counts) are stored
-// for the write pass when the store has room.  Returns the tile's match count.
+// Count pass: every wave takes wave tiles wt = wave + k * waves.
 template <int THREADS, int FS>
-__device__ __forceinline__ uint32_t many_count_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
-                                                    uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt,
-                                                    uint32_t *s_w)
+__global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
+{
+    constexpr int NWV = THREADS / 64;
+    __shared__ __align__(16) uint32_t s_pf[(FS & 0x10) ? MANY_PWORDS : 1];
+    extern __shared__ __align__(16) uint32_t s_dyn[];
+    __shared__ uint16_t s_pos[NWV][MANY_WLIST];
+    __shared__ uint8_t s_rem[NWV][MANY_WLIST];
+    constexpr uint32_t PW = (FS & 0x10) ? MANY_PWORDS : 0u;   // host layout: the prefix filter first
+    if (FS & 0x10) {
+        const uint4 *src = (const uint4 *)a.d.filter;
+        for (uint32_t i = threadIdx.x; i < MANY_PWORDS / 4; i += THREADS)
+            ((uint4 *)s_pf)[i] = src[i];
+    }
+    for (uint32_t i = PW + threadIdx.x; i < a.d.fwords; i += THREADS)
+        s_dyn[i - PW] = a.d.filter[i];
+    __syncthreads();
+    const DictDev &d = a.d;
+    const uint32_t *const pf = (FS & 0x10) ? s_pf : s_dyn;
+    const uint32_t *const small = s_dyn - PW;   // filters 1, 2 at their word offsets
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint16_t *const l_pos = s_pos[wid];
+    uint8_t *const l_rem = s_rem[wid];
+    const uint32_t stride = gridDim.x * NWV;
+    uint64_t wave_total = 0;
+    uint32_t wt = blockIdx.x * NWV + wid;
+    ManyWords W;
+    if (wt < a.num_wtiles)
+        W = many_load(a, wt, 0);
+    for (; wt < a.num_wtiles; wt += stride) {
+        const ManyLane m = many_lane_setup(a, wt);
+        const ManyCand cand = many_candidates<FS>(a, wt, wt + stride, m, pf, small, W);
+        const uint32_t total_c = cand.total;
+        uint32_t *const store = a.cands ? a.cands + (size_t)wt * MANY_WCAP : nullptr;
+        uint32_t kept = 0, tile_total = 0;
+        for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_WLIST) {
+            const uint32_t n = min(MANY_WLIST, total_c - c0);
+            many_list_chunk(a, m, cand, c0, l_pos, l_rem);
+            wave_lds_sync();
+            for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                uint32_t lp = 0, rm = 0, c = 0;
+                if (i < n) {
+                    lp = l_pos[i];
+                    rm = l_rem[i];
+                    if (rm)
+                        c = probe_position<(FS & 0x02) != 0>(d, pf, m.g_text, lp, (uint64_t)m.base + lp, rm, false,
+                                                              nullptr, nullptr, 0, 0);
+                }
+                tile_total += c;
+                if (store) {
+                    // keep the candidates with matches, in position order
+                    const uint64_t has = __ballot(c != 0);
+                    const uint32_t at = kept + (uint32_t)__popcll(has & ((1ull << lane) - 1ull));
+                    if (c && at < MANY_WCAP)
+                        store[at] = lp | rm << CAND_POS_BITS | (c < CAND_RECOUNT ? c : CAND_RECOUNT) << CAND_CNT_SHIFT;
+                    kept += (uint32_t)__popcll(has);
+                }
+            }
+            wave_lds_sync();   // the list is rewritten by the next chunk
+        }
+        const uint32_t tt = wave_incl_add(tile_total);
+        const uint32_t tsum = __shfl(tt, 63, 64);
+        if (lane == 0) {
+            a.tile_count[wt] = tsum;
+            a.tile_list[wt] = (store && kept <= MANY_WCAP) ? (uint64_t)kept : TILE_LIST_NONE;
+        }
+        wave_total += tsum;
+    }
+    if (lane == 0 && wave_total)
+        atomicAdd((unsigned long long *)a.meta, (unsigned long long)wave_total);
+}
+
+// A wave tile whose candidates were not stored: per part, each lane filters
+// its positions again (filter read from global memory) and walks its own
+// candidates, counting, then writing from out0 on.
+template <int FS>
+__device__ __forceinline__ void many_write_unstored(const ManyArgs &a, uint32_t wt, uint64_t out0)
 {
     const DictDev &d = a.d;
-    const ManyTile m = many_tile_setup<THREADS>(a, t);
-    const uint64_t cand = many_filter<FS>(d, s_filter, m);
-    uint32_t total_c;
-    const uint32_t cex = block_excl_sum<THREADS>((uint32_t)__popcll(cand), s_w, total_c);
-    uint32_t own = 0, kept = 0;   // kept: candidates with matches stored so far (every thread)
-    const uint64_t off = (total_c && a.cands && total_c <= a.cand_per_tile) ? (uint64_t)t * a.cand_per_tile
-                                                                            : TILE_LIST_NONE;
-    for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_LCAP) {
-        const uint32_t n = min((uint32_t)MANY_LCAP, total_c - c0);
-        many_list_chunk(a, m, cand, cex, c0, s_cand, s_rem);
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
-            const uint32_t lp = s_cand[i], rm = s_rem[i];
-            const uint32_t c =
-                rm ? probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, false, nullptr, nullptr, 0, 0)
-                   : 0u;
-            own += c;
-            s_cnt[i] = c;
-        }
-        __syncthreads();
-        if (off != TILE_LIST_NONE) {
-            // store the candidates that have matches, in position order: each
-            // thread takes a contiguous run of the chunk
-            const uint32_t K = (n + THREADS - 1) / THREADS;
-            const uint32_t i0 = min(n, threadIdx.x * K), i1 = min(n, i0 + K);
-            uint32_t nz = 0;
-            for (uint32_t i = i0; i < i1; i++)
-                nz += s_cnt[i] != 0;
-            uint32_t chunk_kept;
-            uint32_t at = kept + block_excl_sum<THREADS>(nz, s_w, chunk_kept);
-            for (uint32_t i = i0; i < i1; i++) {
-                const uint32_t c = s_cnt[i];
-                if (c)
-                    a.cands[off + at++] = s_cand[i] | (uint32_t)s_rem[i] << 14 | (c < CAND_RECOUNT ? c : CAND_RECOUNT)
-                                                                                    << 21;
+    const ManyLane m = many_lane_setup(a, wt);
+    uint32_t di = m.di0, dlim = m.dlim0, dnext = m.dnext0;
+    for (uint32_t h = 0; h < MANY_H; h++) {
+        const uint32_t g0 = m.base + h * MANY_PART + m.lo;
+        const uint32_t lim = g0 < a.n ? min(a.n - g0, MANY_LB) : 0u;
+        const uint32_t cand = lim ? many_filter<FS>(d, d.filter, d.filter, many_load(a, wt, h), lim) : 0u;
+        uint32_t own = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            uint64_t r = 0;
+            if (pass) {
+                const uint32_t inc = wave_incl_add(own);
+                r = out0 + (inc - own);
+                out0 += __shfl(inc, 63, 64);
             }
-            kept += chunk_kept;
+            uint32_t pdi = di, pdlim = dlim, pdnext = dnext;
+            uint32_t q = cand;
+            while (q) {
+                const uint32_t lp = h * MANY_PART + m.lo + (uint32_t)__builtin_ctz(q);
+                q &= q - 1;
+                const uint32_t gp = m.base + lp;
+                while (gp >= pdnext) {
+                    pdi++;
+                    pdlim = a.doc_ends[pdi];
+                    pdnext = (pdi + 1 < a.ndocs) ? a.doc_starts[pdi + 1] : 0xFFFFFFFFu;
+                }
+                const uint32_t rm = gp < pdlim ? min(pdlim - gp, 64u) : 0u;
+                if (!rm)
+                    continue;
+                if (!pass)
+                    own += probe_position<false>(d, nullptr, m.g_text, lp, gp, rm, false, nullptr, nullptr, 0, 0);
+                else if (r < a.cap)
+                    r += probe_position<false>(d, nullptr, m.g_text, lp, gp, rm, true, a.out_off, a.out_id, r, a.cap);
+            }
+            if (pass) {
+                di = pdi;
+                dlim = pdlim;
+                dnext = pdnext;
+            }
         }
-        __syncthreads();   // the list is rewritten by the next chunk
     }
-    __syncthreads();   // s_w is reused
-    uint32_t total;
-    (void)block_excl_sum<THREADS>(own, s_w, total);
-    if (threadIdx.x == 0)
-        a.tile_list[t] = off == TILE_LIST_NONE ? TILE_LIST_NONE : (off << 24 | kept);
-    __syncthreads();
-    return total;
 }
 
-// Write pass over one tile: its matches from out0 on, in position then
-// pattern-id order.  With stored candidates (those with matches): read them,
-// offsets from their counts, one probe each.  Without: filter and count again
-// first.
+// Write pass.  A wave takes MANY_JOB consecutive wave tiles (a job; lane j
+// holds tile j's stored count and output offset) and walks the stored candidates of
+// all of them as one list, 64 per round, one candidate per lane, so that a
+// round's probes fill the wave whatever the tiles' candidate counts.  Lane k
+// of a round finds its tile as the last tile starting at or before it
+// (markers in LDS, a prefix maximum over the lanes); its output offset is
+// its tile's offset plus the counts of the tile's earlier candidates (a
+// running sum over the list minus that sum at the tile's first candidate).
+// Tiles whose candidates were not stored are redone afterwards, one at a
+// time.
 template <int THREADS, int FS>
-__device__ __forceinline__ void many_write_tile(const ManyArgs &a, uint32_t t, const uint32_t *s_filter,
-                                                uint16_t *s_cand, uint8_t *s_rem, uint32_t *s_cnt, uint32_t *s_w,
-                                                uint64_t out0)
+__global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
 {
+    constexpr int NWV = THREADS / 64;
+    __shared__ uint32_t s_mark[NWV][64];
+    __shared__ uint64_t s_first[NWV][64];
     const DictDev &d = a.d;
-    const ManyTile m = many_tile_setup<THREADS>(a, t);
-    const uint64_t stored = a.tile_list[t];
-    uint64_t cand = 0;
-    uint32_t total_c, cex = 0;
-    if (stored != TILE_LIST_NONE) {
-        total_c = (uint32_t)(stored & 0xFFFFFFu);
-    } else {
-        cand = many_filter<FS>(d, s_filter, m);
-        cex = block_excl_sum<THREADS>((uint32_t)__popcll(cand), s_w, total_c);
-    }
-    uint64_t run = out0;
-    for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_LCAP) {
-        const uint32_t n = min((uint32_t)MANY_LCAP, total_c - c0);
-        if (stored != TILE_LIST_NONE) {
-            const uint32_t *src = a.cands + (stored >> 24) + c0;
-            for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
-                const uint32_t e = src[i];
-                const uint32_t lp = e & 0x3FFFu, rm = (e >> 14) & 0x7Fu;
-                uint32_t c = e >> 21;
-                if (c == CAND_RECOUNT)
-                    c = probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, false, nullptr, nullptr, 0, 0);
-                s_cand[i] = (uint16_t)lp;
-                s_rem[i] = (uint8_t)rm;
-                s_cnt[i] = c;
-            }
-        } else {
-            many_list_chunk(a, m, cand, cex, c0, s_cand, s_rem);
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
-                const uint32_t lp = s_cand[i], rm = s_rem[i];
-                s_cnt[i] = rm ? probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, false, nullptr,
-                                               nullptr, 0, 0)
-                              : 0u;
-            }
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t *const mark = s_mark[wid];
+    uint64_t *const first = s_first[wid];
+    const uint32_t jobs = (a.num_wtiles + MANY_JOB - 1) / MANY_JOB;
+    for (uint32_t job = blockIdx.x * NWV + wid; job < jobs; job += gridDim.x * NWV) {
+        const uint32_t wt = job * MANY_JOB + lane;
+        const bool valid = lane < MANY_JOB && wt < a.num_wtiles;
+        const uint64_t st = valid ? a.tile_list[wt] : 0ull;
+        const uint64_t off = valid ? a.tile_off[wt] : 0ull;
+        const uint32_t n = st == TILE_LIST_NONE ? 0u : (uint32_t)st;
+        const uint32_t inc = wave_incl_add(n);
+        const uint32_t ex = inc - n;
+        const uint32_t total = __shfl(inc, 63, 64);
+        int32_t carry = -1;
+        uint64_t run = 0;   // counts of the list's candidates before this round
+        for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+            mark[lane] = 0xFFFFFFFFu;
+            wave_lds_sync();
+            if (n && ex >= r0 && ex < r0 + 64)
+                mark[ex - r0] = lane;
+            wave_lds_sync();
+            const int32_t j = max(wave_incl_max((int32_t)mark[lane]), carry);
+            carry = __shfl(j, 63, 64);
+            const uint32_t k = r0 + lane;
+            const bool act = k < total;
+            const uint32_t tj = (uint32_t)(j < 0 ? 0 : j);
+            const uint32_t i = k - (uint32_t)__shfl((int)ex, (int)tj, 64);
+            const uint64_t off_j = __shfl(off, (int)tj, 64);
+            const uint32_t wt_j = job * MANY_JOB + tj;
+            const uint32_t base = wt_j * MANY_WT;
+            const uint8_t *const g_text = a.text + base;
+            const uint32_t e = act ? a.cands[(size_t)wt_j * MANY_WCAP + i] : 0u;
+            const uint32_t lp = e & (MANY_WT - 1u), rm = (e >> CAND_POS_BITS) & 0x7Fu;
+            uint32_t c = e >> CAND_CNT_SHIFT;
+            if (c == CAND_RECOUNT)
+                c = probe_position<false>(d, nullptr, g_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0);
+            const uint32_t cinc = wave_incl_add(c);
+            const uint64_t sk = run + (cinc - c);   // counts of the list before candidate k
+            if (act && i == 0)
+                first[tj] = sk;
+            wave_lds_sync();
+            const uint64_t r = off_j + (sk - (act ? first[tj] : sk));
+            if (act && c && r < a.cap)
+                probe_position<false>(d, nullptr, g_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r,
+                                      a.cap);
+            run += __shfl(cinc, 63, 64);
+            wave_lds_sync();   // mark[] is rewritten by the next round
         }
-        __syncthreads();
-        const uint32_t ctot = many_chunk_offsets<THREADS>(s_cnt, n, s_w);
-        for (uint32_t i = threadIdx.x; i < n; i += THREADS) {
-            const uint32_t lp = s_cand[i], rm = s_rem[i];
-            const uint64_t r = run + s_cnt[i];
-            if (rm && r < a.cap)
-                probe_position(d, s_filter, m.g_text, lp, (uint64_t)m.base + lp, rm, true, a.out_off, a.out_id, r,
-                               a.cap);
-        }
-        run += ctot;
-        __syncthreads();   // the list is rewritten by the next chunk
-    }
-    __syncthreads();   // s_w is reused by the next tile
-}
-
-template <int THREADS, bool WRITE, int FS>
-__global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
-{
-    extern __shared__ __align__(16) uint32_t s_filter[];
-    __shared__ uint16_t s_cand[MANY_LCAP];
-    __shared__ uint8_t s_rem[MANY_LCAP];
-    __shared__ uint32_t s_cnt[MANY_LCAP];
-    __shared__ uint32_t s_w[THREADS / 64];
-    for (uint32_t i = threadIdx.x; i < a.d.fwords; i += THREADS)
-        s_filter[i] = a.d.filter[i];
-    __syncthreads();
-    for (uint32_t t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
-        if (WRITE) {
-            many_write_tile<THREADS, FS>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w, a.tile_off[t]);
-        } else {
-            const uint32_t total = many_count_tile<THREADS, FS>(a, t, s_filter, s_cand, s_rem, s_cnt, s_w);
-            if (threadIdx.x == 0) {
-                a.tile_count[t] = total;
-                if (total)
-                    atomicAdd((unsigned long long *)a.meta, (unsigned long long)total);
-            }
+        uint64_t redo = __ballot(valid && st == TILE_LIST_NONE);
+        while (redo) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(redo);
+            redo &= redo - 1;
+            many_write_unstored<FS>(a, job * MANY_JOB + j, __shfl(off, (int)j, 64));
         }
     }
 }
@@ -679,15 +780,17 @@ __global__ __launch_bounds__(THREADS) void many_tiles_kernel(ManyArgs a)
 // nothing later) and adds them up; the last chunk's workgroup writes the
 // grand total to meta[0].  Chunk totals live in `sums` (zero at launch).
 template <int ST>
-__global__ __launch_bounds__(ST) void many_scan_kernel(ManyArgs a, uint64_t *sums)
+__global__ __launch_bounds__(ST) void many_scan_kernel(ManyArgs a)
 {
+    uint64_t *const sums = a.scan_sums;
     constexpr int K = 8;
     constexpr int CH = ST * K;
+    static_assert(CH == MANY_SCAN_CHUNK, "scan chunk");
     constexpr int NWS = ST / 64;
     __shared__ uint64_t s_wt[NWS];
     __shared__ uint64_t s_carry;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t T = a.num_tiles;
+    const uint32_t T = a.num_wtiles;
     const uint32_t c0 = blockIdx.x * CH;
     uint32_t loc[K];
     uint64_t m = 0;
@@ -743,40 +846,45 @@ __global__ __launch_bounds__(ST) void many_scan_kernel(ManyArgs a, uint64_t *sum
         a.meta[0] = s_carry + tot;
 }
 
-size_t ol_many_lds_bytes(uint32_t fwords)
+// Dynamic LDS of the count pass: the filters after the prefix filter.
+size_t ol_many_lds_bytes(const DictDev &d)
 {
-    return (size_t)fwords * 4;
+    return (size_t)(d.fwords - (d.fbits[4] ? MANY_PWORDS : 0u)) * 4;
 }
 
 template <bool WRITE, int FS>
-static hipError_t launch_many_tiles_fs(const ManyArgs &a, hipStream_t s)
+static hipError_t launch_many_fs(const ManyArgs &a, hipStream_t s)
 {
-    const size_t lds = ol_many_lds_bytes(a.d.fwords);
-    const void *f = (const void *)many_tiles_kernel<SEARCH_THREADS, WRITE, FS>;
+    const size_t lds = WRITE ? 0 : ol_many_lds_bytes(a.d);
+    const void *f = WRITE ? (const void *)many_write_kernel<MANY_THREADS, FS>
+                          : (const void *)many_count_kernel<MANY_THREADS, FS>;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    uint32_t cap = max_grid(f, SEARCH_THREADS, lds);
+    const uint32_t cap = max_grid(f, MANY_THREADS, lds);
     if (!cap)
         return hipErrorInvalidValue;
-    const uint32_t grid = a.num_tiles < 4 * cap ? a.num_tiles : 4 * cap;
-    hipLaunchKernelGGL((many_tiles_kernel<SEARCH_THREADS, WRITE, FS>), dim3(grid), dim3(SEARCH_THREADS), lds, s, a);
+    // persistent waves: workgroups beyond the co-resident ones would only
+    // copy the filter again
+    const uint32_t units = WRITE ? (a.num_wtiles + MANY_JOB - 1) / MANY_JOB : a.num_wtiles;   // per wave: jobs / wave tiles
+    const uint32_t need = (units + MANY_THREADS / 64 - 1) / (MANY_THREADS / 64);
+    const uint32_t grid = need < cap ? need : cap;
+    if (WRITE)
+        hipLaunchKernelGGL((many_write_kernel<MANY_THREADS, FS>), dim3(grid), dim3(MANY_THREADS), 0, s, a);
+    else
+        hipLaunchKernelGGL((many_count_kernel<MANY_THREADS, FS>), dim3(grid), dim3(MANY_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
-// The kernel for the filters the dictionary has: 4-byte prefixes (with the
-// 3-byte keys folded in, the usual case), plus 3-byte keys, plus 1/2-byte keys.
+// The kernels for the filters the dictionary has: the prefix filter (classes
+// 3..5, the usual case), the 1/2-byte filters, or both.
 template <bool WRITE>
-static hipError_t launch_many_tiles(const ManyArgs &a, hipStream_t s)
+static hipError_t launch_many(const ManyArgs &a, hipStream_t s)
 {
-    const int fs = (a.d.fbits[4] ? 0x10 : 0) | (a.d.fbits[3] ? 0x08 : 0) | (a.d.fbits[1] || a.d.fbits[2] ? 0x02 : 0);
+    const int fs = (a.d.fbits[4] ? 0x10 : 0) | (a.d.fbits[1] || a.d.fbits[2] ? 0x02 : 0);
     switch (fs) {
-    case 0x10: return launch_many_tiles_fs<WRITE, 0x10>(a, s);
-    case 0x18: return launch_many_tiles_fs<WRITE, 0x18>(a, s);
-    case 0x12: return launch_many_tiles_fs<WRITE, 0x12>(a, s);
-    case 0x1A: return launch_many_tiles_fs<WRITE, 0x1A>(a, s);
-    case 0x08: return launch_many_tiles_fs<WRITE, 0x08>(a, s);
-    case 0x0A: return launch_many_tiles_fs<WRITE, 0x0A>(a, s);
-    case 0x02: return launch_many_tiles_fs<WRITE, 0x02>(a, s);
+    case 0x10: return launch_many_fs<WRITE, 0x10>(a, s);
+    case 0x12: return launch_many_fs<WRITE, 0x12>(a, s);
+    case 0x02: return launch_many_fs<WRITE, 0x02>(a, s);
     default: return hipErrorInvalidValue;   // a dictionary has at least one filter
     }
 }
@@ -784,14 +892,16 @@ static hipError_t launch_many_tiles(const ManyArgs &a, hipStream_t s)
 // Counts (total in meta[0]); with outputs also the scan and the writes.
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
 {
-    if (a.num_tiles == 0)
+    if (a.num_wtiles == 0)
         return hipSuccess;
-    hipError_t e = launch_many_tiles<false>(a, s);
+    if (a.num_wtiles != a.num_tiles * MANY_WPT)
+        return hipErrorInvalidValue;
+    hipError_t e = launch_many<false>(a, s);
     if (e != hipSuccess || (!a.out_off && !a.out_id))
         return e;
-    const uint32_t chunks = (a.num_tiles + 8 * 1024 - 1) / (8 * 1024);
-    hipLaunchKernelGGL((many_scan_kernel<1024>), dim3(chunks), dim3(1024), 0, s, a, a.tile_list + a.num_tiles);
+    const uint32_t chunks = (a.num_wtiles + MANY_SCAN_CHUNK - 1) / MANY_SCAN_CHUNK;
+    hipLaunchKernelGGL((many_scan_kernel<1024>), dim3(chunks), dim3(1024), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    return launch_many_tiles<true>(a, s);
+    return launch_many<true>(a, s);
 }

@@ -122,6 +122,24 @@ def test_start_before_text_and_saturation(parser):
     assert_many_matches(parser, b"a" * (1 << 18), [b"aaaa", b"a", b"aa", b"zq001x"], "saturated")
 
 
+def test_dense_candidates_few_matches(parser):
+    """Every position passes the prefix filter (the filter keeps only the low 5
+    bits of a key's fourth byte: 'a' and 'A' agree there) but few match, so a
+    wave tile's candidates take several list chunks while its matches are
+    stored."""
+    text = bytearray(b"a" * 20000)
+    for at in (0, 700, 2047, 2048, 5000, 19995):
+        text[at:at + 5] = b"aaaAx"
+    assert_many_matches(parser, bytes(text), [b"aaaAx", b"aaA", b"zzzz"], "dense candidates")
+
+
+def test_duplicate_patterns_count_again(parser):
+    """17,000 copies of one pattern: a position with more matches than a
+    stored candidate's count field holds is counted again by the write pass."""
+    text = b"q" * 3000 + b"abc" + b"q" * 100 + b"abcabc" + b"q" * 10
+    assert_many_matches(parser, text, [b"abc"] * 17000 + [b"bca", b"qqqq"], "duplicates")
+
+
 def test_many_tiles_per_workgroup_equal_single_pattern_searches(parser):
     """64 MiB (4,096 tiles, more than one per resident workgroup): each
     pattern's hits equal its own literal search (the reference's
