@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--size", type=int, default=1 << 31, help="document bytes")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--search", type=str, default="literal,dict1024,dict65536,query,olb",
+                    help="search-side workloads to run (comma list of literal, dict<count>, query, olb)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 31, help="CPU baseline bytes (a prefix of the document)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -207,9 +209,12 @@ def main():
 
     extra = {}
     if not args.no_search:
-        extra = search_benchmarks(p, doc, text, args)
-        extra["query"] = query_benchmark(p, doc, tree, L)
-        extra["olb"] = olb_benchmark(p, text)
+        which = set(x for x in args.search.split(",") if x)
+        extra = search_benchmarks(p, doc, text, which)
+        if "query" in which:
+            extra["query"] = query_benchmark(p, doc, tree, L)
+        if "olb" in which:
+            extra["olb"] = olb_benchmark(p, text)
     del doc, tree, text
     if not args.no_batch:
         extra["batch"] = batch_benchmark(p, args)
@@ -501,7 +506,7 @@ def olb_benchmark(p, text, size=256 << 20):
     return res
 
 
-def search_benchmarks(p, doc, text, args):
+def search_benchmarks(p, doc, text, which):
     """Literal and dictionary search over the resident document (device output)."""
     import ctypes
 
@@ -520,7 +525,7 @@ def search_benchmarks(p, doc, text, args):
 
     # literal (C3a): lengths 4..64, vocabulary phrases (kind 0) and random lowercase (kind 1)
     lits = []
-    for kind in (0, 1):
+    for kind in (0, 1) if "literal" in which else ():
         for ln in (4, 8, 16, 32, 64):
             pat = datagen.literal(ln, kind=kind, seed=0xC3A)
             tot = ctypes.c_uint64()
@@ -544,9 +549,11 @@ def search_benchmarks(p, doc, text, args):
                          "frac_of_hbm_peak": round(b / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
             hip.hipFree(d_off)
             log(f"[bench] literal {lits[-1]}")
-    res["literal_search"] = lits
+    if lits:
+        res["literal_search"] = lits
 
-    for count in (1024, 65536):
+    tot = ctypes.c_uint64()
+    for count in sorted(int(x[4:]) for x in which if x.startswith("dict")):
         pats = datagen.patterns(count, seed=0xC3B)
         pset = p.compile_patterns(doc, pats)
         ol._check(lib.ol_search_many_device(p._ctx, doc._h, pset._h, None, None, 0, ctypes.byref(tot)))

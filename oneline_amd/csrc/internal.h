@@ -371,6 +371,7 @@ struct ManyArgs {
     uint32_t *tile_count;    // [num_wtiles] matches per wave tile
     uint64_t *tile_off;      // [num_wtiles] first output index of each wave tile
     uint64_t *scan_sums;     // [num_wtiles / MANY_SCAN_CHUNK + 1] chunk totals of the scan (zero at launch)
+    uint32_t job_tiles;      // write pass: wave tiles per wave job (set by the launcher)
     // Candidate store (nullable): the candidates of wave tile wt that have
     // matches go to cands[wt * MANY_WT_BYTES / 4 ..] when they fit;
     // tile_list[wt] records their number, or ~0 when they were not stored.

@@ -410,7 +410,7 @@ constexpr uint32_t MANY_WPT = SEARCH_TILE / MANY_WT;        // wave tiles per se
 constexpr uint32_t MANY_WCAP = MANY_WT / 4;                 // stored candidates per wave tile
 constexpr uint32_t MANY_WLIST = 512;                        // per-wave LDS list chunk
 constexpr uint32_t MANY_PWORDS = 1u << (DICT_PBITS - 5);    // prefix filter words
-constexpr uint32_t MANY_JOB = MANY_WT <= 4096 ? 64 : 64 * 4096 / MANY_WT;   // wave tiles per write job
+constexpr uint32_t MANY_JOB_MAX = MANY_WT <= 4096 ? 64 : 64 * 4096 / MANY_WT;   // wave tiles per write job, at most
 static_assert(MANY_WPT * MANY_WT == SEARCH_TILE && MANY_H * MANY_PART == MANY_WT, "wave tiles");
 
 // Stored candidate: wave-tile position (CAND_POS_BITS) | bytes before the
@@ -700,8 +700,8 @@ __device__ __forceinline__ void many_write_unstored(const ManyArgs &a, uint32_t 
     }
 }
 
-// Write pass.  A wave takes MANY_JOB consecutive wave tiles (a job; lane j
-// holds tile j's stored count and output offset) and walks the stored candidates of
+// Write pass.  A wave takes job_tiles (at most MANY_JOB_MAX) consecutive wave
+// tiles (a job; lane j holds tile j's stored count and output offset) and walks the stored candidates of
 // all of them as one list, 64 per round, one candidate per lane, so that a
 // round's probes fill the wave whatever the tiles' candidate counts.  Lane k
 // of a round finds its tile as the last tile starting at or before it
@@ -720,10 +720,11 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     uint32_t *const mark = s_mark[wid];
     uint64_t *const first = s_first[wid];
-    const uint32_t jobs = (a.num_wtiles + MANY_JOB - 1) / MANY_JOB;
+    const uint32_t JT = a.job_tiles;
+    const uint32_t jobs = (a.num_wtiles + JT - 1) / JT;
     for (uint32_t job = blockIdx.x * NWV + wid; job < jobs; job += gridDim.x * NWV) {
-        const uint32_t wt = job * MANY_JOB + lane;
-        const bool valid = lane < MANY_JOB && wt < a.num_wtiles;
+        const uint32_t wt = job * JT + lane;
+        const bool valid = lane < JT && wt < a.num_wtiles;
         const uint64_t st = valid ? a.tile_list[wt] : 0ull;
         const uint64_t off = valid ? a.tile_off[wt] : 0ull;
         const uint32_t n = st == TILE_LIST_NONE ? 0u : (uint32_t)st;
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
             const uint32_t tj = (uint32_t)(j < 0 ? 0 : j);
             const uint32_t i = k - (uint32_t)__shfl((int)ex, (int)tj, 64);
             const uint64_t off_j = __shfl(off, (int)tj, 64);
-            const uint32_t wt_j = job * MANY_JOB + tj;
+            const uint32_t wt_j = job * JT + tj;
             const uint32_t base = wt_j * MANY_WT;
             const uint8_t *const g_text = a.text + base;
             const uint32_t e = act ? a.cands[(size_t)wt_j * MANY_WCAP + i] : 0u;
@@ -769,7 +770,7 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
         while (redo) {
             const uint32_t j = (uint32_t)__builtin_ctzll(redo);
             redo &= redo - 1;
-            many_write_unstored<FS>(a, job * MANY_JOB + j, __shfl(off, (int)j, 64));
+            many_write_unstored<FS>(a, job * JT + j, __shfl(off, (int)j, 64));
         }
     }
 }
@@ -865,13 +866,22 @@ static hipError_t launch_many_fs(const ManyArgs &a, hipStream_t s)
         return hipErrorInvalidValue;
     // persistent waves: workgroups beyond the co-resident ones would only
     // copy the filter again
-    const uint32_t units = WRITE ? (a.num_wtiles + MANY_JOB - 1) / MANY_JOB : a.num_wtiles;   // per wave: jobs / wave tiles
-    const uint32_t need = (units + MANY_THREADS / 64 - 1) / (MANY_THREADS / 64);
-    const uint32_t grid = need < cap ? need : cap;
-    if (WRITE)
-        hipLaunchKernelGGL((many_write_kernel<MANY_THREADS, FS>), dim3(grid), dim3(MANY_THREADS), 0, s, a);
-    else
-        hipLaunchKernelGGL((many_count_kernel<MANY_THREADS, FS>), dim3(grid), dim3(MANY_THREADS), lds, s, a);
+    constexpr uint32_t NWV = MANY_THREADS / 64;
+    if (WRITE) {
+        // jobs of up to MANY_JOB_MAX wave tiles, fewer when the text is too
+        // small to give every resident wave one
+        ManyArgs w = a;
+        const uint32_t per_wave = a.num_wtiles / (cap * NWV);
+        w.job_tiles = per_wave < 1 ? 1 : per_wave > MANY_JOB_MAX ? MANY_JOB_MAX : per_wave;
+        const uint32_t jobs = (a.num_wtiles + w.job_tiles - 1) / w.job_tiles;
+        const uint32_t need = (jobs + NWV - 1) / NWV;
+        hipLaunchKernelGGL((many_write_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap), dim3(MANY_THREADS), 0,
+                           s, w);
+        return hipGetLastError();
+    }
+    const uint32_t need = (a.num_wtiles + NWV - 1) / NWV;
+    hipLaunchKernelGGL((many_count_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap), dim3(MANY_THREADS), lds, s,
+                       a);
     return hipGetLastError();
 }
 
