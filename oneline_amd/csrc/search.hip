@@ -9,9 +9,10 @@
 // crates/oneline-simd/src/search.rs:14-99; GPU: crates/oneline-gpu/src/search.wgsl:100-173).
 // Dictionary: every (start, id) where pattern id occurs, duplicates reporting
 // each id (ManyPatterns, crates/oneline-simd/src/search.rs:206-283; GPU:
-// crates/oneline-gpu/src/many_fused.wgsl:71-141).  Patterns are keyed by their
-// first min(len, 4) bytes; a filter bitmap per key length sits in LDS, the
-// hash tables, entries and pattern bytes are read through L2.
+// crates/oneline-gpu/src/many_fused.wgsl:71-141).  Patterns are keyed by class
+// (their first 1, 2, 3, 4 or 8 bytes); one prefix filter (plus the 1- and
+// 2-byte bitmaps, when used) sits in LDS, the hash tables, entries and pattern
+// bytes are read through L2.
 #include "internal.h"
 
 // Block-wide exclusive sum of `v` (THREADS/64 waves); also returns the total.
