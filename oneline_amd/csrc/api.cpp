@@ -12,10 +12,13 @@
 #include <sys/stat.h>
 #include <unistd.h>
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -83,7 +86,11 @@ struct ol_ctx {
     uint32_t scan_max_grid[3] = {0, 0, 0}; // co-resident limit per bpt (64, 128, 256), from the occupancy API
     uint32_t tree_rounds = 16;             // rounds of 64 lines per tree wave (tile = 256 x rounds lines)
     bool scan_ticket = false;              // scan tiles from an atomic ticket (ordinary launch)
-    uint8_t *stage = nullptr;              // pinned host staging for batch uploads (lazily allocated)
+    // pinned staging ring for host -> HBM uploads: two slots per copy worker
+    // (each allocated on first use), each with the event of its last DMA
+    uint8_t *ring[2 * 8] = {};
+    hipEvent_t ring_ev[2 * 8] = {};
+    bool ring_used[2 * 8] = {};
     // profiling
     bool profiling = false;
     struct Rec {
@@ -268,6 +275,71 @@ int launch_scan(ol_ctx *c, ol_doc *d)
     return launch_scan_args(c, a);
 }
 
+// Host -> HBM upload through the context's pinned staging ring: STAGE_WORKERS
+// threads each take every STAGE_WORKERS-th chunk of STAGE_SLOT bytes, fill one
+// of their two pinned slots (`fill` writes bytes [off, off + len) of the
+// source), and queue its DMA on the context stream; a slot is filled again
+// once the event after its last DMA has completed, so host copies overlap the
+// DMA of earlier chunks.  Kernels queued on the stream afterwards see the
+// whole text.  Sources under STAGE_MIN bytes go through one slot, in order.
+constexpr size_t STAGE_SLOT = 8u << 20;
+constexpr int STAGE_WORKERS = 8;   // the context holds 2 * 8 slots
+constexpr size_t STAGE_MIN = 32u << 20;
+
+int stage_upload(ol_ctx *c, uint8_t *dst, uint64_t n, const std::function<void(uint8_t *, uint64_t, uint64_t)> &fill)
+{
+    if (!n)
+        return OL_OK;
+    const uint64_t chunks = (n + STAGE_SLOT - 1) / STAGE_SLOT;
+    const int workers = n < STAGE_MIN ? 1 : (int)std::min<uint64_t>(STAGE_WORKERS, chunks);
+    for (int i = 0; i < 2 * workers; i++) {
+        if (!c->ring[i] && hipHostMalloc((void **)&c->ring[i], STAGE_SLOT, hipHostMallocDefault) != hipSuccess) {
+            c->ring[i] = nullptr;
+            return fail(OL_ERR_NOMEM, "pinned staging allocation failed");
+        }
+        if (!c->ring_ev[i] && hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming) != hipSuccess) {
+            c->ring_ev[i] = nullptr;
+            return fail(OL_ERR_DEVICE, "staging event creation failed");
+        }
+    }
+    std::atomic<bool> failed{false};
+    auto work = [&](int w) {
+        int k = 0;
+        for (uint64_t ch = (uint64_t)w; ch < chunks && !failed.load(std::memory_order_relaxed);
+             ch += (uint64_t)workers, k ^= 1) {
+            const int slot = 2 * w + k;
+            uint8_t *buf = c->ring[slot];
+            if (c->ring_used[slot] && hipEventSynchronize(c->ring_ev[slot]) != hipSuccess) {
+                failed = true;
+                return;
+            }
+            const uint64_t off = ch * STAGE_SLOT, len = std::min<uint64_t>(STAGE_SLOT, n - off);
+            fill(buf, off, len);
+            if (hipMemcpyAsync(dst + off, buf, len, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                hipEventRecord(c->ring_ev[slot], c->stream) != hipSuccess) {
+                failed = true;
+                return;
+            }
+            c->ring_used[slot] = true;
+        }
+    };
+    if (workers == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        pool.reserve(workers);
+        try {
+            for (int w = 0; w < workers; w++)
+                pool.emplace_back(work, w);
+        } catch (...) {
+            failed = true;
+        }
+        for (auto &t : pool)
+            t.join();
+    }
+    return failed ? fail(OL_ERR_DEVICE, "text upload failed") : OL_OK;
+}
+
 void sample_histogram(const uint8_t *t, size_t n, uint32_t *freq)
 {
     // sample_byte_histogram, crates/oneline-gpu/src/search.rs:176-201
@@ -316,8 +388,12 @@ int upload_doc(ol_ctx *c, const uint8_t *text, size_t len, bool with_nodes, ol_d
         d->text = nullptr;
         return bail(fail(OL_ERR_NOMEM, "device allocation of the text failed"));
     }
-    if (hipMemcpyAsync(d->text, text, len, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemsetAsync(d->text + len, 0, padded - len, c->stream) != hipSuccess)
+    int up = stage_upload(c, d->text, len, [text](uint8_t *buf, uint64_t off, uint64_t k) {
+        std::memcpy(buf, text + off, k);
+    });
+    if (up)
+        return bail(up);
+    if (hipMemsetAsync(d->text + len, 0, padded - len, c->stream) != hipSuccess)
         return bail(fail(OL_ERR_DEVICE, "text upload failed"));
     uint64_t cap = std::min<uint64_t>(len, len / 16 + 4096);
     int rc = alloc_doc_lines(d, cap);
@@ -850,8 +926,10 @@ void ol_ctx_destroy(ol_ctx *c)
     c->scratch.release();
     c->tree_sum.release();
     c->fix.release();
-    if (c->stage)
-        (void)hipHostFree(c->stage);
+    for (int i = 0; i < 16; i++) {
+        if (c->ring[i]) (void)hipHostFree(c->ring[i]);
+        if (c->ring_ev[i]) (void)hipEventDestroy(c->ring_ev[i]);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1931,42 +2009,40 @@ int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, s
         b->text = nullptr;
         return bail(fail(OL_ERR_NOMEM, "device allocation of the batch text failed"));
     }
-    // copy through the context's pinned staging buffer
-    const size_t STAGE = 32u << 20;
-    if (!c->stage && hipHostMalloc((void **)&c->stage, STAGE, hipHostMallocDefault) != hipSuccess) {
-        c->stage = nullptr;
-        return bail(fail(OL_ERR_NOMEM, "pinned staging allocation failed"));
-    }
-    uint8_t *stage = c->stage;
-    size_t fill = 0;
-    uint64_t dst = 0;
-    int rc = OL_OK;
-    auto flush = [&]() -> bool {
-        if (!fill)
-            return true;
-        if (hipMemcpy(b->text + dst, stage, fill, hipMemcpyHostToDevice) != hipSuccess)
-            return false;
-        dst += fill;
-        fill = 0;
-        return true;
+    // the documents back to back, each followed by '\n' unless it ends in one,
+    // through the staging ring
+    std::vector<size_t> order;   // non-empty documents in order
+    order.reserve(b->nd);
+    for (size_t i = 0; i < count; i++)
+        if (lens[i])
+            order.push_back(i);
+    auto fill = [&](uint8_t *buf, uint64_t off, uint64_t k) {
+        // the last non-empty document starting at or before off
+        size_t lo = 0, hi = order.size();
+        while (hi - lo > 1) {
+            const size_t mid = (lo + hi) / 2;
+            if (b->h_start[order[mid]] <= off) lo = mid; else hi = mid;
+        }
+        uint64_t done = 0;
+        for (size_t j = lo; done < k && j < order.size(); j++) {
+            const size_t i = order[j];
+            const uint64_t s0 = b->h_start[i];
+            const uint64_t span = lens[i] + (docs[i][lens[i] - 1] != '\n' ? 1 : 0);
+            uint64_t at = off + done;
+            if (at >= s0 + span)
+                continue;
+            const uint64_t in = at - s0;
+            if (in < lens[i]) {
+                const uint64_t m = std::min<uint64_t>(lens[i] - in, k - done);
+                std::memcpy(buf + done, docs[i] + in, m);
+                done += m;
+            }
+            if (done < k && off + done == s0 + lens[i] && span > lens[i])
+                buf[done++] = '\n';
+        }
     };
-    for (size_t i = 0; i < count && rc == OL_OK; i++) {
-        size_t done = 0;
-        while (done < lens[i]) {
-            const size_t k = std::min(lens[i] - done, STAGE - fill);
-            std::memcpy(stage + fill, docs[i] + done, k);
-            fill += k;
-            done += k;
-            if (fill == STAGE && !flush())
-                rc = fail(OL_ERR_DEVICE, "batch upload failed");
-        }
-        if (lens[i] && docs[i][lens[i] - 1] != '\n') {
-            if (fill == STAGE && !flush())
-                rc = fail(OL_ERR_DEVICE, "batch upload failed");
-            stage[fill++] = '\n';
-        }
-    }
-    if (rc == OL_OK && !flush())
+    int rc = stage_upload(c, b->text, b->n, fill);
+    if (rc == OL_OK && hipStreamSynchronize(c->stream) != hipSuccess)
         rc = fail(OL_ERR_DEVICE, "batch upload failed");
     if (rc)
         return bail(rc);

@@ -75,6 +75,28 @@ def test_generated_documents_all_tree_tiles(parser):
     parser.tune_set("tree_rounds", 16)
 
 
+def test_staged_upload_of_a_large_batch(parser):
+    """A batch above the staging threshold (32 MiB) goes through several copy
+    workers and many 8 MiB slots.  Documents end one byte before, at and after
+    slot boundaries, every other one without its final newline (the batch
+    appends one), with empty documents between."""
+    slot = 8 << 20
+    docs, pos = [], 0
+    for i in range(6):
+        end = (i + 1) * slot + (i % 3) - 1   # where the document's bytes (and newline) end
+        span = end - pos
+        d = datagen.document(span, seed=300 + i).tobytes()
+        if i % 2:
+            d = d[:-2] + b"x"   # no final newline: the batch adds it back, same span
+        docs += [d, b""]
+        pos = end
+    docs += edge_docs()
+    b = parser.batch(docs)
+    assert b.text_bytes() == sum(len(d) for d in docs) and b.text_bytes() > (40 << 20)
+    b.run()
+    check_all(b, docs)
+
+
 def test_many_tiny_documents(parser):
     """Thousands of documents of 64..300 bytes: many documents per scan tile."""
     rng = np.random.default_rng(5)
