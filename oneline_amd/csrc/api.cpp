@@ -564,11 +564,13 @@ int search_doc_host(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, si
 // The dictionary search's per-wave-tile state in the scratch status words:
 // the scan's chunk totals first (the only words cleared), then tile_count
 // (u32), tile_off and tile_list.  The candidate store (when matches are
-// written) holds one candidate in four text positions per wave tile (4 bytes
-// each: the text size in total); a wave tile with more candidates that have
-// matches is filtered and counted again by the write pass.
+// written) holds one candidate in four text positions per wave tile (16 bytes
+// each, with the ids of its first three matches: 4 bytes per text byte); a
+// wave tile with more candidates that have matches is filtered and counted
+// again by the write pass.
 hipError_t setup_many(ol_ctx *c, ManyArgs &a, uint64_t n, bool writes)
 {
+    static_assert(OL_MAX_PATTERNS <= (1u << 22), "stored candidates keep 22-bit pattern ids");
     const uint32_t tiles = (uint32_t)ceil_div(n, SEARCH_TILE);
     const uint32_t wtiles = tiles * (SEARCH_TILE / MANY_WT_BYTES);
     const size_t nsums = wtiles / MANY_SCAN_CHUNK + 1;
@@ -587,7 +589,7 @@ hipError_t setup_many(ol_ctx *c, ManyArgs &a, uint64_t n, bool writes)
     a.cands = nullptr;
     if (!writes)
         return hipSuccess;
-    if ((e = c->cands.ensure((size_t)wtiles * MANY_WT_BYTES)) != hipSuccess)
+    if ((e = c->cands.ensure((size_t)wtiles * MANY_WT_BYTES * 4)) != hipSuccess)
         return e;
     a.cands = (uint32_t *)c->cands.p;
     return hipSuccess;

@@ -373,8 +373,9 @@ struct ManyArgs {
     uint64_t *scan_sums;     // [num_wtiles / MANY_SCAN_CHUNK + 1] chunk totals of the scan (zero at launch)
     uint32_t job_tiles;      // write pass: wave tiles per wave job (set by the launcher)
     // Candidate store (nullable): the candidates of wave tile wt that have
-    // matches go to cands[wt * MANY_WT_BYTES / 4 ..] when they fit;
-    // tile_list[wt] records their number, or ~0 when they were not stored.
+    // matches go, 16 bytes each, to entries wt * MANY_WT_BYTES / 4 .. when
+    // they fit; tile_list[wt] records their number, or ~0 when they were not
+    // stored.
     uint32_t *cands;
     uint64_t *tile_list;     // [num_wtiles]
     // Batches (ndocs > 0): a match must end inside the document it starts in

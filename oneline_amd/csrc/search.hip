@@ -300,11 +300,14 @@ __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
 // (1..64) before the text or document end; written from out[r] in pattern-id
 // order (up to cap) when `emit`.  Returns their number.  GATE: read the
 // tables of classes 3..5 only where the prefix filter (s_filter) passes; a
-// candidate that passed it needs no gate.
-template <bool GATE>
+// candidate that passed it needs no gate.  KEEP: the ids of the first
+// MANY_KEEP matches go to keep[].
+constexpr uint32_t MANY_KEEP = 4;
+template <bool GATE, bool KEEP = false>
 __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
                                                    uint32_t lp, uint64_t gp, uint32_t rem, bool emit, uint64_t *out_off,
-                                                   uint32_t *out_id, uint64_t r, uint64_t cap)
+                                                   uint32_t *out_id, uint64_t r, uint64_t cap,
+                                                   uint32_t *keep = nullptr)
 {
     // s_text: the tile's first byte in global memory (word aligned)
     const uint32_t *wp = (const uint32_t *)(s_text + (lp & ~3u));
@@ -395,6 +398,12 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
             if (out_off) out_off[r + count] = gp;
             if (out_id) out_id[r + count] = bid;
         }
+        if (KEEP) {
+            if (count == 0) keep[0] = bid;
+            if (count == 1) keep[1] = bid;
+            if (count == 2) keep[2] = bid;
+            if (count == 3) keep[3] = bid;
+        }
         count++;
     }
     return count;
@@ -414,14 +423,27 @@ constexpr uint32_t MANY_PWORDS = 1u << (DICT_PBITS - 5);    // prefix filter wor
 constexpr uint32_t MANY_JOB_MAX = MANY_WT <= 4096 ? 64 : 64 * 4096 / MANY_WT;   // wave tiles per write job, at most
 static_assert(MANY_WPT * MANY_WT == SEARCH_TILE && MANY_H * MANY_PART == MANY_WT, "wave tiles");
 
-// Stored candidate: wave-tile position (CAND_POS_BITS) | bytes before the
-// text or document end (7 bits) | match count (the rest; CAND_RECOUNT: count
-// again).
+// Stored candidate (16 bytes): x = wave-tile position (CAND_POS_BITS) | bytes
+// before the text or document end (7 bits) | match count (the rest;
+// CAND_RECOUNT: count again); y, z, w = the ids of its first four matches,
+// 22 bits each (pattern ids are below OL_MAX_PATTERNS = 2^22); a candidate
+// with more is probed again by the write pass.
 constexpr uint32_t CAND_POS_BITS = __builtin_ctz(MANY_WT);
 constexpr uint32_t CAND_CNT_SHIFT = CAND_POS_BITS + 7;
 constexpr uint32_t CAND_RECOUNT = (1u << (32 - CAND_CNT_SHIFT)) - 1u;
 constexpr uint64_t TILE_LIST_NONE = ~0ull;   // tile_list: candidates not stored
 static_assert(32 - CAND_CNT_SHIFT >= 10, "count bits");
+__device__ __forceinline__ uint4 cand_pack(uint32_t x, const uint32_t (&id)[4])
+{
+    return make_uint4(x, id[0] | id[1] << 22, id[1] >> 10 | id[2] << 12, id[2] >> 20 | id[3] << 2);
+}
+__device__ __forceinline__ void cand_ids(const uint4 &e, uint32_t (&id)[4])
+{
+    id[0] = e.y & 0x3FFFFFu;
+    id[1] = e.y >> 22 | (e.z & 0xFFFu) << 10;
+    id[2] = e.z >> 12 | (e.w & 0x3u) << 20;
+    id[3] = e.w >> 2;
+}
 
 // A lane's share of wave tile wt (its positions lo + h * MANY_PART + p of the
 // tile, p < MANY_LB, in part h): the text and the document bounds (batches)
@@ -612,7 +634,7 @@ __global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
         const ManyLane m = many_lane_setup(a, wt);
         const ManyCand cand = many_candidates<FS>(a, wt, wt + stride, m, pf, small, W);
         const uint32_t total_c = cand.total;
-        uint32_t *const store = a.cands ? a.cands + (size_t)wt * MANY_WCAP : nullptr;
+        uint4 *const store = a.cands ? (uint4 *)a.cands + (size_t)wt * MANY_WCAP : nullptr;
         uint32_t kept = 0, tile_total = 0;
         for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_WLIST) {
             const uint32_t n = min(MANY_WLIST, total_c - c0);
@@ -621,12 +643,13 @@ __global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
             for (uint32_t i0 = 0; i0 < n; i0 += 64) {
                 const uint32_t i = i0 + lane;
                 uint32_t lp = 0, rm = 0, c = 0;
+                uint32_t ids[MANY_KEEP] = {0, 0, 0, 0};
                 if (i < n) {
                     lp = l_pos[i];
                     rm = l_rem[i];
                     if (rm)
-                        c = probe_position<(FS & 0x02) != 0>(d, pf, m.g_text, lp, (uint64_t)m.base + lp, rm, false,
-                                                              nullptr, nullptr, 0, 0);
+                        c = probe_position<(FS & 0x02) != 0, true>(d, pf, m.g_text, lp, (uint64_t)m.base + lp, rm,
+                                                                    false, nullptr, nullptr, 0, 0, ids);
                 }
                 tile_total += c;
                 if (store) {
@@ -634,7 +657,8 @@ __global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
                     const uint64_t has = __ballot(c != 0);
                     const uint32_t at = kept + (uint32_t)__popcll(has & ((1ull << lane) - 1ull));
                     if (c && at < MANY_WCAP)
-                        store[at] = lp | rm << CAND_POS_BITS | (c < CAND_RECOUNT ? c : CAND_RECOUNT) << CAND_CNT_SHIFT;
+                        store[at] = cand_pack(
+                            lp | rm << CAND_POS_BITS | (c < CAND_RECOUNT ? c : CAND_RECOUNT) << CAND_CNT_SHIFT, ids);
                     kept += (uint32_t)__popcll(has);
                 }
             }
@@ -702,9 +726,10 @@ __device__ __forceinline__ void many_write_unstored(const ManyArgs &a, uint32_t 
 }
 
 // Write pass.  A wave takes job_tiles (at most MANY_JOB_MAX) consecutive wave
-// tiles (a job; lane j holds tile j's stored count and output offset) and walks the stored candidates of
-// all of them as one list, 64 per round, one candidate per lane, so that a
-// round's probes fill the wave whatever the tiles' candidate counts.  Lane k
+// tiles (a job; lane j holds tile j's stored count and output offset) and
+// walks the stored candidates of all of them as one list, 64 per round, one
+// candidate per lane.  A candidate with at most four matches writes the ids
+// stored with it; one with more is probed again.  Lane k
 // of a round finds its tile as the last tile starting at or before it
 // (markers in LDS, a prefix maximum over the lanes); its output offset is
 // its tile's offset plus the counts of the tile's earlier candidates (a
@@ -717,10 +742,22 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
     constexpr int NWV = THREADS / 64;
     __shared__ uint32_t s_mark[NWV][64];
     __shared__ uint64_t s_first[NWV][64];
+    __shared__ uint4 s_queue[NWV][128];   // candidates to probe again: {position, end bound, output index}
     const DictDev &d = a.d;
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     uint32_t *const mark = s_mark[wid];
     uint64_t *const first = s_first[wid];
+    uint4 *const queue = s_queue[wid];
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t qn = 0;   // queued candidates (every lane)
+    // probes the first `count` queued candidates, one per lane
+    auto probe_queued = [&](uint32_t count) {
+        if (lane < count) {
+            const uint4 q = queue[lane];
+            probe_position<false>(d, nullptr, a.text, q.x, q.x, q.y, true, a.out_off, a.out_id,
+                                  (uint64_t)q.w << 32 | q.z, a.cap);
+        }
+    };
     const uint32_t JT = a.job_tiles;
     const uint32_t jobs = (a.num_wtiles + JT - 1) / JT;
     for (uint32_t job = blockIdx.x * NWV + wid; job < jobs; job += gridDim.x * NWV) {
@@ -750,7 +787,8 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
             const uint32_t wt_j = job * JT + tj;
             const uint32_t base = wt_j * MANY_WT;
             const uint8_t *const g_text = a.text + base;
-            const uint32_t e = act ? a.cands[(size_t)wt_j * MANY_WCAP + i] : 0u;
+            const uint4 ev = act ? ((const uint4 *)a.cands)[(size_t)wt_j * MANY_WCAP + i] : make_uint4(0, 0, 0, 0);
+            const uint32_t e = ev.x;
             const uint32_t lp = e & (MANY_WT - 1u), rm = (e >> CAND_POS_BITS) & 0x7Fu;
             uint32_t c = e >> CAND_CNT_SHIFT;
             if (c == CAND_RECOUNT)
@@ -761,11 +799,40 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
                 first[tj] = sk;
             wave_lds_sync();
             const uint64_t r = off_j + (sk - (act ? first[tj] : sk));
-            if (act && c && r < a.cap)
-                probe_position<false>(d, nullptr, g_text, lp, (uint64_t)base + lp, rm, true, a.out_off, a.out_id, r,
-                                      a.cap);
+            // more matches than stored ids: queued, and probed 64 at a time so
+            // that rounds without them never wait for a probe
+            const bool again = act && c > MANY_KEEP && r < a.cap;
+            const uint64_t qb = __ballot(again);
+            if (again)
+                queue[qn + (uint32_t)__popcll(qb & below)] = make_uint4(base + lp, rm, (uint32_t)r, (uint32_t)(r >> 32));
+            qn += (uint32_t)__popcll(qb);
+            if (act && c && c <= MANY_KEEP) {   // the stored ids
+                const uint64_t gp = (uint64_t)base + lp;
+                uint32_t id4[MANY_KEEP];
+                cand_ids(ev, id4);
+#pragma unroll
+                for (uint32_t q = 0; q < MANY_KEEP; q++)
+                    if (q < c && r + q < a.cap) {
+                        if (a.out_off) a.out_off[r + q] = gp;
+                        if (a.out_id) a.out_id[r + q] = id4[q];
+                    }
+            }
             run += __shfl(cinc, 63, 64);
             wave_lds_sync();   // mark[] is rewritten by the next round
+            if (qn >= 64) {
+                probe_queued(64);
+                const uint4 rest = lane < qn - 64 ? queue[64 + lane] : make_uint4(0, 0, 0, 0);
+                wave_lds_sync();
+                if (lane < qn - 64)
+                    queue[lane] = rest;
+                qn -= 64;
+                wave_lds_sync();
+            }
+        }
+        if (qn) {
+            probe_queued(qn);
+            qn = 0;
+            wave_lds_sync();
         }
         uint64_t redo = __ballot(valid && st == TILE_LIST_NONE);
         while (redo) {

@@ -133,6 +133,14 @@ def test_dense_candidates_few_matches(parser):
     assert_many_matches(parser, bytes(text), [b"aaaAx", b"aaA", b"zzzz"], "dense candidates")
 
 
+def test_positions_with_one_to_six_matches(parser):
+    """Nested prefixes match 1..6 patterns at one position: up to three ids
+    travel with a stored candidate, more are found again by the write pass."""
+    pats = [b"ab", b"abc", b"abcd", b"abcde", b"abcdefgh", b"abcdefghij", b"bcd", b"zz"]
+    text = (b"abcdefghijk " * 500 + b"abcde abcd abc ab zz\n") * 40
+    assert_many_matches(parser, text, pats, "nested prefixes")
+
+
 def test_duplicate_patterns_count_again(parser):
     """17,000 copies of one pattern: a position with more matches than a
     stored candidate's count field holds is counted again by the write pass."""
