@@ -480,7 +480,7 @@ int check_tree_error(ol_ctx *c)
 int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_t *out64,
                uint32_t *out32, uint64_t cap, uint64_t *total)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->n, SEARCH_TILE);
+    uint32_t tiles = (uint32_t)ceil_div(d->n, LITERAL_TILE);
     HIPCHK(ensure_scratch(c, tiles));
     HIPCHK(clear_scratch(c, tiles));
     SearchArgs a{};

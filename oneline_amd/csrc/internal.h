@@ -409,6 +409,11 @@ constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks
 constexpr uint32_t SEARCH_TILE = SEARCH_THREADS * SEARCH_BPT;
+#ifndef OL_LITERAL_NV
+#define OL_LITERAL_NV 8
+#endif
+constexpr uint32_t LITERAL_NV = OL_LITERAL_NV;                     // literal search: 16-byte vectors per lane
+constexpr uint32_t LITERAL_TILE = SEARCH_THREADS * LITERAL_NV * 16;
 constexpr uint32_t MANY_THREADS = 512;     // dictionary search: 8 independent waves per workgroup
 #ifndef OL_MANY_WT_BYTES
 #define OL_MANY_WT_BYTES 8192

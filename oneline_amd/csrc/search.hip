@@ -67,21 +67,23 @@ static uint32_t max_grid(const void *f, int threads, size_t lds) { return ol_cor
 // Persistent, cooperative launch: the last workgroup is the prefix scanner
 // (scan_prefixes); the others take tiles t = blockIdx + k * workers in
 // increasing order.  No LDS staging: lane l holds the tile's 16-byte vectors
-// k * 256 + l (k < 4, coalesced loads) plus the first word of the vector after
-// each, so every 4-byte window of its 64 positions is an alignbyte of its own
-// registers.  Windows that equal the pattern's first (up to) four bytes are
-// verified against the rest from global memory (an L2 hit).  Matches are
-// ordered by vector (k major), counted per vector in 16-bit fields, and kept
-// as one 64-bit mask per lane until the next iteration, when the scanner has
-// (almost always) produced the tile's prefix; then they are written.  The next
-// tile is loaded while the current one is processed.
-template <int THREADS>
+// k * 256 + l (k < NV, coalesced loads) plus the first word of the vector
+// after each, so every 4-byte window of its 16 NV positions is an alignbyte of
+// its own registers.  Windows that equal the pattern's first (up to) four
+// bytes are verified against the rest from global memory (an L2 hit).
+// Matches are ordered by vector (k major), counted per vector in 16-bit
+// fields, and kept as 16-bit masks per vector until the next iteration, when
+// the scanner has (almost always) produced the tile's prefix; then they are
+// written.  The next tile is loaded while the current one is processed.
+template <int THREADS, int NV>
 __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
 {
-    constexpr int NV = 4;                     // vectors per lane
+    static_assert(NV % 4 == 0, "vectors per lane: 16-bit counts, two per word");
     constexpr int TILE = THREADS * NV * 16;
     constexpr int NW = THREADS / 64;
-    __shared__ uint32_t s_w[2][NW];
+    constexpr int NC = NV / 2;   // count words (two 16-bit fields each)
+    constexpr int NM = NV / 4;   // match words (four 16-bit vector masks each)
+    __shared__ uint32_t s_w[NC][NW];
     __shared__ uint64_t s_prefix;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
     bool pend = false;
     uint32_t p_t = 0, p_total = 0;
     uint32_t p_off[NV];
-    uint64_t p_match = 0;
+    uint64_t p_match[NM];
     auto flush = [&]() {   // writes the previous tile's matches (all threads)
         if (wid == 0 && lane == 0) {
             uint64_t w = 0;
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         const uint32_t pbase = p_t * TILE;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
-            uint32_t m = (uint32_t)(p_match >> (16 * k)) & 0xFFFFu;
+            uint32_t m = (uint32_t)(p_match[k / 4] >> (16 * (k % 4))) & 0xFFFFu;
             uint64_t r = prefix + p_off[k];
             while (m) {
                 const uint32_t bit = (uint32_t)__builtin_ctz(m);
@@ -179,8 +181,14 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
             load(t + G);
         // positions gp with gp + m <= n may match
         const uint32_t last_start = a.n >= mlen ? a.n - mlen : 0u;   // inclusive, when n >= m
-        uint64_t match = 0;
-        uint32_t cnt01 = 0, cnt23 = 0;
+        uint64_t match[NM];
+        uint32_t cnt[NC];
+#pragma unroll
+        for (int k = 0; k < NM; k++)
+            match[k] = 0;
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+            cnt[k] = 0;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             const uint32_t W[5] = {cv[k].x, cv[k].y, cv[k].z, cv[k].w, cn[k]};
@@ -206,34 +214,37 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
                         hits &= ~(1u << bit);
                 }
             }
-            match |= (uint64_t)hits << (16 * k);
-            const uint32_t c = (uint32_t)__builtin_popcount(hits);
-            if (k == 0) cnt01 |= c;
-            if (k == 1) cnt01 |= c << 16;
-            if (k == 2) cnt23 |= c;
-            if (k == 3) cnt23 |= c << 16;
+            match[k / 4] |= (uint64_t)hits << (16 * (k % 4));
+            cnt[k / 2] |= (uint32_t)__builtin_popcount(hits) << (16 * (k % 2));
         }
         // offsets of each vector's matches in the tile (vector order k major):
         // two 16-bit counts per word, block-scanned together
-        const uint32_t i01 = wave_incl_add(cnt01), i23 = wave_incl_add(cnt23);
-        if (lane == 63) {
-            s_w[0][wid] = i01;
-            s_w[1][wid] = i23;
+        uint32_t inc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            inc[k] = wave_incl_add(cnt[k]);
+            if (lane == 63)
+                s_w[k][wid] = inc[k];
         }
         __syncthreads();
-        uint32_t b01 = 0, b23 = 0, t01 = 0, t23 = 0;
+        uint32_t run = 0;   // matches of the tile's earlier vectors (k major)
+        uint32_t total = 0;
 #pragma unroll
-        for (int w = 0; w < NW; w++) {
-            if (w < wid) {
-                b01 += s_w[0][w];
-                b23 += s_w[1][w];
+        for (int k = 0; k < NC; k++) {
+            uint32_t before = 0, all = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                if (w < wid)
+                    before += s_w[k][w];
+                all += s_w[k][w];
             }
-            t01 += s_w[0][w];
-            t23 += s_w[1][w];
+            const uint32_t ex = before + inc[k] - cnt[k];   // this lane's place among the lanes, per field
+            const uint32_t t_lo = all & 0xFFFFu, t_hi = all >> 16;
+            p_off[2 * k] = run + (ex & 0xFFFFu);
+            p_off[2 * k + 1] = run + t_lo + (ex >> 16);
+            run += t_lo + t_hi;
         }
-        const uint32_t e01 = b01 + i01 - cnt01, e23 = b23 + i23 - cnt23;
-        const uint32_t T0 = t01 & 0xFFFFu, T1 = t01 >> 16, T2 = t23 & 0xFFFFu, T3 = t23 >> 16;
-        const uint32_t total = T0 + T1 + T2 + T3;
+        total = run;
         if (tid == 0)
             st_coherent_u64(a.status + (size_t)t * SCAN_STATUS_WORDS, (LB_FLAG_AGG << LB_SHIFT) | total);
         if (pend)
@@ -241,11 +252,9 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         pend = true;
         p_t = t;
         p_total = total;
-        p_off[0] = e01 & 0xFFFFu;
-        p_off[1] = T0 + (e01 >> 16);
-        p_off[2] = T0 + T1 + (e23 & 0xFFFFu);
-        p_off[3] = T0 + T1 + T2 + (e23 >> 16);
-        p_match = match;
+#pragma unroll
+        for (int k = 0; k < NM; k++)
+            p_match[k] = match[k];
         __syncthreads();   // s_w and s_prefix are reused
     }
     if (pend)
@@ -255,7 +264,7 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 {
     static uint32_t cap = 0;   // co-resident limit (same device for the process)
-    const void *f = (const void *)search_kernel<SEARCH_THREADS>;
+    const void *f = (const void *)search_kernel<SEARCH_THREADS, LITERAL_NV>;
     if (!cap)
         cap = max_grid(f, SEARCH_THREADS, 0);
     if (!cap)
