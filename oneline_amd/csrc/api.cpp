@@ -13,6 +13,7 @@
 #include <unistd.h>
 #include <cstdlib>
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -113,6 +114,12 @@ struct ol_doc {
     uint8_t *classes = nullptr, *keys = nullptr;
     ol_node_dev *nodes = nullptr;
     uint32_t byte_freq[256];
+    // literal search: a text sample (4 windows of 16 KiB spread over the
+    // text) and, built at the first search, its 4-byte windows' counts
+    // (sorted keys, counts)
+    std::vector<uint8_t> sample;
+    mutable std::mutex win_mu;
+    mutable std::vector<uint32_t> win_key, win_count;
 };
 
 struct ol_tree {
@@ -340,6 +347,24 @@ int stage_upload(ol_ctx *c, uint8_t *dst, uint64_t n, const std::function<void(u
     return failed ? fail(OL_ERR_DEVICE, "text upload failed") : OL_OK;
 }
 
+// A text sample for the literal search's window counts: the whole text up
+// to 64 KiB, else 4 windows of 16 KiB spread over it (window edges cut no
+// real window in two, since each is counted on its own).
+void sample_text(const uint8_t *t, size_t n, std::vector<uint8_t> &out)
+{
+    const size_t SAMPLE = 64u << 10, WINDOWS = 4;
+    out.clear();
+    if (n <= SAMPLE) {
+        out.assign(t, t + n);
+        return;
+    }
+    const size_t w = SAMPLE / WINDOWS, span = n - w;
+    for (size_t k = 0; k < WINDOWS; k++) {
+        const size_t s = span / (WINDOWS - 1) * k;
+        out.insert(out.end(), t + s, t + s + w);
+    }
+}
+
 void sample_histogram(const uint8_t *t, size_t n, uint32_t *freq)
 {
     // sample_byte_histogram, crates/oneline-gpu/src/search.rs:176-201
@@ -415,6 +440,7 @@ int upload_doc(ol_ctx *c, const uint8_t *text, size_t len, bool with_nodes, ol_d
             return bail(rc);
     }
     sample_histogram(text, len, d->byte_freq);
+    sample_text(text, len, d->sample);
     *out = d;
     return OL_OK;
 }
@@ -477,13 +503,103 @@ int check_tree_error(ol_ctx *c)
     return OL_OK;
 }
 
+// Occurrences of every 4-byte window of the document's text sample (each of
+// its 16 KiB windows counted on its own), sorted by window; built at the
+// document's first literal search.
+void build_window_counts(const ol_doc *d)
+{
+    std::lock_guard<std::mutex> g(d->win_mu);
+    if (!d->win_key.empty() || d->sample.size() < 4)
+        return;
+    const size_t part = d->sample.size() <= (64u << 10) && d->n <= (64u << 10) ? d->sample.size() : (16u << 10);
+    std::vector<uint32_t> all;
+    all.reserve(d->sample.size());
+    for (size_t p0 = 0; p0 < d->sample.size(); p0 += part) {
+        const size_t end = std::min(d->sample.size(), p0 + part);
+        for (size_t i = p0; i + 4 <= end; i++) {
+            uint32_t w;
+            std::memcpy(&w, d->sample.data() + i, 4);
+            all.push_back(w);
+        }
+    }
+    std::sort(all.begin(), all.end());
+    for (size_t i = 0; i < all.size();) {
+        size_t j = i + 1;
+        while (j < all.size() && all[j] == all[i])
+            j++;
+        d->win_key.push_back(all[i]);
+        d->win_count.push_back((uint32_t)(j - i));
+        i = j;
+    }
+}
+
+uint32_t window_count(const ol_doc *d, uint32_t w)   // after build_window_counts
+{
+    auto it = std::lower_bound(d->win_key.begin(), d->win_key.end(), w);
+    return (it != d->win_key.end() && *it == w) ? d->win_count[it - d->win_key.begin()] : 0u;
+}
+
+// The pattern window the literal kernel tests.  Each 4-byte window j of the
+// pattern costs (its occurrences in the document's text sample) x (2 + the
+// pattern words that verification then reads: those holding a byte outside
+// the window); the cheapest wins, then the one whose bytes are rarest in the
+// sampled byte histogram (the sum of ln(count + 1) over its bytes, the
+// reference's choose_window, crates/oneline-gpu/src/search.rs:205-221), then
+// the earliest.  Patterns shorter than 4 bytes have one window.  The choice
+// never changes the matches, only how many positions go on to the full
+// comparison and what that costs; counting real windows, not bytes, matters
+// on vocabulary text, where a word's letters are each common but the word is
+// rarer or more frequent than their product says.  `hits` gets the chosen
+// window's count in the sample.
+uint32_t choose_window(const ol_doc *d, const uint8_t *pat, size_t m, uint32_t *hits)
+{
+    const size_t k = std::min<size_t>(m, 4);
+    uint32_t best = 0, best_count = 0;
+    uint64_t best_cost = 0;
+    double best_score = 0;
+    for (size_t j = 0; j + k <= m; j++) {
+        uint32_t cnt = 0;
+        if (k == 4) {
+            uint32_t w;
+            std::memcpy(&w, pat + j, 4);
+            cnt = window_count(d, w);
+        }
+        uint32_t words = 0;   // pattern words with a byte outside the window
+        for (size_t w0 = 0; w0 < m; w0 += 4)
+            for (size_t b = w0; b < std::min(m, w0 + 4); b++)
+                if (b < j || b >= j + k) {
+                    words++;
+                    break;
+                }
+        const uint64_t cost = (uint64_t)cnt * (2 + words);
+        double sc = 0;
+        for (size_t b = 0; b < k; b++)
+            sc += std::log((double)d->byte_freq[pat[j + b]] + 1.0);
+        if (j == 0 || cost < best_cost || (cost == best_cost && sc < best_score)) {
+            best = (uint32_t)j;
+            best_count = cnt;
+            best_cost = cost;
+            best_score = sc;
+        }
+    }
+    *hits = best_count;
+    return best;
+}
+
 int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_t *out64,
                uint32_t *out32, uint64_t cap, uint64_t *total)
 {
-    uint32_t tiles = (uint32_t)ceil_div(d->n, LITERAL_TILE);
+    SearchArgs a{};
+    build_window_counts(d);
+    uint32_t hits = 0;
+    a.wo = choose_window(d, pat, m, &hits);
+    // a window that occurs often (more than once in 8 KiB of the sample) in
+    // a pattern longer than it makes the kernel wait on verifications: it then
+    // takes 4 vectors per lane, which leaves registers for more waves
+    a.dense = m > 4 && d->sample.size() && (uint64_t)hits * 8192 > d->sample.size();
+    uint32_t tiles = (uint32_t)ceil_div(d->n, ol_literal_tile(a.dense));
     HIPCHK(ensure_scratch(c, tiles));
     HIPCHK(clear_scratch(c, tiles));
-    SearchArgs a{};
     a.text = d->text;
     a.n = (uint32_t)d->n;
     a.num_tiles = tiles;
@@ -491,10 +607,19 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
     uint32_t k = (uint32_t)std::min<size_t>(m, 4);
     a.win = 0;
     for (uint32_t i = 0; i < k; i++)
-        a.win |= (uint32_t)pat[i] << (8 * i);
+        a.win |= (uint32_t)pat[a.wo + i] << (8 * i);
     a.win_mask = (k == 4) ? 0xFFFFFFFFu : ((1u << (8 * k)) - 1u);
-    std::memset(a.pat, 0, sizeof a.pat);
-    std::memcpy(a.pat, pat, m);
+    for (uint32_t w = 0; w < 16; w++) {
+        a.pw[w] = a.vmask[w] = 0;
+        for (uint32_t b = 0; b < 4; b++) {
+            const uint32_t j = 4 * w + b;
+            if (j >= m)
+                continue;
+            a.pw[w] |= (uint32_t)pat[j] << (8 * b);
+            if (j < a.wo || j >= a.wo + k)
+                a.vmask[w] |= 0xFFu << (8 * b);
+        }
+    }
     a.cap = cap;
     a.out64 = out64;
     a.out32 = out32;
@@ -844,6 +969,7 @@ int load_olb(ol_ctx *c, const char *path, bool verify, ol_doc **out)
         }
         d->lines = lines;
         sample_histogram(h + toff, n, d->byte_freq);
+        sample_text(h + toff, n, d->sample);
         if (hipStreamSynchronize(c->stream) != hipSuccess)
             return fail_doc(fail(OL_ERR_DEVICE, "upload failed"));
     }
@@ -1234,6 +1360,8 @@ int ol_search_raw(ol_ctx *c, const uint8_t *text, size_t text_len, const uint8_t
     // The one-shot path needs only the text on the device, no line scan.
     ol_doc tmp;
     tmp.n = text_len;
+    sample_histogram(text, text_len, tmp.byte_freq);
+    sample_text(text, text_len, tmp.sample);
     uint64_t padded = round_up(text_len, TEXT_ALIGN) + TEXT_SLACK;
     HIPCHK(hipMalloc(&tmp.text, padded));
     if (hipMemcpyAsync(tmp.text, text, text_len, hipMemcpyHostToDevice, c->stream) != hipSuccess ||

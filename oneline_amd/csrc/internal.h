@@ -306,8 +306,11 @@ struct SearchArgs {
     uint32_t n;
     uint32_t num_tiles;
     uint32_t m;              // pattern length 1..64
-    uint32_t win, win_mask;  // first min(m,4) pattern bytes and their mask
-    uint8_t pat[64];
+    uint32_t wo;             // offset in the pattern of the window the kernel tests
+    uint32_t dense;          // the window is frequent: 4 vectors per lane (16 KiB tiles), else 8 (32 KiB)
+    uint32_t win, win_mask;  // pattern bytes wo .. wo + min(m,4) and their mask
+    uint32_t pw[16];         // the pattern as words (zero past its end)
+    uint32_t vmask[16];      // per word, the bytes left to verify: inside the pattern, outside the window
     uint64_t cap;            // output capacity (matches)
     uint32_t *lb_err;        // set when a look-back wait gave up
     uint64_t *out64;         // offsets as u64 (nullable)
@@ -398,6 +401,7 @@ hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s);   // reads th
 hipError_t ol_launch_tree(const TreeArgs &a, uint32_t rounds, hipStream_t s);
 uint32_t ol_tree_tile_lines(uint32_t rounds);
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
+uint32_t ol_literal_tile(bool dense);   // text bytes per literal search tile
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
 
 // Tile geometry of each kernel.
@@ -409,11 +413,6 @@ constexpr uint32_t TREE_D = 32;
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks
 constexpr uint32_t SEARCH_TILE = SEARCH_THREADS * SEARCH_BPT;
-#ifndef OL_LITERAL_NV
-#define OL_LITERAL_NV 8
-#endif
-constexpr uint32_t LITERAL_NV = OL_LITERAL_NV;                     // literal search: 16-byte vectors per lane
-constexpr uint32_t LITERAL_TILE = SEARCH_THREADS * LITERAL_NV * 16;
 constexpr uint32_t MANY_THREADS = 512;     // dictionary search: 8 independent waves per workgroup
 #ifndef OL_MANY_WT_BYTES
 #define OL_MANY_WT_BYTES 8192

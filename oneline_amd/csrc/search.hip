@@ -69,8 +69,9 @@ static uint32_t max_grid(const void *f, int threads, size_t lds) { return ol_cor
 // increasing order.  No LDS staging: lane l holds the tile's 16-byte vectors
 // k * 256 + l (k < NV, coalesced loads) plus the first word of the vector
 // after each, so every 4-byte window of its 16 NV positions is an alignbyte of
-// its own registers.  Windows that equal the pattern's first (up to) four
-// bytes are verified against the rest from global memory (an L2 hit).
+// its own registers.  Windows that equal the pattern's window (its rarest
+// min(m, 4) bytes, at offset wo, chosen on the host) are verified against the
+// whole pattern from global memory (an L2 hit).
 // Matches are ordered by vector (k major), counted per vector in 16-bit
 // fields, and kept as 16-bit masks per vector until the next iteration, when
 // the scanner has (almost always) produced the tile's prefix; then they are
@@ -104,25 +105,27 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
             nx[k] = *(const uint32_t *)(src + (size_t)(k * THREADS + tid + 1) * 16);   // text is padded
         }
     };
-    const uint32_t mlen = a.m;
+    const uint32_t mlen = a.m, wo = a.wo;
     const uint32_t pat4 = a.win, mask4 = a.win_mask;
-    // the rest of a candidate at text position gp: pattern bytes 4 .. m
+    // a candidate starting at text position gp: the pattern bytes outside its
+    // window (vmask; the window matched already).  The first 16 bytes are
+    // loaded together, so most candidates cost one memory round trip.
+    const uint32_t nw = (mlen + 3) / 4;
+    auto tword = [&](uint32_t x) -> uint32_t {
+        const uint32_t *q = (const uint32_t *)(a.text + (x & ~3u));
+        const uint32_t sx = x & 3u;
+        return sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
+    };
     auto verify = [&](uint32_t gp) -> bool {
-        for (uint32_t i = 4; i < mlen; i += 4) {
-            const uint32_t x = gp + i;
-            const uint32_t *q = (const uint32_t *)(a.text + (x & ~3u));
-            const uint32_t sx = x & 3u;
-            const uint32_t w = sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
-            uint32_t pw = 0;
+        uint32_t bad = 0;
 #pragma unroll
-            for (int b = 0; b < 4; b++)
-                pw |= (uint32_t)a.pat[(i + b) & 63] << (8 * b);
-            const uint32_t nb = mlen - i;
-            const uint32_t msk = nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u);
-            if ((w ^ pw) & msk)
-                return false;
-        }
-        return true;
+        for (uint32_t w = 0; w < 4; w++)
+            if (w < nw && a.vmask[w])
+                bad |= (tword(gp + 4 * w) ^ a.pw[w]) & a.vmask[w];
+        for (uint32_t w = 4; !bad && w < nw; w++)
+            if (a.vmask[w])
+                bad |= (tword(gp + 4 * w) ^ a.pw[w]) & a.vmask[w];
+        return bad == 0;
     };
     // the previous tile: its matches, waiting for its prefix
     bool pend = false;
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
             while (m) {
                 const uint32_t bit = (uint32_t)__builtin_ctz(m);
                 m &= m - 1;
-                const uint32_t gp = pbase + (uint32_t)(k * THREADS + tid) * 16 + bit;
+                const uint32_t gp = pbase + (uint32_t)(k * THREADS + tid) * 16 + bit - wo;   // the match's start
                 if (r < a.cap) {
                     if (a.out64) a.out64[r] = gp;
                     if (a.out32) a.out32[r] = gp;
@@ -179,8 +182,9 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         }
         if (t + G < T)
             load(t + G);
-        // positions gp with gp + m <= n may match
-        const uint32_t last_start = a.n >= mlen ? a.n - mlen : 0u;   // inclusive, when n >= m
+        // window positions q hold a match start q - wo when wo <= q and
+        // q - wo + m <= n
+        const uint32_t last_q = a.n >= mlen ? a.n - mlen + wo : 0u;   // inclusive, when n >= m
         uint64_t match[NM];
         uint32_t cnt[NC];
 #pragma unroll
@@ -200,17 +204,19 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
                                              : W[p >> 2];
                 hits |= (uint32_t)((win & mask4) == pat4) << p;
             }
-            // positions past the last possible start
-            if (a.n < mlen || g > last_start)
+            // window positions past the last possible one, or before wo
+            if (a.n < mlen || g > last_q)
                 hits = 0;
-            else if (last_start - g < 15)
-                hits &= (2u << (last_start - g)) - 1u;
+            else if (last_q - g < 15)
+                hits &= (2u << (last_q - g)) - 1u;
+            if (g < wo)
+                hits = (wo - g) >= 16 ? 0u : hits & ~((1u << (wo - g)) - 1u);
             if (hits && mlen > 4) {
                 uint32_t q = hits;
                 while (q) {
                     const uint32_t bit = (uint32_t)__builtin_ctz(q);
                     q &= q - 1;
-                    if (!verify(g + bit))
+                    if (!verify(g + bit - wo))
                         hits &= ~(1u << bit);
                 }
             }
@@ -261,10 +267,11 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         flush();
 }
 
-hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
+template <int NV>
+static hipError_t launch_search_nv(const SearchArgs &a, hipStream_t s)
 {
     static uint32_t cap = 0;   // co-resident limit (same device for the process)
-    const void *f = (const void *)search_kernel<SEARCH_THREADS, LITERAL_NV>;
+    const void *f = (const void *)search_kernel<SEARCH_THREADS, NV>;
     if (!cap)
         cap = max_grid(f, SEARCH_THREADS, 0);
     if (!cap)
@@ -276,6 +283,15 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
     SearchArgs args = a;
     void *params[] = {&args};
     return hipLaunchCooperativeKernel(f, dim3(grid), dim3(SEARCH_THREADS), params, 0, s);
+}
+
+uint32_t ol_literal_tile(bool dense) { return SEARCH_THREADS * (dense ? 4u : 8u) * 16u; }
+
+// 8 vectors per lane halve the per-tile synchronisation; 4 leave registers
+// for more waves, which hide verifications when the window is frequent.
+hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
+{
+    return a.dense ? launch_search_nv<4>(a, s) : launch_search_nv<8>(a, s);
 }
 
 // ---- dictionary -----------------------------------------------------------------

@@ -88,6 +88,19 @@ def test_anchor_positions_and_nul(parser):
     assert_search_matches(parser, b"xa", b"a\0")
 
 
+def test_rare_window_inside_the_pattern(parser):
+    """The kernel tests the pattern's rarest window (here at offsets 2 and
+    60); matches start before it, at the very start and end of the text."""
+    body = b"ab" * 60_000
+    text = b"abxyzQ" + body + b"abxyzQ" + body + b"abxyzQ"
+    for p in (b"abxyzQ", b"bxyzQ", b"babxyzQ", b"xyzQab"):
+        assert_search_matches(parser, text, p)
+    long = (b"ab" * 30) + b"xyzQ"   # 64 bytes, the rare bytes last
+    text = long + body + long + b"b" + long
+    assert_search_matches(parser, text, long)
+    assert_search_matches(parser, text[:-1], long)   # the last one cut short
+
+
 def test_matches_across_tile_edges(parser):
     rec = b"%-46sMARKER%-47s\n" % (b"header", b"x")
     assert len(rec) == 100
