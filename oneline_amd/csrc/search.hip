@@ -234,7 +234,7 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         }
         __syncthreads();
         uint32_t run = 0;   // matches of the tile's earlier vectors (k major)
-        uint32_t total = 0;
+        uint32_t off[NV];   // this tile's: flush() still needs the previous tile's p_off
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             uint32_t before = 0, all = 0;
@@ -246,11 +246,11 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
             }
             const uint32_t ex = before + inc[k] - cnt[k];   // this lane's place among the lanes, per field
             const uint32_t t_lo = all & 0xFFFFu, t_hi = all >> 16;
-            p_off[2 * k] = run + (ex & 0xFFFFu);
-            p_off[2 * k + 1] = run + t_lo + (ex >> 16);
+            off[2 * k] = run + (ex & 0xFFFFu);
+            off[2 * k + 1] = run + t_lo + (ex >> 16);
             run += t_lo + t_hi;
         }
-        total = run;
+        const uint32_t total = run;
         if (tid == 0)
             st_coherent_u64(a.status + (size_t)t * SCAN_STATUS_WORDS, (LB_FLAG_AGG << LB_SHIFT) | total);
         if (pend)
@@ -258,6 +258,9 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         pend = true;
         p_t = t;
         p_total = total;
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            p_off[k] = off[k];
 #pragma unroll
         for (int k = 0; k < NM; k++)
             p_match[k] = match[k];
@@ -326,9 +329,10 @@ __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
 // order (up to cap) when `emit`.  Returns their number.  GATE: read the
 // tables of classes 3..5 only where the prefix filter (s_filter) passes; a
 // candidate that passed it needs no gate.  KEEP: the ids of the first
-// MANY_KEEP matches go to keep[].
+// MANY_KEEP matches go to keep[].  SMALL: the dictionary may have 1- and
+// 2-byte patterns (without them their classes' code is compiled out).
 constexpr uint32_t MANY_KEEP = 4;
-template <bool GATE, bool KEEP = false>
+template <bool GATE, bool KEEP = false, bool SMALL = true>
 __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
                                                    uint32_t lp, uint64_t gp, uint32_t rem, bool emit, uint64_t *out_off,
                                                    uint32_t *out_id, uint64_t r, uint64_t cap,
@@ -350,7 +354,8 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
     }
 #define OL_KEY(K, LO)                                                                       \
     const uint32_t klo##K = (LO), khi##K = (K) == 5 ? w1 : 0u;                              \
-    const bool use##K = ((d.class_mask >> (K)) & 1u) && ((K) <= 4 ? (K) : 8u) <= rem &&    \
+    const bool use##K = ((K) >= 3 || SMALL) && ((d.class_mask >> (K)) & 1u) &&               \
+                        ((K) <= 4 ? (K) : 8u) <= rem &&                                      \
                         ((K) >= 3 ? pass345 : true);                                         \
     uint32_t h##K = use##K ? dict_thash(klo##K, khi##K, d.tbits[K]) : 0u;                  \
     uint4 v##K = use##K ? d.table[K][h##K] : make_uint4(0, 0, 0, 0);
@@ -673,7 +678,8 @@ __global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
                     lp = l_pos[i];
                     rm = l_rem[i];
                     if (rm)
-                        c = probe_position<(FS & 0x02) != 0, true>(d, pf, m.g_text, lp, (uint64_t)m.base + lp, rm,
+                        c = probe_position<(FS & 0x02) != 0, true, (FS & 0x02) != 0>(d, pf, m.g_text, lp,
+                                                                                      (uint64_t)m.base + lp, rm,
                                                                     false, nullptr, nullptr, 0, 0, ids);
                 }
                 tile_total += c;
@@ -737,9 +743,11 @@ __device__ __forceinline__ void many_write_unstored(const ManyArgs &a, uint32_t 
                 if (!rm)
                     continue;
                 if (!pass)
-                    own += probe_position<false>(d, nullptr, m.g_text, lp, gp, rm, false, nullptr, nullptr, 0, 0);
+                    own += probe_position<false, false, (FS & 0x02) != 0>(d, nullptr, m.g_text, lp, gp, rm, false,
+                                                                          nullptr, nullptr, 0, 0);
                 else if (r < a.cap)
-                    r += probe_position<false>(d, nullptr, m.g_text, lp, gp, rm, true, a.out_off, a.out_id, r, a.cap);
+                    r += probe_position<false, false, (FS & 0x02) != 0>(d, nullptr, m.g_text, lp, gp, rm, true,
+                                                                        a.out_off, a.out_id, r, a.cap);
             }
             if (pass) {
                 di = pdi;
@@ -779,7 +787,7 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
     auto probe_queued = [&](uint32_t count) {
         if (lane < count) {
             const uint4 q = queue[lane];
-            probe_position<false>(d, nullptr, a.text, q.x, q.x, q.y, true, a.out_off, a.out_id,
+            probe_position<false, false, (FS & 0x02) != 0>(d, nullptr, a.text, q.x, q.x, q.y, true, a.out_off, a.out_id,
                                   (uint64_t)q.w << 32 | q.z, a.cap);
         }
     };
@@ -817,7 +825,8 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
             const uint32_t lp = e & (MANY_WT - 1u), rm = (e >> CAND_POS_BITS) & 0x7Fu;
             uint32_t c = e >> CAND_CNT_SHIFT;
             if (c == CAND_RECOUNT)
-                c = probe_position<false>(d, nullptr, g_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0);
+                c = probe_position<false, false, (FS & 0x02) != 0>(d, nullptr, g_text, lp, (uint64_t)base + lp, rm,
+                                                                    false, nullptr, nullptr, 0, 0);
             const uint32_t cinc = wave_incl_add(c);
             const uint64_t sk = run + (cinc - c);   // counts of the list before candidate k
             if (act && i == 0)

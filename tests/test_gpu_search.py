@@ -88,6 +88,18 @@ def test_anchor_positions_and_nul(parser):
     assert_search_matches(parser, b"xa", b"a\0")
 
 
+def test_irregular_matches_in_consecutive_tiles(parser):
+    """Matches in every tile of an 8 MiB generated document, unevenly spread
+    over the lanes and vectors: each tile's matches are written only after
+    the next tile is counted, so the offsets of one tile must not leak into
+    the next (a periodic text would hide that)."""
+    text = datagen.document(8 << 20, seed=21, crlf=10, empty=5)
+    doc = parser.upload(text, nodes=False)
+    pats = [datagen.vocab_word(i) for i in (0, 1, 3, 20, 200)] + datagen.patterns(12, seed=0xC3B)
+    for pat in pats:
+        assert_search_matches(parser, text, pat, doc=doc)
+
+
 def test_rare_window_inside_the_pattern(parser):
     """The kernel tests the pattern's rarest window (here at offsets 2 and
     60); matches start before it, at the very start and end of the text."""
