@@ -157,9 +157,9 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-// One workgroup per level (arrays level-major, one row of num_tiles per
-// level): in_last[t] = max of sum_last over the earlier tiles of t's document,
-// out_first[t] = min of sum_first over its later tiles.  The scan runs over
+// Two workgroups per level, one per direction (arrays level-major, one row of
+// num_tiles per level): in_last[t] = max of sum_last over the earlier tiles of
+// t's document, out_first[t] = min of sum_first over its later tiles.  The scan runs over
 // (document << 32 | value): the maximum over earlier tiles carries t's own
 // document exactly when one of them belongs to it (documents are contiguous
 // runs of tiles), and likewise the minimum over later tiles.  The row goes
@@ -174,13 +174,15 @@ __global__ __launch_bounds__(ST) void tree_scan_kernel(TreeArgs a)
     __shared__ uint64_t s_v[CH];
     __shared__ uint64_t s_w[NWS];
     const uint32_t T = a.num_tiles;
-    const size_t row = (size_t)blockIdx.x * T;
+    // workgroup 2 d takes level d forward, 2 d + 1 backward
+    const size_t row = (size_t)(blockIdx.x >> 1) * T;
+    const bool fwd = (blockIdx.x & 1u) == 0;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     auto seg = [&](uint32_t t) -> uint64_t { return a.nsegs ? (uint64_t)a.tile_seg[t] : 0ull; };
 
     // forward: exclusive prefix max
     uint64_t carry = 0;
-    for (uint32_t c0 = 0; c0 < T; c0 += CH) {
+    for (uint32_t c0 = 0; fwd && c0 < T; c0 += CH) {
         const uint32_t n = (T - c0) < (uint32_t)CH ? T - c0 : (uint32_t)CH;
         for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
             s_v[i] = i < n ? (seg(c0 + i) << 32 | a.sum_last[row + c0 + i]) : 0ull;
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(ST) void tree_scan_kernel(TreeArgs a)
 
     // backward: exclusive suffix min
     carry = ~0ull;
-    for (uint32_t c1 = T; c1 > 0;) {
+    for (uint32_t c1 = T; !fwd && c1 > 0;) {
         const uint32_t n = c1 < (uint32_t)CH ? c1 : (uint32_t)CH;
         const uint32_t c0 = c1 - n;
         for (uint32_t i = tid; i < (uint32_t)CH; i += ST)
@@ -441,7 +443,7 @@ template <int TR>
 static void launch_tree_tr(const TreeArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL((tree_summary_kernel<TR>), dim3(a.num_tiles), dim3(TW), 0, s, a);
-    hipLaunchKernelGGL((tree_scan_kernel<1024>), dim3(D), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL((tree_scan_kernel<1024>), dim3(2 * D), dim3(1024), 0, s, a);
     hipLaunchKernelGGL((tree_walk_kernel<TR>), dim3(a.num_tiles), dim3(TW), 0, s, a);
 }
 
