@@ -100,6 +100,24 @@ def test_resident_document_arrays(parser):
     assert np.array_equal(o2, off) and np.array_equal(c2, c)
 
 
+def test_staged_upload_sizes(parser):
+    """Uploads go through a pinned staging ring: one slot in order below
+    32 MiB, parallel 8 MiB slots above.  Sizes around that threshold and
+    around slot edges, with a distinct byte at each slot edge, arrive intact
+    (the scan's offsets and classes equal the oracle's)."""
+    slot, small = 8 << 20, 32 << 20
+    base = datagen.document(small + slot + 64, seed=9, crlf=10, empty=5)
+    for n in (small - 1, small, small + 1, small + slot + 3):
+        text = base[:n].copy()
+        for edge in range(slot, n, slot):
+            text[edge - 1], text[edge] = ord("\n"), ord("=")
+        off, nodes = orc.parse(text)
+        doc = parser.upload(text, nodes=False)
+        o, c, _ = doc.download()
+        assert np.array_equal(o, off), f"{n} bytes: offsets"
+        assert np.array_equal(c, nodes["kind"].astype(np.uint8)), f"{n} bytes: classes"
+
+
 def test_empty_input(parser):
     assert parser.scan_line_offsets(b"").size == 0
     assert parser.parse_raw(b"").nodes.size == 0
