@@ -63,16 +63,31 @@ __device__ __forceinline__ TileSeg tile_seg(const TreeArgs &a, uint32_t t)
     return ts;
 }
 
-// Keys of this lane's TR rounds (lines past the end read as blank).
+// Keys of this lane's TR rounds (lines past the end read as blank).  The
+// loads are unconditional (past the end they read the document's last key),
+// so all TR are in flight together; a guarded load per round compiled to TR
+// round trips one after another.
 template <int TR>
 __device__ __forceinline__ void load_keys(const uint8_t *keys, const TileSeg &ts, uint32_t wbase, uint32_t (&key)[TR])
 {
     const int lane = threadIdx.x & 63;
+    if (ts.L == 0) {
+#pragma unroll
+        for (int r = 0; r < TR; r++)
+            key[r] = KEY_BLANK;
+        return;
+    }
+    const uint8_t *const kd = keys + ts.g0;
+    const uint64_t last = ts.L - 1;
+    uint32_t v[TR];
 #pragma unroll
     for (int r = 0; r < TR; r++) {
-        const uint32_t g = wbase + r * 64 + lane;
-        key[r] = (g < ts.L) ? (uint32_t)keys[ts.g0 + g] : KEY_BLANK;
+        const uint64_t g = wbase + r * 64 + lane;
+        v[r] = kd[g < last ? g : last];
     }
+#pragma unroll
+    for (int r = 0; r < TR; r++)
+        key[r] = (wbase + r * 64 + lane < ts.L) ? v[r] : KEY_BLANK;
 }
 
 // Bit-sliced level masks of a round (64 lines, lane = line, x = its key):
