@@ -1597,6 +1597,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
             o_t[k] = place(tab[k].size() * 4);
     size_t o_ent = place(ents.size() * 4);
     size_t o_pool = place(pool.size() * 4);
+    size_t o_zero = place(16);   // an empty slot: the table of a class without patterns
     ol_dict *s = new (std::nothrow) ol_dict();
     if (!s)
         return fail(OL_ERR_NOMEM, "out of host memory");
@@ -1621,7 +1622,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     }
     dd.filter = (const uint32_t *)(base + o_f);
     for (uint32_t k = 1; k <= 5; k++)
-        dd.table[k] = tab[k].empty() ? nullptr : (const uint4 *)(base + o_t[k]);
+        dd.table[k] = (const uint4 *)(base + (tab[k].empty() ? o_zero : o_t[k]));
     dd.ents = (const uint4 *)(base + o_ent);
     dd.pool = (const uint32_t *)(base + o_pool);
     s->d = dd;

@@ -346,7 +346,8 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
     const uint32_t w1 = sh ? __builtin_amdgcn_alignbyte(a2, a1, sh) : a1;
     // One run per class (1..5): the patterns whose key matches the text here.
     // Named variables per class (no arrays: nothing may be indexed at run time).
-    // First the table slots of every class are loaded together, then resolved.
+    // First the table slots of every class are loaded together (absent
+    // classes read an empty slot), then resolved.
     bool pass345 = true;
     if (GATE && d.fbits[4]) {
         const uint32_t ph = dict_ph(w0);
@@ -358,7 +359,9 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
                         ((K) <= 4 ? (K) : 8u) <= rem &&                                      \
                         ((K) >= 3 ? pass345 : true);                                         \
     uint32_t h##K = use##K ? dict_thash(klo##K, khi##K, d.tbits[K]) : 0u;                  \
-    uint4 v##K = use##K ? d.table[K][h##K] : make_uint4(0, 0, 0, 0);
+    const uint4 t##K = d.table[K][h##K];   /* unconditional: all classes' loads in flight */   \
+    const uint32_t m##K = use##K ? ~0u : 0u;   /* masked, not selected: keeps the load */      \
+    uint4 v##K = make_uint4(t##K.x & m##K, t##K.y & m##K, t##K.z & m##K, t##K.w & m##K);
     OL_KEY(1, w0 & 0xFFu)
     OL_KEY(2, w0 & 0xFFFFu)
     OL_KEY(3, w0 & 0xFFFFFFu)
@@ -387,8 +390,16 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
     };
     auto tail_mask = [](uint32_t nb) -> uint32_t { return nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u); };
     const uint4 none = make_uint4(0xFFFFFFFFu, 0, 0, 0);
-    uint4 c1 = b1 < e1 ? d.ents[b1] : none, c2 = b2 < e2 ? d.ents[b2] : none, c3 = b3 < e3 ? d.ents[b3] : none,
-          c4 = b4 < e4 ? d.ents[b4] : none, c5 = b5 < e5 ? d.ents[b5] : none;
+    // each run's first entry, loaded unconditionally (entry 0 always exists)
+    // so that the loads are in flight together
+    const uint4 f1 = d.ents[b1 < e1 ? b1 : 0u], f2 = d.ents[b2 < e2 ? b2 : 0u], f3 = d.ents[b3 < e3 ? b3 : 0u],
+                f4 = d.ents[b4 < e4 ? b4 : 0u], f5 = d.ents[b5 < e5 ? b5 : 0u];
+    auto first_or_none = [&](const uint4 &f, bool has) -> uint4 {   // masked, not selected: keeps the load
+        const uint32_t m = has ? ~0u : 0u;
+        return make_uint4((f.x & m) | ~m, f.y & m, f.z & m, f.w & m);
+    };
+    uint4 c1 = first_or_none(f1, b1 < e1), c2 = first_or_none(f2, b2 < e2), c3 = first_or_none(f3, b3 < e3),
+          c4 = first_or_none(f4, b4 < e4), c5 = first_or_none(f5, b5 < e5);
     uint32_t count = 0;
     while (true) {
         // the run whose next entry has the smallest pattern id (runs are sorted by id)
