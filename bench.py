@@ -472,8 +472,9 @@ def query_benchmark(p, doc, tree, L):
         p.profile_enable(False)
         c, ms = p.profile_take().get("query", (1, 0.0))
         kms = ms / max(1, c)
-        # bytes: classes + tree keys read twice (count, write) + 4 B per match written
-        b = 4 * L + 4 * q.total_matches
+        # bytes: the class bytes (a kind asked) and the key bytes (a depth asked),
+        # each read by the count and the write pass, + 4 B per match written
+        b = 2 * L * ((kind is not None) + (depth is not None)) + 4 * q.total_matches
         res.append({"kind": kind, "depth": depth, "matches": q.total_matches, "call_ms": round(dt * 1e3, 3),
                     "kernel_ms": round(kms, 4), "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1)})
         log(f"[bench] query {res[-1]}")

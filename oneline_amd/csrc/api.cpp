@@ -734,7 +734,7 @@ int run_query(ol_ctx *c, const ol_doc *d, uint32_t r0, uint32_t r1, uint32_t kin
     *n_indices = 0;
     if (r1 <= r0)
         return OL_OK;
-    const uint32_t blocks = ol_query_blocks(r1 - r0);
+    const uint32_t blocks = ol_query_blocks(r0, r1);
     HIPCHK(ensure_scratch(c, blocks));
     HIPCHK(clear_scratch(c, blocks));
     const uint64_t cap = std::min<uint64_t>(max_matches, r1 - r0);

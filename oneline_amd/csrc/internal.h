@@ -434,7 +434,7 @@ struct QueryArgs {
     uint64_t cap;
 };
 hipError_t ol_launch_query(const QueryArgs &a, uint64_t *total, hipStream_t s);
-uint32_t ol_query_blocks(uint32_t lines);   // count blocks of a query over `lines` lines
+uint32_t ol_query_blocks(uint32_t r0, uint32_t r1);   // count blocks of a query over lines [r0, r1)
 hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t *offsets,
                                  uint64_t lines, ol_node_dev *nodes, hipStream_t s);
 hipError_t ol_launch_nodes_to_lines(const ol_node_dev *nodes, uint64_t lines, uint8_t *classes, uint8_t *keys,

@@ -879,39 +879,80 @@ hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t
 constexpr uint32_t QUERY_THREADS = 256, QUERY_PER_THREAD = 16;
 constexpr uint32_t QUERY_BLOCK = QUERY_THREADS * QUERY_PER_THREAD;
 
-__device__ __forceinline__ bool query_match(const QueryArgs &a, uint32_t i)
+// Bit j of the result: byte j of `w` equals the byte `v` (SWAR zero-byte test
+// of w ^ v, exact per byte: the low seven bits are added without carries into
+// the next byte).
+__device__ __forceinline__ uint32_t query_byte_eq4(uint32_t w, uint32_t v)
 {
-    if (a.kind != 0 && a.classes[i] != a.kind)
-        return false;
-    if (a.depth == 0xFFFFFFFFu)
-        return true;
-    uint32_t depth;
-    if (a.nodes) {
-        depth = a.nodes[i].depth;
-    } else {
-        const uint32_t key = a.keys[i];
-        if (key < KEY_DEEP) {
-            depth = key;
-        } else {   // blank or very deep: the line's own depth
+    const uint32_t x = w ^ (v * 0x01010101u);
+    const uint32_t nz = (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+    const uint32_t e = (~nz & 0x80808080u) >> 7;   // bits 0, 8, 16, 24
+    return ((e * 0x204081u) >> 21) & 0xFu;         // gathered into bits 0..3
+}
+
+// Bit j: byte j of `w` is >= KEY_DEEP (254 or 255: depth not in the key).
+__device__ __forceinline__ uint32_t query_byte_deep4(uint32_t w)
+{
+    return query_byte_eq4(w | 0x01010101u, 0xFFu);   // 254 and 255 are the bytes with bits 1..7 set
+}
+
+__device__ __forceinline__ uint32_t query_eq16(uint4 v, uint32_t b)
+{
+    return query_byte_eq4(v.x, b) | query_byte_eq4(v.y, b) << 4 | query_byte_eq4(v.z, b) << 8 |
+           query_byte_eq4(v.w, b) << 12;
+}
+
+__device__ __forceinline__ uint32_t query_deep16(uint4 v)
+{
+    return query_byte_deep4(v.x) | query_byte_deep4(v.y) << 4 | query_byte_deep4(v.z) << 8 |
+           query_byte_deep4(v.w) << 12;
+}
+
+// First line of the thread's QUERY_PER_THREAD lines: blocks start on the
+// 16-line boundary at or below r0, so a thread's lines are one aligned
+// 16-byte word of the class and key arrays.
+__device__ __forceinline__ uint32_t query_line0(const QueryArgs &a, uint32_t blk)
+{
+    return (a.r0 & ~(QUERY_PER_THREAD - 1)) + blk * QUERY_BLOCK + threadIdx.x * QUERY_PER_THREAD;
+}
+
+// Match bits of the lines i0 .. i0 + 15 (bit k = line i0 + k).  The classes
+// and keys arrive in one 16-byte load each (both arrays are padded to
+// LINE_ALIGN lines, and i0 < r1 <= lines); a byte load per line behind the
+// range test compiled to sixteen round trips in a row.  Only lines whose key
+// does not hold the depth (blank, depth >= 254) read more: the FlatNode when
+// the document keeps them, else the line parsed again.
+__device__ __forceinline__ uint32_t query_bits(const QueryArgs &a, uint32_t i0)
+{
+    if (i0 >= a.r1)
+        return 0;
+    uint32_t bits = 0xFFFFu;
+    if (i0 < a.r0)
+        bits &= 0xFFFFu << (a.r0 - i0);
+    if (a.r1 - i0 < QUERY_PER_THREAD)
+        bits &= (1u << (a.r1 - i0)) - 1u;
+    if (a.kind != 0)
+        bits &= a.kind < 256 ? query_eq16(*(const uint4 *)(a.classes + i0), a.kind) : 0u;
+    if (a.depth == 0xFFFFFFFFu || bits == 0)
+        return bits;
+    const uint4 key = *(const uint4 *)(a.keys + i0);
+    uint32_t eq = a.depth < KEY_DEEP ? query_eq16(key, a.depth) : 0u;
+    uint32_t deep = bits & query_deep16(key);
+    while (deep) {
+        const uint32_t k = (uint32_t)__builtin_ctz(deep);
+        deep &= deep - 1;
+        const uint32_t i = i0 + k;
+        uint32_t depth;
+        if (a.nodes) {
+            depth = a.nodes[i].depth;
+        } else {
             const uint32_t s = a.offsets[i];
             const uint32_t e = (i + 1 < a.lines) ? a.offsets[i + 1] - 1 : a.n - 1;
             depth = parse_span(GlobalSrc{a.text}, s, e).depth;
         }
+        eq |= (depth == a.depth) ? 1u << k : 0u;
     }
-    return depth == a.depth;
-}
-
-// The thread's QUERY_PER_THREAD consecutive lines of its block: match bits.
-__device__ __forceinline__ uint32_t query_bits(const QueryArgs &a, uint32_t blk)
-{
-    const uint32_t i0 = a.r0 + blk * QUERY_BLOCK + threadIdx.x * QUERY_PER_THREAD;
-    uint32_t bits = 0;
-    for (uint32_t k = 0; k < QUERY_PER_THREAD; k++) {
-        const uint32_t i = i0 + k;
-        if (i < a.r1 && query_match(a, i))
-            bits |= 1u << k;
-    }
-    return bits;
+    return bits & eq;
 }
 
 __device__ __forceinline__ uint32_t query_block_excl(uint32_t v, uint32_t &total)
@@ -933,7 +974,7 @@ __device__ __forceinline__ uint32_t query_block_excl(uint32_t v, uint32_t &total
 
 __global__ __launch_bounds__(QUERY_THREADS) void query_count_kernel(QueryArgs a)
 {
-    const uint32_t bits = query_bits(a, blockIdx.x);
+    const uint32_t bits = query_bits(a, query_line0(a, blockIdx.x));
     uint32_t total;
     (void)query_block_excl((uint32_t)__builtin_popcount(bits), total);
     if (threadIdx.x == 0)
@@ -942,11 +983,11 @@ __global__ __launch_bounds__(QUERY_THREADS) void query_count_kernel(QueryArgs a)
 
 __global__ __launch_bounds__(QUERY_THREADS) void query_write_kernel(QueryArgs a)
 {
-    const uint32_t bits = query_bits(a, blockIdx.x);
+    const uint32_t i0 = query_line0(a, blockIdx.x);
+    const uint32_t bits = query_bits(a, i0);
     uint32_t total;
     const uint32_t ex = query_block_excl((uint32_t)__builtin_popcount(bits), total);
     uint64_t r = a.block_off[blockIdx.x] + ex;
-    const uint32_t i0 = a.r0 + blockIdx.x * QUERY_BLOCK + threadIdx.x * QUERY_PER_THREAD;
     uint32_t m = bits;
     while (m) {
         const uint32_t k = (uint32_t)__builtin_ctz(m);
@@ -989,7 +1030,7 @@ hipError_t ol_launch_query(const QueryArgs &a, uint64_t *total, hipStream_t s)
 {
     if (a.r1 <= a.r0)
         return hipSuccess;
-    const uint32_t blocks = (a.r1 - a.r0 + QUERY_BLOCK - 1) / QUERY_BLOCK;
+    const uint32_t blocks = ol_query_blocks(a.r0, a.r1);
     hipLaunchKernelGGL(query_count_kernel, dim3(blocks), dim3(QUERY_THREADS), 0, s, a);
     hipLaunchKernelGGL(query_scan_kernel, dim3(1), dim3(1024), 0, s, a, blocks, total);
     if (a.out && a.cap)
@@ -997,7 +1038,11 @@ hipError_t ol_launch_query(const QueryArgs &a, uint64_t *total, hipStream_t s)
     return hipGetLastError();
 }
 
-uint32_t ol_query_blocks(uint32_t lines) { return (lines + QUERY_BLOCK - 1) / QUERY_BLOCK; }
+uint32_t ol_query_blocks(uint32_t r0, uint32_t r1)
+{
+    const uint32_t lines = r1 - (r0 & ~(QUERY_PER_THREAD - 1));
+    return (uint32_t)(((uint64_t)lines + QUERY_BLOCK - 1) / QUERY_BLOCK);
+}
 
 // Classes and tree keys from FlatNodes (documents loaded from a .olb file,
 // whose nodes were parsed when it was written).
