@@ -396,9 +396,9 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
 
     const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t above = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
-    // the wave's first output line: outputs are stored at a per-wave base
-    // (uniform) plus a small per-lane offset
-    const uint64_t wg = ts.g0 + wbase;
+    // the lane's output line in round 0: round r stores 64 r entries further,
+    // an immediate offset on one address per output kept across the rounds
+    const uint64_t wg = ts.g0 + wbase + lane;
     uint32_t *const P = a.parent ? a.parent + wg : nullptr;
     uint32_t *const FC = a.first_child ? a.first_child + wg : nullptr;
     uint32_t *const NS = a.next_sibling ? a.next_sibling + wg : nullptr;
@@ -416,10 +416,10 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
         const uint32_t jp = pm ? 63u - (uint32_t)__clzll(pm) : 0u;
         const uint32_t js = sm ? (uint32_t)__builtin_ctzll(sm) : 0u;
         const uint32_t jf = fm ? (uint32_t)__builtin_ctzll(fm) : 0u;
-        const uint32_t kp = __shfl(x, jp, 64), ks = __shfl(x, js, 64), kf = __shfl(x, jf, 64);
+        const uint32_t kp = __shfl(x, jp, 64), ks = __shfl(x, js, 64);
         const uint32_t gv = __shfl(G, xi, 64), gl = __shfl(GL, xi, 64);
         const uint32_t hv = s_h[w][r][xi], hl = s_hl[w][r][xi];
-        const uint32_t nv = s_h[w][r][D - 1], nl = s_hl[w][r][D - 1];   // first content line after the round
+        const uint32_t nv = s_h[w][r][D - 1];   // first content line after the round
 
         const uint32_t g = rb + lane;
         // content line (blank and too deep lines: below; too deep is reported
@@ -428,16 +428,17 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
         const uint32_t plv = pm ? kp : gl;
         const uint32_t cse = sm ? rb + js : (hv != OL_NONE ? hv : (uint32_t)L);
         const uint32_t slv = sm ? ks : hl;
+        // the next content line is a child exactly when it comes before
+        // subtree_end (the first later line with key <= x); NONE > L >= cse
         const uint32_t fi = fm ? rb + jf : nv;
-        const uint32_t fl = fm ? kf : nl;
-        const uint32_t cfc = (fi != OL_NONE && fl > x) ? fi : OL_NONE;
+        const uint32_t cfc = fi < cse ? fi : OL_NONE;
         const uint32_t cns = (cse < L && (cpar == OL_NONE || plv < slv)) ? cse : OL_NONE;
         const uint32_t par = valid ? cpar : OL_NONE;
         const uint32_t se = valid ? cse : g + 1;
         const uint32_t fc = valid ? cfc : OL_NONE;
         const uint32_t ns = valid ? cns : OL_NONE;
         if (g < L) {
-            const uint32_t o = (uint32_t)(r * 64 + lane);
+            const uint32_t o = (uint32_t)(r * 64);   // the lane is in the pointers
             if (P) P[o] = par;
             if (SE) SE[o] = se;
             if (FC) FC[o] = fc;
