@@ -76,7 +76,7 @@ static uint32_t max_grid(const void *f, int threads, size_t lds) { return ol_cor
 // fields, and kept as 16-bit masks per vector until the next iteration, when
 // the scanner has (almost always) produced the tile's prefix; then they are
 // written.  The next tile is loaded while the current one is processed.
-template <int THREADS, int NV>
+template <int THREADS, int NV, bool FULL>
 __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
 {
     static_assert(NV % 4 == 0, "vectors per lane: 16-bit counts, two per word");
@@ -197,12 +197,28 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         for (int k = 0; k < NV; k++) {
             const uint32_t W[5] = {cv[k].x, cv[k].y, cv[k].z, cv[k].w, cn[k]};
             const uint32_t g = base + (uint32_t)(k * THREADS + tid) * 16;
-            uint32_t hits = 0;
-#pragma unroll
-            for (int p = 0; p < 16; p++) {
+            // FULL: a 4-byte window (no mask).  The 32 KiB-tile kernel gets
+            // windows rarer than one per 8 KiB: it first takes each position's
+            // compare as a wave mask (one compare, the masks ORed by the scalar
+            // unit) and builds the lanes' hit bits only when a lane of the wave
+            // has a hit in this vector.
+            auto window = [&](int p) -> uint32_t {
                 const uint32_t win = (p & 3) ? __builtin_amdgcn_alignbyte(W[(p >> 2) + 1], W[p >> 2], p & 3)
                                              : W[p >> 2];
-                hits |= (uint32_t)((win & mask4) == pat4) << p;
+                return FULL ? win : win & mask4;
+            };
+            uint64_t any = ~0ull;
+            if (NV == 8) {
+                any = 0;
+#pragma unroll
+                for (int p = 0; p < 16; p++)
+                    any |= __ballot(window(p) == pat4);
+            }
+            uint32_t hits = 0;
+            if (any) {
+#pragma unroll
+                for (int p = 0; p < 16; p++)
+                    hits |= (uint32_t)(window(p) == pat4) << p;
             }
             // window positions past the last possible one, or before wo
             if (a.n < mlen || g > last_q)
@@ -270,11 +286,11 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         flush();
 }
 
-template <int NV>
+template <int NV, bool FULL>
 static hipError_t launch_search_nv(const SearchArgs &a, hipStream_t s)
 {
     static uint32_t cap = 0;   // co-resident limit (same device for the process)
-    const void *f = (const void *)search_kernel<SEARCH_THREADS, NV>;
+    const void *f = (const void *)search_kernel<SEARCH_THREADS, NV, FULL>;
     if (!cap)
         cap = max_grid(f, SEARCH_THREADS, 0);
     if (!cap)
@@ -294,7 +310,9 @@ uint32_t ol_literal_tile(bool dense) { return SEARCH_THREADS * (dense ? 4u : 8u)
 // for more waves, which hide verifications when the window is frequent.
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 {
-    return a.dense ? launch_search_nv<4>(a, s) : launch_search_nv<8>(a, s);
+    if (a.win_mask == 0xFFFFFFFFu)
+        return a.dense ? launch_search_nv<4, true>(a, s) : launch_search_nv<8, true>(a, s);
+    return a.dense ? launch_search_nv<4, false>(a, s) : launch_search_nv<8, false>(a, s);
 }
 
 // ---- dictionary -----------------------------------------------------------------
