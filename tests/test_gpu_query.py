@@ -82,3 +82,27 @@ def test_query_raw_and_empty(parser):
     assert e.total_matches == 0 and e.indices.size == 0
     one = parser.query_raw(b"| | x", None, 2)
     assert one.total_matches == 1 and list(one.indices) == [0]
+
+
+def test_query_subtree_across_blocks(parser):
+    # a root off the 16-line boundary whose subtree spans several blocks of
+    # 4,096 lines and ends inside a 16-line word
+    rng = np.random.default_rng(64)
+    bodies = [b"x", b"= h", b":c", b"@e", b"- l", b"-. o", b"// c", b"-x", b"/x", b""]
+    lines = [b"top %d" % i for i in range(7)] + [b"root"]
+    depth = 1
+    for _ in range(10_001):
+        depth = int(np.clip(depth + rng.integers(-1, 2), 1, 5))
+        lines.append(b"| " * depth + bodies[rng.integers(len(bodies))])
+    lines += [b"after", b"| tail"]
+    text = b"\n".join(lines) + b"\n"
+    _, nodes = orc.parse(np.frombuffer(text, np.uint8))
+    _, se, _, _ = orc.links(nodes)
+    assert se[7] == nodes.size - 2 and se[7] % 16 != 0
+    doc = parser.upload(text, nodes=False)
+    tree = parser.build_tree(doc)
+    for root in (7, 8, 9, 4100, 8195):
+        for kind, depth in ((None, None), (4, None), (None, 3), (2, 2), (6, 5)):
+            got = parser.query_subtree(doc, tree, root, kind, depth)
+            w = want(nodes, kind, depth, root, int(se[root]))
+            assert got.total_matches == w.size and np.array_equal(got.indices, w), (root, kind, depth)
