@@ -2,7 +2,8 @@
 properties that need no CPU pass over the whole text: the dictionary's hits
 are grouped by pattern with ascending offsets, a sample of them spell their
 pattern in the text, and per-pattern counts and offsets equal those of the
-literal search kernel (a different algorithm) for a sample of patterns.
+literal search kernel (a different algorithm) for a sample of patterns; the
+query kernels are checked at the same size against the downloaded nodes.
 Smaller documents are compared with the oracle hit by hit in
 test_gpu_many.py and test_gpu_search.py."""
 import numpy as np
@@ -61,3 +62,25 @@ def test_dictionary_65536_counts_against_literal_searches(parser, big):
     for i in sample:
         lit = parser.search_document(doc, pats[i], 0)
         assert lit.total_matches == r.counts[i], (int(i), pats[i])
+
+
+def test_queries_against_downloaded_nodes(parser, big):
+    # the query kernels at 2 GiB (lean document: keys, blank lines parsed again;
+    # kept nodes: depths from the FlatNodes) against the predicate on the
+    # downloaded nodes (the parse itself is checked against the oracle by the
+    # benchmark at this size)
+    text, doc = big
+    kept = parser.upload(text, nodes=True)
+    _, cls, nodes = kept.download(nodes=True)
+    assert np.array_equal(cls, nodes["kind"])
+    for kind, depth in ((4, None), (None, 3), (2, 5), (7, None), (None, 0)):
+        sel = np.ones(nodes.size, bool)
+        if kind is not None:
+            sel &= nodes["kind"] == kind
+        if depth is not None:
+            sel &= nodes["depth"] == depth
+        w = np.flatnonzero(sel).astype(np.uint32)
+        for d in (doc, kept):
+            got = parser.query_document(d, kind, depth, max_matches=1 << 26)
+            assert got.total_matches == w.size and np.array_equal(got.indices, w), (kind, depth)
+    kept.close()
