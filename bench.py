@@ -57,6 +57,8 @@ def cpu_baseline(p, doc, tree, text, args):
         return time.perf_counter() - t0
 
     run()   # warm-up; then agreement with the GPU outputs, as the reference's CLI checks
+    p.rebuild_tree_async(doc, tree)   # every link array (the timed step skips subtree_end)
+    p.check_tree(tree)
     o, c, _ = doc.download()
     got = p.download_links(tree)
     assert np.array_equal(o[:L], out[0]) and np.array_equal(c[:L], out[1]["kind"].astype(np.uint8)), \
@@ -141,27 +143,33 @@ def main():
     log(f"[bench] second upload+scan {upload2_s:.3f}s = {n / upload2_s / 1e9:.1f} GB/s")
 
     verified = False
+    want = None
     if not args.no_verify:
         from oracle import pyoracle as orc
         log("[bench] verifying against the oracle (full document)")
         off, nodes = orc.parse(text)
         o, c, _ = doc.download()
         assert np.array_equal(o, off), "offsets differ"
-        assert np.array_equal(c, nodes["kind"].astype(np.uint8)), "classes differ"
-        want = orc.links(nodes)
+        kinds = nodes["kind"].astype(np.uint8)
+        assert np.array_equal(c, kinds), "classes differ"
+        wl = orc.links(nodes)
         got = p.download_links(tree)
         for name, g, w in zip(("parent", "subtree_end", "first_child", "next_sibling"),
-                              (got.parent, got.subtree_end, got.first_child, got.next_sibling), want):
+                              (got.parent, got.subtree_end, got.first_child, got.next_sibling), wl):
             assert np.array_equal(g, w), f"{name} differs"
-        del off, nodes, o, c, want, got
-        verified = True
+        want = (off, kinds, wl[0], wl[2], wl[3])   # re-checked after the timed steps
+        del nodes, o, c, got, wl
         log("[bench] verified bit-exact")
 
     lib = ol.load_library()
 
+    # the step's outputs: offsets + classes (scan), parent / first_child /
+    # next_sibling (tree); subtree_end is not part of the metric's bytes
+    links = ol.LINK_PARENT | ol.LINK_FIRST_CHILD | ol.LINK_NEXT_SIBLING
+
     def step():
         ol._check(lib.ol_doc_reparse(p._ctx, doc._h))
-        ol._check(lib.ol_tree_rebuild_async(p._ctx, doc._h, tree._h))
+        ol._check(lib.ol_tree_rebuild_links_async(p._ctx, doc._h, tree._h, links))
 
     tuning = None
     if not args.no_tune:
@@ -182,7 +190,7 @@ def main():
     dt = time.perf_counter() - t0
     p.profile_enable(False)
     prof = p.profile_take()
-    ol._check(lib.ol_tree_check(p._ctx, tree._h))
+    p.check_tree(tree)
     ms = dt / args.steps * 1e3
     alg_bytes = n + 17 * L
     value = alg_bytes / (dt / args.steps) / 1e9
@@ -201,9 +209,22 @@ def main():
         step()
     p.synchronize()
     dt2 = time.perf_counter() - t0
+    p.check_tree(tree)
     ms2 = dt2 / args.steps * 1e3
     value2 = alg_bytes / (dt2 / args.steps) / 1e9
     log(f"[bench] without events: {ms2:.3f} ms/step, {value2:.1f} GB/s")
+    if want is not None:
+        # the outputs of the last timed step, against the oracle
+        o, c, _ = doc.download()
+        L3 = tree.line_count()
+        arrs = [np.zeros(L3, np.uint32) for _ in range(3)]
+        ol._check(lib.ol_tree_download(p._ctx, tree._h, ol._ptr(arrs[0]), None, ol._ptr(arrs[1]), ol._ptr(arrs[2])))
+        for name, g, w in zip(("offsets", "classes", "parent", "first_child", "next_sibling"),
+                              (o, c, *arrs), want):
+            assert np.array_equal(g, w), f"{name} of the timed step differs from the oracle"
+        del o, c, arrs, want
+        verified = True
+        log("[bench] timed step's outputs bit-exact against the oracle")
 
     if args.copy_calibration:
         copy_calibration(1 << 30)
@@ -222,6 +243,8 @@ def main():
             extra["query"] = query_benchmark(p, doc, tree, L)
         if "olb" in which:
             extra["olb"] = olb_benchmark(p, text)
+    tree.close()
+    doc.close()
     del doc, tree, text
     if not args.no_batch:
         extra["batch"] = batch_benchmark(p, args)
@@ -269,6 +292,7 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extra)
+    p.close()
     print(json.dumps(out), flush=True)
 
 

@@ -81,7 +81,8 @@ int ol_ctx_adapter_name(ol_ctx *ctx, char *buf, size_t cap);
 int ol_ctx_is_software(ol_ctx *ctx);
 /* GpuParser::timestamps_supported (crates/oneline-gpu/src/document.rs:130). */
 int ol_ctx_timestamps_supported(ol_ctx *ctx);
-/* Waits for all work of the context. */
+/* Waits for all work of the context; OL_ERR_DEVICE if a kernel of it set a
+ * device error flag since the last check (the flags are sticky until read). */
 int ol_ctx_synchronize(ol_ctx *ctx);
 /* Thread-local message of the last failed call ("" if none). */
 const char *ol_last_error(void);
@@ -131,9 +132,19 @@ size_t ol_tree_line_count(const ol_tree *tree);
  * next_sibling.  Any pointer may be NULL; each holds line_count entries. */
 int ol_tree_download(ol_ctx *ctx, const ol_tree *tree, uint32_t *parent, uint32_t *subtree_end,
                      uint32_t *first_child, uint32_t *next_sibling);
-/* Rebuilds the links of `doc` into `tree` (same line count) without a host
- * round trip; the error flag is checked by the next ol_tree_check. */
+/* Rebuilds the links of `doc` into `tree` (capacity at least the document's
+ * lines) without a host round trip; the depth and device error flags are
+ * checked by the next ol_tree_check.  Same arrays as ol_tree_build. */
 int ol_tree_rebuild_async(ol_ctx *ctx, const ol_doc *doc, ol_tree *tree);
+/* The same, writing only the arrays in `links` (OL_LINK_* bits): a caller that
+ * needs parent / first_child / next_sibling skips the subtree_end writes.
+ * ol_tree_download of an array the last build did not write fails. */
+#define OL_LINK_PARENT 1u
+#define OL_LINK_SUBTREE_END 2u
+#define OL_LINK_FIRST_CHILD 4u
+#define OL_LINK_NEXT_SIBLING 8u
+#define OL_LINK_ALL 15u
+int ol_tree_rebuild_links_async(ol_ctx *ctx, const ol_doc *doc, ol_tree *tree, uint32_t links);
 int ol_tree_check(ol_ctx *ctx, const ol_tree *tree);
 
 /* ---- literal search --------------------------------------------------------------- */

@@ -31,6 +31,10 @@ MAX_SET_PATTERN_LEN = 64
 MAX_PATTERNS = 1 << 22
 MAX_TREE_DEPTH = 32
 
+# link arrays of a tree rebuild (OL_LINK_*)
+LINK_PARENT, LINK_SUBTREE_END, LINK_FIRST_CHILD, LINK_NEXT_SIBLING = 1, 2, 4, 8
+LINK_ALL = 15
+
 # NodeKind, crates/oneline-core/src/lib.rs:41-51
 BLANK, TEXT, HEADING, ELEMENT, LIST, CONNECT, ORDERED_LIST, COMMENT = range(8)
 
@@ -88,6 +92,7 @@ def load_library() -> ctypes.CDLL:
         "ol_tree_line_count": [P],
         "ol_tree_download": [P, P, P, P, P, P],
         "ol_tree_rebuild_async": [P, P, P],
+        "ol_tree_rebuild_links_async": [P, P, P, ctypes.c_uint32],
         "ol_tree_check": [P, P],
         "ol_search_document": [P, P, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_raw": [P, P, S, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
@@ -466,6 +471,19 @@ class GpuParser:
         h = ctypes.c_void_p()
         _check(_lib.ol_tree_build(self._ctx, doc._h, reuse, ctypes.byref(h)))
         return GpuTree(h)
+
+    def reparse_async(self, doc: GpuDocument) -> None:
+        """Scan + classify the resident text again into the resident arrays
+        (no host round trip)."""
+        _check(_lib.ol_doc_reparse(self._ctx, doc._h))
+
+    def rebuild_tree_async(self, doc: GpuDocument, tree: GpuTree, links: int = LINK_ALL) -> None:
+        """Links of `doc` into `tree` again, only the arrays in `links`
+        (LINK_* bits); errors surface at check_tree / synchronize."""
+        _check(_lib.ol_tree_rebuild_links_async(self._ctx, doc._h, tree._h, links))
+
+    def check_tree(self, tree: GpuTree) -> None:
+        _check(_lib.ol_tree_check(self._ctx, tree._h))
 
     def download_links(self, tree: GpuTree) -> TreeLinks:
         L = tree.line_count()

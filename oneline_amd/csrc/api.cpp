@@ -68,19 +68,49 @@ struct DevBuf {
 
 }  // namespace
 
-// Scratch layout (u64 words): [0, 8) meta, [8, 16) counters (u32 x 16),
-// [16, ...) per-tile look-back words.
-static constexpr size_t SCR_META = 0, SCR_CTR = 8, SCR_STATUS = 16;
+// Device scratch of one owner: a context, or a batch (whose launch sequence
+// may be captured as a graph, which keeps the addresses).
+//   head     never reallocated: u64 meta[8] (results the host reads back),
+//            u32 ctr[16] (ctr[CTR_TREE_ERR]: the tree's depth flag,
+//            ctr[CTR_DEV_ERR]: the sticky device error bits, OLE_*);
+//   status   u64 words, grown on demand: word 0 the ticket counter of a
+//            ticket-ordered scan, the per-tile words from STATUS_TILES on;
+//   tree_sum the tree's per-tile level states; cands the dictionary search's
+//            candidate store.
+// `dirty` counts the leading status words that may be non-zero: cooperative
+// scans and literal searches need their words zero and leave them zero; every
+// other user (ticket-ordered scans, dictionary searches, queries, a graph
+// replay) marks its range dirty, and the next launch that needs zeros clears
+// it first.
+static constexpr size_t HEAD_META = 0, HEAD_CTR = 8, HEAD_WORDS = 16;
+static constexpr uint32_t CTR_TREE_ERR = 1, CTR_DEV_ERR = 4;
+static constexpr size_t STATUS_TILES = 8;
+
+struct Scratch {
+    DevBuf head, status, tree_sum, cands;
+    size_t dirty = 0;
+    bool frozen = false;   // a captured graph holds the addresses: growing is an error
+    uint64_t *meta() { return (uint64_t *)head.p + HEAD_META; }
+    uint32_t *ctr() { return (uint32_t *)((uint64_t *)head.p + HEAD_CTR); }
+    uint64_t *words() { return (uint64_t *)status.p; }
+    size_t nwords() const { return status.bytes / 8; }
+    void release()
+    {
+        head.release();
+        status.release();
+        tree_sum.release();
+        cands.release();
+        dirty = 0;
+        frozen = false;
+    }
+};
 
 struct ol_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string name;
     std::mutex mu;
-    DevBuf scratch;
-    DevBuf tree_sum;                       // tree per-tile level states (4 arrays of tiles x TREE_D)
-    DevBuf fix;                            // list of lines decoded by the scan fixup pass
-    DevBuf cands;                          // dictionary search: candidates kept for the write pass
+    Scratch scr;
     // launch geometry picked by the autotuner (ol_tune_set)
     uint32_t scan_bpt = SCAN_BPT;          // bytes per lane of scan_classify (tile = 256 lanes x bpt)
     uint32_t scan_grid = 0;                // persistent workgroups of scan_classify (0 = all co-resident)
@@ -100,10 +130,6 @@ struct ol_ctx {
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
-
-    uint64_t *meta() { return (uint64_t *)scratch.p + SCR_META; }
-    uint32_t *ctr() { return (uint32_t *)((uint64_t *)scratch.p + SCR_CTR); }
-    uint64_t *status() { return (uint64_t *)scratch.p + SCR_STATUS; }
 };
 
 struct ol_doc {
@@ -124,6 +150,7 @@ struct ol_doc {
 
 struct ol_tree {
     uint64_t lines = 0, cap = 0;
+    uint32_t links = 0;   // OL_LINK_* arrays holding this document's links
     uint32_t *parent = nullptr, *fc = nullptr, *ns = nullptr, *se = nullptr;
 };
 
@@ -175,31 +202,84 @@ struct Prof {
     }
 };
 
-// Scratch: meta and counters, then `words` u64 status words per tile.
-hipError_t ensure_scratch(ol_ctx *c, size_t tiles, size_t words = LB_STRIDE)
+// The head of a scratch (zeroed once, synchronously).
+hipError_t scratch_init(Scratch &s)
 {
-    return c->scratch.ensure((SCR_STATUS + tiles * words + 64) * 8);
+    hipError_t e = s.head.ensure(HEAD_WORDS * 8);
+    if (e == hipSuccess)
+        e = hipMemset(s.head.p, 0, HEAD_WORDS * 8);
+    return e;
 }
 
-hipError_t clear_scratch(ol_ctx *c, size_t tiles, size_t words = LB_STRIDE)
+// At least `words` status words; new memory counts as dirty.
+hipError_t ensure_status(Scratch &s, size_t words)
+{
+    if (s.nwords() >= words && s.status.p)
+        return hipSuccess;
+    if (s.frozen)
+        return hipErrorInvalidValue;   // a captured graph would point at freed memory
+    const hipError_t e = s.status.ensure(words * 8);
+    if (e == hipSuccess)
+        s.dirty = s.nwords();
+    return e;
+}
+
+// Clears status words [0, words) now (a launch on the stream).
+hipError_t clear_status(ol_ctx *c, Scratch &s, size_t words)
 {
     Prof p(c, "clear");
-    return ol_launch_clear_u64((uint64_t *)c->scratch.p, SCR_STATUS + tiles * words, c->stream);
+    return ol_launch_clear_u64(s.words(), words, c->stream);
 }
 
-// Waits for the stream and reads meta[0]; fails if a look-back wait gave up.
-int read_meta(ol_ctx *c, uint64_t *out)
+// Makes every status word zero if some may not be.
+hipError_t zero_status(ol_ctx *c, Scratch &s)
 {
-    uint64_t head[SCR_STATUS];
-    HIPCHK(hipMemcpyAsync(head, c->scratch.p, sizeof head, hipMemcpyDeviceToHost, c->stream));
+    if (!s.dirty)
+        return hipSuccess;
+    const hipError_t e = clear_status(c, s, s.dirty);
+    if (e == hipSuccess)
+        s.dirty = 0;
+    return e;
+}
+
+void soil(Scratch &s, size_t words) { s.dirty = std::max(s.dirty, std::min(words, s.nwords())); }
+
+std::string dev_error_text(uint32_t bits)
+{
+    std::string m;
+    auto add = [&](const char *t) { m += (m.empty() ? "" : "; ") + std::string(t); };
+    if (bits & OLE_PREFIX_WAIT)
+        add("a tile of a cooperative launch gave up waiting for its prefix (the prefix scanner did not reach it)");
+    if (bits & OLE_SCANNER)
+        add("the prefix scanner of a cooperative launch gave up waiting for a tile count");
+    if (bits & OLE_LOOKBACK)
+        add("a decoupled look-back of a ticket-ordered launch gave up waiting for a predecessor tile's status");
+    if (bits & OLE_DICT_SCAN)
+        add("the dictionary search's tile-count scan gave up waiting for an earlier chunk");
+    return m.empty() ? "device error" : m;
+}
+
+// Waits for the stream; reports (and then clears) the sticky device error
+// bits of `s`, and copies meta[0] to *out when asked.
+int sync_check(ol_ctx *c, Scratch &s, uint64_t *out = nullptr)
+{
+    uint64_t head[HEAD_WORDS];
+    HIPCHK(hipMemcpyAsync(head, s.head.p, sizeof head, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const uint32_t *ctr = (const uint32_t *)(head + SCR_CTR);
-    if (ctr[4])
-        return fail(OL_ERR_DEVICE, ctr[4] & 2u ? "the scan's prefix pass gave up waiting for a tile count"
-                                               : "a look-back wait gave up: not every workgroup of a persistent launch was resident");
-    *out = head[SCR_META];
+    const uint32_t bits = ((const uint32_t *)(head + HEAD_CTR))[CTR_DEV_ERR];
+    if (bits) {
+        // the failed launch may have left status words set
+        s.dirty = s.nwords();
+        HIPCHK(hipMemsetAsync(s.ctr() + CTR_DEV_ERR, 0, 4, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return fail(OL_ERR_DEVICE, dev_error_text(bits));
+    }
+    if (out)
+        *out = head[HEAD_META];
     return OL_OK;
 }
+
+int read_meta(ol_ctx *c, Scratch &s, uint64_t *out) { return sync_check(c, s, out); }
 
 void free_doc_lines(ol_doc *d)
 {
@@ -228,23 +308,30 @@ int alloc_doc_lines(ol_doc *d, uint64_t cap)
 
 // Scan + classify of a text into the per-line arrays set in `a` (text, n,
 // cap, outputs and, for batches, the document fields); fills in the rest.
-int launch_scan_args(ol_ctx *c, ScanArgs &a, bool cooperative = true)
+// Status words of a scan over n bytes in either tile order.
+size_t scan_status_words(ol_ctx *c, uint64_t n, bool coop)
+{
+    const uint64_t tiles = ceil_div(n, (uint64_t)SCAN_THREADS * c->scan_bpt);
+    return STATUS_TILES + tiles * (coop ? SCAN_STATUS_WORDS : LB_STRIDE);
+}
+
+int launch_scan_args(ol_ctx *c, Scratch &s, ScanArgs &a, bool cooperative = true)
 {
     uint32_t tiles = (uint32_t)ceil_div(a.n, (uint64_t)SCAN_THREADS * c->scan_bpt);
     // tiles from a ticket unless the cooperative launch guarantees residency
     const bool coop = cooperative && !c->scan_ticket;
-    const size_t words = coop ? SCAN_STATUS_WORDS : LB_STRIDE;
-    HIPCHK(ensure_scratch(c, tiles, words));
-    HIPCHK(clear_scratch(c, tiles, words));
+    const size_t words = scan_status_words(c, a.n, coop);
+    HIPCHK(ensure_status(s, words));
+    if (coop) {
+        HIPCHK(zero_status(c, s));   // the launch leaves its words zero again
+    } else {
+        HIPCHK(clear_status(c, s, words));   // ticket counter and look-back words
+        soil(s, words);
+    }
     a.num_tiles = tiles;
-    a.status = c->status();
-    a.meta = c->meta();
-    a.lb_err = c->ctr() + 4;
-    // a line needing the fixup pass is longer than the scan halo: at most n / 256 of them
-    a.fix_cap = (uint32_t)(a.n / SCAN_HALO + 64);
-    HIPCHK(c->fix.ensure((size_t)a.fix_cap * 16));
-    a.fix_list = (uint4 *)c->fix.p;
-    a.fix_count = c->ctr() + 2;
+    a.status = s.words() + STATUS_TILES;
+    a.meta = s.meta();
+    a.lb_err = s.ctr() + CTR_DEV_ERR;
     const int vi = c->scan_bpt == 64 ? 0 : (c->scan_bpt == 128 ? 1 : 2);
     if (!c->scan_max_grid[vi])
         c->scan_max_grid[vi] = ol_scan_max_grid(c->scan_bpt);
@@ -259,13 +346,9 @@ int launch_scan_args(ol_ctx *c, ScanArgs &a, bool cooperative = true)
         grid = tiles + (coop ? 1 : 0);
     if (coop && grid < 2)
         return fail(OL_ERR_DEVICE, "the scan needs two resident workgroups");
-    a.tile_ctr = coop ? nullptr : c->ctr() + 5;
-    {
-        Prof p(c, "scan_classify");
-        HIPCHK(ol_launch_scan(a, c->scan_bpt, grid, coop, c->stream));
-    }
-    Prof p(c, "scan_fixup");
-    HIPCHK(ol_launch_scan_fixup(a, c->stream));
+    a.tile_ctr = coop ? nullptr : (uint32_t *)s.words();
+    Prof p(c, "scan_classify");
+    HIPCHK(ol_launch_scan(a, c->scan_bpt, grid, coop, c->stream));
     return OL_OK;
 }
 
@@ -279,7 +362,7 @@ int launch_scan(ol_ctx *c, ol_doc *d)
     a.classes = d->classes;
     a.keys = d->keys;
     a.nodes = d->nodes;
-    return launch_scan_args(c, a);
+    return launch_scan_args(c, c->scr, a);
 }
 
 // Host -> HBM upload through the context's pinned staging ring: STAGE_WORKERS
@@ -429,7 +512,7 @@ int upload_doc(ol_ctx *c, const uint8_t *text, size_t len, bool with_nodes, ol_d
         if (rc)
             return bail(rc);
         uint64_t lines = 0;
-        rc = read_meta(c, &lines);
+        rc = read_meta(c, c->scr, &lines);
         if (rc)
             return bail(rc);
         d->lines = lines;
@@ -464,37 +547,39 @@ int ensure_tree(ol_tree *t, uint64_t lines)
     return OL_OK;
 }
 
-int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t)
+// The links a tree launch writes (OL_LINK_* bits); the others are left as
+// they were.
+int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t, uint32_t links)
 {
     uint32_t tiles = (uint32_t)ceil_div(d->lines ? d->lines : 1, (uint64_t)ol_tree_tile_lines(c->tree_rounds));
     size_t part = (size_t)tiles * TREE_D;
-    HIPCHK(ensure_scratch(c, 0));
-    HIPCHK(c->tree_sum.ensure(part * 4 * 4));
-    HIPCHK(clear_scratch(c, 0));
-    uint32_t *base = (uint32_t *)c->tree_sum.p;
+    HIPCHK(c->scr.tree_sum.ensure(part * 4 * 4));
+    uint32_t *base = (uint32_t *)c->scr.tree_sum.p;
     TreeArgs a{};
     a.keys = d->keys;
     a.lines = d->lines;
     a.num_tiles = tiles;
-    a.parent = t->parent;
-    a.first_child = t->fc;
-    a.next_sibling = t->ns;
-    a.subtree_end = t->se;
+    a.parent = (links & OL_LINK_PARENT) ? t->parent : nullptr;
+    a.first_child = (links & OL_LINK_FIRST_CHILD) ? t->fc : nullptr;
+    a.next_sibling = (links & OL_LINK_NEXT_SIBLING) ? t->ns : nullptr;
+    a.subtree_end = (links & OL_LINK_SUBTREE_END) ? t->se : nullptr;
     a.sum_last = base;
     a.sum_first = base + part;
     a.in_last = base + 2 * part;
     a.out_first = base + 3 * part;
-    a.err = c->ctr() + 1;
+    a.err = c->scr.ctr() + CTR_TREE_ERR;
     Prof p(c, "tree");
     HIPCHK(ol_launch_tree(a, c->tree_rounds, c->stream));
     return OL_OK;
 }
 
-int check_tree_error(ol_ctx *c)
+int check_tree_error(ol_ctx *c, Scratch &s)
 {
     uint32_t err = 0;
-    HIPCHK(hipMemcpyAsync(&err, c->ctr() + 1, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(&err, s.ctr() + CTR_TREE_ERR, 4, hipMemcpyDeviceToHost, c->stream));
+    int rc = sync_check(c, s);
+    if (rc)
+        return rc;
     if (err)
         return fail(OL_ERR_TOO_DEEP,
                     "a content line is nested deeper than the " + std::to_string(OL_MAX_TREE_DEPTH) +
@@ -598,8 +683,9 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
     // takes 4 vectors per lane, which leaves registers for more waves
     a.dense = m > 4 && d->sample.size() && (uint64_t)hits * 8192 > d->sample.size();
     uint32_t tiles = (uint32_t)ceil_div(d->n, ol_literal_tile(a.dense));
-    HIPCHK(ensure_scratch(c, tiles));
-    HIPCHK(clear_scratch(c, tiles));
+    Scratch &s = c->scr;
+    HIPCHK(ensure_status(s, STATUS_TILES + (size_t)tiles * SCAN_STATUS_WORDS));
+    HIPCHK(zero_status(c, s));   // each tile zeroes its words again
     a.text = d->text;
     a.n = (uint32_t)d->n;
     a.num_tiles = tiles;
@@ -623,14 +709,14 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
     a.cap = cap;
     a.out64 = out64;
     a.out32 = out32;
-    a.status = c->status();
-    a.meta = c->meta();
-    a.lb_err = c->ctr() + 4;
+    a.status = s.words() + STATUS_TILES;
+    a.meta = s.meta();
+    a.lb_err = s.ctr() + CTR_DEV_ERR;
     {
         Prof p(c, "search");
         HIPCHK(ol_launch_search(a, c->stream));
     }
-    return read_meta(c, total);
+    return read_meta(c, s, total);
 }
 
 int check_pattern(size_t len)
@@ -693,30 +779,41 @@ int search_doc_host(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, si
 // each, with the ids of its first three matches: 4 bytes per text byte); a
 // wave tile with more candidates that have matches is filtered and counted
 // again by the write pass.
-hipError_t setup_many(ol_ctx *c, ManyArgs &a, uint64_t n, bool writes)
+size_t many_status_words(uint64_t n)
+{
+    const uint64_t wtiles = ceil_div(n, SEARCH_TILE) * (SEARCH_TILE / MANY_WT_BYTES);
+    return wtiles / MANY_SCAN_CHUNK + 1 + (wtiles + 1) / 2 + 2 * wtiles;
+}
+
+hipError_t setup_many(ol_ctx *c, Scratch &s, ManyArgs &a, uint64_t n, bool writes)
 {
     static_assert(OL_MAX_PATTERNS <= (1u << 22), "stored candidates keep 22-bit pattern ids");
     const uint32_t tiles = (uint32_t)ceil_div(n, SEARCH_TILE);
     const uint32_t wtiles = tiles * (SEARCH_TILE / MANY_WT_BYTES);
     const size_t nsums = wtiles / MANY_SCAN_CHUNK + 1;
-    const size_t words = nsums + (wtiles + 1) / 2 + 2 * (size_t)wtiles;
-    hipError_t e = ensure_scratch(c, words, 1);
-    if (e != hipSuccess || (e = clear_scratch(c, nsums, 1)) != hipSuccess)
+    const size_t words = many_status_words(n);
+    hipError_t e = ensure_status(s, words);
+    if (e != hipSuccess || (e = clear_status(c, s, nsums)) != hipSuccess)   // the scan's chunk totals
         return e;
-    uint64_t *st = c->status();
+    soil(s, words);
+    uint64_t *st = s.words();
     a.num_tiles = tiles;
     a.num_wtiles = wtiles;
     a.scan_sums = st;
     a.tile_count = (uint32_t *)(st + nsums);
     a.tile_off = st + nsums + (wtiles + 1) / 2;
     a.tile_list = a.tile_off + wtiles;
-    a.meta = c->meta();
+    a.meta = s.meta();
+    a.lb_err = s.ctr() + CTR_DEV_ERR;
     a.cands = nullptr;
     if (!writes)
         return hipSuccess;
-    if ((e = c->cands.ensure((size_t)wtiles * MANY_WT_BYTES * 4)) != hipSuccess)
+    const size_t cb = (size_t)wtiles * MANY_WT_BYTES * 4;
+    if (s.cands.bytes < cb && s.frozen)
+        return hipErrorInvalidValue;
+    if ((e = s.cands.ensure(cb)) != hipSuccess)
         return e;
-    a.cands = (uint32_t *)c->cands.p;
+    a.cands = (uint32_t *)s.cands.p;
     return hipSuccess;
 }
 
@@ -735,8 +832,9 @@ int run_query(ol_ctx *c, const ol_doc *d, uint32_t r0, uint32_t r1, uint32_t kin
     if (r1 <= r0)
         return OL_OK;
     const uint32_t blocks = ol_query_blocks(r0, r1);
-    HIPCHK(ensure_scratch(c, blocks));
-    HIPCHK(clear_scratch(c, blocks));
+    Scratch &s = c->scr;
+    HIPCHK(ensure_status(s, 2 * (size_t)blocks + 8));
+    soil(s, 2 * (size_t)blocks + 8);
     const uint64_t cap = std::min<uint64_t>(max_matches, r1 - r0);
     DevBuf out;
     HIPCHK(out.ensure((cap ? cap : 1) * 4));
@@ -752,20 +850,20 @@ int run_query(ol_ctx *c, const ol_doc *d, uint32_t r0, uint32_t r1, uint32_t kin
     a.depth = depth;
     a.r0 = r0;
     a.r1 = r1;
-    a.block_count = (uint32_t *)c->status();
-    a.block_off = c->status() + blocks;
+    a.block_count = (uint32_t *)s.words();
+    a.block_off = s.words() + blocks;
     a.out = cap ? (uint32_t *)out.p : nullptr;
     a.cap = cap;
     {
         Prof p(c, "query");
-        const hipError_t e = ol_launch_query(a, c->meta(), c->stream);
+        const hipError_t e = ol_launch_query(a, s.meta(), c->stream);
         if (e != hipSuccess) {
             out.release();
             return fail(OL_ERR_DEVICE, std::string("query: ") + hipGetErrorString(e));
         }
     }
     uint64_t tot = 0;
-    int rc = read_meta(c, &tot);
+    int rc = read_meta(c, s, &tot);
     if (rc) {
         out.release();
         return rc;
@@ -988,12 +1086,12 @@ int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, ui
     a.cap = cap;
     a.out_off = out_off;
     a.out_id = out_id;
-    HIPCHK(setup_many(c, a, d->n, out_off || out_id));
+    HIPCHK(setup_many(c, c->scr, a, d->n, out_off || out_id));
     {
         Prof p(c, "many");
         HIPCHK(ol_launch_many(a, c->stream));
     }
-    return read_meta(c, total);
+    return read_meta(c, c->scr, total);
 }
 
 uint32_t ceil_log2(uint64_t v)
@@ -1031,7 +1129,7 @@ int ol_ctx_create(ol_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
         c->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
-    if (c->scratch.ensure((SCR_STATUS + 4096) * 8) != hipSuccess) {
+    if (scratch_init(c->scr) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(OL_ERR_NOMEM, "scratch allocation failed");
@@ -1051,9 +1149,7 @@ void ol_ctx_destroy(ol_ctx *c)
     }
     for (auto e : c->pool)
         (void)hipEventDestroy(e);
-    c->scratch.release();
-    c->tree_sum.release();
-    c->fix.release();
+    c->scr.release();
     for (int i = 0; i < 16; i++) {
         if (c->ring[i]) (void)hipHostFree(c->ring[i]);
         if (c->ring_ev[i]) (void)hipEventDestroy(c->ring_ev[i]);
@@ -1078,8 +1174,8 @@ int ol_ctx_synchronize(ol_ctx *c)
 {
     if (!c)
         return fail(OL_ERR_INVALID, "null argument");
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return OL_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    return sync_check(c, c->scr);
 }
 
 int ol_doc_upload(ol_ctx *c, const uint8_t *text, size_t len, ol_doc **out)
@@ -1264,28 +1360,37 @@ int ol_tree_build(ol_ctx *c, const ol_doc *d, ol_tree *reuse, ol_tree **out)
     }
     int rc = ensure_tree(t, d->lines);
     if (!rc)
-        rc = launch_tree(c, d, t);
+        rc = launch_tree(c, d, t, OL_LINK_ALL);
     if (!rc)
-        rc = check_tree_error(c);
+        rc = check_tree_error(c, c->scr);
     if (rc) {
         ol_tree_free(t);
         return rc;
     }
+    t->links = OL_LINK_ALL;
     *out = t;
     return OL_OK;
 }
 
-int ol_tree_rebuild_async(ol_ctx *c, const ol_doc *d, ol_tree *t)
+int ol_tree_rebuild_links_async(ol_ctx *c, const ol_doc *d, ol_tree *t, uint32_t links)
 {
     if (!c || !d || !t)
         return fail(OL_ERR_INVALID, "null argument");
+    if (links == 0 || (links & ~OL_LINK_ALL))
+        return fail(OL_ERR_INVALID, "links must be a non-empty set of OL_LINK_* bits");
     std::lock_guard<std::mutex> g(c->mu);
-    if (d->lines == 0)
-        return OL_OK;
-    if (t->cap < d->lines)
+    if (t->cap < d->lines && d->lines)
         return fail(OL_ERR_INVALID, "tree capacity is smaller than the document");
     t->lines = d->lines;
-    return launch_tree(c, d, t);
+    t->links = links;
+    if (d->lines == 0)
+        return OL_OK;
+    return launch_tree(c, d, t, links);
+}
+
+int ol_tree_rebuild_async(ol_ctx *c, const ol_doc *d, ol_tree *t)
+{
+    return ol_tree_rebuild_links_async(c, d, t, OL_LINK_ALL);
 }
 
 int ol_tree_check(ol_ctx *c, const ol_tree *t)
@@ -1294,8 +1399,8 @@ int ol_tree_check(ol_ctx *c, const ol_tree *t)
         return fail(OL_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(c->mu);
     if (t->lines == 0)
-        return OL_OK;
-    return check_tree_error(c);
+        return sync_check(c, c->scr);
+    return check_tree_error(c, c->scr);
 }
 
 void ol_tree_free(ol_tree *t)
@@ -1320,6 +1425,10 @@ int ol_tree_download(ol_ctx *c, const ol_tree *t, uint32_t *parent, uint32_t *su
     size_t b = t->lines * 4;
     if (!b)
         return OL_OK;
+    const uint32_t want = (parent ? OL_LINK_PARENT : 0u) | (subtree_end ? OL_LINK_SUBTREE_END : 0u) |
+                          (first_child ? OL_LINK_FIRST_CHILD : 0u) | (next_sibling ? OL_LINK_NEXT_SIBLING : 0u);
+    if (want & ~t->links)
+        return fail(OL_ERR_INVALID, "the tree's last build did not write every requested link array");
     if (parent) HIPCHK(hipMemcpyAsync(parent, t->parent, b, hipMemcpyDeviceToHost, c->stream));
     if (subtree_end) HIPCHK(hipMemcpyAsync(subtree_end, t->se, b, hipMemcpyDeviceToHost, c->stream));
     if (first_child) HIPCHK(hipMemcpyAsync(first_child, t->fc, b, hipMemcpyDeviceToHost, c->stream));
@@ -1406,6 +1515,8 @@ int ol_query_subtree(ol_ctx *c, const ol_doc *d, const ol_tree *t, size_t root, 
     if (!c || !d || !t)
         return fail(OL_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(c->mu);
+    if (!(t->links & OL_LINK_SUBTREE_END))
+        return fail(OL_ERR_INVALID, "the tree's last build did not write subtree_end");
     if (root >= t->lines || t->lines != d->lines)
         return fail(OL_ERR_INVALID, "line " + std::to_string(root) + " is out of range (document has " +
                                         std::to_string(d->lines) + " lines, tree has " + std::to_string(t->lines) +
@@ -1836,12 +1947,16 @@ struct ol_batch {
     DevBuf d_starts, d_ends, d_line_start;   // [nd]
     DevBuf scan_tile_doc, search_tile_doc;
     uint32_t scan_tile_doc_bpt = 0;
+    // the batch's own scratch: a captured graph keeps its addresses, so it is
+    // sized for every launch form at the sizing run and then frozen
+    mutable Scratch scr;
+    hipStream_t stream = nullptr;   // the stream of the last run (a context's)
     // per-line outputs (global line order)
     uint64_t cap = 0, lines = 0;
     DevBuf offsets, classes, keys, parent, fc, ns, se;
     std::vector<uint32_t> h_line_start, h_line_count;   // [nd]
     // tree segments
-    DevBuf seg_line0, seg_lines, seg_tile0, tile_seg, tree_sum;
+    DevBuf seg_line0, seg_lines, seg_tile0, tile_seg;
     uint32_t tree_tiles = 0;
     // dictionary matches, sorted by (global offset, id)
     DevBuf m_off, m_id;
@@ -1855,9 +1970,12 @@ struct ol_batch {
     const ol_dict *prep_dict = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    size_t graph_words = 0;   // status words a replay leaves set
 
     void drop_graph()
     {
+        if ((exec || graph) && stream)
+            (void)hipStreamSynchronize(stream);   // a replay may still run
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
@@ -1881,16 +1999,16 @@ int batch_scan(ol_ctx *c, ol_batch *b, bool cooperative)
     a.doc_starts = (const uint32_t *)b->d_starts.p;
     a.tile_doc = (const uint32_t *)b->scan_tile_doc.p;
     a.doc_line_start = (uint32_t *)b->d_line_start.p;
-    return launch_scan_args(c, a, cooperative);
+    return launch_scan_args(c, b->scr, a, cooperative);
 }
 
 int batch_tree(ol_ctx *c, ol_batch *b)
 {
     const size_t part = (size_t)b->tree_tiles * TREE_D;
-    HIPCHK(ensure_scratch(c, 0));
-    HIPCHK(b->tree_sum.ensure(part * 16));
-    HIPCHK(clear_scratch(c, 0));
-    uint32_t *base = (uint32_t *)b->tree_sum.p;
+    if (b->scr.tree_sum.bytes < part * 16 && b->scr.frozen)
+        return fail(OL_ERR_DEVICE, "batch tree scratch would move under a captured graph");
+    HIPCHK(b->scr.tree_sum.ensure(part * 16));
+    uint32_t *base = (uint32_t *)b->scr.tree_sum.p;
     TreeArgs a{};
     a.keys = (const uint8_t *)b->keys.p;
     a.lines = b->lines;
@@ -1903,7 +2021,7 @@ int batch_tree(ol_ctx *c, ol_batch *b)
     a.sum_first = base + part;
     a.in_last = base + 2 * part;
     a.out_first = base + 3 * part;
-    a.err = c->ctr() + 1;
+    a.err = b->scr.ctr() + CTR_TREE_ERR;
     a.nsegs = b->nd;
     a.seg_line0 = (const uint32_t *)b->seg_line0.p;
     a.seg_lines = (const uint32_t *)b->seg_lines.p;
@@ -1923,7 +2041,7 @@ int batch_many(ol_ctx *c, ol_batch *b, const ol_dict *s, uint64_t cap)
     a.cap = cap;
     a.out_off = cap ? (uint64_t *)b->m_off.p : nullptr;
     a.out_id = cap ? (uint32_t *)b->m_id.p : nullptr;
-    HIPCHK(setup_many(c, a, b->n, cap != 0));
+    HIPCHK(setup_many(c, b->scr, a, b->n, cap != 0));
     a.ndocs = b->nd;
     a.doc_starts = (const uint32_t *)b->d_starts.p;
     a.doc_ends = (const uint32_t *)b->d_ends.p;
@@ -1980,7 +2098,7 @@ int batch_prepare_parse(ol_ctx *c, ol_batch *b)
         if (rc)
             return rc;
         uint64_t lines = 0;
-        rc = read_meta(c, &lines);
+        rc = read_meta(c, b->scr, &lines);
         if (rc)
             return rc;
         b->lines = lines;
@@ -2020,7 +2138,7 @@ int batch_prepare_parse(ol_ctx *c, ol_batch *b)
     int rc = batch_tree(c, b);
     if (rc)
         return rc;
-    rc = check_tree_error(c);
+    rc = check_tree_error(c, b->scr);
     if (rc)
         return rc;
     return OL_OK;
@@ -2029,6 +2147,8 @@ int batch_prepare_parse(ol_ctx *c, ol_batch *b)
 int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict, bool search_only)
 {
     b->drop_graph();
+    b->stream = c->stream;
+    b->scr.frozen = false;
     b->prepared = false;
     b->h_matches = false;
     b->prep_search_only = search_only;
@@ -2054,7 +2174,7 @@ int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict, bool search_only)
         if (rc)
             return rc;
         uint64_t total = 0;
-        rc = read_meta(c, &total);
+        rc = read_meta(c, b->scr, &total);
         if (rc)
             return rc;
         HIPCHK(b->m_off.ensure((total ? total : 1) * 8));
@@ -2063,11 +2183,21 @@ int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict, bool search_only)
         rc = batch_many(c, b, dict, total ? total : 1);
         if (rc)
             return rc;
-        rc = read_meta(c, &total);
+        rc = read_meta(c, b->scr, &total);
         if (rc)
             return rc;
         b->m_total = total;
     }
+    // every launch form of this geometry (eager: cooperative scan; graph:
+    // ticket-ordered scan) fits the scratch from now on: freeze it
+    size_t words = STATUS_TILES;
+    if (!search_only)
+        words = std::max(scan_status_words(c, b->n, true), scan_status_words(c, b->n, false));
+    if (dict)
+        words = std::max(words, many_status_words(b->n));
+    HIPCHK(ensure_status(b->scr, words));
+    b->graph_words = words;
+    b->scr.frozen = true;
     b->prepared = true;
     b->prep_bpt = c->scan_bpt;
     b->prep_rounds = c->tree_rounds;
@@ -2132,9 +2262,12 @@ int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, s
     }
     auto bail = [&](int rc) {
         if (b->text) (void)hipFree(b->text);
+        b->scr.release();
         delete b;
         return rc;
     };
+    if (scratch_init(b->scr) != hipSuccess)
+        return bail(fail(OL_ERR_NOMEM, "batch scratch allocation failed"));
     const uint64_t padded = round_up(b->n ? b->n : 1, TEXT_ALIGN) + TEXT_SLACK;
     if (hipMalloc(&b->text, padded) != hipSuccess) {
         b->text = nullptr;
@@ -2222,8 +2355,9 @@ void ol_batch_free(ol_batch *b)
         (void)hipFree(b->text);
     for (DevBuf *d : {&b->d_starts, &b->d_ends, &b->d_line_start, &b->scan_tile_doc, &b->search_tile_doc,
                       &b->offsets, &b->classes, &b->keys, &b->parent, &b->fc, &b->ns, &b->se, &b->seg_line0,
-                      &b->seg_lines, &b->seg_tile0, &b->tile_seg, &b->tree_sum, &b->m_off, &b->m_id})
+                      &b->seg_lines, &b->seg_tile0, &b->tile_seg, &b->m_off, &b->m_id})
         d->release();
+    b->scr.release();
     delete b;
 }
 
@@ -2243,8 +2377,10 @@ int ol_batch_run(ol_ctx *c, ol_batch *b, const ol_dict *dict, int flags)
         b->prep_search_only != search_only)
         return batch_prepare(c, b, dict, search_only);
     b->h_matches = false;
+    b->stream = c->stream;
     if (!graph)
         return batch_launch(c, b, true);
+    soil(b->scr, b->graph_words);   // a replay leaves its look-back words and chunk totals set
     if (!b->exec) {
         const bool prof = c->profiling;
         c->profiling = false;
@@ -2300,11 +2436,13 @@ int ol_batch_download_doc(ol_ctx *c, const ol_batch *b, size_t doc, uint32_t *of
     if (b->prep_search_only)
         return fail(OL_ERR_INVALID, "the batch was run for its dictionary search only");
     std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_tree_error(c, b->scr);   // also the sticky device error bits
+    if (rc)
+        return rc;
     const uint32_t j = b->h_dev[doc];
     if (j == ~0u)
         return OL_OK;
     const size_t l0 = b->h_line_start[j], nl = b->h_line_count[j];
-    HIPCHK(hipStreamSynchronize(c->stream));
     auto get = [&](void *dst, const DevBuf &src, size_t width) -> hipError_t {
         if (!dst || !nl)
             return hipSuccess;
@@ -2330,7 +2468,9 @@ int ol_batch_matches(ol_ctx *c, ol_batch *b, size_t doc, uint64_t *count, uint32
         return fail(OL_ERR_INVALID, "document index out of range");
     std::lock_guard<std::mutex> g(c->mu);
     if (!b->h_matches) {
-        HIPCHK(hipStreamSynchronize(c->stream));
+        const int rc = sync_check(c, b->scr);
+        if (rc)
+            return rc;
         b->h_moff.resize(b->m_total);
         b->h_mid.resize(b->m_total);
         if (b->m_total) {

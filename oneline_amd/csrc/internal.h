@@ -117,6 +117,14 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
     return v;
 }
 
+// Orders a wave's LDS accesses around a point (a wave's LDS operations
+// complete in order; the fence keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Exclusive look-back over per-tile status words for the sum of the values of
 // all tiles before `t` (one full wave calls this; every lane gets the result).
 // Tile t must already have published its own aggregate.
@@ -136,6 +144,16 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 // second): it then sets *err and returns, so a launch whose workgroups are not
 // all resident ends with an error instead of waiting forever.
 constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
+
+// Bits of the sticky device error word (one per scratch owner): kernels only
+// OR bits into it; the host reads it after a launch sequence, reports the
+// error and only then clears it, so a give-up is never lost to a later stage.
+enum : uint32_t {
+    OLE_PREFIX_WAIT = 1u,   // cooperative launch: a tile waited too long for its prefix word
+    OLE_SCANNER = 2u,       // cooperative launch: the prefix scanner waited too long for a tile count
+    OLE_LOOKBACK = 4u,      // ticket-ordered launch: a decoupled look-back waited too long for a predecessor
+    OLE_DICT_SCAN = 8u,     // dictionary tile-count scan: a chunk waited too long for an earlier chunk
+};
 
 template <int NWIN = 1>
 __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_t t, uint32_t *err)
@@ -169,7 +187,7 @@ __device__ __forceinline__ uint64_t lookback_sum(const uint64_t *status, uint32_
                            ++spins < LB_SPIN_LIMIT)
                         __builtin_amdgcn_s_sleep(8);
                     if (spins >= LB_SPIN_LIMIT)
-                        atomicOr(err, 1u);
+                        atomicOr(err, OLE_LOOKBACK);
                 }
                 if (__shfl(spins, 0, 64) >= LB_SPIN_LIMIT)
                     return prefix;   // gave up: the result is flagged invalid
@@ -239,7 +257,7 @@ __device__ __forceinline__ void scan_prefixes(uint64_t *status, uint32_t T, uint
         if (done == 0) {
             if (++spins >= LB_SPIN_LIMIT) {
                 if (lane == 0)
-                    atomicOr(err, 2u);
+                    atomicOr(err, OLE_SCANNER);
                 return;
             }
             __builtin_amdgcn_s_sleep(2);
@@ -266,13 +284,15 @@ struct ScanArgs {
     uint8_t *classes;        // NodeKind as u8                     [cap]
     uint8_t *keys;           // tree key byte                      [cap]
     ol_node_dev *nodes;      // FlatNode (nullable)                [cap]
-    uint64_t *status;        // look-back words                    [num_tiles]
+    // Status words per tile.  Cooperative launch: two per tile (count,
+    // prefix), zero at launch, and each tile's workgroup zeroes its own two
+    // once it has its prefix, so the words are zero again after the launch
+    // (no clear launch between calls, and a captured graph replays cleanly).
+    // Ticket order: LB_STRIDE per tile, cleared before each launch.
+    uint64_t *status;
     uint64_t *meta;          // meta[0] <- line count
-    uint32_t *lb_err;        // set when a look-back wait gave up
+    uint32_t *lb_err;        // sticky device error word (OLE_* bits)
     uint32_t *tile_ctr;      // non-null: tiles claimed from this ticket (zero at launch)
-    uint4 *fix_list;         // {line, start, end, 0} of lines whose head was not in LDS (fixup pass)
-    uint32_t *fix_count;     // entries appended to fix_list
-    uint32_t fix_cap;        // capacity of fix_list (>= lines longer than the halo)
     // Batches (ndocs > 0): the text is the documents back to back, each ending
     // in '\n'.  Offsets are written relative to the line's document, and the
     // index of each document's first line to doc_line_start.
@@ -291,7 +311,7 @@ struct TreeArgs {
     uint32_t *sum_first;     // per tile x level: first content line in the tile (OL_NONE = none)
     uint32_t *in_last;       // per tile x level: sum_last over all earlier tiles (max)
     uint32_t *out_first;     // per tile x level: sum_first over all later tiles (min)
-    uint32_t *err;           // set to 1 if a content line is too deep
+    uint32_t *err;           // set to 1 (by the walk) if a content line is too deep; zeroed by the summary
     // Batches (nsegs > 0): one tree per document (segment); a tile holds lines
     // of one document only, and every link is local to its document.
     uint32_t nsegs;
@@ -312,10 +332,10 @@ struct SearchArgs {
     uint32_t pw[16];         // the pattern as words (zero past its end)
     uint32_t vmask[16];      // per word, the bytes left to verify: inside the pattern, outside the window
     uint64_t cap;            // output capacity (matches)
-    uint32_t *lb_err;        // set when a look-back wait gave up
+    uint32_t *lb_err;        // sticky device error word (OLE_* bits)
     uint64_t *out64;         // offsets as u64 (nullable)
     uint32_t *out32;         // offsets as u32 (nullable)
-    uint64_t *status;
+    uint64_t *status;        // two words per tile (count, prefix): zero at launch, zeroed again by each tile
     uint64_t *meta;          // meta[0] <- total matches
 };
 
@@ -370,10 +390,11 @@ struct ManyArgs {
     uint64_t cap;
     uint64_t *out_off;       // u64 offsets (nullable)
     uint32_t *out_id;        // pattern ids (nullable)
-    uint64_t *meta;          // meta[0] <- total matches (zero at launch)
+    uint64_t *meta;          // meta[0] <- total matches (written by the tile-count scan)
     uint32_t *tile_count;    // [num_wtiles] matches per wave tile
     uint64_t *tile_off;      // [num_wtiles] first output index of each wave tile
     uint64_t *scan_sums;     // [num_wtiles / MANY_SCAN_CHUNK + 1] chunk totals of the scan (zero at launch)
+    uint32_t *lb_err;        // sticky device error word (OLE_* bits)
     uint32_t job_tiles;      // write pass: wave tiles per wave job (set by the launcher)
     // Candidate store (nullable): the candidates of wave tile wt that have
     // matches go, 16 bytes each, to entries wt * MANY_WT_BYTES / 4 .. when
@@ -396,7 +417,6 @@ hipError_t ol_launch_clear_u64(uint64_t *p, size_t count, hipStream_t s);
 // tiles of 1, 2 or 4 LDS-staged quarters of 256 x 64 bytes.
 hipError_t ol_launch_scan(const ScanArgs &a, uint32_t bpt, uint32_t grid, bool cooperative, hipStream_t s);
 uint32_t ol_scan_max_grid(uint32_t bpt);
-hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s);   // reads the line count from meta[0]
 // summary + scan + walk; `rounds` of 64 lines per wave: 1, 4 or 16 (tile = 256 x rounds lines)
 hipError_t ol_launch_tree(const TreeArgs &a, uint32_t rounds, hipStream_t s);
 uint32_t ol_tree_tile_lines(uint32_t rounds);

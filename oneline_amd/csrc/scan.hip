@@ -375,10 +375,17 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     static_assert(HALO % 16 == 0 && HALO / 16 <= THREADS, "halo");
     static_assert(TILE <= 65536, "tile positions are u16");
 
-    __shared__ __align__(16) uint8_t s_text[HALO + TILE + 64];
+    // the staged window (halo + tile + 64 bytes of slack), then two 80-byte
+    // line heads (s_head, below): one array, so every decode read is an LDS read
+    constexpr int HEAD0 = HALO + TILE + 64;
+    __shared__ __align__(16) uint8_t s_text[HEAD0 + 2 * 80];
     __shared__ __align__(16) uint16_t s_nm[TILE / 16];   // '\n' bits per 16-byte vector, byte order
     __shared__ uint16_t s_le[2][LIST];                   // tile position of each line end
-    __shared__ uint16_t s_rc[2][LIST];                   // decoded class | key << 8 (FIX: fixup pass)
+    __shared__ uint16_t s_rc[2][LIST];                   // decoded class | key << 8
+    // s_head[b]: the head of the tile's first line when it starts before the
+    // staged window (a line longer than the halo): 80 bytes from its start
+    // rounded down to 16, enough for decode_fast / decode_head
+    uint8_t *const s_head0 = s_text + HEAD0;
     __shared__ uint32_t s_wcnt[NW];
     __shared__ int32_t s_wlast[NW];
     __shared__ uint64_t s_prefix, s_pprefix;
@@ -391,7 +398,6 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
     const uint32_t G = gridDim.x - 1;   // workgroups that take tiles (static order)
     const uint32_t T = a.num_tiles;
     const uint32_t last = a.n - 1;   // always a terminator, '\n' or not
-    constexpr uint16_t FIX = 0xFFFF;  // never a class | key << 8 (classes are < 8)
 
     uint4 v[NV];
     uint4 hv;
@@ -421,9 +427,14 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
                 w = ld_coherent_u64(a.status + (size_t)tt * SCAN_STATUS_WORDS + SCAN_PREFIX_WORD);
             }
             if ((w >> LB_SHIFT) != LB_FLAG_INCL) {
-                atomicOr(a.lb_err, 1u);
+                atomicOr(a.lb_err, OLE_PREFIX_WAIT);
                 w = (LB_FLAG_INCL << LB_SHIFT) | tot;
             }
+            // the scanner has consumed the count word and this tile the
+            // prefix word: zero both for the next launch
+            uint64_t *const st = a.status + (size_t)tt * SCAN_STATUS_WORDS;
+            st_coherent_u64(st, 0ull);
+            st_coherent_u64(st + SCAN_PREFIX_WORD, 0ull);
         }
         return (w & LB_MASK) - tot;
     };
@@ -438,11 +449,6 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             return prefix;
         }
         return finish_prefix(tt, tot, 0);
-    };
-    auto add_fix = [&](uint64_t r, uint32_t s, uint32_t e) {
-        const uint32_t slot = atomicAdd(a.fix_count, 1u);
-        if (slot < a.fix_cap)
-            a.fix_list[slot] = make_uint4((uint32_t)r, s, e, 0u);
     };
     // offsets[r] for the line starting at s (relative to its document in a
     // batch; the first line of a document also records r in doc_line_start)
@@ -493,12 +499,8 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
                 const uint32_t ls = i ? pbase + s_le[pb][i - 1] + 1u : pfirst;
                 const uint32_t rc = s_rc[pb][i];
                 offsets[r] = ls;
-                if (rc != FIX) {
-                    classes[r] = (uint8_t)(rc & 0xFFu);
-                    keys[r] = (uint8_t)(rc >> 8);
-                } else {
-                    add_fix(r, ls, pbase + s_le[pb][i]);
-                }
+                classes[r] = (uint8_t)(rc & 0xFFu);
+                keys[r] = (uint8_t)(rc >> 8);
             }
             return;
         }
@@ -507,12 +509,8 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             const uint32_t ls = i ? pbase + s_le[pb][i - 1] + 1u : pfirst;
             put_offset(pb, r, ls);
             const uint32_t rc = s_rc[pb][i];
-            if (rc == FIX) {
-                add_fix(r, ls, pbase + s_le[pb][i]);
-            } else {
-                classes[r] = (uint8_t)(rc & 0xFFu);
-                keys[r] = (uint8_t)(rc >> 8);
-            }
+            classes[r] = (uint8_t)(rc & 0xFFu);
+            keys[r] = (uint8_t)(rc >> 8);
         }
     };
 
@@ -623,6 +621,7 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         }
         __syncthreads();
 
+        const uint32_t lds_lo = (base >= (uint32_t)HALO) ? base - HALO : 0u;
         uint32_t first_w1 = 0;   // wave 1: start of the tile's first line
         if (wid == 0) {
             // the previous tile's prefix (its predecessors have long published
@@ -654,6 +653,11 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
                 } else {
                     first_w1 = find_line_start_global(a.text, base - HALO);
                 }
+                if (first_w1 < lds_lo) {   // the line's head from global memory (the text is padded)
+                    if (lane < 5)
+                        ((uint4 *)(s_head0 + 80 * b))[lane] = *(const uint4 *)(a.text + (first_w1 & ~15u) + 16u * lane);
+                    wave_lds_sync();   // read back by this wave's lane 0 (line 0's decode)
+                }
             }
             if (lane == 0)
                 s_first[b] = first_w1;
@@ -666,23 +670,39 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             }
         }
 
-        const uint32_t lds_lo = (base >= (uint32_t)HALO) ? base - HALO : 0u;
         auto lds_byte = [&](uint32_t pos) -> uint32_t { return s_text[pos + (uint32_t)HALO - base]; };
-        // parse_line of text[s ..= e] from LDS (s >= lds_lo).  Inlined: as a call it
+        // LDS index in s_text of the line's head bytes: the staged window, or
+        // s_head[b] for a first line starting below it
+        auto head_of = [&](uint32_t s) -> uint32_t {
+            return s >= lds_lo ? s + (uint32_t)HALO - base : (uint32_t)HEAD0 + 80u * b + (s & 15u);
+        };
+        // parse_line of text[s ..= e] (e in the tile).  Inlined: as a call it
         // spilled the registers live across it to scratch on every tile.
+        // Only the tile's first line can start below the window; its bytes
+        // there come from global memory (separate loops: a byte source that
+        // picked LDS or global per access would make every read a flat one).
         auto decode = [&](uint32_t s, uint32_t e) __attribute__((always_inline)) -> ol_node_dev {
+            const uint32_t lo = s > lds_lo ? s : lds_lo;
             uint32_t eff = e + 1;
-            while (eff > s) {
+            while (eff > lo) {
                 const uint32_t c = lds_byte(eff - 1);
                 if (c != '\n' && c != '\r')
                     break;
                 eff--;
             }
+            if (eff == lo) {
+                while (eff > s) {
+                    const uint32_t c = a.text[eff - 1];
+                    if (c != '\n' && c != '\r')
+                        break;
+                    eff--;
+                }
+            }
             ol_node_dev nd;
             if (eff == s) {
                 nd.depth = 0; nd.kind = OLK_BLANK; nd.span_start = s; nd.span_len = 0;
-            } else if (!decode_head(s_text, s + (uint32_t)HALO - base, s, eff, nd)) {
-                nd = parse_span(lds_byte, s, e);
+            } else if (!decode_head(s_text, head_of(s), s, eff, nd)) {
+                nd = s >= lds_lo ? parse_span(lds_byte, s, e) : parse_span(GlobalSrc{a.text}, s, e);
             }
             return nd;
         };
@@ -691,10 +711,6 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             if (r >= a.cap)
                 return;
             put_offset(b, r, s);
-            if (s < lds_lo) {   // head not in LDS: decoded by the fixup pass
-                add_fix(r, s, e);
-                return;
-            }
             const ol_node_dev nd = decode(s, e);
             a.classes[r] = (uint8_t)nd.kind;
             a.keys[r] = key_of(nd);
@@ -708,14 +724,11 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
             for (uint32_t i = (uint32_t)tid - 64u; tid >= 64 && i < total; i += THREADS - 64) {
                 const uint32_t e = base + s_le[b][i];
                 const uint32_t s = i ? base + s_le[b][i - 1] + 1u : first_w1;
-                uint32_t rc = FIX;
-                if (s >= lds_lo) {
-                    // (the text's last byte ends a line without being a '\n')
-                    rc = (e == last) ? RC_SLOW : decode_fast(s_text, s + (uint32_t)HALO - base);
-                    if (rc == RC_SLOW) {
-                        const ol_node_dev nd = decode(s, e);
-                        rc = nd.kind | (uint32_t)key_of(nd) << 8;
-                    }
+                // (the text's last byte ends a line without being a '\n')
+                uint32_t rc = (e == last) ? RC_SLOW : decode_fast(s_text, head_of(s));
+                if (rc == RC_SLOW) {
+                    const ol_node_dev nd = decode(s, e);
+                    rc = nd.kind | (uint32_t)key_of(nd) << 8;
                 }
                 s_rc[b][i] = (uint16_t)rc;
             }
@@ -772,27 +785,6 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         __syncthreads();
         write_staged(b ^ 1, p_t * TILE, s_first[b ^ 1], p_total, s_pprefix, p_t == T - 1);
     }
-}
-
-// Lines whose head was not in LDS: decode them from global memory
-// (each entry carries the line's index and byte span).
-__global__ void scan_fixup_kernel(ScanArgs a)
-{
-    const uint32_t cnt = *a.fix_count < a.fix_cap ? *a.fix_count : a.fix_cap;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const uint4 f = a.fix_list[i];
-        const ol_node_dev nd = parse_span(GlobalSrc{a.text}, f.y, f.z);
-        a.classes[f.x] = (uint8_t)nd.kind;
-        a.keys[f.x] = key_of(nd);
-        if (a.nodes)
-            a.nodes[f.x] = nd;
-    }
-}
-
-hipError_t ol_launch_scan_fixup(const ScanArgs &a, hipStream_t s)
-{
-    hipLaunchKernelGGL(scan_fixup_kernel, dim3(64), dim3(256), 0, s, a);
-    return hipGetLastError();
 }
 
 template <int Q>

@@ -141,10 +141,14 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
                    ++spins < LB_SPIN_LIMIT)
                 __builtin_amdgcn_s_sleep(8);
             if (spins >= LB_SPIN_LIMIT) {
-                atomicOr(a.lb_err, 1u);
+                atomicOr(a.lb_err, OLE_PREFIX_WAIT);
                 w = (LB_FLAG_INCL << LB_SHIFT) | p_total;
             }
             s_prefix = (w & LB_MASK) - p_total;
+            // count and prefix word consumed: zero them for the next launch
+            uint64_t *const st = a.status + (size_t)p_t * SCAN_STATUS_WORDS;
+            st_coherent_u64(st, 0ull);
+            st_coherent_u64(st + SCAN_PREFIX_WORD, 0ull);
         }
         __syncthreads();
         const uint64_t prefix = s_prefix;
@@ -651,13 +655,6 @@ __device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyLan
     }
 }
 
-// Orders the wave's LDS accesses around a point (a wave's lists are its own;
-// a wave's LDS operations complete in order).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Count pass: every wave takes wave tiles wt = wave + k * waves.
 template <int THREADS, int FS>
@@ -684,7 +681,6 @@ __global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
     uint16_t *const l_pos = s_pos[wid];
     uint8_t *const l_rem = s_rem[wid];
     const uint32_t stride = gridDim.x * NWV;
-    uint64_t wave_total = 0;
     uint32_t wt = blockIdx.x * NWV + wid;
     ManyWords W;
     if (wt < a.num_wtiles)
@@ -730,10 +726,7 @@ __global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
             a.tile_count[wt] = tsum;
             a.tile_list[wt] = (store && kept <= MANY_WCAP) ? (uint64_t)kept : TILE_LIST_NONE;
         }
-        wave_total += tsum;
     }
-    if (lane == 0 && wave_total)
-        atomicAdd((unsigned long long *)a.meta, (unsigned long long)wave_total);
 }
 
 // A wave tile whose candidates were not stored: per part, each lane filters
@@ -959,6 +952,8 @@ __global__ __launch_bounds__(ST) void many_scan_kernel(ManyArgs a)
                 uint32_t spins = 0;
                 while (((v = ld_coherent_u64(sums + b0 + lane)) >> LB_SHIFT) == 0 && ++spins < LB_SPIN_LIMIT)
                     __builtin_amdgcn_s_sleep(4);
+                if (spins >= LB_SPIN_LIMIT)
+                    atomicOr(a.lb_err, OLE_DICT_SCAN);
             }
             carry += wave_sum_u64(v & LB_MASK);
         }
@@ -1030,7 +1025,7 @@ static hipError_t launch_many(const ManyArgs &a, hipStream_t s)
     }
 }
 
-// Counts (total in meta[0]); with outputs also the scan and the writes.
+// Counts and their scan (total in meta[0]); with outputs also the writes.
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
 {
     if (a.num_wtiles == 0)
@@ -1038,11 +1033,11 @@ hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
     if (a.num_wtiles != a.num_tiles * MANY_WPT)
         return hipErrorInvalidValue;
     hipError_t e = launch_many<false>(a, s);
-    if (e != hipSuccess || (!a.out_off && !a.out_id))
+    if (e != hipSuccess)
         return e;
     const uint32_t chunks = (a.num_wtiles + MANY_SCAN_CHUNK - 1) / MANY_SCAN_CHUNK;
     hipLaunchKernelGGL((many_scan_kernel<1024>), dim3(chunks), dim3(1024), 0, s, a);
-    if ((e = hipGetLastError()) != hipSuccess)
+    if ((e = hipGetLastError()) != hipSuccess || (!a.out_off && !a.out_id))
         return e;
     return launch_many<true>(a, s);
 }

@@ -123,7 +123,7 @@ __device__ __forceinline__ void lane_bit_masks(uint64_t (&dm)[5])
 }
 
 // Per tile: for each level the last content line (+1, 0 = none) and the first
-// (NONE = none); error flag for content lines at depth >= D.
+// (NONE = none).
 template <int TR>
 __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
 {
@@ -136,8 +136,10 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
     load_keys<TR>(a.keys, ts, wbase, key);
     uint64_t dm[5];
     lane_bit_masks(dm);
+    // the depth error flag is set by the walk (the next launch but one)
+    if (t == 0 && threadIdx.x == 0)
+        *a.err = 0u;
     uint32_t last = 0, first = OL_NONE;
-    bool deep = false;
 #pragma unroll
     for (int r = 0; r < TR; r++) {
         uint64_t lt, eq;   // eq: lanes whose key == lane id
@@ -148,10 +150,7 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
             if (first == OL_NONE)
                 first = rb + (uint32_t)__builtin_ctzll(eq);
         }
-        deep |= (key[r] != KEY_BLANK && key[r] >= (uint32_t)D);
     }
-    if (__ballot(deep) && lane == 0)
-        atomicOr(a.err, 1u);
     if (lane < D) {
         s_l[w][lane] = last;
         s_f[w][lane] = first;
@@ -318,12 +317,14 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     load_keys<TR>(a.keys, ts, wbase, key);
 
     // pass 1: masks; per level the wave's last / first line; per round the
-    // first line with key <= d
+    // first line with key <= d; content lines too deep for the tree
     uint64_t dm[5];
     lane_bit_masks(dm);
     uint32_t last = 0, first = OL_NONE;
+    bool deep = false;
 #pragma unroll
     for (int r = 0; r < TR; r++) {
+        deep |= (key[r] != KEY_BLANK && key[r] >= (uint32_t)D);
         uint64_t lt, eq;
         level_masks(key[r], dm, lt, eq);
         if (lane <= D)
@@ -342,6 +343,8 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
             s_hl[w][r][lane] = (uint8_t)kj;
         }
     }
+    if (__ballot(deep) && lane == 0)
+        atomicOr(a.err, 1u);
     if (lane < D) {
         s_l[w][lane] = last;
         s_f[w][lane] = first;
