@@ -162,6 +162,19 @@ int ol_search_raw(ol_ctx *ctx, const uint8_t *text, size_t text_len, const uint8
 int ol_search_device(ol_ctx *ctx, const ol_doc *doc, const uint8_t *pattern, size_t len,
                      uint64_t *d_offsets, size_t cap, uint64_t *total);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code that must not call hipMalloc: tight serving loops, harnesses
+ * that forbid it): ol_search_device with the same kernel and results, whose
+ * scratch (per-tile status words, the match total, the error word) is a
+ * static device array of 4 MiB + 128 B per GPU instead of context memory, so
+ * the call allocates nothing.  The array covers every document up to
+ * OL_MAX_GPU_INPUT_LEN bytes (a larger tile count fails with
+ * OL_ERR_TOO_LARGE).  All calls on one GPU share the array: calls that overlap
+ * on one GPU (several contexts or threads at once) race; calls through one
+ * context are serialised by its lock.  Synchronous, like ol_search_device. */
+int ol_search_device_noalloc(ol_ctx *ctx, const ol_doc *doc, const uint8_t *pattern, size_t len,
+                             uint64_t *d_offsets, size_t cap, uint64_t *total);
+
 /* ---- .olb files ------------------------------------------------------------------------ */
 /* A parsed document on disk, the ".olb" container the reference specifies
  * (.agents/tasks/phase4-zero-copy-binary-serialization.md:13-36; not built there):

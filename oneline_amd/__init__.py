@@ -97,6 +97,7 @@ def load_library() -> ctypes.CDLL:
         "ol_search_document": [P, P, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_raw": [P, P, S, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_device": [P, P, P, S, P, S, U64P],
+        "ol_search_device_noalloc": [P, P, P, S, P, S, U64P],
         "ol_doc_save": [P, P, ctypes.c_char_p],
         "ol_doc_load": [P, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(P)],
         "ol_query_document": [P, P, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],

@@ -111,6 +111,7 @@ struct ol_ctx {
     std::string name;
     std::mutex mu;
     Scratch scr;
+    uint64_t *noalloc = nullptr;           // this device's static scratch (ol_search_device_noalloc)
     // launch geometry picked by the autotuner (ol_tune_set)
     uint32_t scan_bpt = SCAN_BPT;          // bytes per lane of scan_classify (tile = 256 lanes x bpt)
     uint32_t scan_grid = 0;                // persistent workgroups of scan_classify (0 = all co-resident)
@@ -638,6 +639,7 @@ uint32_t window_count(const ol_doc *d, uint32_t w)   // after build_window_count
 // window's count in the sample.
 uint32_t choose_window(const ol_doc *d, const uint8_t *pat, size_t m, uint32_t *hits)
 {
+    std::lock_guard<std::mutex> g(d->win_mu);   // the counts may be built by another thread
     const size_t k = std::min<size_t>(m, 4);
     uint32_t best = 0, best_count = 0;
     uint64_t best_cost = 0;
@@ -671,21 +673,20 @@ uint32_t choose_window(const ol_doc *d, const uint8_t *pat, size_t m, uint32_t *
     return best;
 }
 
-int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_t *out64,
-               uint32_t *out32, uint64_t cap, uint64_t *total)
+// The literal kernel's arguments for pattern `pat` over document `d`, except
+// the scratch words.  Allocates nothing (the window counts of the document's
+// sample are used when an earlier search built them).
+void literal_args(const ol_doc *d, const uint8_t *pat, size_t m, uint64_t *out64, uint32_t *out32, uint64_t cap,
+                  SearchArgs &a)
 {
-    SearchArgs a{};
-    build_window_counts(d);
+    a = SearchArgs{};
     uint32_t hits = 0;
     a.wo = choose_window(d, pat, m, &hits);
     // a window that occurs often (more than once in 8 KiB of the sample) in
     // a pattern longer than it makes the kernel wait on verifications: it then
     // takes 4 vectors per lane, which leaves registers for more waves
     a.dense = m > 4 && d->sample.size() && (uint64_t)hits * 8192 > d->sample.size();
-    uint32_t tiles = (uint32_t)ceil_div(d->n, ol_literal_tile(a.dense));
-    Scratch &s = c->scr;
-    HIPCHK(ensure_status(s, STATUS_TILES + (size_t)tiles * SCAN_STATUS_WORDS));
-    HIPCHK(zero_status(c, s));   // each tile zeroes its words again
+    const uint32_t tiles = (uint32_t)ceil_div(d->n, ol_literal_tile(a.dense));
     a.text = d->text;
     a.n = (uint32_t)d->n;
     a.num_tiles = tiles;
@@ -709,6 +710,17 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
     a.cap = cap;
     a.out64 = out64;
     a.out32 = out32;
+}
+
+int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_t *out64,
+               uint32_t *out32, uint64_t cap, uint64_t *total)
+{
+    build_window_counts(d);
+    SearchArgs a;
+    literal_args(d, pat, m, out64, out32, cap, a);
+    Scratch &s = c->scr;
+    HIPCHK(ensure_status(s, STATUS_TILES + (size_t)a.num_tiles * SCAN_STATUS_WORDS));
+    HIPCHK(zero_status(c, s));   // each tile zeroes its words again
     a.status = s.words() + STATUS_TILES;
     a.meta = s.meta();
     a.lb_err = s.ctr() + CTR_DEV_ERR;
@@ -717,6 +729,41 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
         HIPCHK(ol_launch_search(a, c->stream));
     }
     return read_meta(c, s, total);
+}
+
+static_assert(OL_NOALLOC_MAX_TILES * 16384 >= OL_MAX_GPU_INPUT_LEN, "the no-allocation scratch covers every document");
+
+// ol_search_device on the static scratch of this device (no allocation): the
+// kernel is the same; the total and the error word are read back from the
+// array through its device address.
+int run_search_noalloc(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_t *out64, uint64_t cap,
+                       uint64_t *total)
+{
+    if (!c->noalloc)
+        return fail(OL_ERR_DEVICE, "the no-allocation scratch of this device is not set up");
+    SearchArgs a;
+    literal_args(d, pat, m, out64, nullptr, cap, a);
+    if (a.num_tiles > OL_NOALLOC_MAX_TILES)
+        return fail(OL_ERR_TOO_LARGE, "the document needs " + std::to_string(a.num_tiles) +
+                                          " literal tiles; the no-allocation scratch holds " +
+                                          std::to_string(OL_NOALLOC_MAX_TILES));
+    uint64_t *const sc = c->noalloc;
+    a.status = sc + OL_NOALLOC_STATUS;
+    a.meta = sc;
+    a.lb_err = (uint32_t *)(sc + 8);
+    HIPCHK(ol_launch_search(a, c->stream));
+    uint64_t head[OL_NOALLOC_STATUS];
+    HIPCHK(hipMemcpyAsync(head, sc, sizeof head, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t bits = (uint32_t)head[8];
+    if (bits) {
+        // a tile gave up: its status words may be left set; clear the array
+        HIPCHK(ol_launch_clear_u64(sc, OL_NOALLOC_WORDS, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return fail(OL_ERR_DEVICE, dev_error_text(bits));
+    }
+    *total = head[0];
+    return OL_OK;
 }
 
 int check_pattern(size_t len)
@@ -1125,6 +1172,11 @@ int ol_ctx_create(ol_ctx **out)
     if (hipSetDevice(0) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(OL_ERR_DEVICE, "stream creation failed");
+    }
+    if (ol_noalloc_scratch(0, &c->noalloc) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(OL_ERR_DEVICE, "the no-allocation scratch's device address is not available");
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
@@ -1559,6 +1611,21 @@ int ol_search_device(ol_ctx *c, const ol_doc *d, const uint8_t *pattern, size_t 
     if (d->n == 0)
         return OL_OK;
     return run_search(c, d, pattern, len, d_offsets, nullptr, d_offsets ? cap : 0, total);
+}
+
+int ol_search_device_noalloc(ol_ctx *c, const ol_doc *d, const uint8_t *pattern, size_t len, uint64_t *d_offsets,
+                             size_t cap, uint64_t *total)
+{
+    if (!c || !d || !total || (!pattern && len))
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = check_pattern(len);
+    if (rc)
+        return rc;
+    *total = 0;
+    if (d->n == 0)
+        return OL_OK;
+    return run_search_noalloc(c, d, pattern, len, d_offsets, d_offsets ? cap : 0, total);
 }
 
 // ---- dictionary ------------------------------------------------------------------

@@ -422,6 +422,15 @@ hipError_t ol_launch_tree(const TreeArgs &a, uint32_t rounds, hipStream_t s);
 uint32_t ol_tree_tile_lines(uint32_t rounds);
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
 uint32_t ol_literal_tile(bool dense);   // text bytes per literal search tile
+// Scratch of the allocation-free literal search: OL_NOALLOC_WORDS u64 words
+// in a static device array (search.hip); [0, 8) meta, word 8 the sticky error
+// word, status words from OL_NOALLOC_STATUS on, two per tile.
+constexpr size_t OL_NOALLOC_STATUS = 16;
+constexpr size_t OL_NOALLOC_MAX_TILES = (4294967292ull + 16383) / 16384;
+constexpr size_t OL_NOALLOC_WORDS = OL_NOALLOC_STATUS + 2 * OL_NOALLOC_MAX_TILES;
+// The array's device address on `device` (which must be current), resolved
+// with hipGetSymbolAddress on the first call for that device.
+hipError_t ol_noalloc_scratch(int device, uint64_t **out);
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
 
 // Tile geometry of each kernel.

@@ -15,6 +15,8 @@
 // bytes are read through L2.
 #include "internal.h"
 
+#include <mutex>
+
 // Block-wide exclusive sum of `v` (THREADS/64 waves); also returns the total.
 template <int THREADS>
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *s_w, uint32_t &total)
@@ -309,6 +311,40 @@ static hipError_t launch_search_nv(const SearchArgs &a, hipStream_t s)
 }
 
 uint32_t ol_literal_tile(bool dense) { return SEARCH_THREADS * (dense ? 4u : 8u) * 16u; }
+
+// ---- the allocation-free literal search's scratch ---------------------------
+// One static device array per GPU (each device holds its own copy until the
+// process ends): meta (8 words: [0] the match total), the sticky error word
+// (word 8, low half), then two status words per tile for every tile of the
+// largest document (16 KiB tiles of OL_MAX_GPU_INPUT_LEN bytes): 4 MiB + 128 B.
+// The literal kernel leaves every status word zero again, so the array needs
+// clearing only after a call that gave up (ol_noalloc_clear).
+__device__ __align__(256) static uint64_t scratch[OL_NOALLOC_WORDS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+uint64_t *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t ol_noalloc_scratch(int device, uint64_t **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        // the array's device address on this device (the current one): its
+        // host-side name is not a device address and is never passed on
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_noalloc_ptr[device] = (uint64_t *)p;
+    }
+    *out = g_noalloc_ptr[device];
+    return hipSuccess;
+}
 
 // 8 vectors per lane halve the per-tile synchronisation; 4 leave registers
 // for more waves, which hide verifications when the window is frequent.
