@@ -172,3 +172,84 @@ def test_empty_and_all_empty_batches(parser):
     b.run()
     total, st, cnt = b.lines()
     assert total == 0 and list(cnt) == [0, 0]
+
+
+def test_graph_replay_after_the_context_scratch_grew(parser):
+    """A captured batch graph keeps its own scratch: growing every scratch
+    buffer of the context (a larger upload, a dictionary search with written
+    matches, a literal search) between replays changes none of its results."""
+    docs = generated_docs(30, seed=12, median=8 << 10, hi=256 << 10)
+    pats = datagen.patterns(128, seed=0xC3B)
+    b = parser.batch(docs)
+    pset = parser.compile_patterns(None, pats)
+    b.run(pset)
+    want = [b.document(i) for i in range(len(docs))]
+    want_m = [b.matches(i) for i in range(len(docs))]
+    b.run(pset, graph=True)
+    parser.synchronize()
+    big = parser.upload(datagen.document(96 << 20, seed=45), nodes=False)
+    parser.search_many(big, pset, 1 << 20)            # candidate store and status words grow
+    parser.search_document(big, b"the", 1 << 10)
+    parser.build_tree(big)
+    for rep in range(2):
+        b.run(pset, graph=True)
+        parser.synchronize()
+        for i in range(len(docs)):
+            off, cls, links = b.document(i)
+            assert np.array_equal(off, want[i][0]) and np.array_equal(cls, want[i][1]), f"replay {rep} doc {i}"
+            for g, w in zip((links.parent, links.subtree_end, links.first_child, links.next_sibling),
+                            (want[i][2].parent, want[i][2].subtree_end, want[i][2].first_child,
+                             want[i][2].next_sibling)):
+                assert np.array_equal(g, w), f"replay {rep} doc {i}: links"
+            mo, mi = b.matches(i)
+            assert np.array_equal(mo, want_m[i][0]) and np.array_equal(mi, want_m[i][1]), f"replay {rep} doc {i}"
+    big.close()
+    b.close()
+
+
+def test_c4_workload_after_graph_replays(parser):
+    """The batched workload at its size: 10,000 documents (log-normal sizes,
+    median 16 KiB, sigma 2, 64 B .. 64 MiB, seed 1) with the 4,096-pattern
+    dictionary, run eagerly, replayed as a graph (with eager work between the
+    replays), then a seeded sample of documents checked against the oracle
+    run on each document alone, and the totals checked against the per-
+    document results."""
+    count = 10_000
+    sizes = datagen.lognormal_sizes(count, 1, 16384.0, 2.0, 64, 64 << 20)
+    print(f"[c4] {count} documents, {int(sum(int(x) for x in sizes))} bytes", flush=True)
+    docs = [datagen.document(int(sz), seed=datagen.mix(1, i)) for i, sz in enumerate(sizes)]
+    pats = datagen.patterns(4096, seed=0xC3B)
+    pset = parser.compile_patterns(None, pats)
+    b = parser.batch(docs)
+    b.run(pset)                                    # sizing run
+    total, st, cnt = b.lines()
+    assert total == int(cnt.astype(np.int64).sum())
+    other = parser.upload(datagen.document(8 << 20, seed=77), nodes=False)
+    for _ in range(2):
+        b.run(pset, graph=True)
+        parser.synchronize()
+        parser.search_many(other, pset, 1 << 16)  # eager work on the context between replays
+        b.run(pset, graph=False)
+    b.run(pset, graph=True)
+    parser.synchronize()
+    rng = np.random.default_rng(21)
+    order = rng.permutation(count)
+    checked, nbytes = 0, 0
+    for i in order:
+        d = docs[int(i)]
+        if nbytes + d.size > (96 << 20):
+            continue
+        nbytes += d.size
+        check_doc(b, int(i), d.tobytes())
+        mo, mi = b.matches(int(i))
+        wo, wi = orc.many_find_all(d, pats)
+        assert np.array_equal(mo, wo.astype(np.uint32)) and np.array_equal(mi, wi.astype(np.uint32)), f"doc {i}"
+        checked += 1
+        if checked == 60:
+            break
+    assert checked == 60
+    # the largest document too
+    j = int(np.argmax(sizes))
+    check_doc(b, j, docs[j].tobytes())
+    other.close()
+    b.close()
