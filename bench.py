@@ -106,8 +106,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the default launch geometry")
     ap.add_argument("--batch-docs", type=int, default=10000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep")
-    ap.add_argument("--serve-qps", type=str, default="250,1000,2000,4000,8000")
-    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-qps", type=str, default="250,1000,4000,16000,32000,64000")
+    ap.add_argument("--serve-serial", type=int, default=1600,
+                    help="requests of the per-document loop beside the serving sweep (0: skip it)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--only", choices=("serve",), default=None,
+                    help="run one secondary workload alone and print its record (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, copy 1 GiB device-to-device twice (counter calibration)")
     args = ap.parse_args()
@@ -116,6 +120,10 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "the path's multi-GPU sharding is a design, not a build", file=sys.stderr)
         sys.exit(2)
+
+    if args.only == "serve":
+        print(json.dumps({"serve": serve_benchmark(args)}), flush=True)
+        return
 
     import oneline_amd as ol
     from oneline_amd import datagen
@@ -454,7 +462,10 @@ def serve_benchmark(args):
     30% dictionary-search requests; document sizes log-normal (median 64 KiB,
     sigma 2.0, 64 B .. 256 MiB, seed 2); each search uses one of 8 resident
     dictionaries of 1,024 .. 65,536 patterns.  p50/p99 latency (completion -
-    scheduled arrival) and served GB/s ((document + result bytes) / wall)."""
+    scheduled arrival) and served GB/s ((document + result bytes) / wall).
+    Beside it, the same request sequence run one request at a time through the
+    single-document entry points (the plain per-document path), and the served
+    capacity's ratio to it."""
     import oneline_amd as ol
     from oneline_amd import datagen
 
@@ -462,7 +473,8 @@ def serve_benchmark(args):
     sizes = datagen.lognormal_sizes(pool, 2, 65536.0, 2.0, 64, 256 << 20)
     docs = [datagen.document(int(sz), seed=datagen.mix(2, i)) for i, sz in enumerate(sizes)]
     log(f"[bench] serve: {pool} documents, {int(sum(int(x) for x in sizes))} bytes in the pool")
-    srv = ol.GpuServer(hbm_budget=64 << 30, slots=4, max_batch_docs=256, max_batch_bytes=256 << 20)
+    slots, budget = 4, 96 << 30
+    srv = ol.GpuServer(hbm_budget=budget, slots=slots, max_batch_docs=256, max_batch_bytes=256 << 20)
     dict_sizes = (1024, 2048, 4096, 8192, 16384, 32768, 65536, 1024)
     for k, cnt in enumerate(dict_sizes):
         srv.add_dictionary(datagen.patterns(cnt, seed=0xC3B + 17 * k))
@@ -475,11 +487,22 @@ def serve_benchmark(args):
         sweep.append(rec)
         log(f"[bench] serve qps {q:.0f}: achieved {st['qps_achieved']:.0f}, p50 {st['p50_ms']:.2f} ms, "
             f"p99 {st['p99_ms']:.2f} ms, {st['gbps']:.2f} GB/s, batches {st['batches']}")
+    # the plain per-document path on the same kind of request sequence
+    serial = None
+    if args.serve_serial:
+        serial = srv.load_serial(docs, requests=args.serve_serial, search_fraction=0.3, seed=7)
+        serial = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in serial.items()}
+        log(f"[bench] serve: per-document loop {serial['qps_achieved']:.0f} requests/s, {serial['gbps']:.2f} GB/s")
+    best = max(sweep, key=lambda r: r["gbps"])
+    arena = srv.arena_bytes()
     srv.close()
     return {
         "workload": "open-loop Poisson arrivals, 70% parse / 30% dictionary search, sizes log-normal (median "
                     "64 KiB, sigma 2.0, max 256 MiB, seed 2), 8 resident dictionaries of 1k..64k patterns",
-        "slots": 4, "hbm_budget_bytes": 64 << 30, "sweep": sweep,
+        "slots": slots, "hbm_budget_bytes": budget, "arena_bytes_per_slot": arena, "sweep": sweep,
+        "per_document_loop": serial,
+        "capacity_gbps": best["gbps"], "capacity_qps": best["qps_achieved"],
+        "speedup_vs_per_document": round(best["gbps"] / serial["gbps"], 2) if serial and serial["gbps"] else None,
     }
 
 

@@ -207,6 +207,14 @@ int ol_query_document(ol_ctx *ctx, const ol_doc *doc, uint32_t kind, uint32_t de
 int ol_query_subtree(ol_ctx *ctx, const ol_doc *doc, const ol_tree *tree, size_t root, uint32_t kind,
                      uint32_t depth, size_t max_matches, uint32_t *total, uint32_t **indices,
                      size_t *n_indices);
+/* GpuParser::query (crates/oneline-gpu/src/query.rs:288-314): the same query
+ * over caller-given line offsets (computed elsewhere, e.g. by a CPU scanner):
+ * line i spans offsets[i] .. offsets[i+1], the last one to the end of the
+ * text, and is parsed from those bytes (no scan).  No offsets: no matches and
+ * no GPU work.  Indices are line numbers of the given offsets, ascending. */
+int ol_query(ol_ctx *ctx, const uint8_t *text, size_t len, const uint32_t *offsets, size_t line_count,
+             uint32_t kind, uint32_t depth, size_t max_matches, uint32_t *total, uint32_t **indices,
+             size_t *n_indices);
 /* GpuParser::query_raw (crates/oneline-gpu/src/query.rs:232): uploads, scans, queries. */
 int ol_query_raw(ol_ctx *ctx, const uint8_t *text, size_t len, uint32_t kind, uint32_t depth,
                  size_t max_matches, uint32_t *total, uint32_t **indices, size_t *n_indices);
@@ -265,20 +273,24 @@ int ol_batch_matches(ol_ctx *ctx, ol_batch *batch, size_t doc, uint64_t *count, 
                      size_t cap);
 
 /* ---- serving (no counterpart in the reference): a continuous-batching
- * request loop.  Worker slots (each a context with its own HIP stream) take
- * the oldest pending requests that belong together (parse requests, or
- * searches with the same dictionary), as far as the HBM budget and batch
- * limits allow, run them as one batch and complete them; requests arriving
- * meanwhile join the next batch a slot forms, so batches overlap on the
- * device.  Dictionaries stay resident.  Results equal the single-document
- * calls.  Documents passed to ol_server_submit must stay valid until the
- * request completes. */
+ * request loop.  Each worker slot is a context (its own HIP stream) plus one
+ * device arena allocated at ol_server_create; a slot takes the oldest pending
+ * requests that belong together (parse requests, or searches with the same
+ * dictionary), as many as fit its arena and the batch limits, runs them as one
+ * batch carved from the arena (no device allocation per batch, one read-back
+ * per batch) and completes them; requests arriving meanwhile join the next
+ * batch any slot forms, so up to `slots` batches run on the device at once.
+ * Dictionaries and documents put with ol_server_put_doc stay resident in HBM.
+ * The HBM budget is charged with the arenas, the dictionaries and the
+ * resident documents at their actual device sizes.  Results equal the
+ * single-document calls.  Documents passed to ol_server_submit must stay
+ * valid until the request completes. */
 typedef struct ol_server ol_server;
 typedef struct ol_server_config {
-    size_t hbm_budget;        /* device bytes of requests in flight (0: 32 GiB) */
+    size_t hbm_budget;        /* device bytes: slot arenas + dictionaries + resident documents (0: 64 GiB) */
     uint32_t slots;           /* concurrent batches (0: 4, at most 16) */
     uint32_t max_batch_docs;  /* requests per batch (0: 256) */
-    size_t max_batch_bytes;   /* document bytes per batch (0: 256 MiB) */
+    size_t max_batch_bytes;   /* document bytes per batch, and the largest request (0: 256 MiB) */
 } ol_server_config;
 #define OL_REQ_PARSE 0        /* offsets, classes and tree links */
 #define OL_REQ_SEARCH 1       /* dictionary matches */
@@ -286,25 +298,42 @@ typedef struct ol_serve_stats {
     uint64_t requests, parse_requests, search_requests, rejected, bytes, batches, batched_requests;
     double seconds, qps_offered, qps_achieved, gbps, p50_ms, p90_ms, p99_ms, max_ms, mean_ms;
 } ol_serve_stats;
+/* Fails with OL_ERR_NOMEM when the slots' arenas (each sized for a batch of
+ * max_batch_bytes) exceed the budget. */
 int ol_server_create(const ol_server_config *cfg, ol_server **out);
 /* Finishes the pending requests, then releases everything. */
 void ol_server_destroy(ol_server *server);
 /* Message of the calling thread's last failed ol_server_* call. */
 const char *ol_server_last_error(void);
+/* Device bytes of one slot's arena; device bytes charged to the budget now. */
+size_t ol_server_arena_bytes(const ol_server *server);
+size_t ol_server_hbm_used(ol_server *server);
+/* Compiles a dictionary into HBM (OL_ERR_NOMEM beyond the budget). */
 int ol_server_add_dict(ol_server *server, const uint8_t *const *patterns, const size_t *lens, size_t count,
                        uint32_t *dict_id);
+/* Uploads, scans and builds the tree of a document once; text, per-line
+ * arrays and links stay resident in HBM until ol_server_drop_doc (charged to
+ * the budget at their device size; OL_ERR_NOMEM beyond it). */
+int ol_server_put_doc(ol_server *server, const uint8_t *doc, size_t len, uint64_t *doc_id);
+int ol_server_drop_doc(ol_server *server, uint64_t doc_id);
 /* kind OL_REQ_PARSE or OL_REQ_SEARCH (dict_id from ol_server_add_dict).  A
- * request needing more HBM than the whole budget fails with OL_ERR_NOMEM. */
+ * request larger than max_batch_bytes fails with OL_ERR_NOMEM. */
 int ol_server_submit(ol_server *server, int kind, uint32_t dict_id, const uint8_t *doc, size_t len,
                      int keep_results, uint64_t *req_id);
+/* A request on a resident document: a parse returns its stored lines and
+ * links (no upload, no scan), a search runs over its resident text. */
+int ol_server_submit_doc(ol_server *server, int kind, uint32_t dict_id, uint64_t doc_id, int keep_results,
+                         uint64_t *req_id);
 /* Waits for a request (timeout_s <= 0: no limit); its status. */
 int ol_server_wait(ol_server *server, uint64_t req_id, double timeout_s);
 /* Results of a completed request (two calls: sizes with NULL arrays, then data). */
 int ol_server_result_lines(ol_server *server, uint64_t req_id, size_t *lines, uint32_t *offsets, uint8_t *classes,
                            uint32_t *parent, uint32_t *subtree_end, uint32_t *first_child, uint32_t *next_sibling);
 int ol_server_result_matches(ol_server *server, uint64_t req_id, size_t *count, uint32_t *offsets, uint32_t *ids);
-/* Submission, batch start and completion times (seconds, steady clock). */
+/* Submission, batch start and completion times (seconds, steady clock), and
+ * the slot that ran the request. */
 int ol_server_timing(ol_server *server, uint64_t req_id, double *submit, double *start, double *done);
+int ol_server_slot_of(ol_server *server, uint64_t req_id, uint32_t *slot);
 int ol_server_release(ol_server *server, uint64_t req_id);
 /* Open-loop load: `nreq` requests with exponential inter-arrival times at
  * `qps` (seeded), documents taken round-robin from `docs`, a search with
@@ -313,6 +342,14 @@ int ol_server_release(ol_server *server, uint64_t req_id);
  * and served GB/s ((document + result bytes) / wall time). */
 int ol_server_load(ol_server *server, const uint8_t *const *docs, const size_t *lens, size_t ndocs, double qps,
                    size_t nreq, double search_fraction, uint64_t seed, ol_serve_stats *out);
+/* The same request sequence (same documents, kinds, dictionaries and seed)
+ * run one request at a time through the single-document entry points on one
+ * context -- upload, scan, tree and download for a parse; upload, dictionary
+ * search and download for a search -- the plain per-document path the
+ * batched server is measured against.  Fills requests, bytes, seconds,
+ * qps_achieved, gbps and the latency fields (each request's own time). */
+int ol_server_load_serial(ol_server *server, const uint8_t *const *docs, const size_t *lens, size_t ndocs,
+                          size_t nreq, double search_fraction, uint64_t seed, ol_serve_stats *out);
 
 /* ---- launch tuning (no counterpart in the reference: its tile sizes are
  * compile-time constants, e.g. crates/oneline-gpu/src/scan.rs:11).  The
@@ -336,6 +373,23 @@ int ol_tune_get(ol_ctx *ctx, const char *name, int *value);
 int ol_profile_enable(ol_ctx *ctx, int on);
 int ol_profile_take(ol_ctx *ctx, char *names, size_t name_stride, uint64_t *counts, double *ms,
                     size_t cap, size_t *n);
+
+/* Per-call profile: the reference's Profile {wall, host_steps, gpu_passes}
+ * returned by its profile_* methods (crates/oneline-gpu/src/timing.rs:109-123).
+ * ol_profile_call_begin starts a record on the context; the entry point calls
+ * made until ol_profile_call_end add host steps (time since the previous step,
+ * in execution order, labelled like the reference's marks) and GPU passes
+ * (each kernel launch sequence with its HIP-event duration, in launch order).
+ * ol_profile_call_end waits for the stream and returns the wall time from the
+ * start to the last host step, and the steps and passes (the first `cap` of
+ * each; *n_* = all of them). */
+typedef struct ol_profile_step {
+    char label[48];
+    double ms;
+} ol_profile_step;
+int ol_profile_call_begin(ol_ctx *ctx);
+int ol_profile_call_end(ol_ctx *ctx, double *wall_ms, ol_profile_step *host_steps, size_t host_cap, size_t *n_host,
+                        ol_profile_step *gpu_passes, size_t gpu_cap, size_t *n_gpu);
 
 #ifdef __cplusplus
 }

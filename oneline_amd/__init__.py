@@ -104,6 +104,9 @@ def load_library() -> ctypes.CDLL:
         "ol_query_subtree": [P, P, P, S, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P),
                              ctypes.POINTER(S)],
         "ol_query_raw": [P, P, S, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_query": [P, P, S, P, S, ctypes.c_uint32, ctypes.c_uint32, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
+        "ol_profile_call_begin": [P],
+        "ol_profile_call_end": [P, ctypes.POINTER(ctypes.c_double), P, S, ctypes.POINTER(S), P, S, ctypes.POINTER(S)],
         "ol_dict_compile": [P, P, P, P, S, ctypes.POINTER(P)],
         "ol_dict_free": [P],
         "ol_dict_len": [P],
@@ -127,6 +130,11 @@ def load_library() -> ctypes.CDLL:
         "ol_server_timing": [P, ctypes.c_uint64, P, P, P],
         "ol_server_release": [P, ctypes.c_uint64],
         "ol_server_load": [P, P, P, S, ctypes.c_double, S, ctypes.c_double, ctypes.c_uint64, P],
+        "ol_server_load_serial": [P, P, P, S, S, ctypes.c_double, ctypes.c_uint64, P],
+        "ol_server_put_doc": [P, P, S, U64P],
+        "ol_server_drop_doc": [P, ctypes.c_uint64],
+        "ol_server_submit_doc": [P, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, U64P],
+        "ol_server_slot_of": [P, ctypes.c_uint64, U32P],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -134,6 +142,9 @@ def load_library() -> ctypes.CDLL:
         fn.restype = ctypes.c_int
     for name in ("ol_doc_line_count", "ol_doc_text_len", "ol_tree_line_count", "ol_dict_len",
                  "ol_batch_doc_count", "ol_batch_text_bytes"):
+        getattr(lib, name).restype = ctypes.c_size_t
+        getattr(lib, name).argtypes = [P]
+    for name in ("ol_server_arena_bytes", "ol_server_hbm_used"):
         getattr(lib, name).restype = ctypes.c_size_t
         getattr(lib, name).argtypes = [P]
     lib.ol_server_last_error.argtypes = []
@@ -239,6 +250,22 @@ class ManyResult:
 
 
 MANY_HIT = np.dtype([("pattern", "<u4"), ("offset", "<u4")])
+
+PROFILE_STEP = np.dtype([("label", "S48"), ("ms", "<f8")])
+
+
+@dataclass
+class Profile:
+    """Where the time of one call went (crates/oneline-gpu/src/timing.rs:109-123):
+    host wall time from the start to the last host step, the host steps (time
+    since the previous one, in execution order) and the GPU passes (HIP-event
+    durations, in launch order).  Durations in seconds."""
+    wall: float
+    host_steps: List[tuple]
+    gpu_passes: List[tuple]
+
+    def gpu_total(self) -> float:
+        return sum(d for _, d in self.gpu_passes)
 
 
 class GpuDocument:
@@ -539,6 +566,16 @@ class GpuParser:
                                      ctypes.byref(p), ctypes.byref(n)))
         return QueryResult(tot.value, _take(p, n.value, np.uint32))
 
+    def query(self, text, line_offsets, kind=None, depth=None, max_matches: int = 1 << 20) -> QueryResult:
+        """GpuParser::query: the query over caller-given line offsets (no scan)."""
+        t = as_bytes_array(text)
+        o = np.ascontiguousarray(line_offsets, dtype=np.uint32)
+        k, dp = self._query_args(kind, depth)
+        tot, p, n = ctypes.c_uint32(), ctypes.c_void_p(), ctypes.c_size_t()
+        _check(_lib.ol_query(self._ctx, _ptr(t), t.size, _ptr(o), o.size, k, dp, max_matches, ctypes.byref(tot),
+                             ctypes.byref(p), ctypes.byref(n)))
+        return QueryResult(tot.value, _take(p, n.value, np.uint32))
+
     def query_raw(self, text, kind=None, depth=None, max_matches: int = 1 << 20) -> QueryResult:
         """Upload, scan and query in one call."""
         t = as_bytes_array(text)
@@ -568,6 +605,48 @@ class GpuParser:
         _check(_lib.ol_search_many(self._ctx, doc._h, pset._h, max_hits, _ptr(counts), ctypes.byref(tot),
                                    ctypes.byref(p), ctypes.byref(n)))
         return ManyResult(counts, tot.value, _take(p, n.value, MANY_HIT))
+
+    # -- per-call profiles (the reference's profile_* methods) ---------------------------
+    def profile_call(self, fn, *args, **kwargs):
+        """(fn(*args), Profile) of one call through this parser."""
+        _check(_lib.ol_profile_call_begin(self._ctx))
+        try:
+            res = fn(*args, **kwargs)
+        finally:
+            wall = ctypes.c_double()
+            nh, ng = ctypes.c_size_t(), ctypes.c_size_t()
+            host = np.zeros(64, PROFILE_STEP)
+            gpu = np.zeros(64, PROFILE_STEP)
+            _check(_lib.ol_profile_call_end(self._ctx, ctypes.byref(wall), _ptr(host), host.size, ctypes.byref(nh),
+                                            _ptr(gpu), gpu.size, ctypes.byref(ng)))
+        prof = Profile(wall.value / 1e3,
+                       [(x["label"].decode(), float(x["ms"]) / 1e3) for x in host[:min(nh.value, host.size)]],
+                       [(x["label"].decode(), float(x["ms"]) / 1e3) for x in gpu[:min(ng.value, gpu.size)]])
+        return res, prof
+
+    def profile_upload(self, text, nodes: bool = True):
+        return self.profile_call(self.upload, text, nodes)
+
+    def profile_query_document(self, doc, kind=None, depth=None, max_matches: int = 1 << 20) -> Profile:
+        return self.profile_call(self.query_document, doc, kind, depth, max_matches)[1]
+
+    def profile_query_raw(self, text, kind=None, depth=None, max_matches: int = 1 << 20) -> Profile:
+        return self.profile_call(self.query_raw, text, kind, depth, max_matches)[1]
+
+    def profile_search_document(self, doc, pattern: bytes, max_matches: int) -> Profile:
+        return self.profile_call(self.search_document, doc, pattern, max_matches)[1]
+
+    def profile_search_raw(self, text, pattern: bytes, max_matches: int) -> Profile:
+        return self.profile_call(self.search_raw, text, pattern, max_matches)[1]
+
+    def profile_search_many(self, doc, pset, max_hits: int) -> Profile:
+        return self.profile_call(self.search_many, doc, pset, max_hits)[1]
+
+    def profile_build_tree_reusing(self, doc, previous):
+        return self.profile_call(self.build_tree_reusing, doc, previous)
+
+    def profile_build_tree(self, doc) -> Profile:
+        return self.profile_build_tree_reusing(doc, None)[1]
 
     # -- launch tuning ------------------------------------------------------------------
     def tune_set(self, name: str, value: int) -> None:
@@ -661,6 +740,32 @@ class GpuServer:
         self._docs[rid.value] = a
         return rid.value
 
+    def put_document(self, doc) -> int:
+        """Uploads, scans and builds the tree of `doc` once; it stays resident."""
+        a = as_bytes_array(doc)
+        out = ctypes.c_uint64()
+        _scheck(_lib.ol_server_put_doc(self._h, _ptr(a) if a.size else None, a.size, ctypes.byref(out)))
+        return out.value
+
+    def drop_document(self, doc_id: int) -> None:
+        _scheck(_lib.ol_server_drop_doc(self._h, doc_id))
+
+    def submit_resident(self, doc_id: int, kind: int = PARSE, dictionary: int = 0, keep: bool = True) -> int:
+        rid = ctypes.c_uint64()
+        _scheck(_lib.ol_server_submit_doc(self._h, kind, dictionary, doc_id, 1 if keep else 0, ctypes.byref(rid)))
+        return rid.value
+
+    def arena_bytes(self) -> int:
+        return int(_lib.ol_server_arena_bytes(self._h))
+
+    def hbm_used(self) -> int:
+        return int(_lib.ol_server_hbm_used(self._h))
+
+    def slot_of(self, rid: int) -> int:
+        v = ctypes.c_uint32()
+        _scheck(_lib.ol_server_slot_of(self._h, rid, ctypes.byref(v)))
+        return v.value
+
     def wait(self, rid: int, timeout: float = 120.0) -> None:
         _scheck(_lib.ol_server_wait(self._h, rid, timeout))
 
@@ -702,6 +807,18 @@ class GpuServer:
         st = ServeStats()
         _scheck(_lib.ol_server_load(self._h, ctypes.cast(ptrs, ctypes.c_void_p), ctypes.cast(lens, ctypes.c_void_p),
                                     len(arrs), qps, requests, search_fraction, seed, ctypes.byref(st)))
+        return st.as_dict()
+
+    def load_serial(self, docs: Sequence, requests: int, search_fraction: float = 0.3, seed: int = 5) -> dict:
+        """The same request sequence one at a time through the single-document
+        entry points (ol_server_load_serial): the plain per-document path."""
+        arrs = [as_bytes_array(d) for d in docs]
+        ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        lens = (ctypes.c_size_t * len(arrs))(*[a.size for a in arrs])
+        st = ServeStats()
+        _scheck(_lib.ol_server_load_serial(self._h, ctypes.cast(ptrs, ctypes.c_void_p),
+                                           ctypes.cast(lens, ctypes.c_void_p), len(arrs), requests, search_fraction,
+                                           seed, ctypes.byref(st)))
         return st.as_dict()
 
     def close(self) -> None:

@@ -13,6 +13,7 @@
 #include <unistd.h>
 #include <cstdlib>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -42,13 +43,18 @@ int fail(int code, const std::string &msg)
 uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 uint64_t round_up(uint64_t a, uint64_t b) { return ceil_div(a, b) * b; }
 
+// A device buffer, owned (grown with hipMalloc on demand) or borrowed (a
+// region of a serving slot's arena: never reallocated or freed).
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     hipError_t ensure(size_t b)
     {
         if (b <= bytes && p)
             return hipSuccess;
+        if (borrowed)
+            return hipErrorOutOfMemory;   // the region was carved too small
         release();
         hipError_t e = hipMalloc(&p, b ? b : 16);
         if (e == hipSuccess)
@@ -57,12 +63,20 @@ struct DevBuf {
             p = nullptr;
         return e;
     }
+    void lend(void *q, size_t b)
+    {
+        release();
+        p = q;
+        bytes = b;
+        borrowed = true;
+    }
     void release()
     {
-        if (p)
+        if (p && !borrowed)
             (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        borrowed = false;
     }
 };
 
@@ -123,15 +137,28 @@ struct ol_ctx {
     uint8_t *ring[2 * 8] = {};
     hipEvent_t ring_ev[2 * 8] = {};
     bool ring_used[2 * 8] = {};
-    // profiling
+    // profiling: per-kernel totals (ol_profile_enable / _take), and the
+    // per-call record (ol_profile_call_begin / _end): host step marks and
+    // the call's GPU passes in launch order
     bool profiling = false;
+    bool call_prof = false;
     struct Rec {
         const char *name;
         hipEvent_t a, b;
     };
-    std::vector<Rec> recs;
+    std::vector<Rec> recs, call_recs;
+    std::vector<std::pair<const char *, std::chrono::steady_clock::time_point>> marks;
     std::vector<hipEvent_t> pool;
 };
+
+// A host step of the per-call profile (the reference's StageTimer::mark,
+// crates/oneline-gpu/src/timing.rs:24-28): the time since the previous mark
+// is attributed to `label`.
+static void mark(ol_ctx *c, const char *label)
+{
+    if (c->call_prof)
+        c->marks.emplace_back(label, std::chrono::steady_clock::now());
+}
 
 struct ol_doc {
     uint64_t n = 0, lines = 0, cap = 0;
@@ -159,6 +186,7 @@ struct ol_dict {
     uint32_t count = 0;
     DictDev d{};
     void *dev = nullptr;   // one allocation holding filter, tables, entries, pool
+    size_t bytes = 0;      // its size
 };
 
 namespace {
@@ -183,7 +211,7 @@ struct Prof {
     hipEvent_t a = nullptr;
     Prof(ol_ctx *c_, const char *n) : c(c_), name(n)
     {
-        if (c->profiling) {
+        if (c->profiling || c->call_prof) {
             a = take_event(c);
             if (a)
                 (void)hipEventRecord(a, c->stream);
@@ -195,7 +223,7 @@ struct Prof {
             hipEvent_t b = take_event(c);
             if (b) {
                 (void)hipEventRecord(b, c->stream);
-                c->recs.push_back({name, a, b});
+                (c->call_prof ? c->call_recs : c->recs).push_back({name, a, b});
             } else {
                 c->pool.push_back(a);
             }
@@ -497,11 +525,13 @@ int upload_doc(ol_ctx *c, const uint8_t *text, size_t len, bool with_nodes, ol_d
         d->text = nullptr;
         return bail(fail(OL_ERR_NOMEM, "device allocation of the text failed"));
     }
+    mark(c, "upload: create device buffer");
     int up = stage_upload(c, d->text, len, [text](uint8_t *buf, uint64_t off, uint64_t k) {
         std::memcpy(buf, text + off, k);
     });
     if (up)
         return bail(up);
+    mark(c, "upload: stream through the staging ring");
     if (hipMemsetAsync(d->text + len, 0, padded - len, c->stream) != hipSuccess)
         return bail(fail(OL_ERR_DEVICE, "text upload failed"));
     uint64_t cap = std::min<uint64_t>(len, len / 16 + 4096);
@@ -512,10 +542,12 @@ int upload_doc(ol_ctx *c, const uint8_t *text, size_t len, bool with_nodes, ol_d
         rc = launch_scan(c, d);
         if (rc)
             return bail(rc);
+        mark(c, "scan: launch");
         uint64_t lines = 0;
         rc = read_meta(c, c->scr, &lines);
         if (rc)
             return bail(rc);
+        mark(c, "scan: wait for the GPU + read the line count");
         d->lines = lines;
         if (lines <= d->cap)
             break;
@@ -525,6 +557,7 @@ int upload_doc(ol_ctx *c, const uint8_t *text, size_t len, bool with_nodes, ol_d
     }
     sample_histogram(text, len, d->byte_freq);
     sample_text(text, len, d->sample);
+    mark(c, "upload: sample byte histogram");
     *out = d;
     return OL_OK;
 }
@@ -718,6 +751,7 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
     build_window_counts(d);
     SearchArgs a;
     literal_args(d, pat, m, out64, out32, cap, a);
+    mark(c, "search: choose window");
     Scratch &s = c->scr;
     HIPCHK(ensure_status(s, STATUS_TILES + (size_t)a.num_tiles * SCAN_STATUS_WORDS));
     HIPCHK(zero_status(c, s));   // each tile zeroes its words again
@@ -728,7 +762,10 @@ int run_search(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, uint64_
         Prof p(c, "search");
         HIPCHK(ol_launch_search(a, c->stream));
     }
-    return read_meta(c, s, total);
+    mark(c, "search: launch");
+    const int rc = read_meta(c, s, total);
+    mark(c, "search: wait for the GPU + read the total");
+    return rc;
 }
 
 static_assert(OL_NOALLOC_MAX_TILES * 16384 >= OL_MAX_GPU_INPUT_LEN, "the no-allocation scratch covers every document");
@@ -813,6 +850,7 @@ int search_doc_host(ol_ctx *c, const ol_doc *d, const uint8_t *pat, size_t m, si
         }
     }
     out.release();
+    mark(c, "search: download offsets");
     *total = (uint32_t)tot;
     *offsets = h;
     *n_offsets = keep;
@@ -909,8 +947,10 @@ int run_query(ol_ctx *c, const ol_doc *d, uint32_t r0, uint32_t r1, uint32_t kin
             return fail(OL_ERR_DEVICE, std::string("query: ") + hipGetErrorString(e));
         }
     }
+    mark(c, "query: launch");
     uint64_t tot = 0;
     int rc = read_meta(c, s, &tot);
+    mark(c, "query: wait for the GPU + read the total");
     if (rc) {
         out.release();
         return rc;
@@ -927,6 +967,7 @@ int run_query(ol_ctx *c, const ol_doc *d, uint32_t r0, uint32_t r1, uint32_t kin
         return fail(OL_ERR_DEVICE, "query download failed");
     }
     out.release();
+    mark(c, "query: copy out");
     *total = (uint32_t)tot;
     *indices = h;
     *n_indices = (size_t)keep;
@@ -1138,7 +1179,10 @@ int run_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, uint64_t *out_off, ui
         Prof p(c, "many");
         HIPCHK(ol_launch_many(a, c->stream));
     }
-    return read_meta(c, c->scr, total);
+    mark(c, "many: launch");
+    const int rc = read_meta(c, c->scr, total);
+    mark(c, "many: wait for the GPU + read the total");
+    return rc;
 }
 
 uint32_t ceil_log2(uint64_t v)
@@ -1411,10 +1455,13 @@ int ol_tree_build(ol_ctx *c, const ol_doc *d, ol_tree *reuse, ol_tree **out)
         return OL_OK;
     }
     int rc = ensure_tree(t, d->lines);
+    mark(c, "tree: prepare buffers");
     if (!rc)
         rc = launch_tree(c, d, t, OL_LINK_ALL);
+    mark(c, "tree: launch");
     if (!rc)
         rc = check_tree_error(c, c->scr);
+    mark(c, "tree: wait for the GPU + read status");
     if (rc) {
         ol_tree_free(t);
         return rc;
@@ -1596,6 +1643,61 @@ int ol_query_raw(ol_ctx *c, const uint8_t *text, size_t len, uint32_t kind, uint
         ol_doc_free(d);
         return rc;
     }
+}
+
+int ol_query(ol_ctx *c, const uint8_t *text, size_t len, const uint32_t *offsets, size_t line_count, uint32_t kind,
+             uint32_t depth, size_t max_matches, uint32_t *total, uint32_t **indices, size_t *n_indices)
+{
+    if (!c || (!text && len) || (!offsets && line_count) || !total || !indices || !n_indices)
+        return fail(OL_ERR_INVALID, "null argument");
+    if (len > OL_MAX_GPU_INPUT_LEN)
+        return fail(OL_ERR_TOO_LARGE, "input is " + std::to_string(len) + " bytes; the GPU path addresses at most " +
+                                          std::to_string(OL_MAX_GPU_INPUT_LEN) + " bytes (32-bit offsets)");
+    if (line_count > 0xFFFFFFFFull)
+        return fail(OL_ERR_TOO_MANY, "line count does not fit u32");
+    std::lock_guard<std::mutex> g(c->mu);
+    *total = 0;
+    *indices = nullptr;
+    *n_indices = 0;
+    if (line_count == 0)
+        return OL_OK;
+    // the caller's lines as a temporary document: text, offsets, FlatNodes
+    // parsed from them, and the class / key bytes derived from those
+    ol_doc d;
+    d.n = len;
+    d.with_nodes = true;
+    auto done = [&](int rc) {
+        free_doc_lines(&d);
+        if (d.text)
+            (void)hipFree(d.text);
+        d.text = nullptr;
+        return rc;
+    };
+    const uint64_t padded = round_up(len ? len : 1, TEXT_ALIGN) + TEXT_SLACK;
+    if (hipMalloc(&d.text, padded) != hipSuccess) {
+        d.text = nullptr;
+        return fail(OL_ERR_NOMEM, "device allocation of the text failed");
+    }
+    int rc = stage_upload(c, d.text, len, [text](uint8_t *buf, uint64_t off, uint64_t k) {
+        std::memcpy(buf, text + off, k);
+    });
+    if (rc)
+        return done(rc);
+    if (hipMemsetAsync(d.text + len, 0, padded - len, c->stream) != hipSuccess)
+        return done(fail(OL_ERR_DEVICE, "text upload failed"));
+    if ((rc = alloc_doc_lines(&d, line_count)))
+        return done(rc);
+    d.lines = line_count;
+    if (hipMemcpyAsync(d.offsets, offsets, line_count * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return done(fail(OL_ERR_DEVICE, "offset upload failed"));
+    mark(c, "query: upload text + offsets");
+    {
+        Prof p(c, "parse_given");
+        if (ol_launch_parse_given(d.text, (uint32_t)len, d.offsets, line_count, d.nodes, c->stream) != hipSuccess ||
+            ol_launch_nodes_to_lines(d.nodes, line_count, d.classes, d.keys, c->stream) != hipSuccess)
+            return done(fail(OL_ERR_DEVICE, "parse launch failed"));
+    }
+    return done(run_query(c, &d, 0, (uint32_t)line_count, kind, depth, max_matches, total, indices, n_indices));
 }
 
 int ol_search_device(ol_ctx *c, const ol_doc *d, const uint8_t *pattern, size_t len, uint64_t *d_offsets,
@@ -1783,6 +1885,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         delete s;
         return fail(OL_ERR_NOMEM, "device allocation of the dictionary failed");
     }
+    s->bytes = bytes;
     uint8_t *base = (uint8_t *)s->dev;
     std::vector<uint8_t> host(bytes, 0);
     for (uint32_t f = 1; f <= 4; f++)
@@ -1872,6 +1975,7 @@ int ol_search_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, size_t max_hits
     }
     for (uint64_t i = 0; i < total; i++)
         counts[ids[i]]++;
+    mark(c, "many: read back the hits + count per pattern");
     if (max_hits == 0)
         return OL_OK;
     // Stable counting sort by pattern: the list is offset-ordered, so the result
@@ -1891,6 +1995,7 @@ int ol_search_many(ol_ctx *c, const ol_doc *d, const ol_dict *s, size_t max_hits
     }
     *hits = h;
     *n_hits = keep;
+    mark(c, "many: sort the hits by pattern");
     return OL_OK;
 }
 
@@ -1989,6 +2094,57 @@ int ol_profile_take(ol_ctx *c, char *names, size_t name_stride, uint64_t *counts
     return OL_OK;
 }
 
+int ol_profile_call_begin(ol_ctx *c)
+{
+    if (!c)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    for (auto &r : c->call_recs) {
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->call_recs.clear();
+    c->marks.clear();
+    c->call_prof = true;
+    mark(c, "start");
+    return OL_OK;
+}
+
+int ol_profile_call_end(ol_ctx *c, double *wall_ms, ol_profile_step *host_steps, size_t host_cap, size_t *n_host,
+                        ol_profile_step *gpu_passes, size_t gpu_cap, size_t *n_gpu)
+{
+    if (!c || !wall_ms || !n_host || !n_gpu)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->call_prof)
+        return fail(OL_ERR_INVALID, "no profiled call: ol_profile_call_begin first");
+    c->call_prof = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const auto &m = c->marks;
+    *wall_ms = m.size() >= 2 ? std::chrono::duration<double, std::milli>(m.back().second - m.front().second).count()
+                             : 0.0;
+    *n_host = m.size() ? m.size() - 1 : 0;
+    for (size_t i = 1; i < m.size() && i - 1 < host_cap && host_steps; i++) {
+        std::snprintf(host_steps[i - 1].label, sizeof host_steps[i - 1].label, "%s", m[i].first);
+        host_steps[i - 1].ms = std::chrono::duration<double, std::milli>(m[i].second - m[i - 1].second).count();
+    }
+    *n_gpu = c->call_recs.size();
+    for (size_t i = 0; i < c->call_recs.size(); i++) {
+        const auto &r = c->call_recs[i];
+        if (gpu_passes && i < gpu_cap) {
+            float t = 0.f;
+            (void)hipEventElapsedTime(&t, r.a, r.b);
+            std::snprintf(gpu_passes[i].label, sizeof gpu_passes[i].label, "%s", r.name);
+            gpu_passes[i].ms = t;
+        }
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->call_recs.clear();
+    c->marks.clear();
+    return OL_OK;
+}
+
 }  // extern "C"
 
 // ---- batches ------------------------------------------------------------------------
@@ -2052,7 +2208,106 @@ struct ol_batch {
 
 namespace {
 
-// scan + fixup of the batch text (outputs sized by b->cap)
+// The documents of a batch laid back to back, each followed by '\n' unless it
+// ends in one (empty documents take no bytes).
+struct BatchLayout {
+    const uint8_t *const *docs = nullptr;
+    const size_t *lens = nullptr;
+    size_t count = 0;
+    std::vector<uint64_t> start;   // per document: byte start in the batch text
+    std::vector<uint32_t> dev;     // per document: index among the non-empty ones (or ~0u)
+    std::vector<size_t> order;     // the non-empty documents, in order
+    uint32_t nd = 0;
+    uint64_t n = 0, payload = 0;
+
+    int build(const uint8_t *const *d, const size_t *l, size_t cnt)
+    {
+        docs = d;
+        lens = l;
+        count = cnt;
+        start.assign(cnt, 0);
+        dev.assign(cnt, ~0u);
+        order.clear();
+        nd = 0;
+        n = payload = 0;
+        for (size_t i = 0; i < cnt; i++) {
+            if (l[i] && !d[i])
+                return fail(OL_ERR_INVALID, "document " + std::to_string(i) + " is null");
+            start[i] = n;
+            payload += l[i];
+            if (l[i]) {
+                dev[i] = nd++;
+                order.push_back(i);
+                n += l[i] + (d[i][l[i] - 1] != '\n' ? 1 : 0);
+            }
+        }
+        if (n > OL_MAX_GPU_INPUT_LEN)
+            return fail(OL_ERR_TOO_LARGE, "batch text is " + std::to_string(n) + " bytes; at most " +
+                                              std::to_string(OL_MAX_GPU_INPUT_LEN) + " (32-bit offsets)");
+        return OL_OK;
+    }
+    // bytes [off, off + k) of the batch text into buf
+    void fill(uint8_t *buf, uint64_t off, uint64_t k) const
+    {
+        // the last non-empty document starting at or before off
+        size_t lo = 0, hi = order.size();
+        while (hi - lo > 1) {
+            const size_t mid = (lo + hi) / 2;
+            if (start[order[mid]] <= off) lo = mid; else hi = mid;
+        }
+        uint64_t done = 0;
+        for (size_t j = lo; done < k && j < order.size(); j++) {
+            const size_t i = order[j];
+            const uint64_t s0 = start[i];
+            const uint64_t span = lens[i] + (docs[i][lens[i] - 1] != '\n' ? 1 : 0);
+            const uint64_t at = off + done;
+            if (at >= s0 + span)
+                continue;
+            const uint64_t in = at - s0;
+            if (in < lens[i]) {
+                const uint64_t m = std::min<uint64_t>(lens[i] - in, k - done);
+                std::memcpy(buf + done, docs[i] + in, m);
+                done += m;
+            }
+            if (done < k && off + done == s0 + lens[i] && span > lens[i])
+                buf[done++] = '\n';
+        }
+    }
+    // starts and ends of the non-empty documents
+    void bounds(uint32_t *st, uint32_t *en) const
+    {
+        for (size_t j = 0; j < order.size(); j++) {
+            if (st) st[j] = (uint32_t)start[order[j]];
+            if (en) en[j] = (uint32_t)(start[order[j]] + lens[order[j]]);
+        }
+    }
+    // scan tile t -> first non-empty document starting at or after it (T + 1 entries)
+    void scan_tile_doc(uint64_t tile_bytes, uint32_t *td) const
+    {
+        const uint32_t T = (uint32_t)ceil_div(n, tile_bytes);
+        uint32_t j = 0;
+        for (uint32_t t = 0; t <= T; t++) {
+            const uint64_t pos = (uint64_t)t * tile_bytes;
+            while (j < nd && start[order[j]] < pos)
+                j++;
+            td[t] = j;
+        }
+    }
+    // search tile t -> last non-empty document starting at or before it
+    void search_tile_doc(uint32_t *sd) const
+    {
+        const uint32_t TS = (uint32_t)ceil_div(n ? n : 1, SEARCH_TILE);
+        uint32_t j = 0;
+        for (uint32_t t = 0; t < TS; t++) {
+            const uint64_t p = (uint64_t)t * SEARCH_TILE;
+            while (j + 1 < nd && start[order[j + 1]] <= p)
+                j++;
+            sd[t] = j;
+        }
+    }
+};
+
+// scan of the batch text (outputs sized by b->cap)
 int batch_scan(ol_ctx *c, ol_batch *b, bool cooperative)
 {
     ScanArgs a{};
@@ -2298,35 +2553,20 @@ int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, s
         return fail(OL_ERR_INVALID, "null argument");
     *out = nullptr;
     std::lock_guard<std::mutex> g(c->mu);
+    BatchLayout L;
+    int rc = L.build(docs, lens, count);
+    if (rc)
+        return rc;
     ol_batch *b = new (std::nothrow) ol_batch();
     if (!b)
         return fail(OL_ERR_NOMEM, "out of host memory");
     b->count = count;
-    b->h_start.resize(count);
-    b->h_len.resize(count);
-    b->h_dev.resize(count);
-    uint64_t pos = 0;
-    for (size_t i = 0; i < count; i++) {
-        if (lens[i] && !docs[i]) {
-            delete b;
-            return fail(OL_ERR_INVALID, "document " + std::to_string(i) + " is null");
-        }
-        b->h_start[i] = pos;
-        b->h_len[i] = lens[i];
-        b->payload += lens[i];
-        if (lens[i]) {
-            b->h_dev[i] = b->nd++;
-            pos += lens[i] + (docs[i][lens[i] - 1] != '\n' ? 1 : 0);
-        } else {
-            b->h_dev[i] = ~0u;
-        }
-    }
-    b->n = pos;
-    if (b->n > OL_MAX_GPU_INPUT_LEN) {
-        delete b;
-        return fail(OL_ERR_TOO_LARGE, "batch text is " + std::to_string(b->n) + " bytes; at most " +
-                                          std::to_string(OL_MAX_GPU_INPUT_LEN) + " (32-bit offsets)");
-    }
+    b->h_start = L.start;
+    b->h_len.assign(lens, lens + count);
+    b->h_dev = L.dev;
+    b->nd = L.nd;
+    b->n = L.n;
+    b->payload = L.payload;
     auto bail = [&](int rc) {
         if (b->text) (void)hipFree(b->text);
         b->scr.release();
@@ -2340,39 +2580,8 @@ int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, s
         b->text = nullptr;
         return bail(fail(OL_ERR_NOMEM, "device allocation of the batch text failed"));
     }
-    // the documents back to back, each followed by '\n' unless it ends in one,
-    // through the staging ring
-    std::vector<size_t> order;   // non-empty documents in order
-    order.reserve(b->nd);
-    for (size_t i = 0; i < count; i++)
-        if (lens[i])
-            order.push_back(i);
-    auto fill = [&](uint8_t *buf, uint64_t off, uint64_t k) {
-        // the last non-empty document starting at or before off
-        size_t lo = 0, hi = order.size();
-        while (hi - lo > 1) {
-            const size_t mid = (lo + hi) / 2;
-            if (b->h_start[order[mid]] <= off) lo = mid; else hi = mid;
-        }
-        uint64_t done = 0;
-        for (size_t j = lo; done < k && j < order.size(); j++) {
-            const size_t i = order[j];
-            const uint64_t s0 = b->h_start[i];
-            const uint64_t span = lens[i] + (docs[i][lens[i] - 1] != '\n' ? 1 : 0);
-            uint64_t at = off + done;
-            if (at >= s0 + span)
-                continue;
-            const uint64_t in = at - s0;
-            if (in < lens[i]) {
-                const uint64_t m = std::min<uint64_t>(lens[i] - in, k - done);
-                std::memcpy(buf + done, docs[i] + in, m);
-                done += m;
-            }
-            if (done < k && off + done == s0 + lens[i] && span > lens[i])
-                buf[done++] = '\n';
-        }
-    };
-    int rc = stage_upload(c, b->text, b->n, fill);
+    // the documents back to back through the staging ring
+    rc = stage_upload(c, b->text, b->n, [&L](uint8_t *buf, uint64_t off, uint64_t k) { L.fill(buf, off, k); });
     if (rc == OL_OK && hipStreamSynchronize(c->stream) != hipSuccess)
         rc = fail(OL_ERR_DEVICE, "batch upload failed");
     if (rc)
@@ -2380,27 +2589,14 @@ int ol_batch_create(ol_ctx *c, const uint8_t *const *docs, const size_t *lens, s
     if (hipMemset(b->text + b->n, 0, padded - b->n) != hipSuccess)
         return bail(fail(OL_ERR_DEVICE, "batch upload failed"));
     // document bounds of the non-empty documents; search tiles -> document
-    std::vector<uint32_t> st, en;
-    st.reserve(b->nd);
-    en.reserve(b->nd);
-    for (size_t i = 0; i < count; i++)
-        if (b->h_dev[i] != ~0u) {
-            st.push_back((uint32_t)b->h_start[i]);
-            en.push_back((uint32_t)(b->h_start[i] + b->h_len[i]));
-        }
-    const uint32_t TS = (uint32_t)ceil_div(b->n ? b->n : 1, SEARCH_TILE);
-    std::vector<uint32_t> std_(TS);
-    uint32_t j = 0;
-    for (uint32_t t = 0; t < TS; t++) {
-        const uint64_t p = (uint64_t)t * SEARCH_TILE;
-        while (j + 1 < b->nd && st[j + 1] <= p)
-            j++;
-        std_[t] = j;
-    }
+    std::vector<uint32_t> st(b->nd), en(b->nd);
+    L.bounds(st.data(), en.data());
+    std::vector<uint32_t> std_((size_t)ceil_div(b->n ? b->n : 1, SEARCH_TILE));
+    L.search_tile_doc(std_.data());
     if (b->d_starts.ensure((size_t)(b->nd ? b->nd : 1) * 4) != hipSuccess ||
         b->d_ends.ensure((size_t)(b->nd ? b->nd : 1) * 4) != hipSuccess ||
         b->d_line_start.ensure((size_t)(b->nd ? b->nd : 1) * 4) != hipSuccess ||
-        b->search_tile_doc.ensure((size_t)TS * 4) != hipSuccess)
+        b->search_tile_doc.ensure(std_.size() * 4) != hipSuccess)
         return bail(fail(OL_ERR_NOMEM, "batch index allocation failed"));
     if ((b->nd && (hipMemcpy(b->d_starts.p, st.data(), st.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
                    hipMemcpy(b->d_ends.p, en.data(), en.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) ||
@@ -2559,3 +2755,529 @@ int ol_batch_matches(ol_ctx *c, ol_batch *b, size_t doc, uint64_t *count, uint32
 }
 
 }  // extern "C"
+
+// ---- serving slots ---------------------------------------------------------------
+// A slot is a context (stream, staging ring, launch tuning) plus one device
+// arena, allocated when the slot is created.  Every batch the slot runs is
+// carved from the arena by bounds (one line per text byte for the per-line
+// arrays; a dictionary's match list after its count), so running a batch
+// allocates no device memory.  Index arrays go up and results come down
+// through a pinned host buffer with asynchronous copies on the slot's
+// stream; each batch has one read-back in the middle (the line counts, or
+// the match total), and the slot's thread waits only for its own stream.
+#include "slot.h"
+
+constexpr uint32_t SLOT_TREE_ROUNDS = 4;   // tree tiles of 1,024 lines: small documents waste less
+
+struct ol_slot {
+    ol_ctx *ctx = nullptr;
+    uint8_t *arena = nullptr;
+    size_t arena_bytes = 0;
+    uint8_t *pin = nullptr;   // pinned host staging (grown on demand)
+    size_t pin_bytes = 0;
+};
+
+namespace {
+
+// Bump allocation from an arena, 256-byte aligned.
+struct Carver {
+    uint8_t *base;
+    size_t cap, off = 0;
+    Carver(uint8_t *b, size_t c) : base(b), cap(c) {}
+    uint8_t *take(size_t b)
+    {
+        const size_t o = round_up(off, 256);
+        if (o + b > cap)
+            return nullptr;
+        off = o + b;
+        return base + o;
+    }
+    static size_t need(size_t &acc, size_t b)
+    {
+        acc = round_up(acc, 256) + b;
+        return acc;
+    }
+};
+
+// Region sizes of a parse batch over n text bytes and nd documents.
+struct ParsePlan {
+    uint64_t padded, cap, tiles_bound;
+    uint32_t scan_tiles;
+    size_t status_words, total;
+};
+ParsePlan parse_plan(uint32_t bpt, uint32_t rounds, uint64_t n, uint64_t nd)
+{
+    ParsePlan P;
+    P.padded = round_up(n ? n : 1, TEXT_ALIGN) + TEXT_SLACK;
+    P.cap = round_up(n ? n : 1, LINE_ALIGN);   // a line has at least one byte
+    P.scan_tiles = (uint32_t)ceil_div(n ? n : 1, (uint64_t)SCAN_THREADS * bpt);
+    P.tiles_bound = ceil_div(P.cap, (uint64_t)ol_tree_tile_lines(rounds)) + nd + 1;
+    P.status_words = STATUS_TILES + (size_t)P.scan_tiles * LB_STRIDE;   // ticket-ordered scan
+    size_t t = 0;
+    Carver::need(t, P.padded);
+    Carver::need(t, nd * 4);                    // starts
+    Carver::need(t, ((size_t)P.scan_tiles + 1) * 4);
+    Carver::need(t, nd * 4);                    // line starts
+    Carver::need(t, P.cap * 4);                 // offsets
+    Carver::need(t, P.cap);                     // classes
+    Carver::need(t, P.cap);                     // keys
+    for (int k = 0; k < 4; k++)
+        Carver::need(t, P.cap * 4);             // links
+    for (int k = 0; k < 3; k++)
+        Carver::need(t, nd * 4);                // segments
+    Carver::need(t, P.tiles_bound * 4);         // tile -> segment
+    Carver::need(t, P.tiles_bound * TREE_D * 16);   // tree level states
+    Carver::need(t, HEAD_WORDS * 8);
+    Carver::need(t, P.status_words * 8);
+    P.total = t;
+    return P;
+}
+
+struct SearchPlan {
+    uint64_t padded;
+    uint32_t search_tiles;
+    size_t status_words, cand_bytes, total;   // without the match list
+};
+SearchPlan search_plan(uint64_t n, uint64_t nd)
+{
+    SearchPlan P;
+    P.padded = round_up(n ? n : 1, TEXT_ALIGN) + TEXT_SLACK;
+    P.search_tiles = (uint32_t)ceil_div(n ? n : 1, SEARCH_TILE);
+    P.status_words = many_status_words(n);
+    P.cand_bytes = (size_t)P.search_tiles * SEARCH_TILE * 4;
+    size_t t = 0;
+    Carver::need(t, P.padded);
+    Carver::need(t, nd * 4);   // starts
+    Carver::need(t, nd * 4);   // ends
+    Carver::need(t, (size_t)P.search_tiles * 4);
+    Carver::need(t, HEAD_WORDS * 8);
+    Carver::need(t, P.status_words * 8);
+    Carver::need(t, P.cand_bytes);
+    P.total = t;
+    return P;
+}
+
+uint64_t bound_text(const size_t *lens, size_t count, uint64_t *nd)
+{
+    uint64_t n = 0, k = 0;
+    for (size_t i = 0; i < count; i++)
+        if (lens[i]) {
+            n += lens[i] + 1;
+            k++;
+        }
+    *nd = k;
+    return n;
+}
+
+int ensure_pin(ol_slot *s, size_t bytes)
+{
+    if (bytes <= s->pin_bytes)
+        return OL_OK;
+    if (s->pin)
+        (void)hipHostFree(s->pin);
+    s->pin = nullptr;
+    s->pin_bytes = 0;
+    const size_t want = round_up(std::max<size_t>(bytes, (size_t)16 << 20), (size_t)1 << 20);
+    if (hipHostMalloc((void **)&s->pin, want, hipHostMallocDefault) != hipSuccess) {
+        s->pin = nullptr;
+        return fail(OL_ERR_NOMEM, "pinned host allocation of " + std::to_string(want) + " bytes failed");
+    }
+    s->pin_bytes = want;
+    return OL_OK;
+}
+
+// Lends the batch's scratch its arena regions (zeroing the head).
+int lend_scratch(ol_ctx *c, Carver &cv, Scratch &scr, size_t status_words)
+{
+    uint8_t *h = cv.take(HEAD_WORDS * 8), *st = cv.take(status_words * 8);
+    if (!h || !st)
+        return fail(OL_ERR_NOMEM, "the batch does not fit the slot's arena");
+    scr.head.lend(h, HEAD_WORDS * 8);
+    scr.status.lend(st, status_words * 8);
+    scr.dirty = status_words;
+    HIPCHK(hipMemsetAsync(h, 0, HEAD_WORDS * 8, c->stream));
+    return OL_OK;
+}
+
+// Waits for the slot's stream; the head's sticky error bits and meta[0].
+int slot_sync(ol_ctx *c, Scratch &scr, uint64_t *head, uint64_t *meta)
+{
+    HIPCHK(hipMemcpyAsync(head, scr.head.p, HEAD_WORDS * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t bits = ((const uint32_t *)(head + HEAD_CTR))[CTR_DEV_ERR];
+    if (bits)
+        return fail(OL_ERR_DEVICE, dev_error_text(bits));
+    if (meta)
+        *meta = head[HEAD_META];
+    return OL_OK;
+}
+
+// The dictionary search of text [0, n) (documents `nd` of batch `b` when nd > 0,
+// else one document) into results: count with stored candidates, one read-back
+// of the total, the match list carved from what is left, the writes.
+int slot_many(ol_slot *s, Carver &cv, ol_batch &b, const ol_dict *dict, std::vector<uint64_t> &moff,
+              std::vector<uint32_t> &mid)
+{
+    ol_ctx *c = s->ctx;
+    ManyArgs a{};
+    a.text = b.text;
+    a.n = (uint32_t)b.n;
+    a.d = dict->d;
+    HIPCHK(setup_many(c, b.scr, a, b.n, true));
+    if (b.nd) {
+        a.ndocs = b.nd;
+        a.doc_starts = (const uint32_t *)b.d_starts.p;
+        a.doc_ends = (const uint32_t *)b.d_ends.p;
+        a.tile_doc = (const uint32_t *)b.search_tile_doc.p;
+    }
+    {
+        Prof p(c, "many");
+        HIPCHK(ol_launch_many(a, c->stream));   // count (storing candidates) + scan
+    }
+    uint64_t *head = (uint64_t *)s->pin;
+    uint64_t total = 0;
+    int rc = slot_sync(c, b.scr, head, &total);
+    if (rc)
+        return rc;
+    moff.resize(total);
+    mid.resize(total);
+    if (!total)
+        return OL_OK;
+    uint8_t *mo = cv.take(total * 8), *mi = cv.take(total * 4);
+    if (!mo || !mi)
+        return fail(OL_ERR_NOMEM, std::to_string(total) + " matches do not fit what is left of the slot's arena");
+    a.out_off = (uint64_t *)mo;
+    a.out_id = (uint32_t *)mi;
+    a.cap = total;
+    {
+        Prof p(c, "many_write");
+        HIPCHK(ol_launch_many_write(a, c->stream));
+    }
+    rc = ensure_pin(s, total * 12 + HEAD_WORDS * 8);
+    if (rc)
+        return rc;
+    uint8_t *pin = s->pin + HEAD_WORDS * 8;
+    HIPCHK(hipMemcpyAsync(pin, mo, total * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(pin + total * 8, mi, total * 4, hipMemcpyDeviceToHost, c->stream));
+    rc = slot_sync(c, b.scr, (uint64_t *)s->pin, nullptr);
+    if (rc)
+        return rc;
+    std::memcpy(moff.data(), pin, total * 8);
+    std::memcpy(mid.data(), pin + total * 8, total * 4);
+    return OL_OK;
+}
+
+}  // namespace
+
+int ol_slot_create(size_t arena_bytes, ol_slot **out)
+{
+    if (!out)
+        return fail(OL_ERR_INVALID, "null argument");
+    *out = nullptr;
+    ol_slot *s = new (std::nothrow) ol_slot();
+    if (!s)
+        return fail(OL_ERR_NOMEM, "out of host memory");
+    int rc = ol_ctx_create(&s->ctx);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    // ticket-ordered scan tiles: several slots share the device
+    (void)ol_tune_set(s->ctx, "scan_ticket", 1);
+    (void)ol_tune_set(s->ctx, "tree_rounds", (int)SLOT_TREE_ROUNDS);
+    if (hipMalloc((void **)&s->arena, arena_bytes) != hipSuccess) {
+        ol_ctx_destroy(s->ctx);
+        delete s;
+        return fail(OL_ERR_NOMEM, "device allocation of a " + std::to_string(arena_bytes) + "-byte slot arena failed");
+    }
+    s->arena_bytes = arena_bytes;
+    rc = ensure_pin(s, 16u << 20);
+    if (rc) {
+        ol_slot_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return OL_OK;
+}
+
+void ol_slot_destroy(ol_slot *s)
+{
+    if (!s)
+        return;
+    if (s->ctx)
+        (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->arena)
+        (void)hipFree(s->arena);
+    if (s->pin)
+        (void)hipHostFree(s->pin);
+    ol_ctx_destroy(s->ctx);
+    delete s;
+}
+
+ol_ctx *ol_slot_ctx(ol_slot *s) { return s ? s->ctx : nullptr; }
+size_t ol_slot_arena_bytes(const ol_slot *s) { return s ? s->arena_bytes : 0; }
+
+size_t ol_slot_parse_bytes(const size_t *lens, size_t count)
+{
+    // the plan depends on the slot's tuning, which every slot shares
+    uint64_t nd = 0;
+    const uint64_t n = bound_text(lens, count, &nd);
+    return parse_plan(SCAN_BPT, SLOT_TREE_ROUNDS, n, nd).total;
+}
+
+size_t ol_slot_search_bytes(const size_t *lens, size_t count)
+{
+    uint64_t nd = 0;
+    const uint64_t n = bound_text(lens, count, &nd);
+    return search_plan(n, nd).total;
+}
+
+size_t ol_doc_device_bytes(const ol_doc *d)
+{
+    if (!d || !d->text)
+        return 0;
+    return (size_t)(round_up(d->n, TEXT_ALIGN) + TEXT_SLACK + d->cap * (4 + 1 + 1 + (d->nodes ? 16 : 0)));
+}
+
+size_t ol_tree_device_bytes(const ol_tree *t) { return t ? (size_t)t->cap * 16 : 0; }
+size_t ol_dict_device_bytes(const ol_dict *d) { return d ? d->bytes : 0; }
+
+int ol_slot_parse(ol_slot *s, const uint8_t *const *docs, const size_t *lens, size_t count, SlotResult *res)
+{
+    if (!s || (count && (!docs || !lens || !res)))
+        return fail(OL_ERR_INVALID, "null argument");
+    ol_ctx *c = s->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    BatchLayout L;
+    int rc = L.build(docs, lens, count);
+    if (rc)
+        return rc;
+    for (size_t i = 0; i < count; i++)
+        res[i] = SlotResult();
+    if (L.nd == 0)
+        return OL_OK;
+    const ParsePlan P = parse_plan(c->scan_bpt, c->tree_rounds, L.n, L.nd);
+    if (P.total > s->arena_bytes)
+        return fail(OL_ERR_NOMEM, "a parse batch of " + std::to_string(L.n) + " bytes needs " +
+                                      std::to_string(P.total) + " arena bytes; the slot has " +
+                                      std::to_string(s->arena_bytes));
+    Carver cv(s->arena, s->arena_bytes);
+    ol_batch b;
+    b.count = count;
+    b.nd = L.nd;
+    b.n = L.n;
+    b.h_start = L.start;
+    b.h_dev = L.dev;
+    b.text = cv.take(P.padded);
+    auto lend = [&](DevBuf &d, size_t bytes) {
+        uint8_t *q = cv.take(bytes);
+        if (q) d.lend(q, bytes);
+        return q != nullptr;
+    };
+    bool ok = b.text && lend(b.d_starts, (size_t)L.nd * 4) && lend(b.scan_tile_doc, ((size_t)P.scan_tiles + 1) * 4) &&
+              lend(b.d_line_start, (size_t)L.nd * 4) && lend(b.offsets, P.cap * 4) && lend(b.classes, P.cap) &&
+              lend(b.keys, P.cap) && lend(b.parent, P.cap * 4) && lend(b.fc, P.cap * 4) && lend(b.ns, P.cap * 4) &&
+              lend(b.se, P.cap * 4) && lend(b.seg_line0, (size_t)L.nd * 4) && lend(b.seg_lines, (size_t)L.nd * 4) &&
+              lend(b.seg_tile0, (size_t)L.nd * 4) && lend(b.tile_seg, P.tiles_bound * 4) &&
+              lend(b.scr.tree_sum, P.tiles_bound * TREE_D * 16);
+    if (!ok)
+        return fail(OL_ERR_NOMEM, "the batch does not fit the slot's arena");
+    if ((rc = lend_scratch(c, cv, b.scr, P.status_words)))
+        return rc;
+    b.cap = P.cap;
+    b.scan_tile_doc_bpt = c->scan_bpt;
+    // index arrays up (pinned), text up (staging ring)
+    const size_t idx_bytes = ((size_t)L.nd + P.scan_tiles + 1) * 4;
+    if ((rc = ensure_pin(s, HEAD_WORDS * 8 + idx_bytes + (size_t)L.nd * 4)))
+        return rc;
+    uint32_t *up = (uint32_t *)(s->pin + HEAD_WORDS * 8);
+    L.bounds(up, nullptr);
+    L.scan_tile_doc((uint64_t)SCAN_THREADS * c->scan_bpt, up + L.nd);
+    HIPCHK(hipMemcpyAsync(b.d_starts.p, up, (size_t)L.nd * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.scan_tile_doc.p, up + L.nd, ((size_t)P.scan_tiles + 1) * 4, hipMemcpyHostToDevice,
+                          c->stream));
+    rc = stage_upload(c, b.text, L.n, [&L](uint8_t *buf, uint64_t off, uint64_t k) { L.fill(buf, off, k); });
+    if (rc)
+        return rc;
+    HIPCHK(hipMemsetAsync(b.text + L.n, 0, P.padded - L.n, c->stream));
+    mark(c, "serve: upload");
+    if ((rc = batch_scan(c, &b, false)))
+        return rc;
+    // read-back: line count and each document's first line
+    uint32_t *ls = (uint32_t *)(s->pin + HEAD_WORDS * 8 + idx_bytes);
+    HIPCHK(hipMemcpyAsync(ls, b.d_line_start.p, (size_t)L.nd * 4, hipMemcpyDeviceToHost, c->stream));
+    uint64_t lines = 0;
+    if ((rc = slot_sync(c, b.scr, (uint64_t *)s->pin, &lines)))
+        return rc;
+    if (lines > P.cap)
+        return fail(OL_ERR_DEVICE, "batch line count exceeds its bound");
+    b.lines = lines;
+    std::vector<uint32_t> line0(ls, ls + L.nd), cnt(L.nd), tile0(L.nd), tseg;
+    const uint64_t TL = ol_tree_tile_lines(c->tree_rounds);
+    uint64_t tiles = 0;
+    for (uint32_t j = 0; j < L.nd; j++) {
+        const uint64_t end = (j + 1 < L.nd) ? line0[j + 1] : lines;
+        cnt[j] = (uint32_t)(end - line0[j]);
+        tile0[j] = (uint32_t)tiles;
+        const uint64_t k = ceil_div(cnt[j] ? cnt[j] : 1, TL);
+        tseg.insert(tseg.end(), k, j);
+        tiles += k;
+    }
+    if (tiles > P.tiles_bound)
+        return fail(OL_ERR_DEVICE, "batch tree tiles exceed their bound");
+    b.tree_tiles = (uint32_t)tiles;
+    // the pinned buffer for the rest of the batch (every copy so far has
+    // completed, so it may move): the segments up, then 21 bytes per line down
+    const size_t base = HEAD_WORDS * 8 + round_up(((size_t)3 * L.nd + tiles) * 4, 256);
+    if ((rc = ensure_pin(s, base + lines * 21)))
+        return rc;
+    uint32_t *sg = (uint32_t *)(s->pin + HEAD_WORDS * 8);
+    std::memcpy(sg, line0.data(), (size_t)L.nd * 4);
+    std::memcpy(sg + L.nd, cnt.data(), (size_t)L.nd * 4);
+    std::memcpy(sg + 2 * (size_t)L.nd, tile0.data(), (size_t)L.nd * 4);
+    std::memcpy(sg + 3 * (size_t)L.nd, tseg.data(), tiles * 4);
+    HIPCHK(hipMemcpyAsync(b.seg_line0.p, sg, (size_t)L.nd * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.seg_lines.p, sg + L.nd, (size_t)L.nd * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.seg_tile0.p, sg + 2 * (size_t)L.nd, (size_t)L.nd * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.tile_seg.p, sg + 3 * (size_t)L.nd, tiles * 4, hipMemcpyHostToDevice, c->stream));
+    if ((rc = batch_tree(c, &b)))
+        return rc;
+    mark(c, "serve: scan + tree");
+    // results down, after the segment area
+    uint8_t *dn = s->pin + base;
+    const struct {
+        const DevBuf *src;
+        size_t w;
+    } arr[6] = {{&b.offsets, 4}, {&b.parent, 4}, {&b.se, 4}, {&b.fc, 4}, {&b.ns, 4}, {&b.classes, 1}};
+    size_t at = 0;
+    for (const auto &x : arr) {
+        if (lines)
+            HIPCHK(hipMemcpyAsync(dn + at, x.src->p, lines * x.w, hipMemcpyDeviceToHost, c->stream));
+        at += lines * x.w;
+    }
+    uint64_t *head = (uint64_t *)s->pin;
+    if ((rc = slot_sync(c, b.scr, head, nullptr)))
+        return rc;
+    if (((const uint32_t *)(head + HEAD_CTR))[CTR_TREE_ERR])
+        return fail(OL_ERR_TOO_DEEP, "a content line is nested deeper than the " + std::to_string(OL_MAX_TREE_DEPTH) +
+                                         " levels the GPU tree tracks");
+    mark(c, "serve: download");
+    const uint32_t *h_off = (const uint32_t *)dn, *h_par = h_off + lines, *h_se = h_par + lines,
+                   *h_fc = h_se + lines, *h_ns = h_fc + lines;
+    const uint8_t *h_cls = (const uint8_t *)(h_ns + lines);
+    for (size_t i = 0; i < count; i++) {
+        const uint32_t j = L.dev[i];
+        if (j == ~0u)
+            continue;
+        const size_t l0 = line0[j], k = cnt[j];
+        SlotResult &r = res[i];
+        r.offsets.assign(h_off + l0, h_off + l0 + k);
+        r.parent.assign(h_par + l0, h_par + l0 + k);
+        r.se.assign(h_se + l0, h_se + l0 + k);
+        r.fc.assign(h_fc + l0, h_fc + l0 + k);
+        r.ns.assign(h_ns + l0, h_ns + l0 + k);
+        r.classes.assign(h_cls + l0, h_cls + l0 + k);
+    }
+    return OL_OK;
+}
+
+int ol_slot_search(ol_slot *s, const uint8_t *const *docs, const size_t *lens, size_t count, const ol_dict *dict,
+                   SlotResult *res)
+{
+    if (!s || !dict || (count && (!docs || !lens || !res)))
+        return fail(OL_ERR_INVALID, "null argument");
+    ol_ctx *c = s->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    BatchLayout L;
+    int rc = L.build(docs, lens, count);
+    if (rc)
+        return rc;
+    for (size_t i = 0; i < count; i++)
+        res[i] = SlotResult();
+    if (L.nd == 0)
+        return OL_OK;
+    const SearchPlan P = search_plan(L.n, L.nd);
+    if (P.total > s->arena_bytes)
+        return fail(OL_ERR_NOMEM, "a search batch of " + std::to_string(L.n) + " bytes needs " +
+                                      std::to_string(P.total) + " arena bytes; the slot has " +
+                                      std::to_string(s->arena_bytes));
+    Carver cv(s->arena, s->arena_bytes);
+    ol_batch b;
+    b.count = count;
+    b.nd = L.nd;
+    b.n = L.n;
+    b.text = cv.take(P.padded);
+    auto lend = [&](DevBuf &d, size_t bytes) {
+        uint8_t *q = cv.take(bytes);
+        if (q) d.lend(q, bytes);
+        return q != nullptr;
+    };
+    if (!b.text || !lend(b.d_starts, (size_t)L.nd * 4) || !lend(b.d_ends, (size_t)L.nd * 4) ||
+        !lend(b.search_tile_doc, (size_t)P.search_tiles * 4))
+        return fail(OL_ERR_NOMEM, "the batch does not fit the slot's arena");
+    if ((rc = lend_scratch(c, cv, b.scr, P.status_words)))
+        return rc;
+    if (!lend(b.scr.cands, P.cand_bytes))
+        return fail(OL_ERR_NOMEM, "the batch does not fit the slot's arena");
+    const size_t idx_bytes = ((size_t)2 * L.nd + P.search_tiles) * 4;
+    if ((rc = ensure_pin(s, HEAD_WORDS * 8 + idx_bytes)))
+        return rc;
+    uint32_t *up = (uint32_t *)(s->pin + HEAD_WORDS * 8);
+    L.bounds(up, up + L.nd);
+    L.search_tile_doc(up + 2 * (size_t)L.nd);
+    HIPCHK(hipMemcpyAsync(b.d_starts.p, up, (size_t)L.nd * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.d_ends.p, up + L.nd, (size_t)L.nd * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.search_tile_doc.p, up + 2 * (size_t)L.nd, (size_t)P.search_tiles * 4,
+                          hipMemcpyHostToDevice, c->stream));
+    rc = stage_upload(c, b.text, L.n, [&L](uint8_t *buf, uint64_t off, uint64_t k) { L.fill(buf, off, k); });
+    if (rc)
+        return rc;
+    HIPCHK(hipMemsetAsync(b.text + L.n, 0, P.padded - L.n, c->stream));
+    mark(c, "serve: upload");
+    std::vector<uint64_t> moff;
+    std::vector<uint32_t> mid;
+    if ((rc = slot_many(s, cv, b, dict, moff, mid)))
+        return rc;
+    mark(c, "serve: search + download");
+    for (size_t i = 0; i < count; i++) {
+        if (L.dev[i] == ~0u)
+            continue;
+        const uint64_t st = L.start[i], en = st + lens[i];
+        const size_t lo = std::lower_bound(moff.begin(), moff.end(), st) - moff.begin();
+        const size_t hi = std::lower_bound(moff.begin(), moff.end(), en) - moff.begin();
+        SlotResult &r = res[i];
+        r.m_off.resize(hi - lo);
+        for (size_t k = lo; k < hi; k++)
+            r.m_off[k - lo] = (uint32_t)(moff[k] - st);
+        r.m_id.assign(mid.begin() + lo, mid.begin() + hi);
+    }
+    return OL_OK;
+}
+
+int ol_slot_search_doc(ol_slot *s, const ol_doc *doc, const ol_dict *dict, SlotResult *res)
+{
+    if (!s || !doc || !dict || !res)
+        return fail(OL_ERR_INVALID, "null argument");
+    ol_ctx *c = s->ctx;
+    std::lock_guard<std::mutex> g(c->mu);
+    *res = SlotResult();
+    if (doc->n == 0)
+        return OL_OK;
+    const SearchPlan P = search_plan(doc->n, 0);
+    Carver cv(s->arena, s->arena_bytes);
+    ol_batch b;
+    b.n = doc->n;
+    b.text = doc->text;
+    int rc = lend_scratch(c, cv, b.scr, P.status_words);
+    if (rc)
+        return rc;
+    uint8_t *cq = cv.take(P.cand_bytes);
+    if (!cq)
+        return fail(OL_ERR_NOMEM, "the search does not fit the slot's arena");
+    b.scr.cands.lend(cq, P.cand_bytes);
+    std::vector<uint64_t> moff;
+    if ((rc = slot_many(s, cv, b, dict, moff, res->m_id)))
+        return rc;
+    res->m_off.assign(moff.begin(), moff.end());
+    return OL_OK;
+}

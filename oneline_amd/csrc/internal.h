@@ -432,6 +432,9 @@ constexpr size_t OL_NOALLOC_WORDS = OL_NOALLOC_STATUS + 2 * OL_NOALLOC_MAX_TILES
 // with hipGetSymbolAddress on the first call for that device.
 hipError_t ol_noalloc_scratch(int device, uint64_t **out);
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s);
+// The write pass alone, after an ol_launch_many of the same arguments that
+// counted and stored candidates without outputs.
+hipError_t ol_launch_many_write(const ManyArgs &a, hipStream_t s);
 
 // Tile geometry of each kernel.
 constexpr uint32_t SCAN_THREADS = 256;

@@ -1062,6 +1062,8 @@ static hipError_t launch_many(const ManyArgs &a, hipStream_t s)
 }
 
 // Counts and their scan (total in meta[0]); with outputs also the writes.
+// Without outputs but with a candidate store, the candidates are stored for a
+// later ol_launch_many_write with the same arguments plus the outputs.
 hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
 {
     if (a.num_wtiles == 0)
@@ -1075,5 +1077,12 @@ hipError_t ol_launch_many(const ManyArgs &a, hipStream_t s)
     hipLaunchKernelGGL((many_scan_kernel<1024>), dim3(chunks), dim3(1024), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess || (!a.out_off && !a.out_id))
         return e;
+    return launch_many<true>(a, s);
+}
+
+hipError_t ol_launch_many_write(const ManyArgs &a, hipStream_t s)
+{
+    if (a.num_wtiles == 0 || (!a.out_off && !a.out_id))
+        return hipSuccess;
     return launch_many<true>(a, s);
 }

@@ -106,3 +106,58 @@ def test_query_subtree_across_blocks(parser):
             got = parser.query_subtree(doc, tree, root, kind, depth)
             w = want(nodes, kind, depth, root, int(se[root]))
             assert got.total_matches == w.size and np.array_equal(got.indices, w), (root, kind, depth)
+
+
+def test_query_with_given_offsets(parser):
+    """GpuParser::query (crates/oneline-gpu/src/query.rs:288-314): the lines are
+    the caller's offsets, each parsed from its bytes; against the oracle's
+    per-line parse of the same spans."""
+    text = mixed_text()
+    t = np.frombuffer(text, np.uint8)
+    offs, nodes = orc.parse(t)
+    for kind in (None, 2, 4):
+        for depth in (None, 0, 3, 300):
+            got = parser.query(text, offs, kind, depth, max_matches=1 << 22)
+            w = want(nodes, kind, depth)
+            assert got.total_matches == w.size and np.array_equal(got.indices, w), (kind, depth)
+    # offsets that are not the scanner's: every third line start, so spans
+    # hold several lines; each span parsed as parse() parses a given line
+    sub = offs[::3].copy()
+    ends = np.append(sub[1:], t.size)
+    sub_nodes = np.zeros(sub.size, orc.NODE)
+    for i, (s, e) in enumerate(zip(sub, ends)):
+        sub_nodes[i] = orc.parse_line(text[s:e], int(s))
+    for kind, depth in ((None, None), (1, None), (None, 2), (4, 1)):
+        got = parser.query(text, sub, kind, depth, max_matches=1 << 22)
+        w = want(sub_nodes, kind, depth)
+        assert got.total_matches == w.size and np.array_equal(got.indices, w), (kind, depth)
+    # truncation keeps the true total; no offsets: nothing
+    got = parser.query(text, offs, None, None, max_matches=5)
+    assert got.total_matches == offs.size and np.array_equal(got.indices, np.arange(5, dtype=np.uint32))
+    empty = parser.query(text, np.zeros(0, np.uint32), 2, None)
+    assert empty.total_matches == 0 and empty.indices.size == 0
+
+
+def test_per_call_profiles(parser):
+    """The profile_* record: host steps in execution order whose sum is the
+    wall time, and the call's GPU passes by name."""
+    text = datagen.document(4 << 20, seed=3)
+    doc, prof = parser.profile_upload(text, nodes=False)
+    names = [k for k, _ in prof.gpu_passes]
+    assert "scan_classify" in names
+    steps = [k for k, _ in prof.host_steps]
+    assert "upload: stream through the staging ring" in steps and "scan: launch" in steps
+    assert prof.wall > 0 and abs(sum(d for _, d in prof.host_steps) - prof.wall) < 1e-6
+    assert prof.gpu_total() > 0
+    pm = parser.profile_search_many(doc, parser.compile_patterns(doc, datagen.patterns(256, seed=1)), 1 << 20)
+    assert any(k == "many" for k, _ in pm.gpu_passes) and any(k.startswith("many:") for k, _ in pm.host_steps)
+    tree, pt = parser.profile_build_tree_reusing(doc, None)
+    assert [k for k, _ in pt.gpu_passes] == ["tree"] and tree.line_count() == doc.line_count()
+    pq = parser.profile_query_document(doc, 2, None)
+    assert any(k == "query" for k, _ in pq.gpu_passes)
+    # a call outside a record leaves nothing behind
+    parser.search_document(doc, b"the", 10)
+    ps = parser.profile_search_document(doc, b"the", 10)
+    assert [k for k, _ in ps.gpu_passes] == ["search"]
+    tree.close()
+    doc.close()

@@ -1,6 +1,7 @@
 """Serving (ol_server_*): continuous batching of parse and dictionary-search
-requests.  Each result equals the oracle's and is the same whether the request
-ran alone or inside a mix of concurrent requests."""
+requests on slots with persistent arenas, resident documents and
+dictionaries.  Each result equals the oracle's and is the same whether the
+request ran alone or inside a mix of concurrent requests."""
 import numpy as np
 import pytest
 
@@ -84,19 +85,105 @@ def test_result_alone_equals_result_in_mix():
 
 
 def test_hbm_budget_admission():
-    srv = ol.GpuServer(hbm_budget=24 << 20, slots=3)
+    # arenas sized for 4 MiB batches: two slots fit a 512 MiB budget, sixteen do not
+    with pytest.raises(ol.OnelineError):
+        ol.GpuServer(hbm_budget=512 << 20, slots=16, max_batch_bytes=4 << 20)
+    srv = ol.GpuServer(hbm_budget=512 << 20, slots=2, max_batch_bytes=4 << 20, max_batch_docs=16)
     try:
-        with pytest.raises(ol.OnelineError):
-            srv.submit(np.zeros(32 << 20, np.uint8) + ord("a"), ol.PARSE)
-        docs = [datagen.document(3 << 20, seed=50 + i).tobytes() for i in range(8)]
+        arenas = srv.hbm_used()
+        assert arenas == 2 * srv.arena_bytes() <= 256 << 20
+        with pytest.raises(ol.OnelineError):          # larger than a batch
+            srv.submit(np.zeros(5 << 20, np.uint8) + ord("a"), ol.PARSE)
+        docs = [datagen.document(3 << 20, seed=50 + i).tobytes() for i in range(6)]
         ids = [srv.submit(d, ol.PARSE) for d in docs]
         for rid, d in zip(ids, docs):
             srv.wait(rid)
             assert_parse_equal(srv.parse_result(rid), oracle_parse(d), "budgeted")
-        # the budget admits about two of these at a time: their runs cannot all overlap
-        spans = sorted(srv.timing(rid)[1:] for rid in ids)
-        most = max(sum(1 for s, e in spans if s <= t0 < e) for t0, _ in spans)
-        assert most <= 3
+        # one 3 MiB document per 4 MiB batch: each batch ran alone on a slot
+        assert len({srv.timing(rid)[1] for rid in ids}) == len(ids)
+        assert {srv.slot_of(rid) for rid in ids} <= {0, 1}
+        # a resident document is charged at its device size, and released
+        big = datagen.document(8 << 20, seed=60).tobytes()
+        did = srv.put_document(big)
+        assert srv.hbm_used() > arenas + (8 << 20)
+        srv.drop_document(did)
+        assert srv.hbm_used() == arenas
+        with pytest.raises(ol.OnelineError):          # beyond the budget
+            srv.put_document(np.zeros(400 << 20, np.uint8) + ord("a"))
+        assert srv.hbm_used() == arenas
+    finally:
+        srv.close()
+
+
+def test_resident_documents():
+    """Requests on resident documents skip the upload and the parse; their
+    results equal the oracle's and those of the same bytes submitted."""
+    srv = ol.GpuServer(slots=2, max_batch_bytes=64 << 20)
+    try:
+        pats = datagen.patterns(4096, seed=0xC3B)
+        d0 = srv.add_dictionary(pats)
+        x = datagen.document(5 << 20, seed=33, crlf=10, empty=5).tobytes()
+        did = srv.put_document(x)
+        rp = srv.submit_resident(did, ol.PARSE)
+        rs = srv.submit_resident(did, ol.SEARCH, d0)
+        bp = srv.submit(x, ol.PARSE)
+        bs = srv.submit(x, ol.SEARCH, d0)
+        for r in (rp, rs, bp, bs):
+            srv.wait(r)
+        want = oracle_parse(x)
+        assert_parse_equal(srv.parse_result(rp), want, "resident")
+        assert_parse_equal(srv.parse_result(bp), want, "submitted")
+        w_off, w_id = orc.many_find_all(ol.as_bytes_array(x), pats)
+        for r in (rs, bs):
+            off, ids = srv.search_result(r)
+            assert np.array_equal(off, w_off.astype(np.uint32)) and np.array_equal(ids, w_id)
+        # dropped while a request on it is queued: the request still completes
+        r2 = srv.submit_resident(did, ol.SEARCH, d0)
+        srv.drop_document(did)
+        srv.wait(r2)
+        assert np.array_equal(srv.search_result(r2)[0], w_off.astype(np.uint32))
+        with pytest.raises(ol.OnelineError):
+            srv.submit_resident(did, ol.PARSE)
+    finally:
+        srv.close()
+
+
+def test_c5_workload_against_oracle():
+    """The serving workload's shape: 8 resident dictionaries of 1,024 to 65,536
+    patterns, documents log-normal (median 64 KiB, sigma 2, up to 256 MiB,
+    seed 2) plus a 64 MiB one, 70% parse / 30% search requests submitted
+    together; a sample of each kind checked against the oracle."""
+    srv = ol.GpuServer(hbm_budget=96 << 30, slots=4, max_batch_docs=256, max_batch_bytes=256 << 20)
+    try:
+        dict_sizes = (1024, 2048, 4096, 8192, 16384, 32768, 65536, 1024)
+        dicts = [datagen.patterns(cnt, seed=0xC3B + 17 * k) for k, cnt in enumerate(dict_sizes)]
+        for p in dicts:
+            srv.add_dictionary(p)
+        sizes = list(datagen.lognormal_sizes(120, 2, 65536.0, 2.0, 64, 256 << 20)) + [64 << 20]
+        docs = [datagen.document(int(sz), seed=datagen.mix(2, i)) for i, sz in enumerate(sizes)]
+        print(f"[c5] {len(docs)} documents, {sum(int(x) for x in sizes)} bytes", flush=True)
+        rng = np.random.default_rng(12)
+        reqs = []
+        for i, d in enumerate(docs):
+            if rng.random() < 0.3:
+                k = int(rng.integers(0, len(dicts)))
+                reqs.append((srv.submit(d, ol.SEARCH, k), i, k))
+            else:
+                reqs.append((srv.submit(d, ol.PARSE), i, None))
+        for rid, _, _ in reqs:
+            srv.wait(rid, timeout=300)
+        checked_p = checked_s = 0
+        for rid, i, k in reqs:
+            d = docs[i]
+            if k is None and (checked_p < 25 or d.size > (32 << 20)):
+                assert_parse_equal(srv.parse_result(rid), oracle_parse(d), f"c5 doc {i}")
+                checked_p += 1
+            elif k is not None and checked_s < 12 and d.size <= (4 << 20):
+                off, ids = srv.search_result(rid)
+                w_off, w_id = orc.many_find_all(d, dicts[k])
+                assert np.array_equal(off, w_off.astype(np.uint32)) and np.array_equal(ids, w_id), f"c5 doc {i}"
+                checked_s += 1
+        assert checked_p >= 20 and checked_s >= 5
     finally:
         srv.close()
 
