@@ -19,6 +19,7 @@
 #include <functional>
 #include <mutex>
 #include <new>
+#include <random>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1747,75 +1748,140 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
                                             "..=" + std::to_string(OL_MAX_SET_PATTERN_LEN));
     }
     std::lock_guard<std::mutex> g(c->mu);
-    // Class and key of each pattern: class = length for 1..3, 4 for 4..7,
-    // 5 for >= 8; key = its first min(len, 3), 4 or 8 bytes (lo, hi words).
-    std::vector<uint32_t> cls(count), klo(count), khi(count), order(count);
-    size_t pool_bytes = 0;
-    uint32_t max_len = 0;
+    // Key class and key of each pattern: class DK_LONG for 4 bytes and more
+    // (key: the first 4), DK_3 / DK_2 / DK_1 for 3, 2 and 1 bytes (key: all
+    // of them).  Patterns with equal class and key form a run, sorted by id.
     auto word = [&](const uint8_t *p, size_t from, size_t n) {
         uint32_t v = 0;
         for (size_t b = 0; b < n; b++)
             v |= (uint32_t)p[from + b] << (8 * b);
         return v;
     };
+    std::vector<uint32_t> cls(count), key(count), order(count);
+    uint32_t max_len = 0;
     for (size_t i = 0; i < count; i++) {
         const size_t L = lens[i];
-        const uint32_t k = L <= 3 ? (uint32_t)L : (L < 8 ? 4u : 5u);
-        cls[i] = k;
-        klo[i] = word(patterns[i], 0, std::min<size_t>(L, 4));
-        khi[i] = k == 5 ? word(patterns[i], 4, 4) : 0u;
+        cls[i] = L >= 4 ? DK_LONG : (L == 3 ? DK_3 : (L == 2 ? DK_2 : DK_1));
+        key[i] = word(patterns[i], 0, std::min<size_t>(L, 4));
         order[i] = (uint32_t)i;
-        pool_bytes += L;
         max_len = std::max<uint32_t>(max_len, (uint32_t)L);
     }
     std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         if (cls[a] != cls[b]) return cls[a] < cls[b];
-        if (klo[a] != klo[b]) return klo[a] < klo[b];
-        if (khi[a] != khi[b]) return khi[a] < khi[b];
+        if (key[a] != key[b]) return key[a] < key[b];
         return a < b;
     });
-    // Entries (in sorted order) and the word-aligned pattern pool.
-    (void)pool_bytes;
-    std::vector<uint32_t> ents((size_t)count * 4);
+    // A pattern's bytes 4..7 (w1) and what follows: bytes 8..11 (w2) when it
+    // ends by byte 12, else the pool offset of its bytes 8.. (word aligned).
     std::vector<uint32_t> pool;
-    for (size_t e = 0; e < count; e++) {
-        const uint32_t i = order[e];
+    auto tail = [&](uint32_t i, uint32_t &w1, uint32_t &aux) {
         const size_t L = lens[i];
-        const uint32_t kb = cls[i] <= 3 ? cls[i] : (cls[i] == 4 ? 4u : 8u);   // key bytes
-        uint32_t chk = 0;
-        for (size_t b = kb; b < L && b < kb + 4; b++)
-            chk |= (uint32_t)patterns[i][b] << (8 * (b - kb));
-        ents[e * 4 + 0] = i;
-        ents[e * 4 + 1] = (uint32_t)L;
-        ents[e * 4 + 2] = (uint32_t)pool.size();
-        ents[e * 4 + 3] = chk;
-        for (size_t b = 0; b < L; b += 4)
-            pool.push_back(word(patterns[i], b, std::min<size_t>(4, L - b)));
+        w1 = L > 4 ? word(patterns[i], 4, std::min<size_t>(L - 4, 4)) : 0u;
+        if (L <= 12) {
+            aux = L > 8 ? word(patterns[i], 8, L - 8) : 0u;
+        } else {
+            aux = (uint32_t)pool.size();
+            for (size_t b = 8; b < L; b += 4)
+                pool.push_back(word(patterns[i], b, std::min<size_t>(4, L - b)));
+        }
+    };
+    auto info_of = [&](uint32_t i) { return dk_info(i, (uint32_t)lens[i], cls[i]); };
+    // slots (one per distinct class and key) and the entries of runs
+    struct Slot {
+        uint32_t key, info, a, b;
+    };
+    std::vector<Slot> slots;
+    std::vector<uint32_t> ents;   // 4 words per entry: info, w1, aux, 0
+    for (size_t e = 0; e < count;) {
+        const uint32_t i = order[e];
+        size_t f = e + 1;
+        while (f < count && cls[order[f]] == cls[i] && key[order[f]] == key[i])
+            f++;
+        Slot sl{key[i], 0, 0, 0};
+        if (f - e == 1) {   // one pattern: decided from the slot
+            sl.info = info_of(i);
+            tail(i, sl.a, sl.b);
+        } else {            // a run of entries
+            sl.info = dk_info(0, 0, cls[i]);
+            sl.a = (uint32_t)(ents.size() / 4);
+            sl.b = (uint32_t)(f - e);
+            for (size_t k = e; k < f; k++) {
+                uint32_t w1, aux;
+                tail(order[k], w1, aux);
+                ents.insert(ents.end(), {info_of(order[k]), w1, aux, 0u});
+            }
+        }
+        slots.push_back(sl);
+        e = f;
     }
     pool.resize(pool.size() + 16, 0);
-    // Per class: distinct keys with their entry ranges.
-    struct Bucket {
-        uint32_t lo, hi, begin, end;
-    };
-    std::vector<Bucket> buckets[6];
-    for (size_t e = 0; e < count;) {
-        uint32_t i = order[e];
-        size_t f = e + 1;
-        while (f < count && cls[order[f]] == cls[i] && klo[order[f]] == klo[i] && khi[order[f]] == khi[i])
-            f++;
-        buckets[cls[i]].push_back({klo[i], khi[i], (uint32_t)e, (uint32_t)f});
-        e = f;
+    if (ents.empty())
+        ents.assign(4, 0);
+    // Buckets of DK_BUCKET slots; each slot goes to one of its key's two
+    // buckets (the emptier one, moving residents to their other bucket when
+    // both are full); the table doubles until every slot has a place.
+    uint32_t bbits = 2;
+    while ((size_t)DK_BUCKET << bbits < slots.size() * 2)
+        bbits++;
+    std::vector<int32_t> place;   // slot index per table position, -1 empty
+    for (;; bbits++) {
+        const size_t nb = (size_t)1 << bbits;
+        place.assign(nb * DK_BUCKET, -1);
+        std::vector<uint8_t> fillc(nb, 0);
+        bool ok = true;
+        std::mt19937 rng(12345);
+        for (size_t k = 0; k < slots.size() && ok; k++) {
+            int32_t cur = (int32_t)k;
+            uint32_t prev_bucket = ~0u;
+            for (int step = 0; step < 500; step++) {
+                const Slot &sl = slots[cur];
+                const uint32_t cl = sl.info >> DK_CLASS_SHIFT;
+                const uint32_t b1 = dk_bucket1(sl.key, cl, bbits), b2 = dk_bucket2(sl.key, cl, bbits);
+                uint32_t tgt = fillc[b1] <= fillc[b2] ? b1 : b2;
+                if (fillc[tgt] < DK_BUCKET) {
+                    place[(size_t)tgt * DK_BUCKET + fillc[tgt]++] = cur;
+                    cur = -1;
+                    break;
+                }
+                // both full: evict a random resident of the bucket not just left
+                tgt = (b1 == prev_bucket) ? b2 : (b2 == prev_bucket ? b1 : (rng() & 1 ? b1 : b2));
+                const uint32_t victim_at = tgt * DK_BUCKET + (rng() % DK_BUCKET);
+                const int32_t victim = place[victim_at];
+                place[victim_at] = cur;
+                cur = victim;
+                prev_bucket = tgt;
+            }
+            if (cur >= 0)
+                ok = false;
+        }
+        if (ok)
+            break;
+    }
+    std::vector<uint32_t> tab(((size_t)DK_BUCKET << bbits) * 4, 0);   // empty: info 0 (not DK_VALID)
+    for (size_t q = 0; q < place.size(); q++) {
+        if (place[q] < 0)
+            continue;
+        const Slot &sl = slots[place[q]];
+        tab[q * 4 + 0] = sl.key;
+        tab[q * 4 + 1] = sl.info;
+        tab[q * 4 + 2] = sl.a;
+        tab[q * 4 + 3] = sl.b;
     }
     DictDev dd{};
     dd.max_len = max_len;
-    // filters: 1 and 2 direct; 4 = the prefix filter of classes 3..5 (4-byte
-    // keys, a 3-byte key with each of the 32 values of the fourth byte's low
-    // 5 bits).  Layout: filter 4 first, then 1, 2, then a zero word that
-    // absent filters point at.
+    dd.bbits = bbits;
+    for (size_t i = 0; i < count; i++)
+        dd.class_mask |= 1u << cls[i];
+    // filters: 1 and 2 direct; 4 = the prefix filter of patterns of 3 bytes and
+    // more (4-byte keys, a 3-byte key with each of the 32 values of the fourth
+    // byte's low 5 bits).  Layout: filter 4 first, then 1, 2, then a zero word
+    // that absent filters point at.
     std::vector<std::vector<uint32_t>> filt(6);
     uint32_t fwords = 0;
+    const bool any3 = (dd.class_mask >> DK_3) & 1u, anylong = (dd.class_mask >> DK_LONG) & 1u;
     for (uint32_t f : {4u, 1u, 2u}) {
-        const bool used = f <= 2 ? !buckets[f].empty() : !(buckets[3].empty() && buckets[4].empty() && buckets[5].empty());
+        const uint32_t dk = f == 1 ? DK_1 : DK_2;
+        const bool used = f == 4 ? (any3 || anylong) : ((dd.class_mask >> dk) & 1u) != 0;
         if (!used)
             continue;
         const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : DICT_PBITS;
@@ -1824,60 +1890,38 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         const uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
         filt[f].assign(words, 0);
         fwords += words;
-        if (f <= 2) {
-            for (const Bucket &b : buckets[f]) filt[f][b.lo >> 5] |= 1u << (b.lo & 31);
-            continue;
+        for (size_t i = 0; i < count; i++) {
+            if (f != 4) {
+                if (cls[i] == dk)
+                    filt[f][key[i] >> 5] |= 1u << (key[i] & 31);
+                continue;
+            }
+            auto setkey = [&](uint32_t w) {
+                const uint32_t h = dict_ph(w);
+                filt[f][dict_ph_word(h)] |= 1u << (h & 31);
+            };
+            if (cls[i] == DK_3)
+                for (uint32_t c4 = 0; c4 < 32; c4++) setkey(key[i] | c4 << 24);
+            else if (cls[i] == DK_LONG)
+                setkey(key[i]);
         }
-        auto setkey = [&](uint32_t w) {
-            const uint32_t h = dict_ph(w);
-            filt[f][dict_ph_word(h)] |= 1u << (h & 31);
-        };
-        for (const Bucket &b : buckets[3])
-            for (uint32_t c = 0; c < 32; c++) setkey(b.lo | c << 24);
-        for (uint32_t k = 4; k <= 5; k++)
-            for (const Bucket &b : buckets[k]) setkey(b.lo);
     }
     for (uint32_t f = 1; f <= 4; f++)
         if (!dd.fbits[f])
             dd.foff[f] = fwords;   // the zero word
     fwords += 1;
     dd.fwords = fwords;
-    // tables
-    std::vector<std::vector<uint32_t>> tab(6);   // 4 words per slot
-    for (uint32_t k = 1; k <= 5; k++) {
-        const size_t nb = buckets[k].size();
-        if (!nb)
-            continue;
-        dd.class_mask |= 1u << k;
-        const uint32_t tb = std::max<uint32_t>(4, ceil_log2(nb * 2));
-        dd.tbits[k] = tb;
-        tab[k].assign((size_t)4 << tb, 0);
-        const uint32_t tmask = (1u << tb) - 1;
-        for (const Bucket &b : buckets[k]) {
-            uint32_t h = dict_thash(b.lo, b.hi, tb);
-            while (tab[k][(size_t)h * 4 + 2] != 0)
-                h = (h + 1) & tmask;
-            tab[k][(size_t)h * 4 + 0] = b.lo;
-            tab[k][(size_t)h * 4 + 1] = b.begin;
-            tab[k][(size_t)h * 4 + 2] = b.end;
-            tab[k][(size_t)h * 4 + 3] = b.hi;
-        }
-    }
-    // One device allocation: [filter][tables][ent_id][ent_len][ent_off][pool]
+    // One device allocation: [filter][table][entries][pool]
     size_t bytes = 0;
-    auto place = [&](size_t n) {
+    auto placeb = [&](size_t n) {
         size_t o = round_up(bytes, 256);
         bytes = o + n;
         return o;
     };
-    size_t o_f = place((size_t)std::max<uint32_t>(fwords, 1) * 4);
-    size_t o_t[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t k = 1; k <= 5; k++)
-        if (!tab[k].empty())
-            o_t[k] = place(tab[k].size() * 4);
-    size_t o_ent = place(ents.size() * 4);
-    size_t o_pool = place(pool.size() * 4);
-    size_t o_zero = place(16);   // an empty slot: the table of a class without patterns
+    const size_t o_f = placeb((size_t)fwords * 4);
+    const size_t o_t = placeb(tab.size() * 4);
+    const size_t o_ent = placeb(ents.size() * 4);
+    const size_t o_pool = placeb(pool.size() * 4);
     ol_dict *s = new (std::nothrow) ol_dict();
     if (!s)
         return fail(OL_ERR_NOMEM, "out of host memory");
@@ -1891,9 +1935,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     for (uint32_t f = 1; f <= 4; f++)
         if (!filt[f].empty())
             std::memcpy(host.data() + o_f + (size_t)dd.foff[f] * 4, filt[f].data(), filt[f].size() * 4);
-    for (uint32_t k = 1; k <= 5; k++)
-        if (!tab[k].empty())
-            std::memcpy(host.data() + o_t[k], tab[k].data(), tab[k].size() * 4);
+    std::memcpy(host.data() + o_t, tab.data(), tab.size() * 4);
     std::memcpy(host.data() + o_ent, ents.data(), ents.size() * 4);
     std::memcpy(host.data() + o_pool, pool.data(), pool.size() * 4);
     if (hipMemcpy(base, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1902,8 +1944,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         return fail(OL_ERR_DEVICE, "dictionary upload failed");
     }
     dd.filter = (const uint32_t *)(base + o_f);
-    for (uint32_t k = 1; k <= 5; k++)
-        dd.table[k] = (const uint4 *)(base + (tab[k].empty() ? o_zero : o_t[k]));
+    dd.table = (const uint4 *)(base + o_t);
     dd.ents = (const uint4 *)(base + o_ent);
     dd.pool = (const uint32_t *)(base + o_pool);
     s->d = dd;

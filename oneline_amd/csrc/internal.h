@@ -339,25 +339,52 @@ struct SearchArgs {
     uint64_t *meta;          // meta[0] <- total matches
 };
 
-// Compiled dictionary.  Patterns are keyed by class: 1, 2 and 3 for patterns
-// of that length (key = all their bytes), 4 for lengths 4..7 (key = first 4
-// bytes), 5 for lengths >= 8 (key = first 8 bytes).  Per class an
-// open-addressing table of {key_lo, begin, end, key_hi} slots (end == 0:
-// empty) maps a key to its run of entries, sorted by pattern id.  LDS filters:
-// class 1 and 2 direct bitmaps (indices 1, 2), and one prefix bitmap for
-// classes 3, 4 and 5 (index 4): the 4-byte key with the fourth byte cut to its
-// low 5 bits, hashed (dict_ph); a class-3 key sets all 32 of its fourth bytes.
+// Compiled dictionary.  Every pattern has a key class and a key: DK_LONG for
+// patterns of 4 bytes and more (key: their first 4 bytes), DK_3 / DK_2 / DK_1
+// for patterns of 3, 2 and 1 bytes (key: all their bytes).  Patterns with the
+// same class and key form a run (sorted by id).  One table maps (class, key)
+// to a slot; a slot lives in one of its key's two buckets (dk_bucket1,
+// dk_bucket2) of DK_BUCKET 16-byte slots (one 64-byte line), so a lookup reads
+// two lines, both at once:
+//   slot = {key, info, a, b}; info = id | len << 22 | DK_VALID | class << 30;
+//   one pattern (len > 0): a = its bytes 4..7, b = bytes 8..11 when it ends by
+//     byte 12, else the pool offset of its bytes 8..; decided from the slot
+//     alone up to 12 bytes;
+//   a run (len == 0): a = first entry, b = its length; entries are
+//     {info, a, b, 0} of each pattern, in id order.
+// LDS filters (unchanged): DK_1 and DK_2 direct bitmaps (indices 1, 2), and
+// one prefix bitmap for DK_3 and DK_LONG (index 4): the 4-byte window with the
+// fourth byte cut to its low 5 bits, hashed (dict_ph); a 3-byte key sets all
+// 32 of its fourth bytes.
+enum : uint32_t { DK_LONG = 0, DK_3 = 1, DK_2 = 2, DK_1 = 3 };
+constexpr uint32_t DK_BUCKET = 4;
+constexpr uint32_t DK_LEN_SHIFT = 22, DK_VALID = 1u << 29, DK_CLASS_SHIFT = 30;
+__host__ __device__ __forceinline__ uint32_t dk_info(uint32_t id, uint32_t len, uint32_t cls)
+{
+    return id | len << DK_LEN_SHIFT | DK_VALID | cls << DK_CLASS_SHIFT;
+}
+__host__ __device__ __forceinline__ uint32_t dk_bucket1(uint32_t key, uint32_t cls, uint32_t bits)
+{
+    return ((key ^ cls * 0x27D4EB2Fu) * 0x9E3779B1u) >> (32 - bits);
+}
+__host__ __device__ __forceinline__ uint32_t dk_bucket2(uint32_t key, uint32_t cls, uint32_t bits)
+{
+    uint32_t x = key ^ (cls + 1u) * 0x165667B1u;
+    x ^= x >> 15;
+    const uint32_t b = (x * 0x85EBCA6Bu) >> (32 - bits);
+    return b ^ (b == dk_bucket1(key, cls, bits) ? 1u : 0u);   // never the first bucket
+}
+
 struct DictDev {
-    uint32_t class_mask;     // bit k set if class k (1..5) is non-empty
+    uint32_t class_mask;     // bit k set if class k (DK_*) has patterns
     uint32_t fbits[6];       // log2 bits of the filters 1, 2 and 4 (0: absent)
     uint32_t foff[6];        // word offset of each filter in `filter`
     uint32_t fwords;         // total filter words
-    uint32_t tbits[6];       // log2 table slots per class 1..5
+    uint32_t bbits;          // log2 buckets of the table
     const uint32_t *filter;
-    const uint4 *table[6];
-    const uint4 *ents;       // per entry {pattern id, length, word offset in pool, check word}: the
-                             // check word holds pattern bytes key..key+3 (zero past the end)
-    const uint32_t *pool;    // pattern bytes, each pattern starting on a word, zero padded
+    const uint4 *table;      // (DK_BUCKET << bbits) slots
+    const uint4 *ents;       // run entries
+    const uint32_t *pool;    // pattern bytes 8.. of patterns longer than 12 bytes, word aligned, zero padded
     uint32_t max_len;
 };
 
@@ -374,12 +401,6 @@ __host__ __device__ __forceinline__ uint32_t dict_mul24(uint32_t a, uint32_t b)
 constexpr uint32_t DICT_PBITS = 19;
 __host__ __device__ __forceinline__ uint32_t dict_ph(uint32_t w) { return dict_mul24(w, 0x9E3779u) + (w << 3); }
 __host__ __device__ __forceinline__ uint32_t dict_ph_word(uint32_t h) { return h >> (32 - (DICT_PBITS - 5)); }
-__host__ __device__ __forceinline__ uint32_t dict_thash(uint32_t lo, uint32_t hi, uint32_t bits)
-{
-    uint32_t x = lo ^ (hi * 0x9E3779B1u);
-    x ^= x >> 15;
-    return (x * 0x85EBCA6Bu) >> (32 - bits);
-}
 
 struct ManyArgs {
     const uint8_t *text;

@@ -384,12 +384,44 @@ __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
 
 // Matches starting at tile position lp (global gp) with at most `rem` bytes
 // (1..64) before the text or document end; written from out[r] in pattern-id
-// order (up to cap) when `emit`.  Returns their number.  GATE: read the
-// tables of classes 3..5 only where the prefix filter (s_filter) passes; a
-// candidate that passed it needs no gate.  KEEP: the ids of the first
+// order (up to cap) when `emit`.  Returns their number.  GATE: look up the
+// classes of 3 bytes and more only where the prefix filter (s_filter) passes;
+// a candidate that passed it needs no gate.  KEEP: the ids of the first
 // MANY_KEEP matches go to keep[].  SMALL: the dictionary may have 1- and
-// 2-byte patterns (without them their classes' code is compiled out).
+// 2-byte patterns (without them their lookups are compiled out).
+//
+// One lookup per key class present: the key's two buckets (two 64-byte lines)
+// are read for every class at once, with the text's bytes 0..15 at the
+// position.  A slot holding one pattern decides it from its own words (up to
+// 12 bytes; longer ones read their bytes 8.. from the pool); a run's entries
+// are read one after another, the runs of the classes merged by pattern id.
 constexpr uint32_t MANY_KEEP = 4;
+
+// The slot of (cls, key) among the 8 slots of its two buckets (info 0: none).
+// The 8 loads are independent, so they are in flight together (an unused
+// class reads bucket 0 and is masked afterwards).
+__device__ __forceinline__ uint4 dk_lookup(const DictDev &d, uint32_t cls, uint32_t key, bool use)
+{
+    const uint32_t b1 = use ? dk_bucket1(key, cls, d.bbits) : 0u, b2 = use ? dk_bucket2(key, cls, d.bbits) : 0u;
+    const uint4 *const t1 = d.table + b1 * DK_BUCKET, *const t2 = d.table + b2 * DK_BUCKET;
+    static_assert(DK_BUCKET == 4, "two buckets of four slots");
+    // named loads, all issued before any compare; bitwise tests and
+    // per-word selects, so nothing branches on a loaded value
+    const uint4 v0 = t1[0], v1 = t1[1], v2 = t1[2], v3 = t1[3], v4 = t2[0], v5 = t2[1], v6 = t2[2], v7 = t2[3];
+    uint32_t fx = 0, fy = 0, fz = 0, fw = 0;
+#define DK_SEL(V)                                                                           \
+    {                                                                                        \
+        const bool h = (V.x == key) & ((V.y & DK_VALID) != 0u) & ((V.y >> DK_CLASS_SHIFT) == cls); \
+        fx = h ? V.x : fx;                                                                   \
+        fy = h ? V.y : fy;                                                                   \
+        fz = h ? V.z : fz;                                                                   \
+        fw = h ? V.w : fw;                                                                   \
+    }
+    DK_SEL(v0) DK_SEL(v1) DK_SEL(v2) DK_SEL(v3) DK_SEL(v4) DK_SEL(v5) DK_SEL(v6) DK_SEL(v7)
+#undef DK_SEL
+    return make_uint4(fx, use ? fy : 0u, fz, fw);
+}
+
 template <bool GATE, bool KEEP = false, bool SMALL = true>
 __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
                                                    uint32_t lp, uint64_t gp, uint32_t rem, bool emit, uint64_t *out_off,
@@ -399,47 +431,35 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
     // s_text: the tile's first byte in global memory (word aligned)
     const uint32_t *wp = (const uint32_t *)(s_text + (lp & ~3u));
     const uint32_t sh = lp & 3u;
-    const uint32_t a0 = wp[0], a1 = wp[1], a2 = wp[2];
-    const uint32_t w0 = sh ? __builtin_amdgcn_alignbyte(a1, a0, sh) : a0;
-    const uint32_t w1 = sh ? __builtin_amdgcn_alignbyte(a2, a1, sh) : a1;
-    // One run per class (1..5): the patterns whose key matches the text here.
-    // Named variables per class (no arrays: nothing may be indexed at run time).
-    // First the table slots of every class are loaded together (absent
-    // classes read an empty slot), then resolved.
-    bool pass345 = true;
+    const uint32_t a0 = wp[0], a1 = wp[1], a2 = wp[2], a3 = wp[3];
+    const uint32_t T0 = __builtin_amdgcn_alignbyte(a1, a0, sh);
+    const uint32_t T1 = __builtin_amdgcn_alignbyte(a2, a1, sh);
+    const uint32_t T2 = __builtin_amdgcn_alignbyte(a3, a2, sh);
+    bool pass = true;
     if (GATE && d.fbits[4]) {
-        const uint32_t ph = dict_ph(w0);
-        pass345 = __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
+        const uint32_t ph = dict_ph(T0);
+        pass = __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
     }
-#define OL_KEY(K, LO)                                                                       \
-    const uint32_t klo##K = (LO), khi##K = (K) == 5 ? w1 : 0u;                              \
-    const bool use##K = ((K) >= 3 || SMALL) && ((d.class_mask >> (K)) & 1u) &&               \
-                        ((K) <= 4 ? (K) : 8u) <= rem &&                                      \
-                        ((K) >= 3 ? pass345 : true);                                         \
-    uint32_t h##K = use##K ? dict_thash(klo##K, khi##K, d.tbits[K]) : 0u;                  \
-    const uint4 t##K = d.table[K][h##K];   /* unconditional: all classes' loads in flight */   \
-    const uint32_t m##K = use##K ? ~0u : 0u;   /* masked, not selected: keeps the load */      \
-    uint4 v##K = make_uint4(t##K.x & m##K, t##K.y & m##K, t##K.z & m##K, t##K.w & m##K);
-    OL_KEY(1, w0 & 0xFFu)
-    OL_KEY(2, w0 & 0xFFFFu)
-    OL_KEY(3, w0 & 0xFFFFFFu)
-    OL_KEY(4, w0)
-    OL_KEY(5, w0)
-#undef OL_KEY
-#define OL_RUN(K)                                                                           \
-    while (v##K.z != 0 && !(v##K.x == klo##K && v##K.w == khi##K)) {                        \
-        h##K = (h##K + 1) & ((1u << d.tbits[K]) - 1u);                                       \
-        v##K = d.table[K][h##K];                                                             \
-    }                                                                                        \
-    uint32_t b##K = v##K.y, e##K = v##K.z != 0 ? v##K.z : v##K.y;
-    OL_RUN(1)
-    OL_RUN(2)
-    OL_RUN(3)
-    OL_RUN(4)
-    OL_RUN(5)
-#undef OL_RUN
-    if (b1 >= e1 && b2 >= e2 && b3 >= e3 && b4 >= e4 && b5 >= e5)
-        return 0;
+    const uint32_t cm = d.class_mask;
+    const bool useL = ((cm >> DK_LONG) & 1u) && rem >= 4 && pass;
+    const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass;
+    const bool use2 = SMALL && ((cm >> DK_2) & 1u) && rem >= 2;
+    const bool use1 = SMALL && ((cm >> DK_1) & 1u) && rem >= 1;
+    const uint4 fL = dk_lookup(d, DK_LONG, T0, useL), f3 = dk_lookup(d, DK_3, T0 & 0xFFFFFFu, use3);
+    const uint4 f2 = SMALL ? dk_lookup(d, DK_2, T0 & 0xFFFFu, use2) : make_uint4(0, 0, 0, 0);
+    const uint4 f1 = SMALL ? dk_lookup(d, DK_1, T0 & 0xFFu, use1) : make_uint4(0, 0, 0, 0);
+    // Cursor per class: the head pattern {info, a, b}, and for a run the next
+    // entry and how many remain.  A run's first entry is read for every class
+    // at once.
+    auto is_run = [](const uint4 &f) { return (f.y & DK_VALID) && ((f.y >> DK_LEN_SHIFT) & 0x7Fu) == 0; };
+#define OL_CUR(K)                                                                           \
+    const bool run##K = is_run(f##K);                                                        \
+    const uint4 e##K = d.ents[run##K ? f##K.z : 0u];                                         \
+    uint32_t hi##K = run##K ? e##K.x : f##K.y, ha##K = run##K ? e##K.y : f##K.z,             \
+             hb##K = run##K ? e##K.z : f##K.w;                                               \
+    uint32_t nx##K = run##K ? f##K.z + 1u : 0u, left##K = run##K ? f##K.w - 1u : 0u;
+    OL_CUR(L) OL_CUR(3) OL_CUR(2) OL_CUR(1)
+#undef OL_CUR
     // text word at tile position x (any alignment)
     auto tword = [&](uint32_t x) -> uint32_t {
         const uint32_t *q = (const uint32_t *)(s_text + (x & ~3u));
@@ -447,49 +467,52 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
         return sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
     };
     auto tail_mask = [](uint32_t nb) -> uint32_t { return nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u); };
-    const uint4 none = make_uint4(0xFFFFFFFFu, 0, 0, 0);
-    // each run's first entry, loaded unconditionally (entry 0 always exists)
-    // so that the loads are in flight together
-    const uint4 f1 = d.ents[b1 < e1 ? b1 : 0u], f2 = d.ents[b2 < e2 ? b2 : 0u], f3 = d.ents[b3 < e3 ? b3 : 0u],
-                f4 = d.ents[b4 < e4 ? b4 : 0u], f5 = d.ents[b5 < e5 ? b5 : 0u];
-    auto first_or_none = [&](const uint4 &f, bool has) -> uint4 {   // masked, not selected: keeps the load
-        const uint32_t m = has ? ~0u : 0u;
-        return make_uint4((f.x & m) | ~m, f.y & m, f.z & m, f.w & m);
-    };
-    uint4 c1 = first_or_none(f1, b1 < e1), c2 = first_or_none(f2, b2 < e2), c3 = first_or_none(f3, b3 < e3),
-          c4 = first_or_none(f4, b4 < e4), c5 = first_or_none(f5, b5 < e5);
     uint32_t count = 0;
     while (true) {
-        // the run whose next entry has the smallest pattern id (runs are sorted by id)
+        // the class whose head has the smallest pattern id
         uint32_t bid = 0xFFFFFFFFu, bk = 0;
-#define OL_PICK(K)                                                                          \
-        if (b##K < e##K && c##K.x < bid) {                                                   \
-            bid = c##K.x;                                                                    \
-            bk = (K);                                                                        \
+#define OL_PICK(K, N)                                                                       \
+        if ((hi##K & DK_VALID) && (hi##K & 0x3FFFFFu) < bid) {                               \
+            bid = hi##K & 0x3FFFFFu;                                                         \
+            bk = (N);                                                                        \
         }
-        OL_PICK(1) OL_PICK(2) OL_PICK(3) OL_PICK(4) OL_PICK(5)
+        OL_PICK(L, 1) OL_PICK(3, 2) OL_PICK(2, 3) OL_PICK(1, 4)
 #undef OL_PICK
         if (!bk)
             break;
-        uint4 en = none;
-        uint32_t kb = 0;
-#define OL_ADV(K)                                                                           \
-        if (bk == (K)) {                                                                     \
-            en = c##K;                                                                       \
-            kb = (K) <= 4 ? (K) : 8u;                                                        \
-            if (++b##K < e##K)                                                               \
-                c##K = d.ents[b##K];                                                         \
+        uint32_t info = 0, pa = 0, pb = 0;
+#define OL_ADV(K, N)                                                                        \
+        if (bk == (N)) {                                                                     \
+            info = hi##K;                                                                    \
+            pa = ha##K;                                                                      \
+            pb = hb##K;                                                                      \
+            if (left##K) {                                                                   \
+                const uint4 en = d.ents[nx##K++];                                            \
+                left##K--;                                                                   \
+                hi##K = en.x;                                                                \
+                ha##K = en.y;                                                                \
+                hb##K = en.z;                                                                \
+            } else {                                                                         \
+                hi##K = 0;                                                                   \
+            }                                                                                \
         }
-        OL_ADV(1) OL_ADV(2) OL_ADV(3) OL_ADV(4) OL_ADV(5)
+        OL_ADV(L, 1) OL_ADV(3, 2) OL_ADV(2, 3) OL_ADV(1, 4)
 #undef OL_ADV
-        const uint32_t len = en.y;
+        const uint32_t len = (info >> DK_LEN_SHIFT) & 0x7Fu;
         if (len > rem)
             continue;
         bool ok = true;
-        if (len > kb) {
-            ok = ((tword(lp + kb) ^ en.w) & tail_mask(len - kb)) == 0;
-            for (uint32_t j = kb + 4; ok && j < len; j += 4)
-                ok = ((tword(lp + j) ^ d.pool[en.z + (j >> 2)]) & tail_mask(len - j)) == 0;
+        if (bk == 1 && len > 4) {   // a pattern of 5 bytes or more: bytes 4.. against the text
+            ok = ((T1 ^ pa) & tail_mask(len - 4)) == 0;
+            if (ok && len > 8) {
+                if (len <= 12) {
+                    ok = ((T2 ^ pb) & tail_mask(len - 8)) == 0;
+                } else {
+                    ok = ((T2 ^ d.pool[pb]) & tail_mask(len - 8)) == 0;
+                    for (uint32_t j = 12; ok && j < len; j += 4)
+                        ok = ((tword(lp + j) ^ d.pool[pb + ((j - 8) >> 2)]) & tail_mask(len - j)) == 0;
+                }
+            }
         }
         if (!ok)
             continue;
@@ -505,6 +528,97 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
         }
         count++;
     }
+    return count;
+}
+
+
+// The count pass's probe: the number of matches at tile position lp and the
+// ids of the MANY_KEEP smallest (keep[], ascending; ~0u past the count).
+// Order matters only for those ids, which are kept in four sorted registers,
+// so each class's run is walked on its own, the next entry's load issued
+// before the current one is verified (the write pass emits in id order with
+// probe_position).  GATE and SMALL as for probe_position.
+template <bool GATE, bool SMALL>
+__device__ __forceinline__ uint32_t probe_count_keep(const DictDev &d, const uint32_t *s_filter, const uint8_t *s_text,
+                                                     uint32_t lp, uint32_t rem, uint32_t *keep)
+{
+    const uint32_t *wp = (const uint32_t *)(s_text + (lp & ~3u));
+    const uint32_t sh = lp & 3u;
+    const uint32_t a0 = wp[0], a1 = wp[1], a2 = wp[2], a3 = wp[3];
+    const uint32_t T0 = __builtin_amdgcn_alignbyte(a1, a0, sh);
+    const uint32_t T1 = __builtin_amdgcn_alignbyte(a2, a1, sh);
+    const uint32_t T2 = __builtin_amdgcn_alignbyte(a3, a2, sh);
+    bool pass = true;
+    if (GATE && d.fbits[4]) {
+        const uint32_t ph = dict_ph(T0);
+        pass = __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
+    }
+    const uint32_t cm = d.class_mask;
+    const bool useL = ((cm >> DK_LONG) & 1u) && rem >= 4 && pass;
+    const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass;
+    const bool use2 = SMALL && ((cm >> DK_2) & 1u) && rem >= 2;
+    const bool use1 = SMALL && ((cm >> DK_1) & 1u) && rem >= 1;
+    const uint4 fL = dk_lookup(d, DK_LONG, T0, useL), f3 = dk_lookup(d, DK_3, T0 & 0xFFFFFFu, use3);
+    const uint4 f2 = SMALL ? dk_lookup(d, DK_2, T0 & 0xFFFFu, use2) : make_uint4(0, 0, 0, 0);
+    const uint4 f1 = SMALL ? dk_lookup(d, DK_1, T0 & 0xFFu, use1) : make_uint4(0, 0, 0, 0);
+    auto tword = [&](uint32_t x) -> uint32_t {
+        const uint32_t *q = (const uint32_t *)(s_text + (x & ~3u));
+        const uint32_t sx = x & 3u;
+        return sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
+    };
+    auto tail_mask = [](uint32_t nb) -> uint32_t { return nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u); };
+    uint32_t count = 0, k0 = ~0u, k1 = ~0u, k2 = ~0u, k3 = ~0u;
+    // a pattern {info, a, b} of class `cls` at this position: verify, count,
+    // and insert its id into the sorted four
+    auto consider = [&](uint32_t info, uint32_t pa, uint32_t pb, bool is_long) {
+        const uint32_t len = (info >> DK_LEN_SHIFT) & 0x7Fu;
+        bool ok = len <= rem;
+        if (ok && is_long && len > 4) {
+            ok = ((T1 ^ pa) & tail_mask(len - 4)) == 0;
+            if (ok && len > 8) {
+                if (len <= 12) {
+                    ok = ((T2 ^ pb) & tail_mask(len - 8)) == 0;
+                } else {
+                    ok = ((T2 ^ d.pool[pb]) & tail_mask(len - 8)) == 0;
+                    for (uint32_t j = 12; ok && j < len; j += 4)
+                        ok = ((tword(lp + j) ^ d.pool[pb + ((j - 8) >> 2)]) & tail_mask(len - j)) == 0;
+                }
+            }
+        }
+        if (!ok)
+            return;
+        count++;
+        uint32_t x = info & 0x3FFFFFu;
+        const uint32_t m0 = min(k0, x); x = max(k0, x); k0 = m0;
+        const uint32_t m1 = min(k1, x); x = max(k1, x); k1 = m1;
+        const uint32_t m2 = min(k2, x); x = max(k2, x); k2 = m2;
+        k3 = min(k3, x);
+    };
+    auto walk = [&](const uint4 &f, bool is_long) {
+        if (!(f.y & DK_VALID))
+            return;
+        if ((f.y >> DK_LEN_SHIFT) & 0x7Fu) {   // one pattern, in the slot
+            consider(f.y, f.z, f.w, is_long);
+            return;
+        }
+        const uint32_t b = f.z, n = f.w;
+        uint4 e = d.ents[b];
+        for (uint32_t i = 0; i < n; i++) {
+            const uint4 nxt = d.ents[b + (i + 1 < n ? i + 1 : i)];   // in flight during the check
+            consider(e.x, e.y, e.z, is_long);
+            e = nxt;
+        }
+    };
+    walk(fL, true);
+    walk(f3, false);
+    if (SMALL) {
+        walk(f2, false);
+        walk(f1, false);
+    }
+    keep[0] = k0;
+    keep[1] = k1;
+    keep[2] = k2;
+    keep[3] = k3;
     return count;
 }
 
@@ -694,7 +808,7 @@ __device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyLan
 
 // Count pass: every wave takes wave tiles wt = wave + k * waves.
 template <int THREADS, int FS>
-__global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void many_count_kernel(ManyArgs a)
 {
     constexpr int NWV = THREADS / 64;
     __shared__ __align__(16) uint32_t s_pf[(FS & 0x10) ? MANY_PWORDS : 1];
@@ -739,9 +853,7 @@ __global__ __launch_bounds__(THREADS) void many_count_kernel(ManyArgs a)
                     lp = l_pos[i];
                     rm = l_rem[i];
                     if (rm)
-                        c = probe_position<(FS & 0x02) != 0, true, (FS & 0x02) != 0>(d, pf, m.g_text, lp,
-                                                                                      (uint64_t)m.base + lp, rm,
-                                                                    false, nullptr, nullptr, 0, 0, ids);
+                        c = probe_count_keep<(FS & 0x02) != 0, (FS & 0x02) != 0>(d, pf, m.g_text, lp, rm, ids);
                 }
                 tile_total += c;
                 if (store) {
