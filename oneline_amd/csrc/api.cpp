@@ -1817,11 +1817,11 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     pool.resize(pool.size() + 16, 0);
     if (ents.empty())
         ents.assign(4, 0);
-    // Buckets of DK_BUCKET slots; each slot goes to one of its key's two
-    // buckets (the emptier one, moving residents to their other bucket when
-    // both are full); the table doubles until every slot has a place.
+    // Cuckoo placement: each slot goes to one of its key's two buckets (an
+    // empty one, else it displaces a resident to that one's other bucket);
+    // the table (at most about 45% full) doubles until every slot has a place.
     uint32_t bbits = 2;
-    while ((size_t)DK_BUCKET << bbits < slots.size() * 2)
+    while (((size_t)DK_BUCKET << bbits) * 9 < slots.size() * 20)
         bbits++;
     std::vector<int32_t> place;   // slot index per table position, -1 empty
     for (;; bbits++) {

@@ -343,21 +343,21 @@ struct SearchArgs {
 // patterns of 4 bytes and more (key: their first 4 bytes), DK_3 / DK_2 / DK_1
 // for patterns of 3, 2 and 1 bytes (key: all their bytes).  Patterns with the
 // same class and key form a run (sorted by id).  One table maps (class, key)
-// to a slot; a slot lives in one of its key's two buckets (dk_bucket1,
-// dk_bucket2) of DK_BUCKET 16-byte slots (one 64-byte line), so a lookup reads
-// two lines, both at once:
+// to a 16-byte slot at one of two places (cuckoo hashing: dk_bucket1,
+// dk_bucket2), so a lookup is two 16-byte loads, both at once (a lane's load
+// instructions, not its bytes, are what the L1 serves one by one):
 //   slot = {key, info, a, b}; info = id | len << 22 | DK_VALID | class << 30;
 //   one pattern (len > 0): a = its bytes 4..7, b = bytes 8..11 when it ends by
 //     byte 12, else the pool offset of its bytes 8..; decided from the slot
 //     alone up to 12 bytes;
 //   a run (len == 0): a = first entry, b = its length; entries are
 //     {info, a, b, 0} of each pattern, in id order.
-// LDS filters (unchanged): DK_1 and DK_2 direct bitmaps (indices 1, 2), and
+// LDS filters: DK_1 and DK_2 direct bitmaps (indices 1, 2), and
 // one prefix bitmap for DK_3 and DK_LONG (index 4): the 4-byte window with the
 // fourth byte cut to its low 5 bits, hashed (dict_ph); a 3-byte key sets all
 // 32 of its fourth bytes.
 enum : uint32_t { DK_LONG = 0, DK_3 = 1, DK_2 = 2, DK_1 = 3 };
-constexpr uint32_t DK_BUCKET = 4;
+constexpr uint32_t DK_BUCKET = 1;   // slots per bucket
 constexpr uint32_t DK_LEN_SHIFT = 22, DK_VALID = 1u << 29, DK_CLASS_SHIFT = 30;
 __host__ __device__ __forceinline__ uint32_t dk_info(uint32_t id, uint32_t len, uint32_t cls)
 {

@@ -390,24 +390,22 @@ __device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
 // MANY_KEEP matches go to keep[].  SMALL: the dictionary may have 1- and
 // 2-byte patterns (without them their lookups are compiled out).
 //
-// One lookup per key class present: the key's two buckets (two 64-byte lines)
-// are read for every class at once, with the text's bytes 0..15 at the
-// position.  A slot holding one pattern decides it from its own words (up to
+// One lookup per key class present: the key's two slots are read for every
+// class at once, with the text's bytes 0..15 at the position.  A slot holding one pattern decides it from its own words (up to
 // 12 bytes; longer ones read their bytes 8.. from the pool); a run's entries
 // are read one after another, the runs of the classes merged by pattern id.
 constexpr uint32_t MANY_KEEP = 4;
 
-// The slot of (cls, key) among the 8 slots of its two buckets (info 0: none).
-// The 8 loads are independent, so they are in flight together (an unused
-// class reads bucket 0 and is masked afterwards).
+// The slot of (cls, key) at its two places (info 0: none).  The loads are
+// independent, so they are in flight together (an unused class reads slot 0
+// and is masked afterwards).
 __device__ __forceinline__ uint4 dk_lookup(const DictDev &d, uint32_t cls, uint32_t key, bool use)
 {
+    static_assert(DK_BUCKET == 1, "one slot per bucket");
     const uint32_t b1 = use ? dk_bucket1(key, cls, d.bbits) : 0u, b2 = use ? dk_bucket2(key, cls, d.bbits) : 0u;
-    const uint4 *const t1 = d.table + b1 * DK_BUCKET, *const t2 = d.table + b2 * DK_BUCKET;
-    static_assert(DK_BUCKET == 4, "two buckets of four slots");
-    // named loads, all issued before any compare; bitwise tests and
-    // per-word selects, so nothing branches on a loaded value
-    const uint4 v0 = t1[0], v1 = t1[1], v2 = t1[2], v3 = t1[3], v4 = t2[0], v5 = t2[1], v6 = t2[2], v7 = t2[3];
+    // both loads issued before any compare; bitwise tests and per-word
+    // selects, so nothing branches on a loaded value
+    const uint4 v0 = d.table[b1], v1 = d.table[b2];
     uint32_t fx = 0, fy = 0, fz = 0, fw = 0;
 #define DK_SEL(V)                                                                           \
     {                                                                                        \
@@ -417,7 +415,7 @@ __device__ __forceinline__ uint4 dk_lookup(const DictDev &d, uint32_t cls, uint3
         fz = h ? V.z : fz;                                                                   \
         fw = h ? V.w : fw;                                                                   \
     }
-    DK_SEL(v0) DK_SEL(v1) DK_SEL(v2) DK_SEL(v3) DK_SEL(v4) DK_SEL(v5) DK_SEL(v6) DK_SEL(v7)
+    DK_SEL(v0) DK_SEL(v1)
 #undef DK_SEL
     return make_uint4(fx, use ? fy : 0u, fz, fw);
 }
@@ -534,6 +532,7 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
 
 // The count pass's probe: the number of matches at tile position lp and the
 // ids of the MANY_KEEP smallest (keep[], ascending; ~0u past the count).
+// s_filter is the prefix filter in LDS.
 // Order matters only for those ids, which are kept in four sorted registers,
 // so each class's run is walked on its own, the next entry's load issued
 // before the current one is verified (the write pass emits in id order with
@@ -548,14 +547,18 @@ __device__ __forceinline__ uint32_t probe_count_keep(const DictDev &d, const uin
     const uint32_t T0 = __builtin_amdgcn_alignbyte(a1, a0, sh);
     const uint32_t T1 = __builtin_amdgcn_alignbyte(a2, a1, sh);
     const uint32_t T2 = __builtin_amdgcn_alignbyte(a3, a2, sh);
-    bool pass = true;
-    if (GATE && d.fbits[4]) {
-        const uint32_t ph = dict_ph(T0);
-        pass = __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
-    }
+    // the prefix filter at the window and at two other values of its fourth
+    // byte's low bits: a 3-byte key sets all 32, so the 3-byte lookup is
+    // made only where all three are set
+    auto fbit = [&](uint32_t w) -> bool {
+        const uint32_t ph = dict_ph(w);
+        return __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
+    };
+    const bool has_pf = d.fbits[4] != 0;
+    const bool pass = !(GATE && has_pf) || fbit(T0);
     const uint32_t cm = d.class_mask;
     const bool useL = ((cm >> DK_LONG) & 1u) && rem >= 4 && pass;
-    const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass;
+    const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass && (!has_pf || (fbit(T0 ^ 1u << 24) && fbit(T0 ^ 2u << 24)));
     const bool use2 = SMALL && ((cm >> DK_2) & 1u) && rem >= 2;
     const bool use1 = SMALL && ((cm >> DK_1) & 1u) && rem >= 1;
     const uint4 fL = dk_lookup(d, DK_LONG, T0, useL), f3 = dk_lookup(d, DK_3, T0 & 0xFFFFFFu, use3);
@@ -570,28 +573,23 @@ __device__ __forceinline__ uint32_t probe_count_keep(const DictDev &d, const uin
     uint32_t count = 0, k0 = ~0u, k1 = ~0u, k2 = ~0u, k3 = ~0u;
     // a pattern {info, a, b} of class `cls` at this position: verify, count,
     // and insert its id into the sorted four
+    // (selects, no branches, except for the pool words of patterns longer
+    // than 12 bytes whose bytes 4..11 already match)
     auto consider = [&](uint32_t info, uint32_t pa, uint32_t pb, bool is_long) {
         const uint32_t len = (info >> DK_LEN_SHIFT) & 0x7Fu;
-        bool ok = len <= rem;
-        if (ok && is_long && len > 4) {
-            ok = ((T1 ^ pa) & tail_mask(len - 4)) == 0;
-            if (ok && len > 8) {
-                if (len <= 12) {
-                    ok = ((T2 ^ pb) & tail_mask(len - 8)) == 0;
-                } else {
-                    ok = ((T2 ^ d.pool[pb]) & tail_mask(len - 8)) == 0;
-                    for (uint32_t j = 12; ok && j < len; j += 4)
-                        ok = ((tword(lp + j) ^ d.pool[pb + ((j - 8) >> 2)]) & tail_mask(len - j)) == 0;
-                }
-            }
+        const uint32_t m1 = is_long && len > 4 ? tail_mask(len - 4) : 0u;
+        const uint32_t m2 = is_long && len > 8 && len <= 12 ? tail_mask(len - 8) : 0u;
+        bool ok = (len <= rem) & (((T1 ^ pa) & m1) == 0u) & (((T2 ^ pb) & m2) == 0u);
+        if (is_long && len > 12 && ok) {
+            ok = ((T2 ^ d.pool[pb]) & tail_mask(len - 8)) == 0;
+            for (uint32_t j = 12; ok && j < len; j += 4)
+                ok = ((tword(lp + j) ^ d.pool[pb + ((j - 8) >> 2)]) & tail_mask(len - j)) == 0;
         }
-        if (!ok)
-            return;
-        count++;
-        uint32_t x = info & 0x3FFFFFu;
-        const uint32_t m0 = min(k0, x); x = max(k0, x); k0 = m0;
-        const uint32_t m1 = min(k1, x); x = max(k1, x); k1 = m1;
-        const uint32_t m2 = min(k2, x); x = max(k2, x); k2 = m2;
+        count += ok ? 1u : 0u;
+        uint32_t x = ok ? (info & 0x3FFFFFu) : ~0u;   // ~0u leaves the four as they are
+        const uint32_t n0 = min(k0, x); x = max(k0, x); k0 = n0;
+        const uint32_t n1 = min(k1, x); x = max(k1, x); k1 = n1;
+        const uint32_t n2 = min(k2, x); x = max(k2, x); k2 = n2;
         k3 = min(k3, x);
     };
     auto walk = [&](const uint4 &f, bool is_long) {
@@ -601,12 +599,15 @@ __device__ __forceinline__ uint32_t probe_count_keep(const DictDev &d, const uin
             consider(f.y, f.z, f.w, is_long);
             return;
         }
-        const uint32_t b = f.z, n = f.w;
-        uint4 e = d.ents[b];
-        for (uint32_t i = 0; i < n; i++) {
-            const uint4 nxt = d.ents[b + (i + 1 < n ? i + 1 : i)];   // in flight during the check
-            consider(e.x, e.y, e.z, is_long);
-            e = nxt;
+        // four entries per step, their loads in flight together
+        const uint32_t b = f.z, n = f.w, last = f.z + f.w - 1u;
+        for (uint32_t i = 0; i < n; i += 4) {
+            const uint4 e0 = d.ents[min(b + i, last)], e1 = d.ents[min(b + i + 1, last)],
+                        e2 = d.ents[min(b + i + 2, last)], e3 = d.ents[min(b + i + 3, last)];
+            consider(e0.x, e0.y, e0.z, is_long);
+            if (i + 1 < n) consider(e1.x, e1.y, e1.z, is_long);
+            if (i + 2 < n) consider(e2.x, e2.y, e2.z, is_long);
+            if (i + 3 < n) consider(e3.x, e3.y, e3.z, is_long);
         }
     };
     walk(fL, true);
