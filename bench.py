@@ -40,7 +40,7 @@ def log(*a):
 
 def cpu_baseline(p, doc, tree, text, args):
     from oracle import pyoracle as orc
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = cpu_threads(args)
     n = text.size
     sample = text[: min(args.cpu_sample, n)]
     if sample.size < n:   # whole lines only
@@ -102,6 +102,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 31, help="CPU baseline bytes (a prefix of the document)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the FlatNode + subtree_end variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload (10,000 documents)")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch geometry")
     ap.add_argument("--batch-docs", type=int, default=10000)
@@ -234,6 +235,35 @@ def main():
         verified = True
         log("[bench] timed step's outputs bit-exact against the oracle")
 
+    # the drop-in variant: the reference's outputs as it returns them -- the
+    # 16-byte FlatNode per line (parse_raw / upload) and the tree with
+    # subtree_end -- beside offsets, classes and the other three links
+    dropin = None
+    if not args.no_dropin:
+        docn = p.upload(text, nodes=True)
+        treen = p.build_tree(docn)
+
+        def step_dropin():
+            ol._check(lib.ol_doc_reparse(p._ctx, docn._h))
+            ol._check(lib.ol_tree_rebuild_links_async(p._ctx, docn._h, treen._h, ol.LINK_ALL))
+
+        for _ in range(args.warmup):
+            step_dropin()
+        p.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step_dropin()
+        p.synchronize()
+        p.check_tree(treen)
+        dms = (time.perf_counter() - t0) / args.steps * 1e3
+        db = n + 37 * L   # offsets 4 + class 1 + FlatNode 16 + four links 16, per line
+        dropin = {"outputs": "offsets, classes, 16-byte FlatNodes, parent, subtree_end, first_child, next_sibling",
+                  "ms_per_step": round(dms, 4), "algorithmic_bytes": db, "gbs": round(db / (dms / 1e3) / 1e9, 1),
+                  "frac_of_hbm_peak": round(db / (dms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] drop-in variant (FlatNodes + subtree_end): {dms:.3f} ms/step, {dropin['gbs']} GB/s")
+        treen.close()
+        docn.close()
+
     if args.copy_calibration:
         copy_calibration(1 << 30)
 
@@ -246,7 +276,7 @@ def main():
     extra = {}
     if not args.no_search:
         which = set(x for x in args.search.split(",") if x)
-        extra = search_benchmarks(p, doc, text, which)
+        extra = search_benchmarks(p, doc, text, which, args)
         if "query" in which:
             extra["query"] = query_benchmark(p, doc, tree, L)
         if "olb" in which:
@@ -284,6 +314,7 @@ def main():
             "upload_scan_gbs_warm": round(n / upload2_s / 1e9, 2),
             "kernels": kernels,
             "tuning": tuning and {"scan_bpt": tuning["scan_bpt"], "tree_rounds": tuning["tree_rounds"]},
+            "dropin_variant": dropin,
         },
         "roofline": {
             "bound": "hbm",
@@ -562,13 +593,22 @@ def olb_benchmark(p, text, size=256 << 20):
     return res
 
 
-def search_benchmarks(p, doc, text, which):
-    """Literal and dictionary search over the resident document (device output)."""
+def cpu_threads(args):
+    return args.cpu_threads or min(16, os.cpu_count() or 1)
+
+
+def search_benchmarks(p, doc, text, which, args):
+    """Literal and dictionary search over the resident document (device
+    output).  Beside each, the oracle's restatement of the reference's CPU
+    search on the host cores (chunked memmem; the overlapping Aho-Corasick DFA),
+    checked against the GPU's matches before it is timed."""
     import ctypes
 
     import oneline_amd as ol
     from oneline_amd import datagen
+    from oracle import pyoracle as orc
 
+    threads = cpu_threads(args)
     lib = ol.load_library()
     hip = ctypes.CDLL("libamdhip64.so")
     n = text.size
@@ -598,11 +638,25 @@ def search_benchmarks(p, doc, text, which):
             prof = p.profile_take()
             k = prof.get("search", (1, 0.0))
             kms = k[1] / max(1, k[0])
-            b = n + 8 * tot.value
-            lits.append({"pattern": pat.decode(), "kind": ("vocabulary", "random")[kind], "matches": tot.value,
-                         "call_ms": round(dt * 1e3, 3), "kernel_ms": round(kms, 4),
-                         "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1),
-                         "frac_of_hbm_peak": round(b / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+            b = n + 12 * tot.value   # text + (u64 offset, u32 id) per match
+            rec = {"pattern": pat.decode(), "kind": ("vocabulary", "random")[kind], "matches": tot.value,
+                   "call_ms": round(dt * 1e3, 3), "kernel_ms": round(kms, 4),
+                   "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1),
+                   "frac_of_hbm_peak": round(b / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+            if not args.no_cpu:
+                # the reference's find_all_par, restated: agreement, then the median of 3
+                got = np.zeros(cap, np.uint64)
+                if cap:
+                    assert hip.hipMemcpy(got.ctypes.data_as(ctypes.c_void_p), d_off, ctypes.c_size_t(cap * 8), 2) == 0
+                assert np.array_equal(orc.find_all_par(text, pat, threads), got), f"CPU literal search disagrees ({pat})"
+                ts = []
+                for _ in range(3):
+                    t1 = time.perf_counter()
+                    orc.find_all_par(text, pat, threads)
+                    ts.append(time.perf_counter() - t1)
+                cms = sorted(ts)[1] * 1e3
+                rec.update({"cpu_ms": round(cms, 2), "cpu_gbs": round(n / (cms / 1e3) / 1e9, 2), "cpu_threads": threads})
+            lits.append(rec)
             hip.hipFree(d_off)
             log(f"[bench] literal {lits[-1]}")
     if lits:
@@ -627,9 +681,39 @@ def search_benchmarks(p, doc, text, which):
         k = prof.get("many", (1, 0.0))
         kms = k[1] / max(1, k[0])
         b = n + 12 * tot.value + sum(len(x) for x in pats)
-        res[f"dict_search_{count}"] = {"matches": tot.value, "call_ms": round(dt * 1e3, 3),
-                                       "kernel_ms": round(kms, 4), "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1),
-                                       "frac_of_hbm_peak": round(b / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        rec = {"matches": tot.value, "call_ms": round(dt * 1e3, 3),
+               "kernel_ms": round(kms, 4), "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1),
+               "frac_of_hbm_peak": round(b / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if not args.no_cpu:
+            # the reference's ManyPatterns (an Aho-Corasick DFA, overlapping),
+            # restated: the total over the whole text and every (start, id) of
+            # a 4 MiB window against the GPU's list, then the median of 3
+            # counting runs over the whole text (the reference's count_par)
+            ac = orc.AhoCorasick(pats)
+            assert ac.count(text, threads) == tot.value, f"CPU dictionary search disagrees ({count} patterns)"
+            w0, w1 = n // 3, n // 3 + (4 << 20)
+            got_off = np.zeros(cap, np.uint64)
+            assert hip.hipMemcpy(got_off.ctypes.data_as(ctypes.c_void_p), d_off, ctypes.c_size_t(cap * 8), 2) == 0
+            lo, hi = int(np.searchsorted(got_off, w0)), int(np.searchsorted(got_off, w1))
+            got_id = np.zeros(hi - lo, np.uint32)
+            if hi > lo:
+                assert hip.hipMemcpy(got_id.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(d_id.value + 4 * lo),
+                                     ctypes.c_size_t((hi - lo) * 4), 2) == 0
+            wo, wi = ac.find(text[w0:w1 + 63], threads)
+            keep = wo < (w1 - w0)
+            assert np.array_equal(got_off[lo:hi] - w0, wo[keep]) and np.array_equal(got_id, wi[keep]), \
+                f"CPU dictionary search disagrees in a window ({count} patterns)"
+            del got_off
+            ts = []
+            for _ in range(3):
+                t1 = time.perf_counter()
+                ac.count(text, threads)
+                ts.append(time.perf_counter() - t1)
+            ac.close()
+            cms = sorted(ts)[1] * 1e3
+            rec.update({"cpu_ms": round(cms, 2), "cpu_gbs": round(n / (cms / 1e3) / 1e9, 2), "cpu_threads": threads,
+                        "cpu_sample": "the whole document, every overlapping (start, pattern) counted"})
+        res[f"dict_search_{count}"] = rec
         hip.hipFree(d_off)
         hip.hipFree(d_id)
         pset.close()

@@ -15,10 +15,15 @@
  *                     (block-local stacks, a serial combine of the block
  *                     summaries into a persistent stack, a per-block resolve of
  *                     the lines whose answer lies before their block), then
- *                     first_child / next_sibling by their closed forms.
+ *                     first_child / next_sibling by their closed forms;
+ *   orc_find_all_par  -- the chunked literal search of crates/oneline-simd/src/search.rs:29-52;
+ *   orc_ac_*          -- the overlapping Aho-Corasick DFA search of
+ *                     crates/oneline-simd/src/search.rs:204-283 (the
+ *                     aho-corasick 1.1.5 crate there, restated here).
  *
  * Threads are plain pthreads, one per chunk / block range.
  */
+#define _GNU_SOURCE   /* memmem */
 #include <pthread.h>
 #include <stdint.h>
 #include <stddef.h>
@@ -357,4 +362,289 @@ int orc_links_par(const orc_node *nodes, size_t n, uint32_t *parent, uint32_t *s
     free(praw);
     free(rev);
     return 0;
+}
+
+/* ---- literal search: find_all_par ----------------------------------------
+ * crates/oneline-simd/src/search.rs:29-52: the text in chunks of
+ * max(n / (threads * 4), 1 MiB) bytes, each searched with a (len - 1)-byte
+ * halo past its end, keeping the starts inside the chunk; every overlapping
+ * occurrence.  The per-chunk search is glibc's memmem (the reference uses the
+ * memchr crate's memmem: a SIMD prefilter plus Two-Way), restarted one byte
+ * after each match. */
+#define MIN_CHUNK ((size_t)1 << 20)
+
+typedef struct {
+    const uint8_t *t, *pat;
+    size_t n, m, start, end;
+    uint64_t *hits;
+    size_t count, cap;
+} lit_job;
+
+static void *lit_job_run(void *p)
+{
+    lit_job *j = (lit_job *)p;
+    const size_t wend = j->end + j->m - 1 < j->n ? j->end + j->m - 1 : j->n;
+    const uint8_t *at = j->t + j->start, *lim = j->t + wend;
+    while (at + j->m <= lim) {
+        const uint8_t *q = (const uint8_t *)memmem(at, (size_t)(lim - at), j->pat, j->m);
+        if (!q || (size_t)(q - j->t) >= j->end)
+            break;
+        if (j->count == j->cap) {
+            size_t nc = j->cap ? 2 * j->cap : 1024;
+            uint64_t *h = (uint64_t *)realloc(j->hits, nc * sizeof(uint64_t));
+            if (!h)
+                break;
+            j->hits = h;
+            j->cap = nc;
+        }
+        j->hits[j->count++] = (uint64_t)(q - j->t);
+        at = q + 1;
+    }
+    return NULL;
+}
+
+/* Every start of `pat` in t (ascending); the first `cap` go to out; returns
+ * how many there are. */
+size_t orc_find_all_par(const uint8_t *t, size_t n, const uint8_t *pat, size_t m, int threads, uint64_t *out,
+                        size_t cap)
+{
+    if (m == 0 || n < m)
+        return 0;
+    const int nt = clamp_threads(threads);
+    size_t chunk = n / ((size_t)nt * 4);
+    if (chunk < MIN_CHUNK)
+        chunk = MIN_CHUNK;
+    const size_t nj = (n + chunk - 1) / chunk;
+    lit_job *jobs = (lit_job *)calloc(nj, sizeof(lit_job));
+    for (size_t k = 0; k < nj; k++) {
+        jobs[k].t = t;
+        jobs[k].pat = pat;
+        jobs[k].n = n;
+        jobs[k].m = m;
+        jobs[k].start = k * chunk;
+        jobs[k].end = (k + 1) * chunk < n ? (k + 1) * chunk : n;
+    }
+    for (size_t k0 = 0; k0 < nj; k0 += (size_t)nt) {
+        const int g = (int)((nj - k0) < (size_t)nt ? nj - k0 : (size_t)nt);
+        run_threads(g, lit_job_run, jobs + k0, sizeof(lit_job));
+    }
+    size_t total = 0;
+    for (size_t k = 0; k < nj; k++) {
+        for (size_t i = 0; i < jobs[k].count; i++, total++)
+            if (out && total < cap)
+                out[total] = jobs[k].hits[i];
+        free(jobs[k].hits);
+    }
+    free(jobs);
+    return total;
+}
+
+/* ---- dictionary search: an Aho-Corasick DFA, overlapping ------------------
+ * The CPU baseline of crates/oneline-simd/src/search.rs:204-283: the
+ * aho-corasick crate (1.1.5) built as a DFA with MatchKind::Standard and
+ * searched with find_overlapping_iter over chunks of max(n / (threads * 4),
+ * 1 MiB) bytes with a (longest - 1)-byte halo, keeping matches that start in
+ * the chunk; every (start, pattern) is reported, duplicates each with its own
+ * id.  Restated from the published algorithm (Aho & Corasick 1975): a trie of
+ * the patterns, failure links by breadth-first search, the transition table
+ * completed into a DFA over byte classes (the bytes the patterns use, plus one
+ * class for every other byte), and per state the patterns ending there plus a
+ * link to the nearest state on its failure chain that has outputs. */
+typedef struct orc_ac {
+    uint32_t nstates, nclass, longest;
+    uint8_t cls[256];
+    uint32_t *delta;       /* nstates x nclass */
+    uint32_t *out_first;   /* per state: first index into out_ids (ORC_NONE: none of its own) */
+    uint32_t *out_count;
+    uint32_t *out_link;    /* nearest state on the failure chain with outputs (0: none) */
+    uint32_t *out_ids;
+    uint32_t *plen;
+} orc_ac;
+
+orc_ac *orc_ac_build(const uint8_t *pool, const uint64_t *poff, const uint32_t *plen, size_t np)
+{
+    orc_ac *a = (orc_ac *)calloc(1, sizeof(orc_ac));
+    if (!a || np == 0)
+        return a;
+    int used[256] = {0};
+    size_t total = 0;
+    for (size_t i = 0; i < np; i++) {
+        total += plen[i];
+        if (plen[i] > a->longest)
+            a->longest = plen[i];
+        for (uint32_t b = 0; b < plen[i]; b++)
+            used[pool[poff[i] + b]] = 1;
+    }
+    a->nclass = 1;
+    for (int b = 0; b < 256; b++)
+        a->cls[b] = used[b] ? (uint8_t)a->nclass++ : 0;
+    const uint32_t nc = a->nclass;
+    const size_t maxs = total + 1;
+    uint32_t *trie = (uint32_t *)calloc(maxs * nc, sizeof(uint32_t));   /* 0: no edge (state 0 is the root) */
+    uint32_t *fail = (uint32_t *)calloc(maxs, sizeof(uint32_t));
+    uint32_t *nown = (uint32_t *)calloc(maxs, sizeof(uint32_t));
+    uint32_t *endstate = (uint32_t *)malloc(np * sizeof(uint32_t));
+    uint32_t ns = 1;
+    for (size_t i = 0; i < np; i++) {
+        uint32_t s = 0;
+        for (uint32_t b = 0; b < plen[i]; b++) {
+            const uint32_t c = a->cls[pool[poff[i] + b]];
+            if (!trie[(size_t)s * nc + c])
+                trie[(size_t)s * nc + c] = ns++;
+            s = trie[(size_t)s * nc + c];
+        }
+        endstate[i] = s;
+        nown[s]++;
+    }
+    /* patterns per state, in id order */
+    uint32_t *first = (uint32_t *)malloc(ns * sizeof(uint32_t));
+    uint32_t acc = 0;
+    for (uint32_t s = 0; s < ns; s++) {
+        first[s] = acc;
+        acc += nown[s];
+    }
+    uint32_t *ids = (uint32_t *)malloc((np ? np : 1) * sizeof(uint32_t));
+    uint32_t *fillp = (uint32_t *)calloc(ns, sizeof(uint32_t));
+    for (size_t i = 0; i < np; i++) {
+        const uint32_t s = endstate[i];
+        ids[first[s] + fillp[s]++] = (uint32_t)i;
+    }
+    /* breadth-first: failure links, the DFA, output links */
+    uint32_t *queue = (uint32_t *)malloc(ns * sizeof(uint32_t));
+    uint32_t *link = (uint32_t *)calloc(ns, sizeof(uint32_t));
+    size_t qh = 0, qt = 0;
+    for (uint32_t c = 0; c < nc; c++) {
+        const uint32_t t = trie[c];
+        if (t) {
+            fail[t] = 0;
+            queue[qt++] = t;
+        }
+    }
+    while (qh < qt) {
+        const uint32_t s = queue[qh++];
+        link[s] = nown[fail[s]] ? fail[s] : link[fail[s]];
+        for (uint32_t c = 0; c < nc; c++) {
+            const uint32_t t = trie[(size_t)s * nc + c];
+            if (t) {
+                fail[t] = trie[(size_t)fail[s] * nc + c];
+                queue[qt++] = t;
+            } else {
+                trie[(size_t)s * nc + c] = trie[(size_t)fail[s] * nc + c];
+            }
+        }
+    }
+    a->nstates = ns;
+    a->delta = (uint32_t *)realloc(trie, (size_t)ns * nc * sizeof(uint32_t));
+    a->out_first = first;
+    a->out_count = (uint32_t *)realloc(nown, ns * sizeof(uint32_t));
+    a->out_link = link;
+    a->out_ids = ids;
+    a->plen = (uint32_t *)malloc(np * sizeof(uint32_t));
+    memcpy(a->plen, plen, np * sizeof(uint32_t));
+    free(fail);
+    free(endstate);
+    free(fillp);
+    free(queue);
+    return a;
+}
+
+void orc_ac_free(orc_ac *a)
+{
+    if (!a)
+        return;
+    free(a->delta);
+    free(a->out_first);
+    free(a->out_count);
+    free(a->out_link);
+    free(a->out_ids);
+    free(a->plen);
+    free(a);
+}
+
+typedef struct {
+    const orc_ac *a;
+    const uint8_t *t;
+    size_t n, start, end;
+    int collect;
+    uint64_t *off;
+    uint32_t *id;
+    size_t count, cap;
+} ac_job;
+
+static void *ac_job_run(void *p)
+{
+    ac_job *j = (ac_job *)p;
+    const orc_ac *a = j->a;
+    const size_t wend = j->end + a->longest - 1 < j->n ? j->end + a->longest - 1 : j->n;
+    const uint32_t nc = a->nclass;
+    uint32_t s = 0;
+    for (size_t i = j->start; i < wend; i++) {
+        s = a->delta[(size_t)s * nc + a->cls[j->t[i]]];
+        for (uint32_t q = a->out_count[s] ? s : a->out_link[s]; q; q = a->out_link[q]) {
+            for (uint32_t k = 0; k < a->out_count[q]; k++) {
+                const uint32_t pid = a->out_ids[a->out_first[q] + k];
+                const size_t st = i + 1 - a->plen[pid];
+                if (st >= j->end)
+                    continue;   /* starts in the next chunk */
+                if (j->collect) {
+                    if (j->count == j->cap) {
+                        size_t ncap = j->cap ? 2 * j->cap : 4096;
+                        uint64_t *o = (uint64_t *)realloc(j->off, ncap * sizeof(uint64_t));
+                        uint32_t *d = (uint32_t *)realloc(j->id, ncap * sizeof(uint32_t));
+                        if (o) j->off = o;
+                        if (d) j->id = d;
+                        if (!o || !d)
+                            return NULL;
+                        j->cap = ncap;
+                    }
+                    j->off[j->count] = st;
+                    j->id[j->count] = pid;
+                }
+                j->count++;
+            }
+        }
+    }
+    return NULL;
+}
+
+/* Every (start, pattern) of the dictionary in t, overlapping, over chunks on
+ * `threads` threads; with out_off / out_id the first `cap` are copied in chunk
+ * order (within a chunk in the automaton's order: by end position, then along
+ * the output chain).  Returns how many there are. */
+size_t orc_ac_find_par(const orc_ac *a, const uint8_t *t, size_t n, int threads, uint64_t *out_off,
+                       uint32_t *out_id, size_t cap)
+{
+    if (!a || !a->nstates || n == 0)
+        return 0;
+    const int nt = clamp_threads(threads);
+    size_t chunk = n / ((size_t)nt * 4);
+    if (chunk < MIN_CHUNK)
+        chunk = MIN_CHUNK;
+    const size_t nj = (n + chunk - 1) / chunk;
+    ac_job *jobs = (ac_job *)calloc(nj, sizeof(ac_job));
+    for (size_t k = 0; k < nj; k++) {
+        jobs[k].a = a;
+        jobs[k].t = t;
+        jobs[k].n = n;
+        jobs[k].start = k * chunk;
+        jobs[k].end = (k + 1) * chunk < n ? (k + 1) * chunk : n;
+        jobs[k].collect = out_off || out_id;
+    }
+    for (size_t k0 = 0; k0 < nj; k0 += (size_t)nt) {
+        const int g = (int)((nj - k0) < (size_t)nt ? nj - k0 : (size_t)nt);
+        run_threads(g, ac_job_run, jobs + k0, sizeof(ac_job));
+    }
+    size_t total = 0;
+    for (size_t k = 0; k < nj; k++) {
+        for (size_t i = 0; jobs[k].collect && i < jobs[k].count; i++)
+            if (total + i < cap) {
+                if (out_off) out_off[total + i] = jobs[k].off[i];
+                if (out_id) out_id[total + i] = jobs[k].id[i];
+            }
+        total += jobs[k].count;
+        free(jobs[k].off);
+        free(jobs[k].id);
+    }
+    free(jobs);
+    return total;
 }

@@ -50,6 +50,14 @@ def lib():
         o.orc_parse_par.restype = S
         o.orc_links_par.argtypes = [P, S, P, P, P, P, ctypes.c_int]
         o.orc_links_par.restype = ctypes.c_int
+        o.orc_find_all_par.argtypes = [P, S, P, S, ctypes.c_int, P, S]
+        o.orc_find_all_par.restype = S
+        o.orc_ac_build.argtypes = [P, P, P, S]
+        o.orc_ac_build.restype = P
+        o.orc_ac_free.argtypes = [P]
+        o.orc_ac_free.restype = None
+        o.orc_ac_find_par.argtypes = [P, P, S, ctypes.c_int, P, P, S]
+        o.orc_ac_find_par.restype = S
         _o = o
     return _o
 
@@ -164,3 +172,53 @@ def many_find_all(text, patterns):
     ii = np.zeros(cnt, np.uint32)
     lib().orc_many_find_all(_p(a), a.size, _p(pool), _p(offs), _p(lens), len(patterns), _p(oo), _p(ii), cnt)
     return oo, ii
+
+
+def find_all_par(text, pattern, threads: int) -> np.ndarray:
+    """find_all on `threads` threads over 1 MiB+ chunks with a halo (the
+    reference's find_all_par); ascending starts."""
+    a, p = _arr(text), _arr(pattern)
+    cnt = lib().orc_find_all_par(_p(a), a.size, _p(p), p.size, threads, None, 0)
+    out = np.zeros(cnt, np.uint64)
+    lib().orc_find_all_par(_p(a), a.size, _p(p), p.size, threads, _p(out), cnt)
+    return out
+
+
+class AhoCorasick:
+    """The overlapping Aho-Corasick DFA search (the reference's ManyPatterns)."""
+
+    def __init__(self, patterns):
+        lens = np.array([len(p) for p in patterns], np.uint32)
+        offs = np.zeros(len(patterns), np.uint64)
+        if len(patterns):
+            offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+        self._pool = _arr(b"".join(bytes(p) for p in patterns) + b"\0")
+        self._lens, self._offs = lens, offs
+        self._h = lib().orc_ac_build(_p(self._pool), _p(offs), _p(lens), len(patterns))
+
+    def count(self, text, threads: int) -> int:
+        a = _arr(text)
+        return int(lib().orc_ac_find_par(self._h, _p(a), a.size, threads, None, None, 0))
+
+    def find(self, text, threads: int, sort: bool = True):
+        """(starts u64, ids u32); sorted by (start, id) when `sort`."""
+        a = _arr(text)
+        cnt = self.count(a, threads)
+        off = np.zeros(cnt, np.uint64)
+        ids = np.zeros(cnt, np.uint32)
+        lib().orc_ac_find_par(self._h, _p(a), a.size, threads, _p(off), _p(ids), cnt)
+        if sort:
+            order = np.lexsort((ids, off))
+            off, ids = off[order], ids[order]
+        return off, ids
+
+    def close(self):
+        if self._h:
+            lib().orc_ac_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -289,3 +289,34 @@ def test_parallel_restatement_equals_sequential(threads, known):
         if n1.size:
             for a, b in zip(orc.links(n1), orc.links_par(n1, threads)):
                 assert np.array_equal(a, b), len(c)
+
+
+def test_find_all_par_equals_find_all_across_chunks():
+    """The chunked literal search (find_all_par's 1 MiB chunks with a halo)
+    finds what the sequential one finds, also for matches that straddle a
+    chunk edge."""
+    from oneline_amd import datagen
+    t = datagen.document(5 << 20, seed=7).copy()
+    pat = b"zqzqzq"
+    for edge in (1 << 20, 2 << 20, 3 << 20):
+        for k in (-5, -3, -1, 0):
+            t[edge + k:edge + k + len(pat)] = np.frombuffer(pat, np.uint8)
+    for p in (pat, b"zq", b"the", datagen.vocab_word(1), b"\n"):
+        want = orc.find_all(t, p)
+        for threads in (1, 3, 8):
+            assert np.array_equal(orc.find_all_par(t, p, threads), want), (p, threads)
+
+
+def test_aho_corasick_equals_closed_form():
+    """The overlapping Aho-Corasick baseline reports every (start, id) of the
+    closed form, duplicates each with its id, across chunk edges."""
+    from oneline_amd import datagen
+    t = datagen.document(3 << 20, seed=8).copy()
+    pats = datagen.patterns(512, seed=9) + [b"a", b"ab", b"abc", b"abcd", b"abcd", b"bcd", b"\n|", b"zz" * 20]
+    t[(1 << 20) - 3:(1 << 20) + 37] = np.frombuffer(b"zz" * 20, np.uint8)
+    ac = orc.AhoCorasick(pats)
+    wo, wi = orc.many_find_all(t, pats)
+    for threads in (1, 4):
+        o, i = ac.find(t, threads)
+        assert np.array_equal(o, wo) and np.array_equal(i, wi), threads
+        assert ac.count(t, threads) == wo.size
