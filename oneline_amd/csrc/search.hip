@@ -88,13 +88,16 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
     constexpr int NM = NV / 4;   // match words (four 16-bit vector masks each)
     __shared__ uint32_t s_w[NC][NW];
     __shared__ uint64_t s_prefix;
+    __shared__ uint32_t s_ahead;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t G = gridDim.x - 1;   // workgroups that take tiles
     const uint32_t T = a.num_tiles;
     if (blockIdx.x == G) {
-        if (wid == 0)
+        if (wid == 0) {
+            __builtin_amdgcn_s_setprio(3);   // never yields to the tile waves
             scan_prefixes<SCAN_PREFIX_WINDOWS>(a.status, T, a.lb_err);
+        }
         return;
     }
     uint4 v[NV];
@@ -147,12 +150,19 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
                 w = (LB_FLAG_INCL << LB_SHIFT) | p_total;
             }
             s_prefix = (w & LB_MASK) - p_total;
+            s_ahead = spins ? 1u : 0u;
             // count and prefix word consumed: zero them for the next launch
             uint64_t *const st = a.status + (size_t)p_t * SCAN_STATUS_WORDS;
             st_coherent_u64(st, 0ull);
             st_coherent_u64(st + SCAN_PREFIX_WORD, 0ull);
         }
         __syncthreads();
+        // a workgroup ahead of the prefix front yields issue slots to the
+        // ones behind it (the oldest workgroups on a CU otherwise win by age)
+        if (s_ahead)
+            __builtin_amdgcn_s_setprio(0);
+        else
+            __builtin_amdgcn_s_setprio(2);
         const uint64_t prefix = s_prefix;
         if (tid == 0 && p_t == T - 1)
             a.meta[0] = prefix + p_total;
