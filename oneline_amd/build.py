@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["api.cpp", "scan.hip", "tree.hip", "search.hip", "batch.hip", "serve.cpp"]
+HIP_SOURCES = ["api.cpp", "scan.hip", "tree.hip", "search.hip", "serve.cpp"]
 HIP_LIB = os.path.join(HERE, "liboneline_hip.so")
 GEN_LIB = os.path.join(HERE, "libol_datagen.so")
 
@@ -38,12 +38,12 @@ def _run(cmd):
 
 def build_datagen():
     _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
-          "-o", GEN_LIB, os.path.join(CSRC, "datagen.c"), "-lm", "-lpthread"])
+          "-o", GEN_LIB, os.path.join(CSRC, "datagen.c"), os.path.join(CSRC, "profiles.c"), "-lm", "-lpthread"])
 
 
 def build_hip():
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = HIP_LIB + ".tmp"
     _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs + ["-lpthread"])
     os.replace(tmp, HIP_LIB)

@@ -6,6 +6,8 @@ Document shape: see csrc/datagen.c.  Named workloads:
 * ``document(1 << 31, seed=0, crlf=10, empty=5)``  -- 2 GiB, 1% CRLF, 0.5% empty lines
 * ``patterns(count, seed)``                        -- dictionaries of 3..32-byte patterns
 * ``lognormal_sizes(count, seed, median, sigma, lo, hi)`` -- batch / serving document sizes
+* ``profile(name, target_bytes, seed)``            -- the reference's six document profiles
+  (``PROFILES``: Repeat, ShortLines, Mixed, LongLines, Prose, Nested; csrc/profiles.c)
 """
 from __future__ import annotations
 
@@ -41,6 +43,11 @@ def _lib():
         g.ol_gen_lognormal_sizes.restype = None
         g.ol_gen_mix.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         g.ol_gen_mix.restype = ctypes.c_uint64
+        g.ol_gen_profile.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_void_p,
+                                     ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        g.ol_gen_profile.restype = ctypes.c_int
+        g.ol_gen_prose_vocab.argtypes = [ctypes.c_void_p]
+        g.ol_gen_prose_vocab.restype = None
         _gen = g
     return _gen
 
@@ -87,3 +94,31 @@ def lognormal_sizes(count: int, seed: int, median: float, sigma: float, lo: int,
 
 def mix(seed: int, idx: int) -> int:
     return int(_lib().ol_gen_mix(seed, idx))
+
+
+# Profile::ALL order and Profile::name, crates/oneline-core/src/datagen.rs:29-49
+PROFILES = ("repeat", "short-lines", "mixed", "long-lines", "prose", "nested")
+
+# a generated line adds at most ~4,100 bytes past the target (profiles.c)
+_PROFILE_SLACK = 8192
+
+
+def profile(name, target_bytes: int, seed: int) -> np.ndarray:
+    """``datagen::generate(profile, target_bytes, seed)`` of the reference: at
+    least target_bytes bytes, ending in '\n'.  ``name`` is a Profile::name or
+    an index into PROFILES."""
+    idx = PROFILES.index(name) if isinstance(name, str) else int(name)
+    buf = np.empty(target_bytes + _PROFILE_SLACK, np.uint8)
+    n = ctypes.c_size_t()
+    rc = _lib().ol_gen_profile(idx, target_bytes, seed, buf.ctypes.data, buf.size, ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(f"profile generation failed ({rc}) for {name!r}")
+    return buf[:n.value]
+
+
+def prose_vocabulary() -> list:
+    """``datagen::prose_vocabulary()``: 4096 words, index 0 the most frequent."""
+    raw = ctypes.create_string_buffer(4096 * 11)
+    _lib().ol_gen_prose_vocab(raw)
+    b = raw.raw
+    return [b[11 * k:11 * k + 11].split(b"\0", 1)[0] for k in range(4096)]

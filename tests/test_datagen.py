@@ -68,3 +68,82 @@ def test_golden_hash_of_the_1mib_document():
     links = orc.links(nodes)
     for name, arr in zip(("parent", "subtree_end", "first_child", "next_sibling"), links):
         assert hashlib.sha256(arr.tobytes()).hexdigest() == g[name + "_sha256"], name
+
+
+# ---- the reference's six profiles (csrc/profiles.c) ------------------------
+
+REPEAT_SAMPLE = (b'|=[ Header ]\n| Is this syntax necessary?\n||@card(title: "x")\n'
+                 b'||:rule(allow:list(card))\n|| Content.\n|||- Item A\n|||- Item B\n')
+
+
+def _lines(text):
+    return int(np.count_nonzero(text == 0x0A))
+
+
+def test_profiles_deterministic_and_sized():
+    """crates/oneline-core/src/datagen.rs:258-266."""
+    for p in datagen.PROFILES:
+        a = datagen.profile(p, 100_000, 7)
+        b = datagen.profile(p, 100_000, 7)
+        assert np.array_equal(a, b), p
+        assert a.size >= 100_000 and a[-1] == 0x0A, p
+        assert a.size < 100_000 + 4_200, p   # one line past the target at most
+        assert not np.array_equal(a[:50_000], datagen.profile(p, 100_000, 8)[:50_000]) or p == "repeat"
+
+
+def test_repeat_profile_is_the_sample_repeated():
+    for n in (1, 118, 119, 100_000):
+        t = bytes(datagen.profile("repeat", n, 3))
+        k = -(-n // len(REPEAT_SAMPLE))
+        assert t == REPEAT_SAMPLE * k
+
+
+def test_profile_line_counts_reported_by_the_reference():
+    """The reference's notes quote line counts of its generated documents:
+    100 MB `mixed` (--synthetic-mb 100, seed 1) has 1.3 M lines
+    (.agents/tasks/gpu-profiling-and-overhead.md:8), 1 GiB `mixed` 13 M and
+    1 GiB `nested` 12 M+ (.agents/tasks/gpu-tree-construction.md:14,23).  A
+    wrong length range, kind mix or heavy-tail rate moves these."""
+    assert round(_lines(datagen.profile("mixed", 100 << 20, 1)) / 1e5) == 13
+    assert round(_lines(datagen.profile("mixed", 1 << 30, 1)) / 1e6) == 13
+    assert 12_000_000 <= _lines(datagen.profile("nested", 1 << 30, 1)) < 13_000_000
+
+
+def test_profile_shapes():
+    """Invariants of generate() (datagen.rs:160-253) on 3 MiB of each profile."""
+    words = {w.encode() for w in ("lorem", "ipsum", "dolor", "sit", "amet", "parse", "line", "node", "depth",
+                                  "tomet", "SIMD", "GPU", "縦棒", "階層", "構文", "解析", "naïve", "café",
+                                  "データ", "|pipe|", "a", "of", "the")}
+    vocab = set(datagen.prose_vocabulary())
+    assert len(vocab) <= 4096 and all(2 <= len(w) <= 10 and w.isalpha() and w.islower() for w in vocab)
+    for p in datagen.PROFILES[1:]:
+        t = bytes(datagen.profile(p, 3 << 20, 11))
+        lines = t.split(b"\n")[:-1]
+        cr = sum(1 for ln in lines if ln.endswith(b"\r"))
+        if p == "mixed":
+            assert 0.015 < cr / len(lines) < 0.025, p      # 2% CRLF
+        else:
+            assert cr == 0, p
+        heads = [ln for ln in lines if ln.startswith(b"=[ Section ")]
+        if p == "nested":
+            assert [int(h[11:-2]) for h in heads] == list(range(1, len(heads) + 1))
+            assert all(ln.startswith(b"|") or ln == b"" or ln in heads for ln in lines)
+        else:
+            assert not heads, p
+        # the words between the prefix and the markup come from the profile's word list
+        allowed = vocab if p == "prose" else words | {b"pipe|"}
+        toks = set()
+        for ln in lines:
+            ln = ln.rstrip(b"\r")
+            if ln in heads:
+                continue
+            i = len(ln) - len(ln.lstrip(b"|"))
+            body = ln[i + 1:] if i and ln[i:i + 1] == b" " else ln[i:]
+            for a, z in ((b"=[ ", b" ]"), (b":rule(", b")"), (b'@card(title: "', b'")'), (b"-. ", b""), (b"- ", b"")):
+                if body.startswith(a) and body.endswith(z):
+                    body = body[len(a):len(body) - len(z)]
+                    break
+            toks.update(w for w in body.split(b" ") if w)
+        assert toks <= allowed, (p, sorted(toks - allowed)[:5])
+        depth_runs = [len(ln) - len(ln.lstrip(b"|")) for ln in lines if ln[:1] == b"|"]
+        assert max(depth_runs) >= 7, p   # the walk reaches its cap of 7 (words add more '|')
