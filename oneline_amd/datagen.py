@@ -17,7 +17,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-GEN_PATH = os.path.join(HERE, "libol_datagen.so")
+# OL_DATAGEN_LIB: another build of the same sources (the sanitizer build)
+GEN_PATH = os.environ.get("OL_DATAGEN_LIB") or os.path.join(HERE, "libol_datagen.so")
 
 _gen = None
 

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboneline_oracle.so")
+# OL_ORACLE_LIB: another build of the same sources (the sanitizer build)
+LIB = os.environ.get("OL_ORACLE_LIB") or os.path.join(HERE, "liboneline_oracle.so")
 NONE = 0xFFFFFFFF
 NODE = np.dtype([("depth", "<u4"), ("kind", "<u4"), ("span_start", "<u4"), ("span_len", "<u4")])
 
