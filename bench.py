@@ -10,7 +10,8 @@ value    = algorithmic bytes / second = (N + 17 L) / step time, N = text bytes,
            L = lines (4 B offset + 1 B class + 3 x 4 B links per line);
 roofline = the dominant kernel (scan_classify): its algorithmic bytes per launch
            (N + 5 L) / its mean launch time from HIP events on the engine stream,
-           against the 8 TB/s HBM peak;
+           against the 8 TB/s HBM peak; beside it the floor: the time of a
+           kernel that only moves the scan's bytes (calib.hip);
 cpu_baseline = the C oracle's multi-threaded restatement of oneline-simd's
            parallel parse (two-pass chunked) and tree (three-phase nearest
            smaller values) over the same document, on up to 16 host threads,
@@ -235,6 +236,18 @@ def main():
         verified = True
         log("[bench] timed step's outputs bit-exact against the oracle")
 
+    # the floor under the scan: a kernel that moves the scan's bytes (the text
+    # in 32 KiB tiles; the document's mean lines per tile of u32 + 2 x u8
+    # writes) with none of its work
+    tiles32 = -(-n // 32768)
+    lpt = max(1, round(L / tiles32))
+    floor_r = p.calibrate_scan_floor(doc, lpt, write=False, reps=10)
+    floor_rw = p.calibrate_scan_floor(doc, lpt, write=True, reps=10)
+    floor = {"kernel": "stream_floor_kernel (calib.hip): the scan's reads and writes without its work",
+             "lines_per_tile": lpt, "read_us": round(floor_r, 2), "read_write_us": round(floor_rw, 2),
+             "read_write_gbs": round((n + 6 * lpt * tiles32) / (floor_rw / 1e6) / 1e9, 1)}
+    log(f"[bench] floor: read {floor_r:.1f} us, read + write {floor_rw:.1f} us (scan {k_avg_ms * 1e3:.1f} us)")
+
     # the drop-in variant: the reference's outputs as it returns them -- the
     # 16-byte FlatNode per line (parse_raw / upload) and the tree with
     # subtree_end -- beside offsets, classes and the other three links
@@ -327,6 +340,9 @@ def main():
             "kernel": kname,
             "kernel_avg_us": round(k_avg_ms * 1e3, 2),
             "kernel_algorithmic_bytes": k_bytes,
+            "floor_us": floor["read_write_us"],
+            "frac_of_floor": round(floor["read_write_us"] / (k_avg_ms * 1e3), 4) if k_avg_ms > 0 else None,
+            "floor": floor,
         },
         "cpu_baseline": cpu,
     }

@@ -374,6 +374,14 @@ int ol_profile_enable(ol_ctx *ctx, int on);
 int ol_profile_take(ol_ctx *ctx, char *names, size_t name_stride, uint64_t *counts, double *ms,
                     size_t cap, size_t *n);
 
+/* Roofline calibration (no counterpart in the reference): the mean time in
+ * microseconds of `reps` launches of a kernel that moves the scan's bytes over
+ * the document's text with none of its work -- the text read in the scan's
+ * 32 KiB tiles, and when `write` is set, lines_per_tile u32 + 2 x u8 written
+ * per tile at consecutive positions (the scan's offsets, classes, keys).  The
+ * write targets are allocated for the call; the document is not changed. */
+int ol_calib_scan_floor(ol_ctx *ctx, const ol_doc *doc, uint32_t lines_per_tile, int write, int reps, double *us);
+
 /* Per-call profile: the reference's Profile {wall, host_steps, gpu_passes}
  * returned by its profile_* methods (crates/oneline-gpu/src/timing.rs:109-123).
  * ol_profile_call_begin starts a record on the context; the entry point calls

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -116,6 +117,7 @@ def load_library() -> ctypes.CDLL:
         "ol_tune_set": [P, ctypes.c_char_p, ctypes.c_int],
         "ol_tune_get": [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
         "ol_profile_take": [P, P, S, P, P, S, ctypes.POINTER(S)],
+        "ol_calib_scan_floor": [P, P, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
         "ol_batch_create": [P, P, P, S, ctypes.POINTER(P)],
         "ol_batch_run": [P, P, P, ctypes.c_int],
         "ol_batch_lines": [P, P, U64P, P, P],
@@ -298,6 +300,8 @@ class GpuDocument:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():   # the runtime may be gone: the process reclaims it
+            return
         try:
             self.close()
         except Exception:
@@ -324,6 +328,8 @@ class GpuTree:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():   # the runtime may be gone: the process reclaims it
+            return
         try:
             self.close()
         except Exception:
@@ -345,6 +351,8 @@ class GpuPatternSet:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():   # the runtime may be gone: the process reclaims it
+            return
         try:
             self.close()
         except Exception:
@@ -408,6 +416,8 @@ class GpuBatch:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():   # the runtime may be gone: the process reclaims it
+            return
         try:
             self.close()
         except Exception:
@@ -429,6 +439,8 @@ class GpuParser:
             self._ctx = None
 
     def __del__(self):
+        if sys.is_finalizing():   # the runtime may be gone: the process reclaims it
+            return
         try:
             self.close()
         except Exception:
@@ -607,6 +619,13 @@ class GpuParser:
         return ManyResult(counts, tot.value, _take(p, n.value, MANY_HIT))
 
     # -- per-call profiles (the reference's profile_* methods) ---------------------------
+    def calibrate_scan_floor(self, doc, lines_per_tile: int, write: bool = True, reps: int = 10) -> float:
+        """Mean microseconds of a kernel moving the scan's bytes over doc's text
+        with none of its work (ol_calib_scan_floor): the roofline's floor."""
+        us = ctypes.c_double()
+        _check(_lib.ol_calib_scan_floor(self._ctx, doc._h, lines_per_tile, 1 if write else 0, reps, ctypes.byref(us)))
+        return us.value
+
     def profile_call(self, fn, *args, **kwargs):
         """(fn(*args), Profile) of one call through this parser."""
         _check(_lib.ol_profile_call_begin(self._ctx))
@@ -828,6 +847,8 @@ class GpuServer:
             self._docs.clear()
 
     def __del__(self):
+        if sys.is_finalizing():   # the runtime may be gone: the process reclaims it
+            return
         try:
             self.close()
         except Exception:

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["api.cpp", "scan.hip", "tree.hip", "search.hip", "serve.cpp"]
+HIP_SOURCES = ["api.cpp", "scan.hip", "tree.hip", "search.hip", "calib.hip", "serve.cpp"]
 HIP_LIB = os.path.join(HERE, "liboneline_hip.so")
 GEN_LIB = os.path.join(HERE, "libol_datagen.so")
 

@@ -1333,6 +1333,55 @@ int ol_doc_reparse(ol_ctx *c, ol_doc *d)
     return launch_scan(c, d);
 }
 
+int ol_calib_scan_floor(ol_ctx *c, const ol_doc *d, uint32_t lines_per_tile, int write, int reps, double *us)
+{
+    if (!c || !d || !us || reps <= 0)
+        return fail(OL_ERR_INVALID, "null argument or no repetitions");
+    if (lines_per_tile > 32768)
+        return fail(OL_ERR_INVALID, "a 32 KiB tile holds at most 32768 lines");
+    std::lock_guard<std::mutex> g(c->mu);
+    *us = 0.0;
+    const uint32_t tiles = (uint32_t)((d->n + 32767) / 32768);   // the text is padded to 64 KiB
+    if (tiles == 0)
+        return OL_OK;
+    const size_t lines = (size_t)tiles * lines_per_tile;
+    struct Held {   // released on every return
+        DevBuf off, cls, key, sink;
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Held()
+        {
+            off.release();
+            cls.release();
+            key.release();
+            sink.release();
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } h;
+    HIPCHK(h.sink.ensure(64));
+    if (write) {
+        HIPCHK(h.off.ensure(lines * 4 + 4));
+        HIPCHK(h.cls.ensure(lines + 4));
+        HIPCHK(h.key.ensure(lines + 4));
+    }
+    HIPCHK(hipEventCreate(&h.a));
+    HIPCHK(hipEventCreate(&h.b));
+    auto launch = [&]() {
+        return ol_launch_stream_floor(d->text, tiles, lines_per_tile, write != 0, (uint32_t *)h.off.p,
+                                      (uint8_t *)h.cls.p, (uint8_t *)h.key.p, (uint32_t *)h.sink.p, c->stream);
+    };
+    HIPCHK(launch());   // warm-up
+    HIPCHK(hipEventRecord(h.a, c->stream));
+    for (int i = 0; i < reps; i++)
+        HIPCHK(launch());
+    HIPCHK(hipEventRecord(h.b, c->stream));
+    HIPCHK(hipEventSynchronize(h.b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, h.a, h.b));
+    *us = (double)ms * 1e3 / reps;
+    return OL_OK;
+}
+
 int ol_scan_line_offsets(ol_ctx *c, const uint8_t *text, size_t len, uint32_t **offsets, size_t *line_count)
 {
     if (!c || !offsets || !line_count)

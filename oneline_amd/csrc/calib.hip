@@ -1,0 +1,49 @@
+// calib.hip -- a bandwidth floor for the scan's roofline: a kernel that moves
+// the scan's bytes with none of its work.  It reads the text in the scan's
+// 32 KiB tiles (coalesced 16-byte loads, one workgroup per tile) and writes,
+// per tile, a line count's worth of a u32 array and two u8 arrays at
+// consecutive positions, as the scan writes its offsets, classes and keys.
+// Nothing in it depends on the text beyond an XOR that keeps the loads.
+#include "internal.h"
+
+constexpr int CAL_THREADS = 256;
+constexpr int CAL_TILE = 32768;
+constexpr int CAL_NV = CAL_TILE / 16 / CAL_THREADS;
+
+template <bool WRITE>
+__global__ __launch_bounds__(CAL_THREADS) void stream_floor_kernel(const uint4 *text, uint32_t lpt, uint32_t *off,
+                                                                  uint8_t *cls, uint8_t *key, uint32_t *sink)
+{
+    const uint32_t t = blockIdx.x;
+    const uint4 *src = text + (size_t)t * (CAL_TILE / 16);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < CAL_NV; k++) {
+        const uint4 v = src[k * CAL_THREADS + threadIdx.x];
+        acc += v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (WRITE) {
+        const size_t l0 = (size_t)t * lpt;
+        for (uint32_t i = threadIdx.x; i < lpt; i += CAL_THREADS) {
+            off[l0 + i] = acc + i;
+            cls[l0 + i] = (uint8_t)acc;
+            key[l0 + i] = (uint8_t)(acc >> 8);
+        }
+    } else if (acc == 0x9E3779B9u) {
+        sink[0] = acc;   // never taken on text; keeps the loads
+    }
+}
+
+hipError_t ol_launch_stream_floor(const uint8_t *text, uint32_t tiles, uint32_t lpt, bool write, uint32_t *off,
+                                  uint8_t *cls, uint8_t *key, uint32_t *sink, hipStream_t s)
+{
+    if (tiles == 0)
+        return hipSuccess;
+    if (write)
+        hipLaunchKernelGGL(stream_floor_kernel<true>, dim3(tiles), dim3(CAL_THREADS), 0, s, (const uint4 *)text, lpt,
+                           off, cls, key, sink);
+    else
+        hipLaunchKernelGGL(stream_floor_kernel<false>, dim3(tiles), dim3(CAL_THREADS), 0, s, (const uint4 *)text, lpt,
+                           off, cls, key, sink);
+    return hipGetLastError();
+}

@@ -442,6 +442,10 @@ uint32_t ol_scan_max_grid(uint32_t bpt);
 hipError_t ol_launch_tree(const TreeArgs &a, uint32_t rounds, hipStream_t s);
 uint32_t ol_tree_tile_lines(uint32_t rounds);
 hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s);
+// The scan's traffic without its work (calib.hip): the text read in 32 KiB
+// tiles, and with `write` lpt u32 + 2 x u8 written per tile.
+hipError_t ol_launch_stream_floor(const uint8_t *text, uint32_t tiles, uint32_t lpt, bool write, uint32_t *off,
+                                  uint8_t *cls, uint8_t *key, uint32_t *sink, hipStream_t s);
 uint32_t ol_literal_tile(bool dense);   // text bytes per literal search tile
 // Scratch of the allocation-free literal search: OL_NOALLOC_WORDS u64 words
 // in a static device array (search.hip); [0, 8) meta, word 8 the sticky error

@@ -42,7 +42,8 @@ def test_wrapper_binds_only_declared_symbols():
 def test_datagen_library_exports():
     from oneline_amd import datagen
     g = datagen._lib()
-    for s in ("ol_gen_document", "ol_gen_patterns", "ol_gen_vocab_word", "ol_gen_lognormal_sizes"):
+    for s in ("ol_gen_document", "ol_gen_patterns", "ol_gen_vocab_word", "ol_gen_lognormal_sizes",
+              "ol_gen_profile", "ol_gen_prose_vocab"):
         assert hasattr(g, s)
 
 

@@ -185,3 +185,17 @@ def test_ticket_ordered_scan(parser):
     finally:
         parser.tune_set("scan_ticket", 0)
         parser.tune_set("scan_bpt", 128)
+
+
+def test_scan_floor_calibration(parser):
+    """ol_calib_scan_floor times a kernel over the document without changing it."""
+    text = datagen.document(8 << 20, seed=4)
+    doc = parser.upload(text, nodes=False)
+    before = doc.download()
+    r = parser.calibrate_scan_floor(doc, 500, write=False, reps=3)
+    rw = parser.calibrate_scan_floor(doc, 500, write=True, reps=3)
+    assert r > 0 and rw > 0
+    after = doc.download()
+    assert all(np.array_equal(x, y) for x, y in zip(before[:2], after[:2]))
+    with pytest.raises(ol.OnelineError):
+        parser.calibrate_scan_floor(doc, 40000)
