@@ -587,8 +587,7 @@ int ensure_tree(ol_tree *t, uint64_t lines)
 int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t, uint32_t links)
 {
     uint32_t tiles = (uint32_t)ceil_div(d->lines ? d->lines : 1, (uint64_t)ol_tree_tile_lines(c->tree_rounds));
-    size_t part = (size_t)tiles * TREE_D;
-    HIPCHK(c->scr.tree_sum.ensure(part * 4 * 4));
+    HIPCHK(c->scr.tree_sum.ensure(ol_tree_sum_words(tiles) * 4));
     uint32_t *base = (uint32_t *)c->scr.tree_sum.p;
     TreeArgs a{};
     a.keys = d->keys;
@@ -598,10 +597,7 @@ int launch_tree(ol_ctx *c, const ol_doc *d, ol_tree *t, uint32_t links)
     a.first_child = (links & OL_LINK_FIRST_CHILD) ? t->fc : nullptr;
     a.next_sibling = (links & OL_LINK_NEXT_SIBLING) ? t->ns : nullptr;
     a.subtree_end = (links & OL_LINK_SUBTREE_END) ? t->se : nullptr;
-    a.sum_last = base;
-    a.sum_first = base + part;
-    a.in_last = base + 2 * part;
-    a.out_first = base + 3 * part;
+    ol_tree_set_sums(a, base, tiles);
     a.err = c->scr.ctr() + CTR_TREE_ERR;
     Prof p(c, "tree");
     HIPCHK(ol_launch_tree(a, c->tree_rounds, c->stream));
@@ -2416,10 +2412,10 @@ int batch_scan(ol_ctx *c, ol_batch *b, bool cooperative)
 
 int batch_tree(ol_ctx *c, ol_batch *b)
 {
-    const size_t part = (size_t)b->tree_tiles * TREE_D;
-    if (b->scr.tree_sum.bytes < part * 16 && b->scr.frozen)
+    const size_t sum_bytes = ol_tree_sum_words(b->tree_tiles) * 4;
+    if (b->scr.tree_sum.bytes < sum_bytes && b->scr.frozen)
         return fail(OL_ERR_DEVICE, "batch tree scratch would move under a captured graph");
-    HIPCHK(b->scr.tree_sum.ensure(part * 16));
+    HIPCHK(b->scr.tree_sum.ensure(sum_bytes));
     uint32_t *base = (uint32_t *)b->scr.tree_sum.p;
     TreeArgs a{};
     a.keys = (const uint8_t *)b->keys.p;
@@ -2429,10 +2425,7 @@ int batch_tree(ol_ctx *c, ol_batch *b)
     a.first_child = (uint32_t *)b->fc.p;
     a.next_sibling = (uint32_t *)b->ns.p;
     a.subtree_end = (uint32_t *)b->se.p;
-    a.sum_last = base;
-    a.sum_first = base + part;
-    a.in_last = base + 2 * part;
-    a.out_first = base + 3 * part;
+    ol_tree_set_sums(a, base, b->tree_tiles);
     a.err = b->scr.ctr() + CTR_TREE_ERR;
     a.nsegs = b->nd;
     a.seg_line0 = (const uint32_t *)b->seg_line0.p;
@@ -2916,7 +2909,7 @@ ParsePlan parse_plan(uint32_t bpt, uint32_t rounds, uint64_t n, uint64_t nd)
     for (int k = 0; k < 3; k++)
         Carver::need(t, nd * 4);                // segments
     Carver::need(t, P.tiles_bound * 4);         // tile -> segment
-    Carver::need(t, P.tiles_bound * TREE_D * 16);   // tree level states
+    Carver::need(t, ol_tree_sum_words(P.tiles_bound) * 4);   // tree level states
     Carver::need(t, HEAD_WORDS * 8);
     Carver::need(t, P.status_words * 8);
     P.total = t;
@@ -3169,7 +3162,7 @@ int ol_slot_parse(ol_slot *s, const uint8_t *const *docs, const size_t *lens, si
               lend(b.keys, P.cap) && lend(b.parent, P.cap * 4) && lend(b.fc, P.cap * 4) && lend(b.ns, P.cap * 4) &&
               lend(b.se, P.cap * 4) && lend(b.seg_line0, (size_t)L.nd * 4) && lend(b.seg_lines, (size_t)L.nd * 4) &&
               lend(b.seg_tile0, (size_t)L.nd * 4) && lend(b.tile_seg, P.tiles_bound * 4) &&
-              lend(b.scr.tree_sum, P.tiles_bound * TREE_D * 16);
+              lend(b.scr.tree_sum, ol_tree_sum_words(P.tiles_bound) * 4);
     if (!ok)
         return fail(OL_ERR_NOMEM, "the batch does not fit the slot's arena");
     if ((rc = lend_scratch(c, cv, b.scr, P.status_words)))

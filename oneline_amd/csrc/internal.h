@@ -311,6 +311,8 @@ struct TreeArgs {
     uint32_t *sum_first;     // per tile x level: first content line in the tile (OL_NONE = none)
     uint32_t *in_last;       // per tile x level: sum_last over all earlier tiles (max)
     uint32_t *out_first;     // per tile x level: sum_first over all later tiles (min)
+    uint32_t *wsum_last;     // per wave (tile * TREE_WAVES + wave) x level, wave-major: as sum_last
+    uint32_t *wsum_first;    //   and sum_first over the wave's own lines (written by the summary)
     uint32_t *err;           // set to 1 (by the walk) if a content line is too deep; zeroed by the summary
     // Batches (nsegs > 0): one tree per document (segment); a tile holds lines
     // of one document only, and every link is local to its document.
@@ -467,6 +469,21 @@ constexpr uint32_t SCAN_BPT = 128;                          // default bytes per
 constexpr uint32_t SCAN_TILE_MAX = SCAN_THREADS * 256;
 constexpr uint32_t SCAN_HALO = 256;
 constexpr uint32_t TREE_D = 32;
+constexpr uint32_t TREE_WAVES = 4;   // waves per tree tile
+// The tree's level states for `tiles` tiles in one buffer of
+// ol_tree_sum_words(tiles) u32: per tile and level sum_last, sum_first,
+// in_last, out_first (level-major), then per wave wsum_last, wsum_first.
+inline size_t ol_tree_sum_words(size_t tiles) { return tiles * TREE_D * (4 + 2 * TREE_WAVES); }
+inline void ol_tree_set_sums(TreeArgs &a, uint32_t *base, size_t tiles)
+{
+    const size_t part = tiles * TREE_D;
+    a.sum_last = base;
+    a.sum_first = base + part;
+    a.in_last = base + 2 * part;
+    a.out_first = base + 3 * part;
+    a.wsum_last = base + 4 * part;
+    a.wsum_first = base + (4 + TREE_WAVES) * part;
+}
 constexpr uint32_t SEARCH_THREADS = 256;
 constexpr uint32_t SEARCH_BPT = 64;   // lanes keep 64-bit position masks
 constexpr uint32_t SEARCH_TILE = SEARCH_THREADS * SEARCH_BPT;

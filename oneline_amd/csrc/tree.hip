@@ -40,6 +40,7 @@
 constexpr int TW = 256;              // threads per tile (4 waves); a tile = TW x TR lines
 constexpr int D = TREE_D;            // levels tracked
 static_assert(D == 32, "five key bits, lanes 0..32");
+static_assert(TW / 64 == (int)TREE_WAVES, "per-wave summaries");
 
 __device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
@@ -122,11 +123,15 @@ __device__ __forceinline__ void lane_bit_masks(uint64_t (&dm)[5])
         dm[b] = ((lane >> b) & 1) ? ~0ull : 0ull;
 }
 
-// Per tile: for each level the last content line (+1, 0 = none) and the first
-// (NONE = none).
+// Per wave and per tile: for each level the last content line (+1, 0 = none)
+// and the first (NONE = none).  Per round a lane keeps only whether its level
+// occurs (a bit per round) and the round's {key == d} mask in LDS; the last
+// and first line come from the last and first such round at the end.
 template <int TR>
 __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
 {
+    static_assert(TR <= 32, "a presence bit per round");
+    __shared__ uint64_t s_eq[TW / 64][TR][D];
     __shared__ uint32_t s_l[TW / 64][D], s_f[TW / 64][D];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t t = blockIdx.x;
@@ -139,21 +144,29 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
     // the depth error flag is set by the walk (the next launch but one)
     if (t == 0 && threadIdx.x == 0)
         *a.err = 0u;
-    uint32_t last = 0, first = OL_NONE;
+    uint32_t pres = 0;
 #pragma unroll
     for (int r = 0; r < TR; r++) {
         uint64_t lt, eq;   // eq: lanes whose key == lane id
         level_masks(key[r], dm, lt, eq);
-        const uint32_t rb = wbase + r * 64;
-        if (lane < D && eq) {
-            last = rb + 64 - (uint32_t)__clzll(eq);     // index + 1
-            if (first == OL_NONE)
-                first = rb + (uint32_t)__builtin_ctzll(eq);
-        }
+        if (lane < D)
+            s_eq[w][r][lane] = eq;
+        pres |= (eq != 0 ? 1u : 0u) << r;
+    }
+    wave_lds_sync();
+    uint32_t last = 0, first = OL_NONE;
+    if (lane < D && pres) {
+        const uint32_t rl = 31u - (uint32_t)__clz(pres), rf = (uint32_t)__builtin_ctz(pres);
+        const uint64_t el = s_eq[w][rl][lane], ef = s_eq[w][rf][lane];
+        last = wbase + rl * 64 + 64 - (uint32_t)__clzll(el);   // index + 1
+        first = wbase + rf * 64 + (uint32_t)__builtin_ctzll(ef);
     }
     if (lane < D) {
         s_l[w][lane] = last;
         s_f[w][lane] = first;
+        const size_t wi = ((size_t)t * TREE_WAVES + w) * D + lane;   // wave-major
+        a.wsum_last[wi] = last;
+        a.wsum_first[wi] = first;
     }
     __syncthreads();
     if (w == 0 && lane < D) {
@@ -305,7 +318,6 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     __shared__ uint64_t s_m[NWV][TR][D + 1];   // lane d's {key < d} of each round
     __shared__ uint32_t s_h[NWV][TR][D];       // first line with key <= d: in the round, then after it
     __shared__ uint8_t s_hl[NWV][TR][D];       // that line's key
-    __shared__ uint32_t s_l[NWV][D], s_f[NWV][D];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t t = blockIdx.x;
@@ -316,11 +328,11 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     uint32_t key[TR];
     load_keys<TR>(a.keys, ts, wbase, key);
 
-    // pass 1: masks; per level the wave's last / first line; per round the
-    // first line with key <= d; content lines too deep for the tree
+    // pass 1: masks; per round the first line with key <= d; content lines
+    // too deep for the tree.  The waves share no LDS: each works on its own
+    // slices, and the other waves' per-level lines come from the summary.
     uint64_t dm[5];
     lane_bit_masks(dm);
-    uint32_t last = 0, first = OL_NONE;
     bool deep = false;
 #pragma unroll
     for (int r = 0; r < TR; r++) {
@@ -334,33 +346,26 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
         const int j = le ? (int)__builtin_ctzll(le) : 0;
         const uint32_t kj = __shfl(key[r], j, 64);
         if (lane < D) {
-            if (eq) {
-                last = rb + 64 - (uint32_t)__clzll(eq);
-                if (first == OL_NONE)
-                    first = rb + (uint32_t)__builtin_ctzll(eq);
-            }
             s_h[w][r][lane] = le ? rb + (uint32_t)j : OL_NONE;
             s_hl[w][r][lane] = (uint8_t)kj;
         }
     }
     if (__ballot(deep) && lane == 0)
         atomicOr(a.err, 1u);
-    if (lane < D) {
-        s_l[w][lane] = last;
-        s_f[w][lane] = first;
-    }
-    __syncthreads();
+    wave_lds_sync();
 
     // per-level states at the wave's first line (F, last line + 1) and after
-    // its last line (B, first line)
+    // its last line (B, first line): the tile's, and the tile's other waves'
     uint32_t F = 0, Bw = OL_NONE;
     if (lane < D) {
         F = a.in_last[(size_t)lane * T + t];
         Bw = a.out_first[(size_t)lane * T + t];
+        const uint32_t *const wl = a.wsum_last + (size_t)t * TREE_WAVES * D + lane;
+        const uint32_t *const wf = a.wsum_first + (size_t)t * TREE_WAVES * D + lane;
         for (int k = 0; k < w; k++)
-            F = umax(F, s_l[k][lane]);
+            F = umax(F, wl[k * D]);
         for (int k = w + 1; k < NWV; k++)
-            Bw = umin(Bw, s_f[k][lane]);
+            Bw = umin(Bw, wf[k * D]);
     }
     // G = exclusive prefix max of F over levels, H = inclusive prefix min of B
     uint32_t pv = (lane < D) ? F : 0u, pl = (uint32_t)lane;
@@ -395,7 +400,7 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
             }
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 
     const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t above = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
