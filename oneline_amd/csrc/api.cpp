@@ -1924,6 +1924,14 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     std::vector<std::vector<uint32_t>> filt(6);
     uint32_t fwords = 0;
     const bool any3 = (dd.class_mask >> DK_3) & 1u, anylong = (dd.class_mask >> DK_LONG) & 1u;
+    // prefix filter keying: sparse (free bit offset) unless the filter would
+    // hold more than DICT_PF_DENSE_BITS bits
+    {
+        uint64_t bits = 0;
+        for (size_t i = 0; i < count; i++)
+            bits += cls[i] == DK_3 ? 32 : cls[i] == DK_LONG ? 1 : 0;
+        dd.pf_hashed = bits > DICT_PF_DENSE_BITS ? 1u : 0u;
+    }
     for (uint32_t f : {4u, 1u, 2u}) {
         const uint32_t dk = f == 1 ? DK_1 : DK_2;
         const bool used = f == 4 ? (any3 || anylong) : ((dd.class_mask >> dk) & 1u) != 0;
@@ -1945,10 +1953,16 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
                 const uint32_t h = dict_ph(w);
                 filt[f][dict_ph_word(h)] |= 1u << (h & 31);
             };
-            if (cls[i] == DK_3)
-                for (uint32_t c4 = 0; c4 < 32; c4++) setkey(key[i] | c4 << 24);
-            else if (cls[i] == DK_LONG)
-                setkey(key[i]);
+            if (dd.pf_hashed) {
+                if (cls[i] == DK_3)   // every value of the window's byte 3
+                    for (uint32_t c4 = 0; c4 < 32; c4++) setkey(key[i] | c4 << 24);
+                else if (cls[i] == DK_LONG)
+                    setkey(key[i]);
+            } else if (cls[i] == DK_3) {   // every value of the window's byte 3
+                filt[f][dict_pword(key[i])] = ~0u;
+            } else if (cls[i] == DK_LONG) {
+                filt[f][dict_pword(key[i])] |= 1u << dict_pbit(key[i]);
+            }
         }
     }
     for (uint32_t f = 1; f <= 4; f++)

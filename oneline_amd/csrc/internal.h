@@ -354,10 +354,10 @@ struct SearchArgs {
 //     alone up to 12 bytes;
 //   a run (len == 0): a = first entry, b = its length; entries are
 //     {info, a, b, 0} of each pattern, in id order.
-// LDS filters: DK_1 and DK_2 direct bitmaps (indices 1, 2), and
-// one prefix bitmap for DK_3 and DK_LONG (index 4): the 4-byte window with the
-// fourth byte cut to its low 5 bits, hashed (dict_ph); a 3-byte key sets all
-// 32 of its fourth bytes.
+// LDS filters: DK_1 and DK_2 direct bitmaps (indices 1, 2), and one prefix
+// bitmap for DK_3 and DK_LONG (index 4): a word per hash of the window's
+// bytes 0..2, a bit per low 5 bits of its byte 3 (dict_pword, dict_pbit); a
+// 3-byte key sets its whole word.
 enum : uint32_t { DK_LONG = 0, DK_3 = 1, DK_2 = 2, DK_1 = 3 };
 constexpr uint32_t DK_BUCKET = 1;   // slots per bucket
 constexpr uint32_t DK_LEN_SHIFT = 22, DK_VALID = 1u << 29, DK_CLASS_SHIFT = 30;
@@ -388,6 +388,7 @@ struct DictDev {
     const uint4 *ents;       // run entries
     const uint32_t *pool;    // pattern bytes 8.. of patterns longer than 12 bytes, word aligned, zero padded
     uint32_t max_len;
+    uint32_t pf_hashed;      // prefix filter keying: 0 dict_pword/dict_pbit, 1 dict_ph (dense filters)
 };
 
 // Hashes shared by the compiler (host) and the search kernel (device).
@@ -395,14 +396,30 @@ __host__ __device__ __forceinline__ uint32_t dict_mul24(uint32_t a, uint32_t b)
 {
     return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
 }
-// Prefix filter: 2^DICT_PBITS bits, 64 KiB of LDS.  The hash of a 4-byte
-// window w (first byte lowest) depends on bytes 0..2 and the low 5 bits of
-// byte 3 only (w << 3 drops byte 3's top 3 bits); two full-rate operations on
-// the device (24-bit multiply, shift-add).  Word = top DICT_PBITS - 5 bits,
-// bit = low 5 bits.
+// Prefix filter: 2^DICT_PBITS bits, 64 KiB of LDS, keyed on a 4-byte window w
+// (first byte lowest) cut to its bytes 0..2 and the low 5 bits of byte 3.
+// Two keyings, chosen per dictionary:
+// - sparse filters: the word is a multiplicative hash of bytes 0..2 (one
+//   full-rate 24-bit multiply, its top DICT_PBITS - 5 bits), the bit is byte
+//   3's low 5 bits, and a 3-byte key sets its whole word.  On the device the
+//   bit costs nothing: a bit-field extract takes the low 5 bits of its offset
+//   operand, and the window three positions on has this window's byte 3 as
+//   its lowest byte;
+// - dense filters: word and bit both from a hash of the cut window (a 24-bit
+//   multiply-add of the window and the window shifted by 3, which drops byte
+//   3's top bits), and a 3-byte key sets its 32 windows' bits.  One operation
+//   more per position, but hot 3-byte prefixes of the text no longer share a
+//   word with every pattern that starts with them.
 constexpr uint32_t DICT_PBITS = 19;
+__host__ __device__ __forceinline__ uint32_t dict_pword(uint32_t w) { return dict_mul24(w, 0x9E3779u) >> (32 - (DICT_PBITS - 5)); }
+__host__ __device__ __forceinline__ uint32_t dict_pbit(uint32_t w) { return (w >> 24) & 31u; }
 __host__ __device__ __forceinline__ uint32_t dict_ph(uint32_t w) { return dict_mul24(w, 0x9E3779u) + (w << 3); }
 __host__ __device__ __forceinline__ uint32_t dict_ph_word(uint32_t h) { return h >> (32 - (DICT_PBITS - 5)); }
+// The dense keying above this many prefix filter bits set (1/8 of them).  On
+// the 2 GiB benchmark text, sparse / dense: 1,024 patterns 0.78 / 0.82 ms,
+// 4,096 1.58 / 1.60 ms, 16,384 (6% of the bits) 4.9 / 5.2 ms, 65,536 (25%)
+// 25.2 / 20.8 ms.
+constexpr uint32_t DICT_PF_DENSE_BITS = (1u << DICT_PBITS) / 8;
 
 struct ManyArgs {
     const uint8_t *text;

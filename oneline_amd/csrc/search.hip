@@ -446,7 +446,8 @@ __device__ __forceinline__ uint32_t probe_position(const DictDev &d, const uint3
     bool pass = true;
     if (GATE && d.fbits[4]) {
         const uint32_t ph = dict_ph(T0);
-        pass = __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
+        pass = d.pf_hashed ? __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1)
+                           : __builtin_amdgcn_ubfe(s_filter[dict_pword(T0)], dict_pbit(T0), 1);
     }
     const uint32_t cm = d.class_mask;
     const bool useL = ((cm >> DK_LONG) & 1u) && rem >= 4 && pass;
@@ -557,18 +558,29 @@ __device__ __forceinline__ uint32_t probe_count_keep(const DictDev &d, const uin
     const uint32_t T0 = __builtin_amdgcn_alignbyte(a1, a0, sh);
     const uint32_t T1 = __builtin_amdgcn_alignbyte(a2, a1, sh);
     const uint32_t T2 = __builtin_amdgcn_alignbyte(a3, a2, sh);
-    // the prefix filter at the window and at two other values of its fourth
-    // byte's low bits: a 3-byte key sets all 32, so the 3-byte lookup is
-    // made only where all three are set
-    auto fbit = [&](uint32_t w) -> bool {
+    // The 3-byte lookup is made only where the prefix filter can hold a 3-byte
+    // key of the window: sparse keying, the window's word is full (a 3-byte
+    // key sets all of it); dense keying, the bits of two other values of
+    // byte 3's low bits are set too (a 3-byte key sets all 32).
+    const bool has_pf = d.fbits[4] != 0;
+    auto hbit = [&](uint32_t w) -> bool {
         const uint32_t ph = dict_ph(w);
         return __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
     };
-    const bool has_pf = d.fbits[4] != 0;
-    const bool pass = !(GATE && has_pf) || fbit(T0);
+    bool pass, gate3;
+    if (!has_pf) {
+        pass = gate3 = true;
+    } else if (d.pf_hashed) {
+        pass = !GATE || hbit(T0);
+        gate3 = hbit(T0 ^ 1u << 24) && hbit(T0 ^ 2u << 24);
+    } else {
+        const uint32_t pfw = s_filter[dict_pword(T0)];
+        pass = !GATE || __builtin_amdgcn_ubfe(pfw, dict_pbit(T0), 1);
+        gate3 = pfw == ~0u;
+    }
     const uint32_t cm = d.class_mask;
     const bool useL = ((cm >> DK_LONG) & 1u) && rem >= 4 && pass;
-    const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass && (!has_pf || (fbit(T0 ^ 1u << 24) && fbit(T0 ^ 2u << 24)));
+    const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass && gate3;
     const bool use2 = SMALL && ((cm >> DK_2) & 1u) && rem >= 2;
     const bool use1 = SMALL && ((cm >> DK_1) & 1u) && rem >= 1;
     const uint4 fL = dk_lookup(d, DK_LONG, T0, useL), f3 = dk_lookup(d, DK_3, T0 & 0xFFFFFFu, use3);
@@ -727,8 +739,8 @@ __device__ __forceinline__ ManyWords many_load(const ManyArgs &a, uint32_t wt, u
 // the prefix filter at pf, bit 1: the 1- and 2-byte filters at their word
 // offsets from small); compile-time, so the lookups are straight-line code
 // whose reads the compiler can overlap.  A position of the prefix filter
-// costs a byte align, a shift, a 24-bit multiply-add, the word address, one
-// read and the bit extract.
+// costs a byte align, a 24-bit multiply, the word address, one read and the
+// bit extract, whose offset is the window three positions on.
 template <int FS>
 __device__ __forceinline__ uint32_t many_filter(const DictDev &d, const uint32_t *pf, const uint32_t *small,
                                                 const ManyWords &W, uint32_t lim)
@@ -736,13 +748,22 @@ __device__ __forceinline__ uint32_t many_filter(const DictDev &d, const uint32_t
     const uint32_t *f1 = small + d.foff[1], *f2 = small + d.foff[2];
     const uint32_t m1 = d.fbits[1] ? 0xFFu : 0u, m2 = d.fbits[2] ? 0xFFFFu : 0u;
     uint32_t cand = 0;
+    // the window at lane position q (q < MANY_LB), and a word whose lowest
+    // byte is the byte at q (q < MANY_LB + 3)
+    auto win = [&](int q) -> uint32_t {
+        return (q & 3) ? __builtin_amdgcn_alignbyte(W.w[(q >> 2) + 1], W.w[q >> 2], q & 3) : W.w[q >> 2];
+    };
+    auto low_byte_at = [&](int q) -> uint32_t { return q < (int)MANY_LB ? win(q) : W.w[q >> 2] >> (8 * (q & 3)); };
+    static_assert(MANY_LB % 4 == 0 && MANY_LB / 4 + 1 <= sizeof(W.w) / 4, "the words past the lane's bytes");
 #pragma unroll
     for (int p = 0; p < (int)MANY_LB; p++) {
-        const uint32_t w = (p & 3) ? __builtin_amdgcn_alignbyte(W.w[(p >> 2) + 1], W.w[p >> 2], p & 3) : W.w[p >> 2];
+        const uint32_t w = win(p);
         uint32_t hit = 0;
-        if (FS & 0x10) {
+        if ((FS & 0x30) == 0x10) {   // offset: the low 5 bits of byte 3
+            hit |= __builtin_amdgcn_ubfe(pf[dict_pword(w)], low_byte_at(p + 3), 1);
+        } else if (FS & 0x10) {      // offset: the hash's low 5 bits
             const uint32_t h = dict_ph(w);
-            hit |= __builtin_amdgcn_ubfe(pf[dict_ph_word(h)], h, 1);   // offset: h's low 5 bits
+            hit |= __builtin_amdgcn_ubfe(pf[dict_ph_word(h)], h, 1);
         }
         if (FS & 0x02) hit |= filter_bit(f1, w & m1) | filter_bit(f2, w & m2);
         cand |= hit << p;
@@ -1175,10 +1196,12 @@ static hipError_t launch_many_fs(const ManyArgs &a, hipStream_t s)
 template <bool WRITE>
 static hipError_t launch_many(const ManyArgs &a, hipStream_t s)
 {
-    const int fs = (a.d.fbits[4] ? 0x10 : 0) | (a.d.fbits[1] || a.d.fbits[2] ? 0x02 : 0);
+    const int fs = (a.d.fbits[4] ? (a.d.pf_hashed ? 0x30 : 0x10) : 0) | (a.d.fbits[1] || a.d.fbits[2] ? 0x02 : 0);
     switch (fs) {
     case 0x10: return launch_many_fs<WRITE, 0x10>(a, s);
     case 0x12: return launch_many_fs<WRITE, 0x12>(a, s);
+    case 0x30: return launch_many_fs<WRITE, 0x30>(a, s);
+    case 0x32: return launch_many_fs<WRITE, 0x32>(a, s);
     case 0x02: return launch_many_fs<WRITE, 0x02>(a, s);
     default: return hipErrorInvalidValue;   // a dictionary has at least one filter
     }
