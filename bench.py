@@ -210,6 +210,7 @@ def main():
     k_bytes = n + 5 * L
     k_gbs = k_bytes / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
     traffic = load_traffic(kname, n)
+    prof_us = load_profile_mean("void scan_classify_kernel<256, 2>", n)
     kernels = {k: {"launches": c, "avg_us": round(t / max(1, c) * 1e3, 2)} for k, (c, t) in prof.items()}
     log(f"[bench] {ms:.3f} ms/step, {value:.1f} GB/s; kernels {kernels}")
 
@@ -340,6 +341,10 @@ def main():
             "kernel": kname,
             "kernel_avg_us": round(k_avg_ms * 1e3, 2),
             "kernel_algorithmic_bytes": k_bytes,
+            "profile_kernel_avg_us": prof_us,
+            "profile_frac": round(k_bytes / (prof_us / 1e6) / 1e9 / HBM_PEAK_GBS, 4) if prof_us else None,
+            "profile_source": "profiles/scan_tree_2GiB_kernel_trace.txt (rocprofv3 --kernel-trace, last 20 launches)"
+            if prof_us else None,
             "floor_us": floor["read_write_us"],
             "frac_of_floor": round(floor["read_write_us"] / (k_avg_ms * 1e3), 4) if k_avg_ms > 0 else None,
             "floor": floor,
@@ -364,6 +369,23 @@ def load_traffic(kname, n):
     if not rec or rec.get("text_bytes") != n:
         return {}
     return {"bytes_per_launch": rec["bytes_per_launch"], "source": f"profiles/traffic.json ({rec['profile']})"}
+
+
+def load_profile_mean(kernel_prefix, n):
+    """The mean duration (us) of a kernel in the committed rocprofv3 kernel
+    trace of the headline (profiles/scan_tree_2GiB_kernel_trace.txt: its last
+    20 launches, the timed steps), for the 2 GiB workload only."""
+    if n != 1 << 31:
+        return None
+    path = os.path.join(ROOT, "profiles", "scan_tree_2GiB_kernel_trace.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(kernel_prefix):
+                    return float(line.split()[-1])
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def copy_calibration(nbytes):
