@@ -22,6 +22,7 @@
 #include <random>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -1835,29 +1836,70 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     struct Slot {
         uint32_t key, info, a, b;
     };
-    std::vector<Slot> slots;
-    std::vector<uint32_t> ents;   // 4 words per entry: info, w1, aux, 0
+    DictDev dd{};
+    // groups: the patterns of one (class, key), a range of `order`
+    struct Group {
+        uint32_t cls, key;
+        size_t b, e;
+    };
+    std::vector<Group> groups;
     for (size_t e = 0; e < count;) {
         const uint32_t i = order[e];
         size_t f = e + 1;
         while (f < count && cls[order[f]] == cls[i] && key[order[f]] == key[i])
             f++;
-        Slot sl{key[i], 0, 0, 0};
-        if (f - e == 1) {   // one pattern: decided from the slot
-            sl.info = info_of(i);
-            tail(i, sl.a, sl.b);
-        } else {            // a run of entries
-            sl.info = dk_info(0, 0, cls[i]);
+        groups.push_back({cls[i], key[i], e, f});
+        e = f;
+    }
+    // Merged runs: the run of a key also holds the shorter patterns that are
+    // prefixes of it (the groups of classes 3, 2, 1 on its first 3, 2, 1
+    // bytes), so one run per position holds all its candidates.  Short
+    // patterns are copied into every longer key that extends them; when that
+    // would more than double the entries, runs keep their own class only.
+    std::unordered_map<uint64_t, uint32_t> gidx;
+    gidx.reserve(groups.size() * 2);
+    for (size_t g = 0; g < groups.size(); g++)
+        gidx[(uint64_t)groups[g].cls << 32 | groups[g].key] = (uint32_t)g;
+    auto prefix_groups = [&](const Group &g, auto &&visit) {
+        for (uint32_t c = g.cls + 1; c <= DK_1; c++) {   // DK_3, DK_2, DK_1: 3, 2, 1 bytes
+            const uint32_t nb = 4 - c;
+            const auto it = gidx.find((uint64_t)c << 32 | (g.key & ((1u << (8 * nb)) - 1u)));
+            if (it != gidx.end())
+                visit(groups[it->second]);
+        }
+    };
+    {
+        uint64_t total = 0;
+        for (const Group &g : groups) {
+            total += g.e - g.b;
+            prefix_groups(g, [&](const Group &s) { total += s.e - s.b; });
+        }
+        dd.merged = total <= 2 * (uint64_t)count + 65536 ? 1u : 0u;
+    }
+    std::vector<Slot> slots;
+    std::vector<uint32_t> ents;   // 4 words per entry: info, w1, aux, 0
+    std::vector<uint32_t> mem;
+    for (const Group &g : groups) {
+        mem.assign(order.begin() + g.b, order.begin() + g.e);
+        if (dd.merged) {
+            prefix_groups(g, [&](const Group &s) { mem.insert(mem.end(), order.begin() + s.b, order.begin() + s.e); });
+            std::sort(mem.begin(), mem.end());
+        }
+        Slot sl{g.key, 0, 0, 0};
+        if (mem.size() == 1) {   // one pattern (of the key's own class): decided from the slot
+            sl.info = info_of(mem[0]);
+            tail(mem[0], sl.a, sl.b);
+        } else {                 // a run of entries, in id order
+            sl.info = dk_info(0, 0, g.cls);
             sl.a = (uint32_t)(ents.size() / 4);
-            sl.b = (uint32_t)(f - e);
-            for (size_t k = e; k < f; k++) {
+            sl.b = (uint32_t)mem.size();
+            for (uint32_t p : mem) {
                 uint32_t w1, aux;
-                tail(order[k], w1, aux);
-                ents.insert(ents.end(), {info_of(order[k]), w1, aux, 0u});
+                tail(p, w1, aux);
+                ents.insert(ents.end(), {info_of(p), w1, aux, 0u});
             }
         }
         slots.push_back(sl);
-        e = f;
     }
     pool.resize(pool.size() + 16, 0);
     if (ents.empty())
@@ -1912,7 +1954,6 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
         tab[q * 4 + 2] = sl.a;
         tab[q * 4 + 3] = sl.b;
     }
-    DictDev dd{};
     dd.max_len = max_len;
     dd.bbits = bbits;
     for (size_t i = 0; i < count; i++)

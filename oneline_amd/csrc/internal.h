@@ -389,6 +389,14 @@ struct DictDev {
     const uint32_t *pool;    // pattern bytes 8.. of patterns longer than 12 bytes, word aligned, zero padded
     uint32_t max_len;
     uint32_t pf_hashed;      // prefix filter keying: 0 dict_pword/dict_pbit, 1 dict_ph (dense filters)
+    // Merged runs (1): the run of a key also holds every shorter pattern that
+    // is a prefix of the key (a class-L run the 3-, 2- and 1-byte patterns on
+    // its first 3, 2 and 1 bytes; a class-3 run the 2- and 1-byte ones; a
+    // class-2 run the 1-byte ones), sorted by id.  A position's matches are
+    // then all in the run of the first class whose slot holds its key, in
+    // the order L, 3, 2, 1.  0: each run holds its own class only (the
+    // compiler's choice when merging would copy too many short patterns).
+    uint32_t merged;
 };
 
 // Hashes shared by the compiler (host) and the search kernel (device).

@@ -646,6 +646,120 @@ __device__ __forceinline__ uint32_t probe_count_keep(const DictDev &d, const uin
 }
 
 
+// ---- merged runs (DictDev::merged) -------------------------------------------
+// The slot of the first class, in the order L, 3, 2, 1, that holds the key of
+// the window T0 (info 0: none).  Its run holds every pattern that can start
+// there.  FILT: the prefix filter s_filter is at hand (LDS): the 3-byte
+// lookup then needs the filter's 3-byte gate (as in probe_count_keep) and,
+// with GATE, the classes of 3 bytes and more the window's filter bit.
+template <bool FILT, bool GATE, bool SMALL>
+__device__ __forceinline__ uint4 merged_slot(const DictDev &d, const uint32_t *s_filter, uint32_t T0, uint32_t rem)
+{
+    bool pass = true, gate3 = true;
+    if (FILT && d.fbits[4]) {
+        if (d.pf_hashed) {
+            auto hbit = [&](uint32_t w) -> bool {
+                const uint32_t ph = dict_ph(w);
+                return __builtin_amdgcn_ubfe(s_filter[dict_ph_word(ph)], ph, 1);
+            };
+            pass = !GATE || hbit(T0);
+            gate3 = hbit(T0 ^ 1u << 24) && hbit(T0 ^ 2u << 24);
+        } else {
+            const uint32_t pfw = s_filter[dict_pword(T0)];
+            pass = !GATE || __builtin_amdgcn_ubfe(pfw, dict_pbit(T0), 1);
+            gate3 = pfw == ~0u;
+        }
+    }
+    const uint32_t cm = d.class_mask;
+    const bool useL = ((cm >> DK_LONG) & 1u) && rem >= 4 && pass;
+    const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass && gate3;
+    const uint4 fL = dk_lookup(d, DK_LONG, T0, useL), f3 = dk_lookup(d, DK_3, T0 & 0xFFFFFFu, use3);
+    uint4 f = (fL.y & DK_VALID) ? fL : f3;
+    if (SMALL) {
+        const bool use2 = ((cm >> DK_2) & 1u) && rem >= 2, use1 = ((cm >> DK_1) & 1u) && rem >= 1;
+        const uint4 f2 = dk_lookup(d, DK_2, T0 & 0xFFFFu, use2), f1 = dk_lookup(d, DK_1, T0 & 0xFFu, use1);
+        f = (f.y & DK_VALID) ? f : (f2.y & DK_VALID) ? f2 : f1;
+    }
+    return f;
+}
+
+// Whether pattern {info, pa, pb} of a run or slot that holds the position's
+// key occurs at tile position lp: its length fits `rem`, its bytes 4..11
+// equal the text words T1, T2 there, and its bytes 12.. the text (pool
+// words, read only when bytes 4..11 match).  The key bytes need no check.
+__device__ __forceinline__ bool pattern_here(const DictDev &d, const uint8_t *s_text, uint32_t lp, uint32_t T1,
+                                             uint32_t T2, uint32_t info, uint32_t pa, uint32_t pb, uint32_t rem)
+{
+    auto tail_mask = [](uint32_t nb) -> uint32_t { return nb >= 4 ? ~0u : ((1u << (8 * nb)) - 1u); };
+    const uint32_t len = (info >> DK_LEN_SHIFT) & 0x7Fu;
+    const uint32_t m1 = len > 4 ? tail_mask(len - 4) : 0u;
+    const uint32_t m2 = len > 8 && len <= 12 ? tail_mask(len - 8) : 0u;
+    bool ok = (len <= rem) & (((T1 ^ pa) & m1) == 0u) & (((T2 ^ pb) & m2) == 0u);
+    if (len > 12 && ok) {
+        ok = (T2 ^ d.pool[pb]) == 0u;
+        for (uint32_t j = 12; ok && j < len; j += 4) {
+            const uint32_t x = lp + j;
+            const uint32_t *q = (const uint32_t *)(s_text + (x & ~3u));
+            const uint32_t sx = x & 3u;
+            const uint32_t tw = sx ? __builtin_amdgcn_alignbyte(q[1], q[0], sx) : q[0];
+            ok = ((tw ^ d.pool[pb + ((j - 8) >> 2)]) & tail_mask(len - j)) == 0u;
+        }
+    }
+    return ok;
+}
+
+// The text words at tile position lp: bytes 0..3, 4..7, 8..11.
+__device__ __forceinline__ void text_words(const uint8_t *s_text, uint32_t lp, uint32_t &T0, uint32_t &T1,
+                                           uint32_t &T2)
+{
+    const uint32_t *wp = (const uint32_t *)(s_text + (lp & ~3u));
+    const uint32_t sh = lp & 3u;
+    const uint32_t a0 = wp[0], a1 = wp[1], a2 = wp[2], a3 = wp[3];
+    T0 = __builtin_amdgcn_alignbyte(a1, a0, sh);
+    T1 = __builtin_amdgcn_alignbyte(a2, a1, sh);
+    T2 = __builtin_amdgcn_alignbyte(a3, a2, sh);
+}
+
+// Merged layout, one lane on its own: the matches at tile position lp
+// (global gp), counted, and written in id order from out[r] (up to cap) when
+// `emit`.  For the write pass's rare cases (runs longer than 64 entries,
+// wave tiles whose candidates were not stored).
+template <bool SMALL>
+__device__ uint32_t probe_merged(const DictDev &d, const uint8_t *s_text, uint32_t lp, uint64_t gp, uint32_t rem,
+                                 bool emit, uint64_t *out_off, uint32_t *out_id, uint64_t r, uint64_t cap)
+{
+    uint32_t T0, T1, T2;
+    text_words(s_text, lp, T0, T1, T2);
+    const uint4 f = merged_slot<false, false, SMALL>(d, nullptr, T0, rem);
+    if (!(f.y & DK_VALID))
+        return 0;
+    const uint32_t slen = (f.y >> DK_LEN_SHIFT) & 0x7Fu;
+    const uint32_t n = slen ? 1u : f.w;
+    uint32_t count = 0;
+    for (uint32_t j = 0; j < n; j++) {
+        const uint4 e = slen ? make_uint4(f.y, f.z, f.w, 0u) : d.ents[f.z + j];
+        if (!pattern_here(d, s_text, lp, T1, T2, e.x, e.y, e.z, rem))
+            continue;
+        if (emit && r + count < cap) {
+            if (out_off) out_off[r + count] = gp;
+            if (out_id) out_id[r + count] = e.x & 0x3FFFFFu;
+        }
+        count++;
+    }
+    return count;
+}
+
+// The write pass's probe of one position for either layout.
+template <bool MERGED, bool SMALL>
+__device__ __forceinline__ uint32_t probe_any(const DictDev &d, const uint8_t *s_text, uint32_t lp, uint64_t gp,
+                                              uint32_t rem, bool emit, uint64_t *out_off, uint32_t *out_id, uint64_t r,
+                                              uint64_t cap)
+{
+    if (MERGED)
+        return probe_merged<SMALL>(d, s_text, lp, gp, rem, emit, out_off, out_id, r, cap);
+    return probe_position<false, false, SMALL>(d, nullptr, s_text, lp, gp, rem, emit, out_off, out_id, r, cap);
+}
+
 // Wave tiles: MANY_H parts of 64 lanes x MANY_LB positions (a lane's
 // positions are contiguous within a part; parts follow each other).
 constexpr uint32_t MANY_LB = 32;
@@ -909,10 +1023,148 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
 }
 
+// Stored candidate of the merged layout (16 bytes): x as above (position |
+// bytes left | match count; CAND_RECOUNT: a run of more than 64 entries,
+// probed again by the write pass); y = the first entry of the key's run, or
+// CAND_SINGLE | the pattern id when the key has one pattern; z, w = which of
+// the run's entries match (bit j: entry y + j), low word first.
+constexpr uint32_t CAND_SINGLE = 1u << 31;
+constexpr uint32_t MERGED_BITS = 64;   // run entries a stored candidate records
+constexpr uint32_t MANY_STAGE = 512;   // write pass: matches a wave stages per round
+
+// Count pass of the merged layout.  Per 64 candidates (one per lane): the
+// key's slot; a single pattern is checked in the lane, and the runs' entries
+// are checked 64 at a time over the whole wave (entry k of the concatenated
+// runs on lane k mod 64, its candidate found from markers in LDS and a
+// prefix maximum over the lanes), so a wave's work follows the sum of its
+// candidates' runs, not their maximum.  A candidate collects its entries'
+// results from the round's ballot.
+template <int THREADS, int FS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void many_count_m_kernel(ManyArgs a)
+{
+    constexpr int NWV = THREADS / 64;
+    constexpr bool SMALL = (FS & 0x02) != 0;
+    __shared__ __align__(16) uint32_t s_pf[(FS & 0x10) ? MANY_PWORDS : 1];
+    extern __shared__ __align__(16) uint32_t s_dyn[];
+    __shared__ uint16_t s_pos[NWV][MANY_WLIST];
+    __shared__ uint8_t s_rem[NWV][MANY_WLIST];
+    __shared__ int32_t s_mark[NWV][64];
+    constexpr uint32_t PW = (FS & 0x10) ? MANY_PWORDS : 0u;   // host layout: the prefix filter first
+    if (FS & 0x10) {
+        const uint4 *src = (const uint4 *)a.d.filter;
+        for (uint32_t i = threadIdx.x; i < MANY_PWORDS / 4; i += THREADS)
+            ((uint4 *)s_pf)[i] = src[i];
+    }
+    for (uint32_t i = PW + threadIdx.x; i < a.d.fwords; i += THREADS)
+        s_dyn[i - PW] = a.d.filter[i];
+    __syncthreads();
+    const DictDev &d = a.d;
+    const uint32_t *const pf = (FS & 0x10) ? s_pf : s_dyn;
+    const uint32_t *const small = s_dyn - PW;   // filters 1, 2 at their word offsets
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint16_t *const l_pos = s_pos[wid];
+    uint8_t *const l_rem = s_rem[wid];
+    int32_t *const mark = s_mark[wid];
+    const uint32_t stride = gridDim.x * NWV;
+    uint32_t wt = blockIdx.x * NWV + wid;
+    ManyWords W;
+    if (wt < a.num_wtiles)
+        W = many_load(a, wt, 0);
+    for (; wt < a.num_wtiles; wt += stride) {
+        const ManyLane m = many_lane_setup(a, wt);
+        const ManyCand cand = many_candidates<FS>(a, wt, wt + stride, m, pf, small, W);
+        const uint32_t total_c = cand.total;
+        uint4 *const store = a.cands ? (uint4 *)a.cands + (size_t)wt * MANY_WCAP : nullptr;
+        uint32_t kept = 0, tile_total = 0;
+        for (uint32_t c0 = 0; c0 < total_c; c0 += MANY_WLIST) {
+            const uint32_t n = min(MANY_WLIST, total_c - c0);
+            many_list_chunk(a, m, cand, c0, l_pos, l_rem);
+            wave_lds_sync();
+            for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                uint32_t lp = 0, rm = 0, T0 = 0, T1 = 0, T2 = 0;
+                uint4 f = make_uint4(0, 0, 0, 0);
+                if (i < n) {
+                    lp = l_pos[i];
+                    rm = l_rem[i];
+                }
+                if (rm) {
+                    text_words(m.g_text, lp, T0, T1, T2);
+                    f = merged_slot<true, SMALL, SMALL>(d, pf, T0, rm);
+                }
+                const bool valid = (f.y & DK_VALID) != 0;
+                const uint32_t slen = (f.y >> DK_LEN_SHIFT) & 0x7Fu;
+                const uint32_t nrun = valid && !slen ? f.w : 0u;
+                uint32_t c = valid && slen && pattern_here(d, m.g_text, lp, T1, T2, f.y, f.z, f.w, rm) ? 1u : 0u;
+                uint64_t bits = c;
+                // the runs' entries: entry k of the wave's concatenated runs
+                // (this lane's at [P, P + nrun)) is checked on lane k mod 64
+                const uint32_t incl = wave_incl_add(nrun);
+                const uint32_t P = incl - nrun, M = __shfl(incl, 63, 64);
+                int32_t carry = -1;
+                for (uint32_t r0 = 0; r0 < M; r0 += 64) {
+                    mark[lane] = -1;
+                    wave_lds_sync();
+                    if (nrun && P >= r0 && P < r0 + 64)
+                        mark[P - r0] = (int32_t)lane;
+                    wave_lds_sync();
+                    const int32_t o = max(wave_incl_max(mark[lane]), carry);
+                    carry = __shfl(o, 63, 64);
+                    const int ol = o < 0 ? 0 : o;
+                    const uint32_t oP = __shfl(P, ol, 64), ob = __shfl(f.z, ol, 64);
+                    const uint32_t oT1 = __shfl(T1, ol, 64), oT2 = __shfl(T2, ol, 64);
+                    const uint32_t olr = __shfl(lp | rm << 16, ol, 64);
+                    const uint32_t k = r0 + lane;
+                    bool ok = false;
+                    if (k < M) {
+                        const uint4 e = d.ents[ob + (k - oP)];
+                        ok = pattern_here(d, m.g_text, olr & 0xFFFFu, oT1, oT2, e.x, e.y, e.z, olr >> 16);
+                    }
+                    const uint64_t okm = __ballot(ok);
+                    // this lane's entries in the round: lanes [at, at + nrun)
+                    const int32_t at = (int32_t)P - (int32_t)r0;
+                    if (nrun && at < 64 && at + (int32_t)nrun > 0) {
+                        const int32_t lo = at > 0 ? at : 0, hi = min(at + (int32_t)nrun, 64);
+                        const uint64_t rmask = (hi >= 64 ? ~0ull : (1ull << hi) - 1ull) & ~((1ull << lo) - 1ull);
+                        const uint64_t mine = okm & rmask;
+                        c += (uint32_t)__popcll(mine);
+                        if (at >= 0)
+                            bits |= mine >> at;
+                        else if (at > -64)
+                            bits |= mine << (-at);
+                    }
+                    wave_lds_sync();   // mark[] is rewritten by the next round
+                }
+                tile_total += c;
+                if (store) {
+                    // keep the candidates with matches, in position order
+                    const uint64_t has = __ballot(c != 0);
+                    const uint32_t at = kept + (uint32_t)__popcll(has & below);
+                    if (c && at < MANY_WCAP) {
+                        const uint32_t cc = nrun > MERGED_BITS ? CAND_RECOUNT : c;
+                        store[at] = make_uint4(lp | rm << CAND_POS_BITS | cc << CAND_CNT_SHIFT,
+                                               slen ? (CAND_SINGLE | (f.y & 0x3FFFFFu)) : f.z, (uint32_t)bits,
+                                               (uint32_t)(bits >> 32));
+                    }
+                    kept += (uint32_t)__popcll(has);
+                }
+            }
+            wave_lds_sync();   // the list is rewritten by the next chunk
+        }
+        const uint32_t tt = wave_incl_add(tile_total);
+        const uint32_t tsum = __shfl(tt, 63, 64);
+        if (lane == 0) {
+            a.tile_count[wt] = tsum;
+            a.tile_list[wt] = (store && kept <= MANY_WCAP) ? (uint64_t)kept : TILE_LIST_NONE;
+        }
+    }
+}
+
 // A wave tile whose candidates were not stored: per part, each lane filters
 // its positions again (filter read from global memory) and walks its own
 // candidates, counting, then writing from out0 on.
-template <int FS>
+template <int FS, bool MERGED>
 __device__ __forceinline__ void many_write_unstored(const ManyArgs &a, uint32_t wt, uint64_t out0)
 {
     const DictDev &d = a.d;
@@ -945,11 +1197,10 @@ __device__ __forceinline__ void many_write_unstored(const ManyArgs &a, uint32_t 
                 if (!rm)
                     continue;
                 if (!pass)
-                    own += probe_position<false, false, (FS & 0x02) != 0>(d, nullptr, m.g_text, lp, gp, rm, false,
-                                                                          nullptr, nullptr, 0, 0);
+                    own += probe_any<MERGED, (FS & 0x02) != 0>(d, m.g_text, lp, gp, rm, false, nullptr, nullptr, 0, 0);
                 else if (r < a.cap)
-                    r += probe_position<false, false, (FS & 0x02) != 0>(d, nullptr, m.g_text, lp, gp, rm, true,
-                                                                        a.out_off, a.out_id, r, a.cap);
+                    r += probe_any<MERGED, (FS & 0x02) != 0>(d, m.g_text, lp, gp, rm, true, a.out_off, a.out_id, r,
+                                                             a.cap);
             }
             if (pass) {
                 di = pdi;
@@ -1074,7 +1325,164 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
         while (redo) {
             const uint32_t j = (uint32_t)__builtin_ctzll(redo);
             redo &= redo - 1;
-            many_write_unstored<FS>(a, job * JT + j, __shfl(off, (int)j, 64));
+            many_write_unstored<FS, false>(a, job * JT + j, __shfl(off, (int)j, 64));
+        }
+    }
+}
+
+// Write pass of the merged layout: as many_write_kernel, but a stored
+// candidate names its matches (the single pattern's id, or its run's first
+// entry and which entries match), so its ids are read from the entries, in
+// id order, and nothing is probed again except runs longer than 64 entries.
+template <int THREADS, int FS>
+__global__ __launch_bounds__(THREADS) void many_write_m_kernel(ManyArgs a)
+{
+    constexpr int NWV = THREADS / 64;
+    constexpr bool SMALL = (FS & 0x02) != 0;
+    __shared__ uint32_t s_mark[NWV][64];
+    __shared__ uint64_t s_first[NWV][64];
+    __shared__ uint4 s_queue[NWV][128];   // candidates to probe again: {position, end bound, output index}
+    __shared__ uint32_t s_sgp[NWV][MANY_STAGE], s_sid[NWV][MANY_STAGE];   // a round's matches, staged
+    const DictDev &d = a.d;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t *const mark = s_mark[wid];
+    uint64_t *const first = s_first[wid];
+    uint4 *const queue = s_queue[wid];
+    uint32_t *const sgp = s_sgp[wid], *const sid = s_sid[wid];
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t qn = 0;   // queued candidates (every lane)
+    auto probe_queued = [&](uint32_t count) {
+        if (lane < count) {
+            const uint4 q = queue[lane];
+            probe_merged<SMALL>(d, a.text, q.x, q.x, q.y, true, a.out_off, a.out_id, (uint64_t)q.w << 32 | q.z, a.cap);
+        }
+    };
+    const uint32_t JT = a.job_tiles;
+    const uint32_t jobs = (a.num_wtiles + JT - 1) / JT;
+    for (uint32_t job = blockIdx.x * NWV + wid; job < jobs; job += gridDim.x * NWV) {
+        const uint32_t wt = job * JT + lane;
+        const bool valid = lane < JT && wt < a.num_wtiles;
+        const uint64_t st = valid ? a.tile_list[wt] : 0ull;
+        const uint64_t off = valid ? a.tile_off[wt] : 0ull;
+        const uint32_t n = st == TILE_LIST_NONE ? 0u : (uint32_t)st;
+        const uint32_t inc = wave_incl_add(n);
+        const uint32_t ex = inc - n;
+        const uint32_t total = __shfl(inc, 63, 64);
+        int32_t carry = -1;
+        uint64_t run = 0;   // counts of the list's candidates before this round
+        for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+            mark[lane] = 0xFFFFFFFFu;
+            wave_lds_sync();
+            if (n && ex >= r0 && ex < r0 + 64)
+                mark[ex - r0] = lane;
+            wave_lds_sync();
+            const int32_t j = max(wave_incl_max((int32_t)mark[lane]), carry);
+            carry = __shfl(j, 63, 64);
+            const uint32_t k = r0 + lane;
+            const bool act = k < total;
+            const uint32_t tj = (uint32_t)(j < 0 ? 0 : j);
+            const uint32_t i = k - (uint32_t)__shfl((int)ex, (int)tj, 64);
+            const uint64_t off_j = __shfl(off, (int)tj, 64);
+            const uint32_t wt_j = job * JT + tj;
+            const uint32_t base = wt_j * MANY_WT;
+            const uint8_t *const g_text = a.text + base;
+            const uint4 ev = act ? ((const uint4 *)a.cands)[(size_t)wt_j * MANY_WCAP + i] : make_uint4(0, 0, 0, 0);
+            const uint32_t e = ev.x;
+            const uint32_t lp = e & (MANY_WT - 1u), rm = (e >> CAND_POS_BITS) & 0x7Fu;
+            uint32_t c = e >> CAND_CNT_SHIFT;
+            const bool recount = c == CAND_RECOUNT;
+            if (recount)
+                c = probe_merged<SMALL>(d, g_text, lp, (uint64_t)base + lp, rm, false, nullptr, nullptr, 0, 0);
+            const uint32_t cinc = wave_incl_add(c);
+            const uint64_t sk = run + (cinc - c);   // counts of the list before candidate k
+            if (act && i == 0)
+                first[tj] = sk;
+            wave_lds_sync();
+            const uint64_t r = off_j + (sk - (act ? first[tj] : sk));
+            // runs longer than the stored bits: queued, and probed 64 at a time
+            const bool again = act && recount && c && r < a.cap;
+            const uint64_t qb = __ballot(again);
+            if (again)
+                queue[qn + (uint32_t)__popcll(qb & below)] = make_uint4(base + lp, rm, (uint32_t)r, (uint32_t)(r >> 32));
+            qn += (uint32_t)__popcll(qb);
+            // The round's outputs [R0, R0 + span) go through LDS and out in
+            // coalesced stores when its candidates fill them without a gap (no
+            // queued run, no unstored wave tile between) and they fit the
+            // stage; otherwise each lane writes its own.
+            const uint64_t R0 = __shfl(r, 0, 64);   // lane 0 is active in every round
+            const uint32_t lastl = min(63u, total - 1u - r0);
+            const uint64_t span = __shfl(r + c, (int)lastl, 64) - R0;
+            const bool staged = __ballot(act && recount) == 0 && span == __shfl(cinc, 63, 64) && span <= MANY_STAGE;
+            if (act && c && !recount) {
+                const uint32_t gp = base + lp;
+                const uint32_t s0 = (uint32_t)(r - R0);
+                if (ev.y & CAND_SINGLE) {
+                    if (staged) {
+                        sgp[s0] = gp;
+                        sid[s0] = ev.y & 0x3FFFFFu;
+                    } else if (r < a.cap) {
+                        if (a.out_off) a.out_off[r] = gp;
+                        if (a.out_id) a.out_id[r] = ev.y & 0x3FFFFFu;
+                    }
+                } else {
+                    // the matching entries' ids, two loads in flight at a time
+                    uint64_t bm = (uint64_t)ev.w << 32 | ev.z;
+                    for (uint32_t q = 0; bm; q += 2) {
+                        const uint32_t b0 = (uint32_t)__builtin_ctzll(bm);
+                        bm &= bm - 1;
+                        const bool two = bm != 0;
+                        const uint32_t b1 = two ? (uint32_t)__builtin_ctzll(bm) : b0;
+                        bm &= two ? bm - 1 : bm;
+                        const uint32_t id0 = d.ents[ev.y + b0].x & 0x3FFFFFu, id1 = d.ents[ev.y + b1].x & 0x3FFFFFu;
+                        if (staged) {
+                            sgp[s0 + q] = gp;
+                            sid[s0 + q] = id0;
+                            if (two) {
+                                sgp[s0 + q + 1] = gp;
+                                sid[s0 + q + 1] = id1;
+                            }
+                        } else {
+                            if (r + q < a.cap) {
+                                if (a.out_off) a.out_off[r + q] = gp;
+                                if (a.out_id) a.out_id[r + q] = id0;
+                            }
+                            if (two && r + q + 1 < a.cap) {
+                                if (a.out_off) a.out_off[r + q + 1] = gp;
+                                if (a.out_id) a.out_id[r + q + 1] = id1;
+                            }
+                        }
+                    }
+                }
+            }
+            if (staged) {
+                wave_lds_sync();
+                for (uint32_t j = lane; j < (uint32_t)span && R0 + j < a.cap; j += 64) {
+                    if (a.out_off) a.out_off[R0 + j] = sgp[j];
+                    if (a.out_id) a.out_id[R0 + j] = sid[j];
+                }
+            }
+            run += __shfl(cinc, 63, 64);
+            wave_lds_sync();   // mark[] is rewritten by the next round
+            if (qn >= 64) {
+                probe_queued(64);
+                const uint4 rest = lane < qn - 64 ? queue[64 + lane] : make_uint4(0, 0, 0, 0);
+                wave_lds_sync();
+                if (lane < qn - 64)
+                    queue[lane] = rest;
+                qn -= 64;
+                wave_lds_sync();
+            }
+        }
+        if (qn) {
+            probe_queued(qn);
+            qn = 0;
+            wave_lds_sync();
+        }
+        uint64_t redo = __ballot(valid && st == TILE_LIST_NONE);
+        while (redo) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(redo);
+            redo &= redo - 1;
+            many_write_unstored<FS, true>(a, job * JT + j, __shfl(off, (int)j, 64));
         }
     }
 }
@@ -1159,12 +1567,14 @@ size_t ol_many_lds_bytes(const DictDev &d)
     return (size_t)(d.fwords - (d.fbits[4] ? MANY_PWORDS : 0u)) * 4;
 }
 
-template <bool WRITE, int FS>
+template <bool WRITE, int FS, bool MERGED>
 static hipError_t launch_many_fs(const ManyArgs &a, hipStream_t s)
 {
     const size_t lds = WRITE ? 0 : ol_many_lds_bytes(a.d);
-    const void *f = WRITE ? (const void *)many_write_kernel<MANY_THREADS, FS>
-                          : (const void *)many_count_kernel<MANY_THREADS, FS>;
+    const void *f = MERGED ? (WRITE ? (const void *)many_write_m_kernel<MANY_THREADS, FS>
+                                    : (const void *)many_count_m_kernel<MANY_THREADS, FS>)
+                           : (WRITE ? (const void *)many_write_kernel<MANY_THREADS, FS>
+                                    : (const void *)many_count_kernel<MANY_THREADS, FS>);
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const uint32_t cap = max_grid(f, MANY_THREADS, lds);
@@ -1181,13 +1591,21 @@ static hipError_t launch_many_fs(const ManyArgs &a, hipStream_t s)
         w.job_tiles = per_wave < 1 ? 1 : per_wave > MANY_JOB_MAX ? MANY_JOB_MAX : per_wave;
         const uint32_t jobs = (a.num_wtiles + w.job_tiles - 1) / w.job_tiles;
         const uint32_t need = (jobs + NWV - 1) / NWV;
-        hipLaunchKernelGGL((many_write_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap), dim3(MANY_THREADS), 0,
-                           s, w);
+        if (MERGED)
+            hipLaunchKernelGGL((many_write_m_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap),
+                               dim3(MANY_THREADS), 0, s, w);
+        else
+            hipLaunchKernelGGL((many_write_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap),
+                               dim3(MANY_THREADS), 0, s, w);
         return hipGetLastError();
     }
     const uint32_t need = (a.num_wtiles + NWV - 1) / NWV;
-    hipLaunchKernelGGL((many_count_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap), dim3(MANY_THREADS), lds, s,
-                       a);
+    if (MERGED)
+        hipLaunchKernelGGL((many_count_m_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap), dim3(MANY_THREADS),
+                           lds, s, a);
+    else
+        hipLaunchKernelGGL((many_count_kernel<MANY_THREADS, FS>), dim3(need < cap ? need : cap), dim3(MANY_THREADS),
+                           lds, s, a);
     return hipGetLastError();
 }
 
@@ -1197,12 +1615,13 @@ template <bool WRITE>
 static hipError_t launch_many(const ManyArgs &a, hipStream_t s)
 {
     const int fs = (a.d.fbits[4] ? (a.d.pf_hashed ? 0x30 : 0x10) : 0) | (a.d.fbits[1] || a.d.fbits[2] ? 0x02 : 0);
+    const bool m = a.d.merged != 0;
     switch (fs) {
-    case 0x10: return launch_many_fs<WRITE, 0x10>(a, s);
-    case 0x12: return launch_many_fs<WRITE, 0x12>(a, s);
-    case 0x30: return launch_many_fs<WRITE, 0x30>(a, s);
-    case 0x32: return launch_many_fs<WRITE, 0x32>(a, s);
-    case 0x02: return launch_many_fs<WRITE, 0x02>(a, s);
+    case 0x10: return m ? launch_many_fs<WRITE, 0x10, true>(a, s) : launch_many_fs<WRITE, 0x10, false>(a, s);
+    case 0x12: return m ? launch_many_fs<WRITE, 0x12, true>(a, s) : launch_many_fs<WRITE, 0x12, false>(a, s);
+    case 0x30: return m ? launch_many_fs<WRITE, 0x30, true>(a, s) : launch_many_fs<WRITE, 0x30, false>(a, s);
+    case 0x32: return m ? launch_many_fs<WRITE, 0x32, true>(a, s) : launch_many_fs<WRITE, 0x32, false>(a, s);
+    case 0x02: return m ? launch_many_fs<WRITE, 0x02, true>(a, s) : launch_many_fs<WRITE, 0x02, false>(a, s);
     default: return hipErrorInvalidValue;   // a dictionary has at least one filter
     }
 }

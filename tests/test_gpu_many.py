@@ -166,3 +166,48 @@ def test_many_tiles_per_workgroup_equal_single_pattern_searches(parser):
     assert tot == want_off.size
     assert np.array_equal(off, want_off)
     assert np.array_equal(ids, want_id)
+
+
+def merged_entries(patterns):
+    """Entries merged runs would hold for `patterns`, by the compiler's rule
+    (runs merge when this is at most 2 x count + 65,536): each pattern once in
+    the run of its own class and key, plus once more in the run of every
+    longer class's key that starts with it."""
+    from collections import Counter
+    groups = Counter((min(len(p), 4), bytes(p[:4])) for p in patterns)
+    total = len(patterns)
+    for (klen, key) in groups:
+        for shorter in range(klen - 1, 0, -1):
+            total += groups.get((shorter, key[:shorter]), 0)
+    return total
+
+
+def test_runs_around_the_stored_bits(parser):
+    """Merged runs of 63, 64 and 65 entries on one key (62..64 patterns on
+    "abcd" plus the 3-byte "abc" merged in), with up to 21 matches at one
+    position: a stored candidate records which of up to 64 entries match, a
+    longer run is probed again by the write pass."""
+    tail = b"efghijklmnopqrstuvwxyz0123456789ABCDEFGHIJKLMNOPQRSTUVWXYZ!?"
+    assert len(tail) == 60
+    text = b"".join(b"abcd" + tail[:k] + b"#" + b"." * (k % 5) for k in range(0, 61, 3)) * 30
+    for n in (63, 64, 65):
+        pats = [b"abc"] + [b"abcd" + tail[:i % 61] for i in range(n - 1)]
+        assert merged_entries(pats) <= 2 * len(pats) + 65536
+        assert_many_matches(parser, text, pats, f"run of {n}")
+
+
+def test_per_class_runs_when_merging_would_copy_too_much(parser):
+    """300 copies of b"e" under 400 longer keys that start with it: merged runs
+    would copy it 120,000 times, so each run keeps its own class and the
+    classes' runs are merged by id at every position."""
+    rng = np.random.default_rng(5)
+    words = [w for w in (datagen.vocab_word(i) for i in range(10_000)) if w[:1] == b"e" and len(w) >= 4]
+    longs = list(dict.fromkeys(words))[:300]
+    while len(longs) < 400:
+        w = b"e" + bytes(rng.integers(97, 123, size=int(rng.integers(3, 12))).astype(np.uint8))
+        if w not in longs:
+            longs.append(w)
+    pats = [b"e"] * 300 + longs + [b"ex", b"exa", b"q", b"ee"]
+    assert merged_entries(pats) > 2 * len(pats) + 65536
+    text = datagen.document(1 << 20, seed=21)
+    assert_many_matches(parser, text, pats, "per-class runs")
