@@ -923,9 +923,10 @@ __device__ __forceinline__ ManyCand many_candidates(const ManyArgs &a, uint32_t 
 }
 
 // Candidates [c0, c0 + MANY_WLIST) of the wave tile into the wave's LDS list:
-// wave-tile position and end bound (0: no match may start there).
-__device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyLane &m, const ManyCand &cand,
-                                                uint32_t c0, uint16_t *l_pos, uint8_t *l_rem)
+// wave-tile position and end bound (0: no match may start there).  Chunks are
+// taken in order; each removes the bits it lists from `cand`.
+__device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyLane &m, ManyCand &cand, uint32_t c0,
+                                                uint16_t *l_pos, uint8_t *l_rem)
 {
     uint32_t di = m.di0, dlim = m.dlim0, dnext = m.dnext0;
 #pragma unroll
@@ -935,19 +936,20 @@ __device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyLan
         while (q && at < c0 + MANY_WLIST) {
             const uint32_t lp = h * MANY_PART + m.lo + (uint32_t)__builtin_ctz(q);
             q &= q - 1;
-            if (at >= c0) {
-                const uint32_t gp = m.base + lp;
-                while (gp >= dnext) {
-                    di++;
-                    dlim = a.doc_ends[di];
-                    dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
-                }
-                const uint32_t rm = gp < dlim ? dlim - gp : 0u;
-                l_pos[at - c0] = (uint16_t)lp;
-                l_rem[at - c0] = (uint8_t)(rm < 64u ? rm : 64u);
+            const uint32_t gp = m.base + lp;
+            while (gp >= dnext) {
+                di++;
+                dlim = a.doc_ends[di];
+                dnext = (di + 1 < a.ndocs) ? a.doc_starts[di + 1] : 0xFFFFFFFFu;
             }
+            const uint32_t rm = gp < dlim ? dlim - gp : 0u;
+            l_pos[at - c0] = (uint16_t)lp;
+            l_rem[at - c0] = (uint8_t)(rm < 64u ? rm : 64u);
             at++;
         }
+        // the bits listed are consumed: the next chunk starts at the first not listed
+        cand.q[h] = q;
+        cand.ex[h] = at;
     }
 }
 
@@ -983,7 +985,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         W = many_load(a, wt, 0);
     for (; wt < a.num_wtiles; wt += stride) {
         const ManyLane m = many_lane_setup(a, wt);
-        const ManyCand cand = many_candidates<FS>(a, wt, wt + stride, m, pf, small, W);
+        ManyCand cand = many_candidates<FS>(a, wt, wt + stride, m, pf, small, W);
         const uint32_t total_c = cand.total;
         uint4 *const store = a.cands ? (uint4 *)a.cands + (size_t)wt * MANY_WCAP : nullptr;
         uint32_t kept = 0, tile_total = 0;
@@ -1073,7 +1075,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         W = many_load(a, wt, 0);
     for (; wt < a.num_wtiles; wt += stride) {
         const ManyLane m = many_lane_setup(a, wt);
-        const ManyCand cand = many_candidates<FS>(a, wt, wt + stride, m, pf, small, W);
+        ManyCand cand = many_candidates<FS>(a, wt, wt + stride, m, pf, small, W);
         const uint32_t total_c = cand.total;
         uint4 *const store = a.cands ? (uint4 *)a.cands + (size_t)wt * MANY_WCAP : nullptr;
         uint32_t kept = 0, tile_total = 0;
@@ -1369,24 +1371,41 @@ __global__ __launch_bounds__(THREADS) void many_write_m_kernel(ManyArgs a)
         const uint32_t ex = inc - n;
         const uint32_t total = __shfl(inc, 63, 64);
         int32_t carry = -1;
-        uint64_t run = 0;   // counts of the list's candidates before this round
-        for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+        // round rr's list entry on this lane: its tile (within the job) and
+        // index in the tile's stored candidates, and the stored record
+        auto locate = [&](uint32_t rr, uint32_t &tj_out, uint32_t &i_out) {
             mark[lane] = 0xFFFFFFFFu;
             wave_lds_sync();
-            if (n && ex >= r0 && ex < r0 + 64)
-                mark[ex - r0] = lane;
+            if (n && ex >= rr && ex < rr + 64)
+                mark[ex - rr] = lane;
             wave_lds_sync();
             const int32_t j = max(wave_incl_max((int32_t)mark[lane]), carry);
             carry = __shfl(j, 63, 64);
+            tj_out = (uint32_t)(j < 0 ? 0 : j);
+            i_out = rr + lane - (uint32_t)__shfl((int)ex, (int)tj_out, 64);
+            wave_lds_sync();   // mark[] is rewritten by the next call
+        };
+        auto record = [&](uint32_t rr, uint32_t tj_, uint32_t i_) -> uint4 {
+            return rr + lane < total ? ((const uint4 *)a.cands)[(size_t)(job * JT + tj_) * MANY_WCAP + i_]
+                                     : make_uint4(0, 0, 0, 0);
+        };
+        uint32_t tj = 0, i = 0;
+        if (total)
+            locate(0, tj, i);
+        uint4 ev = record(0, tj, i);
+        uint64_t run = 0;   // counts of the list's candidates before this round
+        for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+            // the next round's records load while this round is written
+            uint32_t tjn = 0, in = 0;
+            if (r0 + 64 < total)
+                locate(r0 + 64, tjn, in);
+            const uint4 evn = record(r0 + 64, tjn, in);
             const uint32_t k = r0 + lane;
             const bool act = k < total;
-            const uint32_t tj = (uint32_t)(j < 0 ? 0 : j);
-            const uint32_t i = k - (uint32_t)__shfl((int)ex, (int)tj, 64);
             const uint64_t off_j = __shfl(off, (int)tj, 64);
             const uint32_t wt_j = job * JT + tj;
             const uint32_t base = wt_j * MANY_WT;
             const uint8_t *const g_text = a.text + base;
-            const uint4 ev = act ? ((const uint4 *)a.cands)[(size_t)wt_j * MANY_WCAP + i] : make_uint4(0, 0, 0, 0);
             const uint32_t e = ev.x;
             const uint32_t lp = e & (MANY_WT - 1u), rm = (e >> CAND_POS_BITS) & 0x7Fu;
             uint32_t c = e >> CAND_CNT_SHIFT;
@@ -1462,7 +1481,10 @@ __global__ __launch_bounds__(THREADS) void many_write_m_kernel(ManyArgs a)
                 }
             }
             run += __shfl(cinc, 63, 64);
-            wave_lds_sync();   // mark[] is rewritten by the next round
+            tj = tjn;
+            i = in;
+            ev = evn;
+            wave_lds_sync();   // first[] and the stage are rewritten by the next round
             if (qn >= 64) {
                 probe_queued(64);
                 const uint4 rest = lane < qn - 64 ? queue[64 + lane] : make_uint4(0, 0, 0, 0);
