@@ -2022,6 +2022,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     const size_t o_t = placeb(tab.size() * 4);
     const size_t o_ent = placeb(ents.size() * 4);
     const size_t o_pool = placeb(pool.size() * 4);
+    const size_t o_ids = placeb(ents.size());   // one u32 per 16-byte entry
     ol_dict *s = new (std::nothrow) ol_dict();
     if (!s)
         return fail(OL_ERR_NOMEM, "out of host memory");
@@ -2038,6 +2039,10 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     std::memcpy(host.data() + o_t, tab.data(), tab.size() * 4);
     std::memcpy(host.data() + o_ent, ents.data(), ents.size() * 4);
     std::memcpy(host.data() + o_pool, pool.data(), pool.size() * 4);
+    for (size_t e = 0; e < ents.size() / 4; e++) {
+        const uint32_t id = ents[4 * e] & 0x3FFFFFu;
+        std::memcpy(host.data() + o_ids + 4 * e, &id, 4);
+    }
     if (hipMemcpy(base, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(s->dev);
         delete s;
@@ -2047,6 +2052,7 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     dd.table = (const uint4 *)(base + o_t);
     dd.ents = (const uint4 *)(base + o_ent);
     dd.pool = (const uint32_t *)(base + o_pool);
+    dd.ent_ids = (const uint32_t *)(base + o_ids);
     s->d = dd;
     s->count = (uint32_t)count;
     *out = s;

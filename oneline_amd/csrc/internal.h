@@ -387,6 +387,7 @@ struct DictDev {
     const uint4 *table;      // (DK_BUCKET << bbits) slots
     const uint4 *ents;       // run entries
     const uint32_t *pool;    // pattern bytes 8.. of patterns longer than 12 bytes, word aligned, zero padded
+    const uint32_t *ent_ids; // the pattern id of each entry (the entries' info words without length and class)
     uint32_t max_len;
     uint32_t pf_hashed;      // prefix filter keying: 0 dict_pword/dict_pbit, 1 dict_ph (dense filters)
     // Merged runs (1): the run of a key also holds every shorter pattern that

@@ -1334,8 +1334,10 @@ __global__ __launch_bounds__(THREADS) void many_write_kernel(ManyArgs a)
 
 // Write pass of the merged layout: as many_write_kernel, but a stored
 // candidate names its matches (the single pattern's id, or its run's first
-// entry and which entries match), so its ids are read from the entries, in
-// id order, and nothing is probed again except runs longer than 64 entries.
+// entry and which entries match), so its ids are read from the entries' id
+// array (4 bytes per entry, which L2 keeps better than the 16-byte entries),
+// in id order, and nothing is probed again except runs longer than 64
+// entries.
 template <int THREADS, int FS>
 __global__ __launch_bounds__(THREADS) void many_write_m_kernel(ManyArgs a)
 {
@@ -1444,30 +1446,30 @@ __global__ __launch_bounds__(THREADS) void many_write_m_kernel(ManyArgs a)
                         if (a.out_id) a.out_id[r] = ev.y & 0x3FFFFFu;
                     }
                 } else {
-                    // the matching entries' ids, two loads in flight at a time
+                    // the matching entries' ids, four loads in flight at a time
                     uint64_t bm = (uint64_t)ev.w << 32 | ev.z;
-                    for (uint32_t q = 0; bm; q += 2) {
-                        const uint32_t b0 = (uint32_t)__builtin_ctzll(bm);
-                        bm &= bm - 1;
-                        const bool two = bm != 0;
-                        const uint32_t b1 = two ? (uint32_t)__builtin_ctzll(bm) : b0;
-                        bm &= two ? bm - 1 : bm;
-                        const uint32_t id0 = d.ents[ev.y + b0].x & 0x3FFFFFu, id1 = d.ents[ev.y + b1].x & 0x3FFFFFu;
-                        if (staged) {
-                            sgp[s0 + q] = gp;
-                            sid[s0 + q] = id0;
-                            if (two) {
-                                sgp[s0 + q + 1] = gp;
-                                sid[s0 + q + 1] = id1;
-                            }
-                        } else {
-                            if (r + q < a.cap) {
-                                if (a.out_off) a.out_off[r + q] = gp;
-                                if (a.out_id) a.out_id[r + q] = id0;
-                            }
-                            if (two && r + q + 1 < a.cap) {
-                                if (a.out_off) a.out_off[r + q + 1] = gp;
-                                if (a.out_id) a.out_id[r + q + 1] = id1;
+                    for (uint32_t q = 0; bm; q += 4) {
+                        uint32_t b[4], nb = 0;
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            b[t] = bm ? (uint32_t)__builtin_ctzll(bm) : b[0];
+                            nb += bm ? 1u : 0u;
+                            bm &= bm ? bm - 1 : bm;
+                        }
+                        uint32_t id[4];
+#pragma unroll
+                        for (int t = 0; t < 4; t++)
+                            id[t] = d.ent_ids[ev.y + b[t]];
+#pragma unroll
+                        for (uint32_t t = 0; t < 4; t++) {
+                            if (t >= nb)
+                                break;
+                            if (staged) {
+                                sgp[s0 + q + t] = gp;
+                                sid[s0 + q + t] = id[t];
+                            } else if (r + q + t < a.cap) {
+                                if (a.out_off) a.out_off[r + q + t] = gp;
+                                if (a.out_id) a.out_id[r + q + t] = id[t];
                             }
                         }
                     }
