@@ -147,6 +147,28 @@ int ol_tree_rebuild_async(ol_ctx *ctx, const ol_doc *doc, ol_tree *tree);
 int ol_tree_rebuild_links_async(ol_ctx *ctx, const ol_doc *doc, ol_tree *tree, uint32_t links);
 int ol_tree_check(ol_ctx *ctx, const ol_tree *tree);
 
+/* ---- AST bridge: structural pass ---------------------------------------------------- */
+/* The line-structure half of to_tomet_ast_with_links
+ * (crates/oneline-tomet/src/lib.rs:20-90): literal_pipe_lines (lib.rs:96-125),
+ * has_pipe_child (lib.rs:33-41), which lines get a block (lib.rs:44-60) and
+ * the stitching walk's parent for each block (lib.rs:66-86).  The blocks
+ * themselves need tomet's own parser (lib.rs:128-310) and stay with the
+ * caller.  Per line (host arrays of ol_doc_line_count entries; either may be
+ * NULL):
+ *   flags[i]     = OL_BRIDGE_* bits;
+ *   container[i] = the line whose block receives line i's block in the
+ *                  stitching walk (before finish() keeps or hoists it), or
+ *                  OL_NONE when the block goes to the top level or line i has
+ *                  no block.
+ * `tree` must hold the links of `doc` with parent written. */
+#define OL_BRIDGE_LITERAL 1u      /* a `|` line tomet reads as plain text */
+#define OL_BRIDGE_BLOCK 2u        /* the line becomes a block (not blank, not a comment, text after trim) */
+#define OL_BRIDGE_CUT 4u          /* a bare blank line lies between the line and its parent: parent NONE */
+#define OL_BRIDGE_PIPE_CHILD 8u   /* an element whose `|` lines are its content */
+#define OL_BRIDGE_BARE_BLANK 16u  /* an empty line with no `|` */
+int ol_tree_bridge(ol_ctx *ctx, const ol_doc *doc, const ol_tree *tree, uint8_t *flags,
+                   uint32_t *container);
+
 /* ---- literal search --------------------------------------------------------------- */
 /* GpuParser::search_document (crates/oneline-gpu/src/search.rs:84): 1..=64-byte
  * pattern; total = every overlapping occurrence; offsets = the first

@@ -17,7 +17,7 @@ import ctypes
 import os
 import sys
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -25,6 +25,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboneline_hip.so")
 
 NONE = 0xFFFFFFFF
+# ol_tree_bridge flags (include/oneline_hip.h)
+BRIDGE_LITERAL, BRIDGE_BLOCK, BRIDGE_CUT, BRIDGE_PIPE_CHILD, BRIDGE_BARE_BLANK = 1, 2, 4, 8, 16
 MAX_GPU_INPUT_LEN = 4294967292
 MAX_PATTERN_LEN = 64
 MIN_SET_PATTERN_LEN = 1
@@ -95,6 +97,7 @@ def load_library() -> ctypes.CDLL:
         "ol_tree_rebuild_async": [P, P, P],
         "ol_tree_rebuild_links_async": [P, P, P, ctypes.c_uint32],
         "ol_tree_check": [P, P],
+        "ol_tree_bridge": [P, P, P, P, P],
         "ol_search_document": [P, P, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_raw": [P, P, S, P, S, S, U32P, ctypes.POINTER(P), ctypes.POINTER(S)],
         "ol_search_device": [P, P, P, S, P, S, U64P],
@@ -530,6 +533,18 @@ class GpuParser:
         arrs = [np.zeros(L, np.uint32) for _ in range(4)]
         _check(_lib.ol_tree_download(self._ctx, tree._h, *[_ptr(a) for a in arrs]))
         return TreeLinks(*arrs)
+
+    # -- AST bridge ---------------------------------------------------------------------
+    def bridge(self, doc: GpuDocument, tree: GpuTree) -> Tuple[np.ndarray, np.ndarray]:
+        """The structural pass of to_tomet_ast_with_links
+        (crates/oneline-tomet/src/lib.rs:20-90): per line the BRIDGE_* flags and
+        the line whose block receives its block (NONE at the top level or
+        without a block)."""
+        L = doc.line_count()
+        flags = np.zeros(L, np.uint8)
+        container = np.zeros(L, np.uint32)
+        _check(_lib.ol_tree_bridge(self._ctx, doc._h, tree._h, _ptr(flags), _ptr(container)))
+        return flags, container
 
     # -- literal search ---------------------------------------------------------------
     def search_document(self, doc: GpuDocument, pattern: bytes, max_matches: int) -> SearchResult:

@@ -1583,6 +1583,64 @@ int ol_tree_download(ol_ctx *c, const ol_tree *t, uint32_t *parent, uint32_t *su
     return OL_OK;
 }
 
+// ---- AST bridge: structural pass ---------------------------------------------------
+int ol_tree_bridge(ol_ctx *c, const ol_doc *d, const ol_tree *t, uint8_t *flags, uint32_t *container)
+{
+    if (!c || !d || !t)
+        return fail(OL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (t->lines != d->lines)
+        return fail(OL_ERR_INVALID, "the tree has " + std::to_string(t->lines) + " lines, the document " +
+                                        std::to_string(d->lines));
+    if (t->lines && !(t->links & OL_LINK_PARENT))
+        return fail(OL_ERR_INVALID, "the tree's last build did not write parent");
+    const uint64_t lines = d->lines;
+    if (lines == 0)
+        return OL_OK;
+    const uint32_t blocks = ol_bridge_blocks(lines);
+    Scratch &s = c->scr;
+    HIPCHK(ensure_status(s, (size_t)blocks + 8));
+    soil(s, (size_t)blocks + 8);
+    DevBuf fl, ct;
+    hipError_t e = fl.ensure((size_t)blocks * 4096);
+    if (e == hipSuccess)
+        e = ct.ensure((size_t)blocks * 4096 * 4);
+    if (e != hipSuccess) {
+        fl.release();
+        return fail(OL_ERR_NOMEM, "device allocation of the bridge outputs failed");
+    }
+    BridgeArgs a{};
+    a.text = d->text;
+    a.n = (uint32_t)d->n;
+    a.lines = lines;
+    a.offsets = d->offsets;
+    a.classes = d->classes;
+    a.keys = d->keys;
+    a.nodes = d->nodes;
+    a.parent = t->parent;
+    a.flags = (uint8_t *)fl.p;
+    a.container = (uint32_t *)ct.p;
+    a.blk_last = (uint32_t *)s.words();
+    a.blk_in = a.blk_last + blocks;
+    {
+        Prof p(c, "bridge");
+        e = ol_launch_bridge(a, c->stream);
+    }
+    mark(c, "bridge: launch");
+    if (e == hipSuccess && flags)
+        e = hipMemcpyAsync(flags, fl.p, lines, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && container)
+        e = hipMemcpyAsync(container, ct.p, lines * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    mark(c, "bridge: wait for the GPU + download");
+    fl.release();
+    ct.release();
+    if (e != hipSuccess)
+        return fail(OL_ERR_DEVICE, std::string("bridge: ") + hipGetErrorString(e));
+    return OL_OK;
+}
+
 // ---- literal search -------------------------------------------------------------
 int ol_search_document(ol_ctx *c, const ol_doc *d, const uint8_t *pattern, size_t len, size_t max_matches,
                        uint32_t *total, uint32_t **offsets, size_t *n_offsets)

@@ -534,6 +534,24 @@ struct QueryArgs {
     uint64_t cap;
 };
 hipError_t ol_launch_query(const QueryArgs &a, uint64_t *total, hipStream_t s);
+// The AST bridge's structural pass (scan.hip): per line the bridge flags
+// (1 literal, 2 block, 4 cut, 8 pipe child, 16 bare blank) and the container
+// line of each block.  flags and container cover whole blocks of 4,096 lines.
+struct BridgeArgs {
+    const uint8_t *text;
+    uint32_t n;
+    uint64_t lines;
+    const uint32_t *offsets;
+    const uint8_t *classes, *keys;
+    const ol_node_dev *nodes;   // nullable
+    const uint32_t *parent;
+    uint8_t *flags;             // [ol_bridge_blocks * 4096]
+    uint32_t *container;        // [ol_bridge_blocks * 4096]
+    uint32_t *blk_last;         // [blocks] each block's last bare blank line + 1 (0: none)
+    uint32_t *blk_in;           // [blocks] the last one before each block
+};
+hipError_t ol_launch_bridge(const BridgeArgs &a, hipStream_t s);
+uint32_t ol_bridge_blocks(uint64_t lines);
 uint32_t ol_query_blocks(uint32_t r0, uint32_t r1);   // count blocks of a query over lines [r0, r1)
 hipError_t ol_launch_parse_given(const uint8_t *text, uint32_t n, const uint32_t *offsets,
                                  uint64_t lines, ol_node_dev *nodes, hipStream_t s);

@@ -1058,6 +1058,261 @@ uint32_t ol_query_blocks(uint32_t r0, uint32_t r1)
     return (uint32_t)(((uint64_t)lines + QUERY_BLOCK - 1) / QUERY_BLOCK);
 }
 
+// ---- AST bridge: the structural pass of to_tomet_ast_with_links -------------
+// crates/oneline-tomet/src/lib.rs:20-90 walks the lines in order
+// (literal_pipe_lines keeps the last bare blank line seen and each line's
+// literal flag depends on its parent's) and stitches blocks from the last
+// line back.  Both are sequential there; here every line is independent
+// given one number, b(i) = the last bare blank line before i (a prefix
+// maximum over the lines):
+//   - a line of depth > 0 is literal when, walking up its parent chain, a
+//     parent is NONE, or lies before b(i) (a bare blank line between them:
+//     the cut), or is not an element, before a depth-0 parent is reached.
+//     The chain's lines are content lines, so no bare blank lies at them,
+//     and b(i) is also the last bare blank before every chain line above a
+//     cut, which is why one b serves the whole chain;
+//   - the line's parent after the cut is NONE when b(i) > parent(i);
+//   - a line's block goes to the nearest ancestor along the cut parents that
+//     has a block (lines without one hand their children on, lib.rs:66-86);
+//   - has_pipe_child(p) (lib.rs:33-41): the non-cut children of p are
+//     literal exactly when p is not an element or is literal, and if any
+//     child is not cut then the first is not (a cut of the first child cuts
+//     all later ones): so p is a non-literal element whose next content line
+//     is deeper, with no bare blank line before it.
+// Three launches over blocks of BRIDGE_BLOCK lines (16 per thread): marks
+// (block and bare-blank bits, each block's last bare blank), a prefix
+// maximum of those over the blocks, and the per-line chains.
+constexpr uint32_t BRIDGE_THREADS = 256, BRIDGE_PER_THREAD = 16;
+constexpr uint32_t BRIDGE_BLOCK = BRIDGE_THREADS * BRIDGE_PER_THREAD;
+enum : uint32_t { BR_LITERAL = 1u, BR_BLOCK = 2u, BR_CUT = 4u, BR_PIPE = 8u, BR_BARE = 16u };
+
+// char::is_whitespace (Unicode White_Space) at text[p .. e): its length in
+// bytes, 0 when it is not whitespace (or not valid UTF-8 there).
+__device__ __forceinline__ uint32_t bridge_space(const uint8_t *t, uint32_t p, uint32_t e)
+{
+    const uint32_t c = t[p];
+    if ((c >= 0x09u && c <= 0x0Du) || c == 0x20u)
+        return 1;
+    if (c < 0xC2u || c > 0xE3u)
+        return 0;
+    const uint32_t c1 = p + 1 < e ? t[p + 1] : 0u;
+    if (c == 0xC2u)
+        return (c1 == 0x85u || c1 == 0xA0u) ? 2u : 0u;   // U+0085, U+00A0
+    const uint32_t c2 = p + 2 < e ? t[p + 2] : 0u;
+    if (c == 0xE1u)
+        return (c1 == 0x9Au && c2 == 0x80u) ? 3u : 0u;   // U+1680
+    if (c == 0xE2u) {
+        if (c1 == 0x80u && ((c2 >= 0x80u && c2 <= 0x8Au) || c2 == 0xA8u || c2 == 0xA9u || c2 == 0xAFu))
+            return 3;                                    // U+2000..200A, U+2028, U+2029, U+202F
+        return (c1 == 0x81u && c2 == 0x9Fu) ? 3u : 0u;   // U+205F
+    }
+    if (c == 0xE3u)
+        return (c1 == 0x80u && c2 == 0x80u) ? 3u : 0u;   // U+3000
+    return 0;
+}
+
+// Line i's depth (when its key does not hold it) and content span: from the
+// FlatNodes when the document keeps them, else the line parsed again.
+__device__ __forceinline__ ol_node_dev bridge_node(const BridgeArgs &a, uint32_t i)
+{
+    if (a.nodes)
+        return a.nodes[i];
+    const uint32_t s = a.offsets[i];
+    const uint32_t e = (i + 1 < a.lines) ? a.offsets[i + 1] - 1 : a.n - 1;
+    return parse_span(GlobalSrc{a.text}, s, e);
+}
+
+__global__ __launch_bounds__(BRIDGE_THREADS) void bridge_mark_kernel(BridgeArgs a)
+{
+    __shared__ uint32_t s_w[BRIDGE_THREADS / 64];
+    const uint32_t i0 = blockIdx.x * BRIDGE_BLOCK + threadIdx.x * BRIDGE_PER_THREAD;
+    uint32_t f[BRIDGE_PER_THREAD];
+    uint32_t last = 0;   // the thread's last bare blank line + 1
+    if (i0 < a.lines) {
+        const uint4 cl = *(const uint4 *)(a.classes + i0), ky = *(const uint4 *)(a.keys + i0);
+        const uint32_t cw[4] = {cl.x, cl.y, cl.z, cl.w}, kw[4] = {ky.x, ky.y, ky.z, ky.w};
+#pragma unroll
+        for (uint32_t k = 0; k < BRIDGE_PER_THREAD; k++) {
+            const uint32_t i = i0 + k;
+            const uint32_t kind = (cw[k >> 2] >> (8 * (k & 3))) & 0xFFu, key = (kw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+            uint32_t fl = 0;
+            if (i < a.lines) {
+                if (key == KEY_BLANK) {
+                    if (bridge_node(a, i).depth == 0) {
+                        fl = BR_BARE;
+                        last = i + 1;
+                    }
+                } else if (kind != OLK_COMMENT) {
+                    // a block unless the content is whitespace only
+                    const ol_node_dev nd = bridge_node(a, i);
+                    const uint32_t e = nd.span_start + nd.span_len;
+                    uint32_t p = nd.span_start, w = 1;
+                    while (p < e && (w = bridge_space(a.text, p, e)) != 0)
+                        p += w;
+                    fl = p < e ? BR_BLOCK : 0u;
+                }
+            }
+            f[k] = fl;
+        }
+        *(uint4 *)(a.flags + i0) = make_uint4(f[0] | f[1] << 8 | f[2] << 16 | f[3] << 24,
+                                              f[4] | f[5] << 8 | f[6] << 16 | f[7] << 24,
+                                              f[8] | f[9] << 8 | f[10] << 16 | f[11] << 24,
+                                              f[12] | f[13] << 8 | f[14] << 16 | f[15] << 24);
+    }
+    // the block's last bare blank line (+ 1, 0: none)
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t m = (uint32_t)wave_incl_max((int32_t)last);
+    if (lane == 63)
+        s_w[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t k = 0; k < BRIDGE_THREADS / 64; k++)
+            t = max(t, s_w[k]);
+        a.blk_last[blockIdx.x] = t;
+    }
+}
+
+// Exclusive prefix maximum of blk_last into blk_in (one workgroup, chunks of 1024).
+__global__ __launch_bounds__(1024) void bridge_scan_kernel(BridgeArgs a, uint32_t blocks)
+{
+    __shared__ uint32_t s_w[16];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < blocks; c0 += 1024) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < blocks ? a.blk_last[i] : 0u;
+        const uint32_t inc = (uint32_t)wave_incl_max((int32_t)v);
+        if (lane == 63)
+            s_w[w] = inc;
+        __syncthreads();
+        uint32_t before = carry, tot = carry;
+        for (uint32_t k = 0; k < 16; k++) {
+            before = k < w ? max(before, s_w[k]) : before;
+            tot = max(tot, s_w[k]);
+        }
+        const uint32_t prev = __shfl_up(inc, 1, 64);
+        if (i < blocks)
+            a.blk_in[i] = max(before, lane ? prev : 0u);
+        carry = tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(BRIDGE_THREADS) void bridge_link_kernel(BridgeArgs a)
+{
+    __shared__ uint32_t s_w[BRIDGE_THREADS / 64];
+    const uint32_t i0 = blockIdx.x * BRIDGE_BLOCK + threadIdx.x * BRIDGE_PER_THREAD;
+    const bool any = i0 < a.lines;
+    uint4 fv = make_uint4(0, 0, 0, 0);
+    if (any)
+        fv = *(const uint4 *)(a.flags + i0);
+    const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
+    // the thread's last bare blank line (+ 1), then the last one before its lines
+    uint32_t last = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < BRIDGE_PER_THREAD; k++)
+        if ((fw[k >> 2] >> (8 * (k & 3))) & BR_BARE)
+            last = i0 + k + 1;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t inc = (uint32_t)wave_incl_max((int32_t)last);
+    if (lane == 63)
+        s_w[w] = inc;
+    __syncthreads();
+    uint32_t b1 = a.blk_in[blockIdx.x];
+    for (uint32_t k = 0; k < BRIDGE_THREADS / 64; k++)
+        b1 = k < w ? max(b1, s_w[k]) : b1;
+    const uint32_t prev = __shfl_up(inc, 1, 64);
+    b1 = max(b1, lane ? prev : 0u);
+    if (!any)
+        return;
+    const uint4 cl = *(const uint4 *)(a.classes + i0), ky = *(const uint4 *)(a.keys + i0);
+    const uint32_t cw[4] = {cl.x, cl.y, cl.z, cl.w}, kw[4] = {ky.x, ky.y, ky.z, ky.w};
+    uint32_t out[BRIDGE_PER_THREAD], cont[BRIDGE_PER_THREAD];
+    for (uint32_t k = 0; k < BRIDGE_PER_THREAD; k++) {
+        const uint32_t i = i0 + k;
+        const uint32_t kind = (cw[k >> 2] >> (8 * (k & 3))) & 0xFFu, key = (kw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+        uint32_t fl = (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+        uint32_t ct = OL_NONE;
+        // b1: the last bare blank line before line i, + 1 (0: none)
+        auto cut_at = [&](uint32_t p) { return b1 > p + 1; };
+        if (i < a.lines) {
+            if (key == KEY_BLANK) {
+                if (!(fl & BR_BARE))
+                    fl |= BR_LITERAL;   // a `|` line with no content: parent NONE
+            } else if (key != 0) {
+                // literal: up the parent chain to a depth-0 parent
+                uint32_t x = i, lit = 0;
+                for (uint32_t step = 0; step < TREE_D; step++) {
+                    const uint32_t p = a.parent[x];
+                    if (p == OL_NONE || cut_at(p) || a.classes[p] != OLK_ELEMENT) {
+                        lit = 1;
+                        break;
+                    }
+                    if (a.keys[p] == 0)
+                        break;
+                    x = p;
+                }
+                const uint32_t p0 = a.parent[i];
+                fl |= lit ? BR_LITERAL : 0u;
+                fl |= (p0 != OL_NONE && cut_at(p0)) ? BR_CUT : 0u;
+            }
+            if ((fl & BR_BLOCK) && key != 0) {
+                // the nearest ancestor with a block along the cut parents
+                uint32_t x = i;
+                for (uint32_t step = 0; step < TREE_D; step++) {
+                    const uint32_t p = a.parent[x];
+                    if (p == OL_NONE || cut_at(p))
+                        break;
+                    if (a.flags[p] & BR_BLOCK) {
+                        ct = p;
+                        break;
+                    }
+                    x = p;
+                }
+            }
+            if (kind == OLK_ELEMENT && !(fl & BR_LITERAL)) {
+                // a `|` child: the next content line is deeper, no bare blank line before it
+                uint32_t j = i + 1;
+                bool blocked = false;
+                while (j < a.lines && a.keys[j] == KEY_BLANK) {
+                    if (a.flags[j] & BR_BARE) {
+                        blocked = true;
+                        break;
+                    }
+                    j++;
+                }
+                if (!blocked && j < a.lines && a.keys[j] < KEY_DEEP && a.keys[j] > key)
+                    fl |= BR_PIPE;
+            }
+            if (fl & BR_BARE)
+                b1 = i + 1;
+        }
+        out[k] = fl;
+        cont[k] = ct;
+    }
+    *(uint4 *)(a.flags + i0) = make_uint4(out[0] | out[1] << 8 | out[2] << 16 | out[3] << 24,
+                                          out[4] | out[5] << 8 | out[6] << 16 | out[7] << 24,
+                                          out[8] | out[9] << 8 | out[10] << 16 | out[11] << 24,
+                                          out[12] | out[13] << 8 | out[14] << 16 | out[15] << 24);
+#pragma unroll
+    for (uint32_t q = 0; q < BRIDGE_PER_THREAD / 4; q++)
+        *(uint4 *)(a.container + i0 + 4 * q) = make_uint4(cont[4 * q], cont[4 * q + 1], cont[4 * q + 2], cont[4 * q + 3]);
+}
+
+hipError_t ol_launch_bridge(const BridgeArgs &a, hipStream_t s)
+{
+    if (a.lines == 0)
+        return hipSuccess;
+    const uint32_t blocks = ol_bridge_blocks(a.lines);
+    hipLaunchKernelGGL(bridge_mark_kernel, dim3(blocks), dim3(BRIDGE_THREADS), 0, s, a);
+    hipLaunchKernelGGL(bridge_scan_kernel, dim3(1), dim3(1024), 0, s, a, blocks);
+    hipLaunchKernelGGL(bridge_link_kernel, dim3(blocks), dim3(BRIDGE_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+uint32_t ol_bridge_blocks(uint64_t lines) { return (uint32_t)((lines + BRIDGE_BLOCK - 1) / BRIDGE_BLOCK); }
+
 // Classes and tree keys from FlatNodes (documents loaded from a .olb file,
 // whose nodes were parsed when it was written).
 __global__ void nodes_to_lines_kernel(const ol_node_dev *nodes, uint64_t lines, uint8_t *classes, uint8_t *keys)

@@ -314,3 +314,123 @@ size_t orc_many_find_all(const uint8_t *t, size_t n, const uint8_t *pool,
     free(ids); free(order);
     return k;
 }
+
+/* ---- the AST bridge's structural pass ----------------------------------------
+ * to_tomet_ast_with_links, crates/oneline-tomet/src/lib.rs:20-90, up to the
+ * point where it needs tomet's own parser (the per-line blocks and
+ * finish/nest_sections, lib.rs:143-310): which lines are literal `|` text,
+ * where a bare blank line cuts a line from its element, which lines have a
+ * block, which elements get `|` content, and the block each block is attached
+ * to by the stitching walk.  Per line:
+ *   flags[i]     = 1 literal | 2 has a block | 4 cut by a bare blank line
+ *                  | 8 has a `|` child (has_pipe_child) | 16 bare blank line
+ *   container[i] = the line whose block receives line i's block, NONE at the
+ *                  top level or when line i has no block.
+ *
+ * char::is_whitespace (Unicode White_Space) at s: the character's length in
+ * bytes, 0 when it is not whitespace (or not valid UTF-8 there). */
+static size_t orc_utf8_space(const uint8_t *s, size_t left)
+{
+    const uint8_t c = s[0];
+    if ((c >= 0x09 && c <= 0x0D) || c == 0x20)
+        return 1;
+    if (c == 0xC2 && left >= 2 && (s[1] == 0x85 || s[1] == 0xA0))
+        return 2;                                   /* U+0085, U+00A0 */
+    if (left >= 3) {
+        if (c == 0xE1 && s[1] == 0x9A && s[2] == 0x80)
+            return 3;                               /* U+1680 */
+        if (c == 0xE2 && s[1] == 0x80 &&
+            ((s[2] >= 0x80 && s[2] <= 0x8A) || s[2] == 0xA8 || s[2] == 0xA9 || s[2] == 0xAF))
+            return 3;                               /* U+2000..200A, U+2028, U+2029, U+202F */
+        if (c == 0xE2 && s[1] == 0x81 && s[2] == 0x9F)
+            return 3;                               /* U+205F */
+        if (c == 0xE3 && s[1] == 0x80 && s[2] == 0x80)
+            return 3;                               /* U+3000 */
+    }
+    return 0;
+}
+
+/* content_of(node).trim() is not empty, crates/oneline-core/src/lib.rs:136-140 */
+static int orc_has_text(const uint8_t *text, const orc_node *nd)
+{
+    size_t p = nd->span_start, e = (size_t)nd->span_start + nd->span_len;
+    while (p < e) {
+        size_t w = orc_utf8_space(text + p, e - p);
+        if (!w)
+            return 1;
+        p += w;
+    }
+    return 0;
+}
+
+void orc_bridge(const uint8_t *text, const orc_node *nodes, const uint32_t *parent, size_t n,
+                uint8_t *flags, uint32_t *container)
+{
+    uint8_t *literal = (uint8_t *)calloc(n ? n : 1, 1);
+    uint8_t *pipe = (uint8_t *)calloc(n ? n : 1, 1);
+    uint32_t *par = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    /* literal_pipe_lines, lib.rs:96-125 */
+    size_t last_blank = (size_t)-1;
+    for (size_t i = 0; i < n; i++) {
+        par[i] = parent[i];
+        if (nodes[i].kind == K_BLANK && nodes[i].depth == 0) {
+            last_blank = i;
+            continue;
+        }
+        if (nodes[i].depth == 0)
+            continue;
+        const uint32_t p = parent[i];
+        if (p == ORC_NONE) {
+            literal[i] = 1;
+        } else if (last_blank != (size_t)-1 && last_blank > p) {
+            par[i] = ORC_NONE;
+            literal[i] = 1;
+        } else {
+            literal[i] = nodes[p].kind != K_ELEMENT || literal[p];
+        }
+    }
+    /* has_pipe_child, lib.rs:33-41 */
+    for (size_t i = 0; i < n; i++)
+        if (!literal[i] && nodes[i].depth > 0 && par[i] != ORC_NONE && par[i] < n)
+            pipe[par[i]] = 1;
+    /* blocks, lib.rs:44-60: none for Blank and Comment lines and for content
+     * that trims to nothing */
+    for (size_t i = 0; i < n; i++) {
+        const int block = nodes[i].kind != K_BLANK && nodes[i].kind != K_COMMENT && orc_has_text(text, &nodes[i]);
+        flags[i] = (uint8_t)(literal[i] | block << 1 | (par[i] != parent[i]) << 2 | pipe[i] << 3 |
+                             (nodes[i].kind == K_BLANK && nodes[i].depth == 0) << 4);
+        container[i] = ORC_NONE;
+    }
+    /* the stitching walk, lib.rs:66-86, with line numbers for blocks: from the
+     * last line back, a line with a block takes the pending blocks of its
+     * children and is itself pending at its parent; a line without one hands
+     * its children's on.  Pending lists as linked lists (head, tail, next). */
+    uint32_t *head = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *tail = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *next = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    for (size_t i = 0; i < n; i++)
+        head[i] = tail[i] = next[i] = ORC_NONE;
+    for (size_t ii = n; ii-- > 0;) {
+        uint32_t fh, ft;   /* the finished list */
+        if (flags[ii] & 2) {
+            for (uint32_t c = head[ii]; c != ORC_NONE; c = next[c])
+                container[c] = (uint32_t)ii;
+            fh = ft = (uint32_t)ii;
+            next[ii] = ORC_NONE;
+        } else {
+            fh = head[ii];
+            ft = tail[ii];
+        }
+        const uint32_t p = par[ii];
+        if (fh == ORC_NONE || p == ORC_NONE)
+            continue;   /* nothing finished, or it goes to the top level (container NONE) */
+        if (head[p] == ORC_NONE) {
+            head[p] = fh;
+        } else {
+            next[tail[p]] = fh;
+        }
+        tail[p] = ft;
+    }
+    free(head); free(tail); free(next);
+    free(literal); free(pipe); free(par);
+}

@@ -53,6 +53,8 @@ def lib():
         o.orc_links_par.restype = ctypes.c_int
         o.orc_find_all_par.argtypes = [P, S, P, S, ctypes.c_int, P, S]
         o.orc_find_all_par.restype = S
+        o.orc_bridge.argtypes = [P, P, P, S, P, P]
+        o.orc_bridge.restype = None
         o.orc_ac_build.argtypes = [P, P, P, S]
         o.orc_ac_build.restype = P
         o.orc_ac_free.argtypes = [P]
@@ -223,3 +225,17 @@ class AhoCorasick:
             self.close()
         except Exception:
             pass
+
+
+def bridge(text, nodes: np.ndarray, parent: np.ndarray):
+    """The AST bridge's structural pass (crates/oneline-tomet/src/lib.rs:20-90):
+    per-line flags (1 literal, 2 block, 4 cut, 8 pipe child, 16 bare blank) and
+    the container line of each block (NONE at the top level or without a block)."""
+    t = _arr(text)
+    nodes = np.ascontiguousarray(nodes, NODE)
+    parent = np.ascontiguousarray(parent, np.uint32)
+    n = nodes.size
+    flags = np.zeros(n, np.uint8)
+    cont = np.zeros(n, np.uint32)
+    lib().orc_bridge(_p(t), _p(nodes), _p(parent), n, _p(flags), _p(cont))
+    return flags, cont
