@@ -98,8 +98,8 @@ def main():
     ap.add_argument("--size", type=int, default=1 << 31, help="document bytes")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-search", action="store_true")
-    ap.add_argument("--search", type=str, default="literal,dict1024,dict65536,query,olb",
-                    help="search-side workloads to run (comma list of literal, dict<count>, query, olb)")
+    ap.add_argument("--search", type=str, default="literal,dict1024,dict65536,query,bridge,olb",
+                    help="search-side workloads to run (comma list of literal, dict<count>, query, bridge, olb)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 31, help="CPU baseline bytes (a prefix of the document)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -293,6 +293,8 @@ def main():
         extra = search_benchmarks(p, doc, text, which, args)
         if "query" in which:
             extra["query"] = query_benchmark(p, doc, tree, L)
+        if "bridge" in which:
+            extra["bridge"] = bridge_benchmark(p, doc, tree, text, L, args)
         if "olb" in which:
             extra["olb"] = olb_benchmark(p, text)
     tree.close()
@@ -594,6 +596,48 @@ def query_benchmark(p, doc, tree, L):
         res.append({"kind": kind, "depth": depth, "matches": q.total_matches, "call_ms": round(dt * 1e3, 3),
                     "kernel_ms": round(kms, 4), "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1)})
         log(f"[bench] query {res[-1]}")
+    return res
+
+
+def bridge_benchmark(p, doc, tree, text, L, args):
+    """The AST bridge's structural pass over the resident document and its
+    tree (the tree's consumer, crates/oneline-tomet/src/lib.rs:20-90), checked
+    against the oracle, then timed with HIP events; the oracle's sequential
+    restatement (one thread) is its CPU baseline."""
+    import oneline_amd as ol
+    from oracle import pyoracle as orc
+    flags, cont = p.bridge(doc, tree)
+    res = {"lines": L, "blocks": int(np.count_nonzero(flags & ol.BRIDGE_BLOCK)),
+           "literal": int(np.count_nonzero(flags & ol.BRIDGE_LITERAL)),
+           "cut": int(np.count_nonzero(flags & ol.BRIDGE_CUT))}
+    if not args.no_verify or not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        _, nodes = orc.parse_par(text, threads)
+        parent = orc.links_par(nodes, threads)[0]
+        t0 = time.perf_counter()
+        w_flags, w_cont = orc.bridge(text, nodes, parent)
+        cpu_s = time.perf_counter() - t0
+        res["verified_bit_exact"] = bool(np.array_equal(flags, w_flags) and np.array_equal(cont, w_cont))
+        if not res["verified_bit_exact"]:
+            raise SystemExit("bridge: the GPU differs from the oracle")
+        res["cpu_ms"] = round(cpu_s * 1e3, 1)
+        res["cpu_threads"] = 1
+        del nodes, parent, w_flags, w_cont
+    del flags, cont
+    p.profile_enable(True)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        p.bridge(doc, tree)
+    dt = (time.perf_counter() - t0) / reps
+    p.profile_enable(False)
+    c, ms = p.profile_take().get("bridge", (1, 0.0))
+    kms = ms / max(1, c)
+    # bytes: classes, keys and parent read, flags and container written (11 per line)
+    b = 11 * L
+    res.update({"call_ms": round(dt * 1e3, 3), "kernel_ms": round(kms, 4), "kernel_gbs": round(b / (kms / 1e3) / 1e9, 1),
+                "algorithmic_bytes": b})
+    log(f"[bench] bridge {res}")
     return res
 
 

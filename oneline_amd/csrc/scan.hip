@@ -1079,9 +1079,10 @@ uint32_t ol_query_blocks(uint32_t r0, uint32_t r1)
 //     child is not cut then the first is not (a cut of the first child cuts
 //     all later ones): so p is a non-literal element whose next content line
 //     is deeper, with no bare blank line before it.
-// Three launches over blocks of BRIDGE_BLOCK lines (16 per thread): marks
-// (block and bare-blank bits, each block's last bare blank), a prefix
-// maximum of those over the blocks, and the per-line chains.
+// Three launches over blocks of BRIDGE_BLOCK lines: marks (block and
+// bare-blank bits, each block's last bare blank; a line per lane), a prefix
+// maximum of those over the blocks, and the per-line chains (16 consecutive
+// lines per thread).
 constexpr uint32_t BRIDGE_THREADS = 256, BRIDGE_PER_THREAD = 16;
 constexpr uint32_t BRIDGE_BLOCK = BRIDGE_THREADS * BRIDGE_PER_THREAD;
 enum : uint32_t { BR_LITERAL = 1u, BR_BLOCK = 2u, BR_CUT = 4u, BR_PIPE = 8u, BR_BARE = 16u };
@@ -1111,53 +1112,98 @@ __device__ __forceinline__ uint32_t bridge_space(const uint8_t *t, uint32_t p, u
     return 0;
 }
 
-// Line i's depth (when its key does not hold it) and content span: from the
-// FlatNodes when the document keeps them, else the line parsed again.
-__device__ __forceinline__ ol_node_dev bridge_node(const BridgeArgs &a, uint32_t i)
+// Whether Text line i's content has a character that is not whitespace.
+// Other kinds need no look: a Heading, Connect, Element or list line's
+// content starts with `=`, `:`, `@` or `-`.  The content's first byte is the
+// line's first byte that is not part of the prefix (`|` and blanks when the
+// line starts with `|`, blanks otherwise; parse_line,
+// crates/oneline-simd/src/line.rs:8-44), and a Text line's content is not
+// empty, so that byte is inside the line.  It is found in a 32-byte window of
+// independent loads; when it is plainly not whitespace (the usual case) that
+// decides, otherwise the span is taken from the FlatNodes or the line parsed
+// again and walked character by character.
+__device__ __forceinline__ bool bridge_text_block_slow(const BridgeArgs &a, uint32_t i)
 {
-    if (a.nodes)
-        return a.nodes[i];
-    const uint32_t s = a.offsets[i];
-    const uint32_t e = (i + 1 < a.lines) ? a.offsets[i + 1] - 1 : a.n - 1;
-    return parse_span(GlobalSrc{a.text}, s, e);
+    ol_node_dev nd;
+    if (a.nodes) {
+        nd = a.nodes[i];
+    } else {
+        const uint32_t s = a.offsets[i];
+        const uint32_t e = (i + 1 < a.lines) ? a.offsets[i + 1] - 1 : a.n - 1;
+        nd = parse_span(GlobalSrc{a.text}, s, e);
+    }
+    const uint32_t e = nd.span_start + nd.span_len;
+    uint32_t p = nd.span_start, w = 1;
+    while (p < e && (w = bridge_space(a.text, p, e)) != 0)
+        p += w;
+    return p < e;
 }
 
+__device__ __forceinline__ bool bridge_text_block(const BridgeArgs &a, uint32_t i)
+{
+    const uint32_t s = a.offsets[i];
+    const uint32_t s0 = s & ~3u, sh = s - s0;
+    const uint32_t *wp = (const uint32_t *)(a.text + s0);   // the text is padded past its end
+    uint32_t w[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        w[j] = wp[j];
+    uint32_t pipes = 0, blanks = 0;   // bit b: byte s0 + b
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t c = (w[j] >> (8 * q)) & 0xFFu;
+            pipes |= (c == '|' ? 1u : 0u) << (4 * j + q);
+            blanks |= (c == ' ' || c == '\t' ? 1u : 0u) << (4 * j + q);
+        }
+    }
+    const uint32_t pre = ((pipes >> sh) & 1u) ? (pipes | blanks) : blanks;
+    const uint32_t rest = ~pre >> sh;   // bit k: byte s + k is not prefix
+    const uint32_t k = rest ? (uint32_t)__builtin_ctz(rest) : 32u;
+    if (sh + k < 32) {
+        const uint32_t p = sh + k;
+        uint32_t word = w[0];
+#pragma unroll
+        for (int j = 1; j < 8; j++)
+            word = (p >> 2) == (uint32_t)j ? w[j] : word;
+        const uint32_t c0 = (word >> (8 * (p & 3))) & 0xFFu;
+        // not the first byte of any Unicode White_Space character
+        const bool plain = (c0 > 0x20u && c0 != 0xC2u && c0 != 0xE1u && c0 != 0xE2u && c0 != 0xE3u) ||
+                           c0 < 0x09u || (c0 > 0x0Du && c0 < 0x20u);
+        if (plain)
+            return true;
+    }
+    return bridge_text_block_slow(a, i);
+}
+
+// Lane per line (line blk * BRIDGE_BLOCK + r * 256 + tid in round r), so the
+// class, offset and text loads of a wave are coalesced: a line's flags are
+// its block bit and its bare-blank bit.
 __global__ __launch_bounds__(BRIDGE_THREADS) void bridge_mark_kernel(BridgeArgs a)
 {
     __shared__ uint32_t s_w[BRIDGE_THREADS / 64];
-    const uint32_t i0 = blockIdx.x * BRIDGE_BLOCK + threadIdx.x * BRIDGE_PER_THREAD;
-    uint32_t f[BRIDGE_PER_THREAD];
     uint32_t last = 0;   // the thread's last bare blank line + 1
-    if (i0 < a.lines) {
-        const uint4 cl = *(const uint4 *)(a.classes + i0), ky = *(const uint4 *)(a.keys + i0);
-        const uint32_t cw[4] = {cl.x, cl.y, cl.z, cl.w}, kw[4] = {ky.x, ky.y, ky.z, ky.w};
-#pragma unroll
-        for (uint32_t k = 0; k < BRIDGE_PER_THREAD; k++) {
-            const uint32_t i = i0 + k;
-            const uint32_t kind = (cw[k >> 2] >> (8 * (k & 3))) & 0xFFu, key = (kw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-            uint32_t fl = 0;
-            if (i < a.lines) {
-                if (key == KEY_BLANK) {
-                    if (bridge_node(a, i).depth == 0) {
-                        fl = BR_BARE;
-                        last = i + 1;
-                    }
-                } else if (kind != OLK_COMMENT) {
-                    // a block unless the content is whitespace only
-                    const ol_node_dev nd = bridge_node(a, i);
-                    const uint32_t e = nd.span_start + nd.span_len;
-                    uint32_t p = nd.span_start, w = 1;
-                    while (p < e && (w = bridge_space(a.text, p, e)) != 0)
-                        p += w;
-                    fl = p < e ? BR_BLOCK : 0u;
-                }
+    const uint32_t base = blockIdx.x * BRIDGE_BLOCK + threadIdx.x;
+#pragma unroll 4
+    for (uint32_t r = 0; r < BRIDGE_PER_THREAD; r++) {
+        const uint32_t i = base + r * BRIDGE_THREADS;
+        if (i >= a.lines)
+            break;
+        const uint32_t kind = a.classes[i];
+        uint32_t fl = 0;
+        if (kind == OLK_BLANK) {
+            // depth 0 exactly when the line does not start with `|`
+            if (a.text[a.offsets[i]] != '|') {
+                fl = BR_BARE;
+                last = i + 1;
             }
-            f[k] = fl;
+        } else if (kind == OLK_TEXT) {
+            fl = bridge_text_block(a, i) ? BR_BLOCK : 0u;
+        } else if (kind != OLK_COMMENT) {
+            fl = BR_BLOCK;
         }
-        *(uint4 *)(a.flags + i0) = make_uint4(f[0] | f[1] << 8 | f[2] << 16 | f[3] << 24,
-                                              f[4] | f[5] << 8 | f[6] << 16 | f[7] << 24,
-                                              f[8] | f[9] << 8 | f[10] << 16 | f[11] << 24,
-                                              f[12] | f[13] << 8 | f[14] << 16 | f[15] << 24);
+        a.flags[i] = (uint8_t)fl;
     }
     // the block's last bare blank line (+ 1, 0: none)
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1199,6 +1245,36 @@ __global__ __launch_bounds__(1024) void bridge_scan_kernel(BridgeArgs a, uint32_
     }
 }
 
+// The rest of a parent chain, from parent p of a line whose last bare blank
+// line before it is b1 - 1 (b1 = 0: none): literal (an element chain that
+// ends at a depth-0 element is not), and the nearest ancestor with a block.
+__device__ __forceinline__ uint32_t bridge_literal_from(const BridgeArgs &a, uint32_t p, uint32_t b1)
+{
+    for (uint32_t step = 0; step < TREE_D; step++) {
+        // p is a depth > 0 element, not cut: its own parent decides
+        const uint32_t q = a.parent[p];
+        if (q == OL_NONE || b1 > q + 1 || a.classes[q] != OLK_ELEMENT)
+            return BR_LITERAL;
+        if (a.keys[q] == 0)
+            return 0;
+        p = q;
+    }
+    return 0;
+}
+__device__ __forceinline__ uint32_t bridge_container_from(const BridgeArgs &a, uint32_t p, uint32_t b1)
+{
+    for (uint32_t step = 0; step < TREE_D; step++) {
+        // p has no block and is not cut: its own parent is next
+        const uint32_t q = a.parent[p];
+        if (q == OL_NONE || b1 > q + 1)
+            return OL_NONE;
+        if (a.flags[q] & BR_BLOCK)
+            return q;
+        p = q;
+    }
+    return OL_NONE;
+}
+
 __global__ __launch_bounds__(BRIDGE_THREADS) void bridge_link_kernel(BridgeArgs a)
 {
     __shared__ uint32_t s_w[BRIDGE_THREADS / 64];
@@ -1228,48 +1304,48 @@ __global__ __launch_bounds__(BRIDGE_THREADS) void bridge_link_kernel(BridgeArgs 
         return;
     const uint4 cl = *(const uint4 *)(a.classes + i0), ky = *(const uint4 *)(a.keys + i0);
     const uint32_t cw[4] = {cl.x, cl.y, cl.z, cl.w}, kw[4] = {ky.x, ky.y, ky.z, ky.w};
-    uint32_t out[BRIDGE_PER_THREAD], cont[BRIDGE_PER_THREAD];
+    uint32_t par[BRIDGE_PER_THREAD];
+#pragma unroll
+    for (uint32_t q = 0; q < BRIDGE_PER_THREAD / 4; q++) {
+        const uint4 v = *(const uint4 *)(a.parent + i0 + 4 * q);   // padded to 4,096 lines
+        par[4 * q] = v.x;
+        par[4 * q + 1] = v.y;
+        par[4 * q + 2] = v.z;
+        par[4 * q + 3] = v.w;
+    }
+    // each line's parent: class | key << 8 | flags << 16, the loads issued together
+    uint32_t pa[BRIDGE_PER_THREAD];
+#pragma unroll
+    for (uint32_t k = 0; k < BRIDGE_PER_THREAD; k++) {
+        const uint32_t p = (i0 + k < a.lines && par[k] != OL_NONE) ? par[k] : i0;
+        pa[k] = (uint32_t)a.classes[p] | (uint32_t)a.keys[p] << 8 | (uint32_t)a.flags[p] << 16;
+    }
+    uint32_t out[4] = {0, 0, 0, 0}, cont[BRIDGE_PER_THREAD];
     for (uint32_t k = 0; k < BRIDGE_PER_THREAD; k++) {
         const uint32_t i = i0 + k;
         const uint32_t kind = (cw[k >> 2] >> (8 * (k & 3))) & 0xFFu, key = (kw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
         uint32_t fl = (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
         uint32_t ct = OL_NONE;
-        // b1: the last bare blank line before line i, + 1 (0: none)
-        auto cut_at = [&](uint32_t p) { return b1 > p + 1; };
         if (i < a.lines) {
+            // b1: the last bare blank line before line i, + 1 (0: none)
+            const uint32_t p = par[k];
+            const bool cut = p != OL_NONE && b1 > p + 1;
             if (key == KEY_BLANK) {
                 if (!(fl & BR_BARE))
                     fl |= BR_LITERAL;   // a `|` line with no content: parent NONE
             } else if (key != 0) {
                 // literal: up the parent chain to a depth-0 parent
-                uint32_t x = i, lit = 0;
-                for (uint32_t step = 0; step < TREE_D; step++) {
-                    const uint32_t p = a.parent[x];
-                    if (p == OL_NONE || cut_at(p) || a.classes[p] != OLK_ELEMENT) {
-                        lit = 1;
-                        break;
-                    }
-                    if (a.keys[p] == 0)
-                        break;
-                    x = p;
-                }
-                const uint32_t p0 = a.parent[i];
-                fl |= lit ? BR_LITERAL : 0u;
-                fl |= (p0 != OL_NONE && cut_at(p0)) ? BR_CUT : 0u;
-            }
-            if ((fl & BR_BLOCK) && key != 0) {
+                uint32_t lit;
+                if (p == OL_NONE || cut || (pa[k] & 0xFFu) != OLK_ELEMENT)
+                    lit = BR_LITERAL;
+                else if (((pa[k] >> 8) & 0xFFu) == 0)
+                    lit = 0;
+                else
+                    lit = bridge_literal_from(a, p, b1);
+                fl |= lit | (cut ? BR_CUT : 0u);
                 // the nearest ancestor with a block along the cut parents
-                uint32_t x = i;
-                for (uint32_t step = 0; step < TREE_D; step++) {
-                    const uint32_t p = a.parent[x];
-                    if (p == OL_NONE || cut_at(p))
-                        break;
-                    if (a.flags[p] & BR_BLOCK) {
-                        ct = p;
-                        break;
-                    }
-                    x = p;
-                }
+                if ((fl & BR_BLOCK) && p != OL_NONE && !cut)
+                    ct = ((pa[k] >> 16) & BR_BLOCK) ? p : bridge_container_from(a, p, b1);
             }
             if (kind == OLK_ELEMENT && !(fl & BR_LITERAL)) {
                 // a `|` child: the next content line is deeper, no bare blank line before it
@@ -1288,13 +1364,10 @@ __global__ __launch_bounds__(BRIDGE_THREADS) void bridge_link_kernel(BridgeArgs 
             if (fl & BR_BARE)
                 b1 = i + 1;
         }
-        out[k] = fl;
+        out[k >> 2] |= fl << (8 * (k & 3));
         cont[k] = ct;
     }
-    *(uint4 *)(a.flags + i0) = make_uint4(out[0] | out[1] << 8 | out[2] << 16 | out[3] << 24,
-                                          out[4] | out[5] << 8 | out[6] << 16 | out[7] << 24,
-                                          out[8] | out[9] << 8 | out[10] << 16 | out[11] << 24,
-                                          out[12] | out[13] << 8 | out[14] << 16 | out[15] << 24);
+    *(uint4 *)(a.flags + i0) = make_uint4(out[0], out[1], out[2], out[3]);
 #pragma unroll
     for (uint32_t q = 0; q < BRIDGE_PER_THREAD / 4; q++)
         *(uint4 *)(a.container + i0 + 4 * q) = make_uint4(cont[4 * q], cont[4 * q + 1], cont[4 * q + 2], cont[4 * q + 3]);

@@ -5,7 +5,8 @@ test_oracle.py).  Searches: the dictionary's hits are grouped by pattern with
 ascending offsets, a sample of them spell their pattern, per-pattern counts
 and offsets equal those of the literal kernel (a different algorithm) for a
 sample of patterns, and at 65,536 patterns every (offset, id) of sampled
-windows equals the oracle's; the query kernels against the downloaded nodes.
+windows equals the oracle's; the query kernels against the downloaded nodes; the
+AST bridge's structural pass against the oracle.
 Smaller documents are compared with the oracle hit by hit in
 test_gpu_many.py and test_gpu_search.py."""
 import ctypes
@@ -124,6 +125,13 @@ def test_parse_and_tree_against_oracle(parser, big):
     # subtree_end was not written by that build: downloading it is an error
     with pytest.raises(ol.OnelineError):
         parser.download_links(tree)
+    # the AST bridge's structural pass over the whole document (lean: depths of
+    # blank lines and content spans parsed again from the text)
+    print("[fullsize] bridge against the oracle", flush=True)
+    flags, cont = parser.bridge(doc, tree)
+    w_flags, w_cont = orc.bridge(text, w_nodes, want[0])
+    assert np.array_equal(flags, w_flags), "bridge flags"
+    assert np.array_equal(cont, w_cont), "bridge containers"
     tree.close()
 
 
