@@ -1221,8 +1221,20 @@ int ol_ctx_create(ol_ctx **out)
         return fail(OL_ERR_DEVICE, "the no-allocation scratch's device address is not available");
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
-        c->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) {
+        // the marketing name can come back empty: then the runtime's device
+        // name, else a generic one; the architecture, CUs and HBM size follow
+        std::string nm = prop.name;
+        if (nm.empty()) {
+            char buf[256] = {0};
+            if (hipDeviceGetName(buf, sizeof(buf) - 1, 0) == hipSuccess)
+                nm = buf;
+        }
+        if (nm.empty())
+            nm = "AMD GPU";
+        c->name = nm + " (" + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) + " CUs, " +
+                  std::to_string((prop.totalGlobalMem + (1ull << 29)) >> 30) + " GiB)";
+    }
     if (scratch_init(c->scr) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -2712,8 +2724,8 @@ int batch_prepare(ol_ctx *c, ol_batch *b, const ol_dict *dict, bool search_only)
             return rc;
         b->m_total = total;
     }
-    // every launch form of this geometry (eager: cooperative scan; graph:
-    // ticket-ordered scan) fits the scratch from now on: freeze it
+    // every launch form of this geometry (cooperative scan; ticket-ordered
+    // scan, as a context set to it uses) fits the scratch from now on: freeze it
     size_t words = STATUS_TILES;
     if (!search_only)
         words = std::max(scan_status_words(c, b->n, true), scan_status_words(c, b->n, false));
@@ -2849,8 +2861,11 @@ int ol_batch_run(ol_ctx *c, ol_batch *b, const ol_dict *dict, int flags)
     if (!b->exec) {
         const bool prof = c->profiling;
         c->profiling = false;
+        // the same cooperative scan as the eager sequence (the capture keeps
+        // the launch cooperative); the replay's first node clears the status
+        // words that the previous run's dictionary pass left set
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = batch_launch(c, b, false);
+        int rc = batch_launch(c, b, true);
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &gr);
         c->profiling = prof;

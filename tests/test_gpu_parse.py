@@ -199,3 +199,11 @@ def test_scan_floor_calibration(parser):
     assert all(np.array_equal(x, y) for x, y in zip(before[:2], after[:2]))
     with pytest.raises(ol.OnelineError):
         parser.calibrate_scan_floor(doc, 40000)
+
+
+def test_adapter_name_names_the_device(parser):
+    """GpuParser::adapter_name (crates/oneline-gpu/src/parser.rs:470): a
+    device name (the runtime's, or a generic one when it reports none), the
+    architecture, compute units and HBM size."""
+    name = parser.adapter_name()
+    assert name and not name.startswith(" ") and "gfx950" in name and " CUs, " in name, name
