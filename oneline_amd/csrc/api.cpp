@@ -2410,6 +2410,7 @@ struct ol_batch {
     const ol_dict *prep_dict = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipGraphNode_t scan_node = nullptr;   // the captured scan (marked cooperative before instantiation)
     size_t graph_words = 0;   // status words a replay leaves set
 
     void drop_graph()
@@ -2749,6 +2750,13 @@ int batch_launch(ol_ctx *c, ol_batch *b, bool cooperative)
     int rc = OL_OK;
     if (!b->prep_search_only) {
         rc = batch_scan(c, b, cooperative);
+        // under capture: the scan's node is the stream's one dependency now
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        const hipGraphNode_t *deps = nullptr;
+        size_t nd = 0;
+        if (!rc && hipStreamGetCaptureInfo_v2(c->stream, &cs, nullptr, nullptr, &deps, &nd) == hipSuccess &&
+            cs == hipStreamCaptureStatusActive && nd == 1)
+            b->scan_node = deps[0];
         if (!rc)
             rc = batch_tree(c, b);
     }
@@ -2861,9 +2869,11 @@ int ol_batch_run(ol_ctx *c, ol_batch *b, const ol_dict *dict, int flags)
     if (!b->exec) {
         const bool prof = c->profiling;
         c->profiling = false;
-        // the same cooperative scan as the eager sequence (the capture keeps
-        // the launch cooperative); the replay's first node clears the status
-        // words that the previous run's dictionary pass left set
+        // the same cooperative scan as the eager sequence: the capture
+        // records it as a plain kernel node, so the node is marked
+        // cooperative before instantiation.  The replay's first node clears
+        // the status words that the previous run's dictionary pass left set.
+        b->scan_node = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = batch_launch(c, b, true);
         hipGraph_t gr = nullptr;
@@ -2875,6 +2885,15 @@ int ol_batch_run(ol_ctx *c, ol_batch *b, const ol_dict *dict, int flags)
         }
         if (e != hipSuccess)
             return fail(OL_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        if (!b->prep_search_only) {
+            hipKernelNodeAttrValue v{};
+            v.cooperative = 1;
+            if (!b->scan_node ||
+                hipGraphKernelNodeSetAttribute(b->scan_node, hipKernelNodeAttributeCooperative, &v) != hipSuccess) {
+                (void)hipGraphDestroy(gr);
+                return fail(OL_ERR_DEVICE, "graph capture: the scan's node could not be marked cooperative");
+            }
+        }
         b->graph = gr;
         HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
