@@ -91,28 +91,34 @@ __device__ __forceinline__ void load_keys(const uint8_t *keys, const TileSeg &ts
         key[r] = (wbase + r * 64 + lane < ts.L) ? v[r] : KEY_BLANK;
 }
 
-// Bit-sliced level masks of a round (64 lines, lane = line, x = its key):
-// lane d (0 <= d < 32) gets lt = {lines with key < d} and eq = {lines with key
-// == d}, lane 32 gets lt = {all content lines}.  Five ballots of the key bits
-// feed a per-lane comparator against d, most significant bit first, instead
-// of one ballot per level.  `dm[b]` is all ones in lanes whose bit b is set.
+// Exclusive prefix OR over the lanes of a 64-bit value (lane 0 gets 0): DPP
+// row shifts within rows of 16 lanes, then the row broadcasts carry rows 0-1
+// into rows 1-3, as wave_incl_add does for sums.
+__device__ __forceinline__ uint64_t wave_excl_or64(uint64_t v)
+{
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#define OL_OR_STEP(CTRL, RM)              \
+    lo |= dpp_move<CTRL, RM>(0u, lo);     \
+    hi |= dpp_move<CTRL, RM>(0u, hi);
+    OL_OR_STEP(0x111, 0xF) OL_OR_STEP(0x112, 0xF) OL_OR_STEP(0x114, 0xF) OL_OR_STEP(0x118, 0xF)
+    OL_OR_STEP(0x142, 0xA) OL_OR_STEP(0x143, 0xC)
+#undef OL_OR_STEP
+    return (uint64_t)wave_shr1(hi, 0u) << 32 | wave_shr1(lo, 0u);
+}
+
+// Level masks of a round (64 lines, lane = line, x = its key): lane d
+// (0 <= d < 32) gets eq = {lines with key == d}, from five ballots of the key
+// bits compared with d's bits (`dm[b]` is all ones in lanes whose bit b is
+// set), and lt = {lines with key < d}, the OR of the lower lanes' eq; lane 32
+// gets lt = {all content lines}, the OR of all 32.
 __device__ __forceinline__ void level_masks(uint32_t x, const uint64_t (&dm)[5], uint64_t &lt, uint64_t &eq)
 {
-    const int lane = threadIdx.x & 63;
-    const uint64_t V = __ballot(x < (uint32_t)D);
-    uint64_t B[5];
+    uint64_t e = __ballot(x < (uint32_t)D);
 #pragma unroll
-    for (int b = 0; b < 5; b++)
-        B[b] = __ballot((x >> b) & 1u);
-    uint64_t l = 0, e = V;
-#pragma unroll
-    for (int b = 4; b >= 0; b--) {
-        const uint64_t t = e & ~B[b];          // equal so far, key bit 0
-        l |= t & dm[b];                        // ... and d's bit 1: key < d
-        e &= ~(B[b] ^ dm[b]);                  // key bit == d's bit
-    }
-    lt = (lane == D) ? V : l;
+    for (int b = 4; b >= 0; b--)
+        e &= ~(__ballot((x >> b) & 1u) ^ dm[b]);   // key bit b == d's bit b
     eq = e;
+    lt = wave_excl_or64(e);
 }
 
 __device__ __forceinline__ void lane_bit_masks(uint64_t (&dm)[5])
