@@ -3,6 +3,9 @@
 // 32 KiB tiles (coalesced 16-byte loads, one workgroup per tile) and writes,
 // per tile, a line count's worth of a u32 array and two u8 arrays at
 // consecutive positions, as the scan writes its offsets, classes and keys.
+// The loads are non-temporal, the fastest way found to stream the text (388
+// us against 442 us with plain loads at 2 GiB), so the floor is the best
+// this data movement achieves, whichever loads the scan uses.
 // Nothing in it depends on the text beyond an XOR that keeps the loads.
 #include "internal.h"
 
@@ -19,7 +22,7 @@ __global__ __launch_bounds__(CAL_THREADS) void stream_floor_kernel(const uint4 *
     uint32_t acc = 0;
 #pragma unroll
     for (int k = 0; k < CAL_NV; k++) {
-        const uint4 v = src[k * CAL_THREADS + threadIdx.x];
+        const uint4 v = ld_stream_u4(src + k * CAL_THREADS + threadIdx.x);
         acc += v.x ^ v.y ^ v.z ^ v.w;
     }
     if (WRITE) {

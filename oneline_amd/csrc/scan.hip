@@ -406,7 +406,7 @@ __global__ __launch_bounds__(THREADS) void scan_classify_kernel(ScanArgs a)
         const uint4 *src = (const uint4 *)(a.text + (size_t)b);
 #pragma unroll
         for (int k = 0; k < NV; k++)
-            v[k] = src[k * THREADS + tid];
+            v[k] = src[k * THREADS + tid];   // plain loads: non-temporal ones measured slower here (the halo and long-line reads miss)
         hv = make_uint4(0, 0, 0, 0);
         if (tt > 0 && tid < HALO / 16)
             hv = ((const uint4 *)(a.text + b - HALO))[tid];

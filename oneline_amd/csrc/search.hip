@@ -106,8 +106,11 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         const uint8_t *src = a.text + (size_t)tt * TILE;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
-            v[k] = ((const uint4 *)src)[k * THREADS + tid];
-            nx[k] = *(const uint32_t *)(src + (size_t)(k * THREADS + tid + 1) * 16);   // text is padded
+            v[k] = ld_stream_u4((const uint4 *)src + k * THREADS + tid);
+            // a wave's last lane loads the word after its vector (the text
+            // is padded); the other lanes take it from the next lane's vector
+            // when the tile is searched
+            nx[k] = lane == 63 ? *(const uint32_t *)(src + (size_t)(k * THREADS + tid + 1) * 16) : 0u;
         }
     };
     const uint32_t mlen = a.m, wo = a.wo;
@@ -198,6 +201,9 @@ __global__ __launch_bounds__(THREADS) void search_kernel(SearchArgs a)
         }
         if (t + G < T)
             load(t + G);
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            cn[k] = dpp_move<0x130>(cn[k], cv[k].x);   // wave_shl:1: the next lane's first word; lane 63 keeps its own
         // window positions q hold a match start q - wo when wo <= q and
         // q - wo + m <= n
         const uint32_t last_q = a.n >= mlen ? a.n - mlen + wo : 0u;   // inclusive, when n >= m
@@ -1433,7 +1439,10 @@ __global__ __launch_bounds__(THREADS) void many_write_m_kernel(ManyArgs a)
             const uint64_t R0 = __shfl(r, 0, 64);   // lane 0 is active in every round
             const uint32_t lastl = min(63u, total - 1u - r0);
             const uint64_t span = __shfl(r + c, (int)lastl, 64) - R0;
-            const bool staged = __ballot(act && recount) == 0 && span == __shfl(cinc, 63, 64) && span <= MANY_STAGE;
+            // A round whose candidates each have one match already writes
+            // consecutive outputs lane by lane, so it skips the stage.
+            const bool staged = __ballot(act && recount) == 0 && __ballot(act && c != 1) != 0 &&
+                                span == __shfl(cinc, 63, 64) && span <= MANY_STAGE;
             if (act && c && !recount) {
                 const uint32_t gp = base + lp;
                 const uint32_t s0 = (uint32_t)(r - R0);
