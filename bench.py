@@ -99,7 +99,9 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--search", type=str, default="literal,dict1024,dict65536,query,bridge,olb",
-                    help="search-side workloads to run (comma list of literal, dict<count>, query, bridge, olb)")
+                    help="search-side workloads to run (comma list of literal, dict<count>, query, bridge, olb; "
+                         "dict<count>s: the same dictionary plus one 2-byte pattern that never occurs, "
+                         "the cost of the 1- and 2-byte classes)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 31, help="CPU baseline bytes (a prefix of the document)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -745,8 +747,9 @@ def search_benchmarks(p, doc, text, which, args):
         res["literal_search"] = lits
 
     tot = ctypes.c_uint64()
-    for count in sorted(int(x[4:]) for x in which if x.startswith("dict")):
-        pats = datagen.patterns(count, seed=0xC3B)
+    dicts = sorted((int(x[4:].rstrip("s")), x.endswith("s")) for x in which if x.startswith("dict"))
+    for count, short in dicts:
+        pats = datagen.patterns(count, seed=0xC3B) + ([b"\x01\x02"] if short else [])
         pset = p.compile_patterns(doc, pats)
         ol._check(lib.ol_search_many_device(p._ctx, doc._h, pset._h, None, None, 0, ctypes.byref(tot)))
         cap = tot.value
@@ -795,11 +798,14 @@ def search_benchmarks(p, doc, text, which, args):
             cms = sorted(ts)[1] * 1e3
             rec.update({"cpu_ms": round(cms, 2), "cpu_gbs": round(n / (cms / 1e3) / 1e9, 2), "cpu_threads": threads,
                         "cpu_sample": "the whole document, every overlapping (start, pattern) counted"})
-        res[f"dict_search_{count}"] = rec
+        name = f"dict_search_{count}" + ("_short" if short else "")
+        if short:
+            rec["short_pattern"] = "one 2-byte pattern (0x01 0x02) that never occurs, added to the set"
+        res[name] = rec
         hip.hipFree(d_off)
         hip.hipFree(d_id)
         pset.close()
-        log(f"[bench] dict {count}: {res[f'dict_search_{count}']}")
+        log(f"[bench] {name}: {res[name]}")
     return res
 
 

@@ -2028,10 +2028,13 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
     dd.bbits = bbits;
     for (size_t i = 0; i < count; i++)
         dd.class_mask |= 1u << cls[i];
-    // filters: 1 and 2 direct; 4 = the prefix filter of patterns of 3 bytes and
-    // more (4-byte keys, a 3-byte key with each of the 32 values of the fourth
-    // byte's low 5 bits).  Layout: filter 4 first, then 1, 2, then a zero word
-    // that absent filters point at.
+    // filters: 2 = a direct bitmap of the 2-byte window (a 2-byte pattern's
+    // key; a 1-byte pattern's byte with each of the 256 second bytes), so a
+    // position takes one lookup for both short classes; 4 = the prefix filter
+    // of patterns of 3 bytes and more (4-byte keys, a 3-byte key with each of
+    // the 32 values of the fourth byte's low 5 bits).  Layout: filter 4 first,
+    // then 2 (the kernels find it right after filter 4), then a zero word that
+    // absent filters point at.
     std::vector<std::vector<uint32_t>> filt(6);
     uint32_t fwords = 0;
     const bool any3 = (dd.class_mask >> DK_3) & 1u, anylong = (dd.class_mask >> DK_LONG) & 1u;
@@ -2043,21 +2046,26 @@ int ol_dict_compile(ol_ctx *c, const ol_doc *doc, const uint8_t *const *patterns
             bits += cls[i] == DK_3 ? 32 : cls[i] == DK_LONG ? 1 : 0;
         dd.pf_hashed = bits > DICT_PF_DENSE_BITS ? 1u : 0u;
     }
-    for (uint32_t f : {4u, 1u, 2u}) {
-        const uint32_t dk = f == 1 ? DK_1 : DK_2;
-        const bool used = f == 4 ? (any3 || anylong) : ((dd.class_mask >> dk) & 1u) != 0;
+    const bool anyshort = ((dd.class_mask >> DK_1) & 1u) || ((dd.class_mask >> DK_2) & 1u);
+    for (uint32_t f : {4u, 2u}) {
+        const bool used = f == 4 ? (any3 || anylong) : anyshort;
         if (!used)
             continue;
-        const uint32_t fb = f == 1 ? 8 : f == 2 ? 16 : DICT_PBITS;
+        const uint32_t fb = f == 2 ? 16 : DICT_PBITS;
         dd.fbits[f] = fb;
         dd.foff[f] = fwords;
-        const uint32_t words = std::max<uint32_t>(1, (1u << fb) / 32);
+        const uint32_t words = (1u << fb) / 32;
         filt[f].assign(words, 0);
         fwords += words;
         for (size_t i = 0; i < count; i++) {
-            if (f != 4) {
-                if (cls[i] == dk)
+            if (f == 2) {
+                if (cls[i] == DK_2)
                     filt[f][key[i] >> 5] |= 1u << (key[i] & 31);
+                else if (cls[i] == DK_1)   // the byte, then any second byte
+                    for (uint32_t b1 = 0; b1 < 256; b1++) {
+                        const uint32_t k2 = key[i] | b1 << 8;
+                        filt[f][k2 >> 5] |= 1u << (k2 & 31);
+                    }
                 continue;
             }
             auto setkey = [&](uint32_t w) {

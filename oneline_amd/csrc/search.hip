@@ -386,17 +386,13 @@ hipError_t ol_launch_search(const SearchArgs &a, hipStream_t s)
 //                (matches of the earlier candidates).
 // Filter: each lane tests its 32 positions against the LDS bitmaps
 // (straight-line code, 4-byte windows from registers): one lookup per
-// position in the 64 KiB prefix filter of classes 3..5, plus the direct 1- and
-// 2-byte bitmaps when the dictionary has such patterns.  Two workgroups of 8
+// position in the 64 KiB prefix filter of classes 3..5, plus one in the
+// direct 2-byte bitmap (which also holds the 1-byte patterns) when the
+// dictionary has such patterns.  Two workgroups of 8
 // waves share a CU.  Probe: the class tables give, per class, the run of
 // patterns sharing the key (sorted by id); the runs are merged by id and each
 // pattern's remaining bytes compared word by word.  The text is read from
 // global memory (the probes hit the cache lines the filter just read).
-
-__device__ __forceinline__ uint32_t filter_bit(const uint32_t *f, uint32_t i)
-{
-    return __builtin_amdgcn_ubfe(f[i >> 5], i & 31, 1);
-}
 
 // Matches starting at tile position lp (global gp) with at most `rem` bytes
 // (1..64) before the text or document end; written from out[r] in pattern-id
@@ -655,11 +651,15 @@ __device__ __forceinline__ uint32_t probe_count_keep(const DictDev &d, const uin
 // ---- merged runs (DictDev::merged) -------------------------------------------
 // The slot of the first class, in the order L, 3, 2, 1, that holds the key of
 // the window T0 (info 0: none).  Its run holds every pattern that can start
-// there.  FILT: the prefix filter s_filter is at hand (LDS): the 3-byte
-// lookup then needs the filter's 3-byte gate (as in probe_count_keep) and,
-// with GATE, the classes of 3 bytes and more the window's filter bit.
+// there.  FILT: the prefix filter s_filter and the 2-byte filter s_small are
+// at hand (LDS): the 3-byte lookup then needs the filter's 3-byte gate (as in
+// probe_count_keep), with GATE the classes of 3 bytes and more the window's
+// filter bit, and the 2- and 1-byte classes the window's 2-byte filter bit.
+// The short classes are looked up only where neither longer class holds the
+// key (a longer key's run holds its 1- and 2-byte prefixes).
 template <bool FILT, bool GATE, bool SMALL>
-__device__ __forceinline__ uint4 merged_slot(const DictDev &d, const uint32_t *s_filter, uint32_t T0, uint32_t rem)
+__device__ __forceinline__ uint4 merged_slot(const DictDev &d, const uint32_t *s_filter, const uint32_t *s_small,
+                                             uint32_t T0, uint32_t rem)
 {
     bool pass = true, gate3 = true;
     if (FILT && d.fbits[4]) {
@@ -681,10 +681,10 @@ __device__ __forceinline__ uint4 merged_slot(const DictDev &d, const uint32_t *s
     const bool use3 = ((cm >> DK_3) & 1u) && rem >= 3 && pass && gate3;
     const uint4 fL = dk_lookup(d, DK_LONG, T0, useL), f3 = dk_lookup(d, DK_3, T0 & 0xFFFFFFu, use3);
     uint4 f = (fL.y & DK_VALID) ? fL : f3;
-    if (SMALL) {
+    if (SMALL && !(f.y & DK_VALID) && (!FILT || __builtin_amdgcn_ubfe(s_small[__builtin_amdgcn_ubfe(T0, 5, 11)], T0, 1))) {
         const bool use2 = ((cm >> DK_2) & 1u) && rem >= 2, use1 = ((cm >> DK_1) & 1u) && rem >= 1;
         const uint4 f2 = dk_lookup(d, DK_2, T0 & 0xFFFFu, use2), f1 = dk_lookup(d, DK_1, T0 & 0xFFu, use1);
-        f = (f.y & DK_VALID) ? f : (f2.y & DK_VALID) ? f2 : f1;
+        f = (f2.y & DK_VALID) ? f2 : f1;
     }
     return f;
 }
@@ -736,7 +736,7 @@ __device__ uint32_t probe_merged(const DictDev &d, const uint8_t *s_text, uint32
 {
     uint32_t T0, T1, T2;
     text_words(s_text, lp, T0, T1, T2);
-    const uint4 f = merged_slot<false, false, SMALL>(d, nullptr, T0, rem);
+    const uint4 f = merged_slot<false, false, SMALL>(d, nullptr, nullptr, T0, rem);
     if (!(f.y & DK_VALID))
         return 0;
     const uint32_t slen = (f.y >> DK_LEN_SHIFT) & 0x7Fu;
@@ -856,8 +856,9 @@ __device__ __forceinline__ ManyWords many_load(const ManyArgs &a, uint32_t wt, u
 
 // One part's positions through the filters: bit p set when a pattern may
 // start at the lane's position p.  FS: the filters the dictionary has (bit 4:
-// the prefix filter at pf, bit 1: the 1- and 2-byte filters at their word
-// offsets from small); compile-time, so the lookups are straight-line code
+// the prefix filter at pf, bit 1: the 2-byte filter, which the host places
+// right after the prefix filter in the layout that small points at);
+// compile-time, so the lookups are straight-line code
 // whose reads the compiler can overlap.  A position of the prefix filter
 // costs a byte align, a 24-bit multiply, the word address, one read and the
 // bit extract, whose offset is the window three positions on.
@@ -865,8 +866,8 @@ template <int FS>
 __device__ __forceinline__ uint32_t many_filter(const DictDev &d, const uint32_t *pf, const uint32_t *small,
                                                 const ManyWords &W, uint32_t lim)
 {
-    const uint32_t *f1 = small + d.foff[1], *f2 = small + d.foff[2];
-    const uint32_t m1 = d.fbits[1] ? 0xFFu : 0u, m2 = d.fbits[2] ? 0xFFFFu : 0u;
+    // the 2-byte filter (both short classes) follows the prefix filter
+    const uint32_t *const f2 = small + ((FS & 0x10) ? MANY_PWORDS : 0u);
     uint32_t cand = 0;
     // the window at lane position q (q < MANY_LB), and a word whose lowest
     // byte is the byte at q (q < MANY_LB + 3)
@@ -885,7 +886,8 @@ __device__ __forceinline__ uint32_t many_filter(const DictDev &d, const uint32_t
             const uint32_t h = dict_ph(w);
             hit |= __builtin_amdgcn_ubfe(pf[dict_ph_word(h)], h, 1);
         }
-        if (FS & 0x02) hit |= filter_bit(f1, w & m1) | filter_bit(f2, w & m2);
+        if (FS & 0x02)   // word: bits 5..15 of the window; bit: its low 5 bits
+            hit |= __builtin_amdgcn_ubfe(f2[__builtin_amdgcn_ubfe(w, 5, 11)], w, 1);
         cand |= hit << p;
     }
     return lim >= MANY_LB ? cand : cand & ((1u << lim) - 1u);
@@ -1048,7 +1050,7 @@ constexpr uint32_t MANY_STAGE = 512;   // write pass: matches a wave stages per 
 // candidates' runs, not their maximum.  A candidate collects its entries'
 // results from the round's ballot.
 template <int THREADS, int FS>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void many_count_m_kernel(ManyArgs a)
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu((FS & 0x12) == 0x12 ? 3 : 4))) void many_count_m_kernel(ManyArgs a)
 {
     constexpr int NWV = THREADS / 64;
     constexpr bool SMALL = (FS & 0x02) != 0;
@@ -1099,7 +1101,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                 }
                 if (rm) {
                     text_words(m.g_text, lp, T0, T1, T2);
-                    f = merged_slot<true, SMALL, SMALL>(d, pf, T0, rm);
+                    f = merged_slot<true, SMALL, SMALL>(d, pf, small + PW, T0, rm);
                 }
                 const bool valid = (f.y & DK_VALID) != 0;
                 const uint32_t slen = (f.y >> DK_LEN_SHIFT) & 0x7Fu;
@@ -1647,7 +1649,7 @@ static hipError_t launch_many_fs(const ManyArgs &a, hipStream_t s)
 template <bool WRITE>
 static hipError_t launch_many(const ManyArgs &a, hipStream_t s)
 {
-    const int fs = (a.d.fbits[4] ? (a.d.pf_hashed ? 0x30 : 0x10) : 0) | (a.d.fbits[1] || a.d.fbits[2] ? 0x02 : 0);
+    const int fs = (a.d.fbits[4] ? (a.d.pf_hashed ? 0x30 : 0x10) : 0) | (a.d.fbits[2] ? 0x02 : 0);
     const bool m = a.d.merged != 0;
     switch (fs) {
     case 0x10: return m ? launch_many_fs<WRITE, 0x10, true>(a, s) : launch_many_fs<WRITE, 0x10, false>(a, s);

@@ -90,6 +90,23 @@ def test_generated_text_and_dictionaries(parser):
         assert_many_matches(parser, text, pats, f"{n} patterns")
 
 
+def test_short_classes_share_one_filter(parser):
+    """1- and 2-byte patterns pass through one 2-byte filter (a 1-byte pattern
+    sets every second byte): each class alone, both, both with longer
+    patterns, and matches on the text's last byte and first bytes."""
+    text = bytearray(datagen.document(1 << 20, seed=21))
+    text[0:2] = b"Qz"
+    text[-1:] = b"Q"    # a 1-byte match on the last byte (no byte after it)
+    text = bytes(text)
+    ones = [b"Q", b"e", b"\x01"]
+    twos = [b"Qz", b"th", b"\x01\x02", b"zQ"]
+    longer = [datagen.vocab_word(i) for i in range(40)]
+    for what, pats in (("1-byte", ones), ("2-byte", twos), ("1- and 2-byte", ones + twos),
+                       ("1-byte and longer", ones + longer), ("2-byte and longer", longer + twos),
+                       ("all", twos + longer + ones)):
+        assert_many_matches(parser, text, pats, what)
+
+
 def test_benchmark_dictionaries(parser):
     text = datagen.document(4 << 20, seed=0, crlf=10, empty=5)
     for count in (1024, 65536):
