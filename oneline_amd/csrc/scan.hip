@@ -1118,10 +1118,13 @@ __device__ __forceinline__ uint32_t bridge_space(const uint8_t *t, uint32_t p, u
 // line's first byte that is not part of the prefix (`|` and blanks when the
 // line starts with `|`, blanks otherwise; parse_line,
 // crates/oneline-simd/src/line.rs:8-44), and a Text line's content is not
-// empty, so that byte is inside the line.  It is found in a 32-byte window of
-// independent loads; when it is plainly not whitespace (the usual case) that
-// decides, otherwise the span is taken from the FlatNodes or the line parsed
-// again and walked character by character.
+// empty, so that byte is inside the line.  It is found in 32-byte windows of
+// independent loads (one for a prefix under 29 bytes, about 92% of the lines
+// of the benchmark document); when it is plainly not whitespace (the usual
+// case) that decides, otherwise the span is taken from the FlatNodes or the
+// line parsed again and walked character by character.  Taking that path for
+// every prefix longer than one window (a lane of most 64-line rounds) made
+// the mark pass 1.36 ms at 2 GiB.
 __device__ __forceinline__ bool bridge_text_block_slow(const BridgeArgs &a, uint32_t i)
 {
     ol_node_dev nd;
@@ -1142,37 +1145,49 @@ __device__ __forceinline__ bool bridge_text_block_slow(const BridgeArgs &a, uint
 __device__ __forceinline__ bool bridge_text_block(const BridgeArgs &a, uint32_t i)
 {
     const uint32_t s = a.offsets[i];
-    const uint32_t s0 = s & ~3u, sh = s - s0;
-    const uint32_t *wp = (const uint32_t *)(a.text + s0);   // the text is padded past its end
-    uint32_t w[8];
+    uint32_t s0 = s & ~3u, sh = s - s0;
+    uint32_t piped = 2;   // whether the line starts with `|` (2: not read yet)
+    // A prefix longer than the window continues in the next one (a prefix of
+    // depth 16 or more fills 32 bytes): up to four windows, then the slow
+    // path.  Every window read ends within 32 bytes of a byte of the line (the
+    // text is padded past its end).
+#pragma unroll 1
+    for (uint32_t round = 0; round < 4; round++) {
+        const uint32_t *wp = (const uint32_t *)(a.text + s0);
+        uint32_t w[8];
 #pragma unroll
-    for (int j = 0; j < 8; j++)
-        w[j] = wp[j];
-    uint32_t pipes = 0, blanks = 0;   // bit b: byte s0 + b
+        for (int j = 0; j < 8; j++)
+            w[j] = wp[j];
+        uint32_t pipes = 0, blanks = 0;   // bit b: byte s0 + b
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+        for (int j = 0; j < 8; j++) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint32_t c = (w[j] >> (8 * q)) & 0xFFu;
-            pipes |= (c == '|' ? 1u : 0u) << (4 * j + q);
-            blanks |= (c == ' ' || c == '\t' ? 1u : 0u) << (4 * j + q);
+            for (int q = 0; q < 4; q++) {
+                const uint32_t c = (w[j] >> (8 * q)) & 0xFFu;
+                pipes |= (c == '|' ? 1u : 0u) << (4 * j + q);
+                blanks |= (c == ' ' || c == '\t' ? 1u : 0u) << (4 * j + q);
+            }
         }
-    }
-    const uint32_t pre = ((pipes >> sh) & 1u) ? (pipes | blanks) : blanks;
-    const uint32_t rest = ~pre >> sh;   // bit k: byte s + k is not prefix
-    const uint32_t k = rest ? (uint32_t)__builtin_ctz(rest) : 32u;
-    if (sh + k < 32) {
-        const uint32_t p = sh + k;
-        uint32_t word = w[0];
+        if (piped == 2)
+            piped = (pipes >> sh) & 1u;
+        const uint32_t pre = piped ? (pipes | blanks) : blanks;
+        const uint32_t rest = ~pre >> sh;   // bit k: byte s0 + sh + k is not prefix
+        if (rest) {
+            const uint32_t p = sh + (uint32_t)__builtin_ctz(rest);
+            uint32_t word = w[0];
 #pragma unroll
-        for (int j = 1; j < 8; j++)
-            word = (p >> 2) == (uint32_t)j ? w[j] : word;
-        const uint32_t c0 = (word >> (8 * (p & 3))) & 0xFFu;
-        // not the first byte of any Unicode White_Space character
-        const bool plain = (c0 > 0x20u && c0 != 0xC2u && c0 != 0xE1u && c0 != 0xE2u && c0 != 0xE3u) ||
-                           c0 < 0x09u || (c0 > 0x0Du && c0 < 0x20u);
-        if (plain)
-            return true;
+            for (int j = 1; j < 8; j++)
+                word = (p >> 2) == (uint32_t)j ? w[j] : word;
+            const uint32_t c0 = (word >> (8 * (p & 3))) & 0xFFu;
+            // not the first byte of any Unicode White_Space character
+            const bool plain = (c0 > 0x20u && c0 != 0xC2u && c0 != 0xE1u && c0 != 0xE2u && c0 != 0xE3u) ||
+                               c0 < 0x09u || (c0 > 0x0Du && c0 < 0x20u);
+            return plain ? true : bridge_text_block_slow(a, i);
+        }
+        s0 += 32;
+        sh = 0;
+        if (s0 >= a.n)
+            break;
     }
     return bridge_text_block_slow(a, i);
 }

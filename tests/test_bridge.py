@@ -215,6 +215,43 @@ def test_gpu_blocks_and_chains_across_block_edges(parser):
     assert_bridge(parser, b"\n".join(lines) + b"\n", "block edges")
 
 
+def long_prefix_doc():
+    """Text lines whose content starts past the first 32-byte window (and the
+    second, third and fourth): deep `| ` prefixes, blank runs inside the
+    prefix, blanks-only prefixes at depth 0, and content that starts with or
+    is only Unicode whitespace."""
+    contents = [b"text", " x".encode(), "　".encode(), "  ".encode(), b"\x0bq", b"x"]
+    lines = [b"@root"]
+    for d in range(1, 32):
+        for k, c in enumerate(contents):
+            pre = b"| " * d
+            if k % 3 == 1:
+                pre = b"|\t" + b"| " * (d - 1)
+            lines.append(pre + c)
+    for blanks in (30, 60, 93, 126, 127, 200):
+        lines.append(b"|" + b" " * blanks + b"deep blank run")
+        lines.append(b"|" + b" " * blanks + "　".encode())
+        lines.append(b" " * blanks + b"depth-0 blanks")
+    lines.append(b"@after")
+    return b"\n".join(lines) + b"\n"
+
+
+def test_oracle_equals_chain_form_on_long_prefixes():
+    text = long_prefix_doc()
+    off, nodes = orc.parse(text)
+    parent = orc.links(nodes)[0]
+    f, c = orc.bridge(text, nodes, parent)
+    f2, c2 = chain_form(text, nodes, parent)
+    assert np.array_equal(f, f2) and np.array_equal(c, c2)
+    # both block and no-block Text lines past the first window
+    assert (f & ol.BRIDGE_BLOCK).any() and not (f & ol.BRIDGE_BLOCK).all()
+
+
+@pytest.mark.gpu
+def test_gpu_long_prefixes(parser):
+    assert_bridge(parser, long_prefix_doc(), "long prefixes")
+
+
 @pytest.mark.gpu
 def test_gpu_generated_profiles(parser):
     for name in datagen.PROFILES:
