@@ -331,7 +331,8 @@ def main():
             "upload_s_pcie_inclusive": round(upload_s, 3),
             "upload_scan_gbs_warm": round(n / upload2_s / 1e9, 2),
             "kernels": kernels,
-            "tuning": tuning and {"scan_bpt": tuning["scan_bpt"], "tree_rounds": tuning["tree_rounds"]},
+            "tuning": tuning and {"scan_bpt": tuning["scan_bpt"], "tree_rounds": tuning["tree_rounds"],
+                                  "sweep_kernel_ms": tuning["sweep"]},
             "dropin_variant": dropin,
         },
         "roofline": {

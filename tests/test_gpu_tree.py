@@ -91,6 +91,23 @@ def test_adversarial_shapes(parser):
         assert_tree_matches(parser, text_from_depths(lines), name)
 
 
+def test_every_tile_geometry(parser):
+    """The tuner's tree geometries (64 x rounds lines per wave): tile edges,
+    random depths and blank runs under each."""
+    rng = np.random.default_rng(9)
+    n = 8192 * 3 + 77
+    lines = [(int(rng.integers(8)), bool(rng.random() < 0.2)) for _ in range(n)]
+    text = text_from_depths(lines)
+    deep = text_from_depths([(min(31, i // 300), i % 97 == 0) for i in range(n)] + [(0, False)])
+    try:
+        for rounds in (4, 1, 16):
+            parser.tune_set("tree_rounds", rounds)
+            assert_tree_matches(parser, text, f"random, {rounds} rounds")
+            assert_tree_matches(parser, deep, f"deep runs, {rounds} rounds")
+    finally:
+        parser.tune_set("tree_rounds", 16)
+
+
 def test_depth_limit_and_recovery(parser, known):
     c = known["tree_depth_limit"]
     parser.build_tree(parser.upload(c["ok"]))
