@@ -79,6 +79,12 @@ __device__ __forceinline__ void load_keys(const uint8_t *keys, const TileSeg &ts
         return;
     }
     const uint8_t *const kd = keys + ts.g0;
+    if ((uint64_t)wbase + TR * 64 <= ts.L) {   // the whole wave inside the document
+#pragma unroll
+        for (int r = 0; r < TR; r++)
+            key[r] = kd[wbase + r * 64 + lane];
+        return;
+    }
     const uint64_t last = ts.L - 1;
     uint32_t v[TR];
 #pragma unroll
@@ -130,42 +136,76 @@ __device__ __forceinline__ void lane_bit_masks(uint64_t (&dm)[5])
 }
 
 // Per wave and per tile: for each level the last content line (+1, 0 = none)
-// and the first (NONE = none).  Per round a lane keeps only whether its level
-// occurs (a bit per round) and the round's {key == d} mask in LDS; the last
-// and first line come from the last and first such round at the end.
+// and the first (NONE = none).  Lane l takes the TR consecutive lines
+// wbase + l TR + k of the wave's range and notes, per level it meets, the
+// run-local index of its last and first line there (LDS, [level][lane]) and a
+// presence bit; 32 ballots of the presence bits then give lane d the runs that
+// hold level d, and the last and first of them hold the wave's answers.  No
+// lane waits on another lane's line.
 template <int TR>
 __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
 {
-    static_assert(TR <= 32, "a presence bit per round");
-    __shared__ uint64_t s_eq[TW / 64][TR][D];
-    __shared__ uint32_t s_l[TW / 64][D], s_f[TW / 64][D];
+    static_assert(TR <= 64 && 64 * TR <= (D + 1) * 64 * 2, "a run fits the level rows");
+    constexpr int NWV = TW / 64;
+    __shared__ uint8_t s_run[NWV][2][D + 1][64];   // [last, first][level][lane]; row D takes non-content lines
+    __shared__ uint32_t s_l[NWV][D], s_f[NWV][D];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t t = blockIdx.x;
     const TileSeg ts = tile_seg(a, t);
     const uint32_t wbase = ts.lt * (TW * TR) + w * (TR * 64);
+    uint8_t (*const lastk)[64] = s_run[w][0];
+    uint8_t (*const firstk)[64] = s_run[w][1];
+
+    // keys: loaded a round of 64 lines at a time (coalesced), turned into runs
     uint32_t key[TR];
-    load_keys<TR>(a.keys, ts, wbase, key);
-    uint64_t dm[5];
-    lane_bit_masks(dm);
+    {
+        uint32_t kr[TR];
+        load_keys<TR>(a.keys, ts, wbase, kr);
+        uint8_t *const kb = &lastk[0][0];
+#pragma unroll
+        for (int r = 0; r < TR; r++)
+            kb[r * 64 + lane] = (uint8_t)kr[r];
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < TR; k++)
+            key[k] = kb[lane * TR + k];
+        wave_lds_sync();
+    }
     // the depth error flag is set by the walk (the next launch but one)
     if (t == 0 && threadIdx.x == 0)
         *a.err = 0u;
-    uint32_t pres = 0;
+
+    uint32_t pres = 0;   // levels met in the run
 #pragma unroll
-    for (int r = 0; r < TR; r++) {
-        uint64_t lt, eq;   // eq: lanes whose key == lane id
-        level_masks(key[r], dm, lt, eq);
-        if (lane < D)
-            s_eq[w][r][lane] = eq;
-        pres |= (eq != 0 ? 1u : 0u) << r;
+    for (int k = 0; k < TR; k++) {
+        const uint32_t x = key[k];
+        const bool c = x < (uint32_t)D;
+        const uint32_t xc = c ? x : (uint32_t)D;
+        const bool fresh = c && ((pres >> x) & 1u) == 0u;
+        lastk[xc][lane] = (uint8_t)k;
+        firstk[fresh ? x : (uint32_t)D][lane] = (uint8_t)k;
+        pres |= c ? 1u << x : 0u;
     }
     wave_lds_sync();
+    // lane d: the runs that hold level d.  The presence words of each half
+    // wave are a 32 x 32 bit matrix (lane = row, level = column), transposed
+    // by swapping ever smaller off-diagonal blocks between lanes l and l ^ s;
+    // lane d then holds column d of its half, and the other half's comes
+    // from lane d + 32.
+    uint32_t col = pres;
+#pragma unroll
+    for (int sh = 16; sh >= 1; sh >>= 1) {
+        const uint32_t m = sh == 16 ? 0x0000FFFFu : sh == 8 ? 0x00FF00FFu : sh == 4 ? 0x0F0F0F0Fu
+                         : sh == 2 ? 0x33333333u : 0x55555555u;   // the columns whose bit sh is 0
+        const uint32_t o = (uint32_t)__shfl_xor((int)col, sh, 64);
+        col = (lane & sh) ? ((col & ~m) | ((o & ~m) >> sh)) : ((col & m) | ((o & m) << sh));
+    }
+    const uint64_t md = (uint64_t)(uint32_t)__shfl_xor((int)col, 32, 64) << 32 | col;
     uint32_t last = 0, first = OL_NONE;
-    if (lane < D && pres) {
-        const uint32_t rl = 31u - (uint32_t)__clz(pres), rf = (uint32_t)__builtin_ctz(pres);
-        const uint64_t el = s_eq[w][rl][lane], ef = s_eq[w][rf][lane];
-        last = wbase + rl * 64 + 64 - (uint32_t)__clzll(el);   // index + 1
-        first = wbase + rf * 64 + (uint32_t)__builtin_ctzll(ef);
+    if (lane < D && md) {
+        const uint32_t rl = 63u - (uint32_t)__clzll(md), rf = (uint32_t)__builtin_ctzll(md);
+        last = wbase + rl * TR + lastk[lane][rl] + 1u;   // index + 1
+        first = wbase + rf * TR + firstk[lane][rf];
     }
     if (lane < D) {
         s_l[w][lane] = last;
