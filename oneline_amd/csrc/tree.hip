@@ -175,17 +175,21 @@ __global__ __launch_bounds__(TW) void tree_summary_kernel(TreeArgs a)
     if (t == 0 && threadIdx.x == 0)
         *a.err = 0u;
 
+    // the last line of each level: written forward, the last write stays; the
+    // first: the same addresses written backward
     uint32_t pres = 0;   // levels met in the run
+    uint32_t at[TR];
 #pragma unroll
     for (int k = 0; k < TR; k++) {
         const uint32_t x = key[k];
         const bool c = x < (uint32_t)D;
-        const uint32_t xc = c ? x : (uint32_t)D;
-        const bool fresh = c && ((pres >> x) & 1u) == 0u;
-        lastk[xc][lane] = (uint8_t)k;
-        firstk[fresh ? x : (uint32_t)D][lane] = (uint8_t)k;
+        at[k] = (c ? x : (uint32_t)D) * 64u + (uint32_t)lane;
+        lastk[0][at[k]] = (uint8_t)k;
         pres |= c ? 1u << x : 0u;
     }
+#pragma unroll
+    for (int k = TR - 1; k >= 0; k--)
+        firstk[0][at[k]] = (uint8_t)k;
     wave_lds_sync();
     // lane d: the runs that hold level d.  The presence words of each half
     // wave are a 32 x 32 bit matrix (lane = row, level = column), transposed
@@ -242,7 +246,7 @@ template <int ST>
 __global__ __launch_bounds__(ST) void tree_scan_kernel(TreeArgs a)
 {
     constexpr int NWS = ST / 64;
-    constexpr int K = 4;                 // tiles per thread per chunk
+    constexpr int K = 8;                 // tiles per thread per chunk (8,192 per chunk: one chunk at 2 GiB)
     constexpr int CH = ST * K;
     __shared__ uint64_t s_v[CH];
     __shared__ uint64_t s_w[NWS];
