@@ -936,6 +936,26 @@ __device__ __forceinline__ ManyCand many_candidates(const ManyArgs &a, uint32_t 
 __device__ __forceinline__ void many_list_chunk(const ManyArgs &a, const ManyLane &m, ManyCand &cand, uint32_t c0,
                                                 uint16_t *l_pos, uint8_t *l_rem)
 {
+    if (a.ndocs == 0) {   // one document: every bound is the text end, nothing to load
+        const uint32_t n = a.n;
+#pragma unroll
+        for (uint32_t h = 0; h < MANY_H; h++) {
+            uint32_t at = cand.ex[h];
+            uint32_t q = cand.q[h];
+            while (q && at < c0 + MANY_WLIST) {
+                const uint32_t lp = h * MANY_PART + m.lo + (uint32_t)__builtin_ctz(q);
+                q &= q - 1;
+                const uint32_t gp = m.base + lp;
+                const uint32_t rm = gp < n ? n - gp : 0u;
+                l_pos[at - c0] = (uint16_t)lp;
+                l_rem[at - c0] = (uint8_t)(rm < 64u ? rm : 64u);
+                at++;
+            }
+            cand.q[h] = q;
+            cand.ex[h] = at;
+        }
+        return;
+    }
     uint32_t di = m.di0, dlim = m.dlim0, dnext = m.dnext0;
 #pragma unroll
     for (uint32_t h = 0; h < MANY_H; h++) {
