@@ -373,7 +373,9 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
     const uint32_t t = blockIdx.x;
     const uint32_t T = a.num_tiles;
     const TileSeg ts = tile_seg(a, t);
-    const uint64_t L = ts.L;
+    // a document's lines are numbered in 32 bits (its offsets are u32), so the
+    // per-line bound checks compare 32-bit values
+    const uint32_t L = (uint32_t)ts.L;
     const uint32_t wbase = ts.lt * (TW * TR) + w * (TR * 64);
     uint32_t key[TR];
     load_keys<TR>(a.keys, ts, wbase, key);
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(TW) void tree_walk_kernel(TreeArgs a)
         // by the error flag)
         const uint32_t cpar = pm ? rb + jp : (gv ? gv - 1 : OL_NONE);
         const uint32_t plv = pm ? kp : gl;
-        const uint32_t cse = sm ? rb + js : (hv != OL_NONE ? hv : (uint32_t)L);
+        const uint32_t cse = sm ? rb + js : (hv != OL_NONE ? hv : L);
         const uint32_t slv = sm ? ks : hl;
         // the next content line is a child exactly when it comes before
         // subtree_end (the first later line with key <= x); NONE > L >= cse
