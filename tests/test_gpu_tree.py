@@ -73,7 +73,11 @@ def test_run_and_tile_boundaries(parser):
 
 def test_adversarial_shapes(parser):
     n = 4096 * 5 + 123
+    rng = np.random.default_rng(11)
+    walk = np.clip(np.cumsum(rng.integers(-1, 2, n)), 0, 31)
     shapes = {
+        "random levels over all 32": [(int(rng.integers(32)), bool(rng.random() < 0.1)) for _ in range(n)],
+        "random walk over all 32": [(int(d), bool(rng.random() < 0.05)) for d in walk],
         "flat depth 1": [(1, False)] * n,
         "flat depth 0": [(0, False)] * n,
         "header then flat": [(0, False)] + [(1, False)] * (n - 1),
@@ -99,11 +103,13 @@ def test_every_tile_geometry(parser):
     lines = [(int(rng.integers(8)), bool(rng.random() < 0.2)) for _ in range(n)]
     text = text_from_depths(lines)
     deep = text_from_depths([(min(31, i // 300), i % 97 == 0) for i in range(n)] + [(0, False)])
+    every = text_from_depths([(int(rng.integers(32)), bool(rng.random() < 0.1)) for _ in range(n)])
     try:
         for rounds in (4, 1, 16):
             parser.tune_set("tree_rounds", rounds)
             assert_tree_matches(parser, text, f"random, {rounds} rounds")
             assert_tree_matches(parser, deep, f"deep runs, {rounds} rounds")
+            assert_tree_matches(parser, every, f"random over all 32 levels, {rounds} rounds")
     finally:
         parser.tune_set("tree_rounds", 16)
 
